@@ -1,0 +1,123 @@
+/*
+ * qmi.h -- C-ABI of libqmi.so, the MI355X (gfx950) engine for Q's qdyn MD step.
+ *
+ * The reference exposes this path as a process and file interface: a
+ * window is one `qdyn <input.inp>` run (argv rule src/q6/md.f90:2568-2577)
+ * that reads the .inp (md.f90:2528-3262), the Qprep topology
+ * (topo.f90:529-1082), the .fep file (qatom.f90:253-548, 874-1745) and an
+ * optional restart, and writes the per-state energy file (.en,
+ * nrgy.f90:94-119, md.f90:4156-4159), the final/backup restart
+ * (md.f90:15452-15470) and the stdout log (md.f90:15292-15438).  Each entry
+ * point below replaces one part of that process; a caller (ctypes, or the
+ * qmi-qdyn command) binds them with plain pointers and sizes.
+ *
+ * Conventions: status 0 = ok, negative = error (text in qmi_last_error());
+ * the library never calls exit().  Atom indices are 0-based.  Buffers are
+ * owned by the caller; `cap` is the element capacity.  One engine per host
+ * thread; calls on one engine are serialised by the caller.
+ */
+#ifndef QMI_H
+#define QMI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qmi_engine qmi_engine;
+
+#define QMI_ABI_VERSION 1
+
+/* Energy layout of qmi_get_energies: the `energies` type of nrgy.f90:43-49
+ * flattened in declaration order: potential, kinetic, LRF, p(bond, angle,
+ * torsion, improper), w(4), q(4), pp(el, vdw), pw(2), ww(2), qx(2),
+ * restraint(total, fix, shell, protein, solvent_radial, water_pol). */
+#define QMI_N_ENERGIES 29
+/* Per-state layout of qmi_get_q_energies: the `q_energies` type of
+ * nrgy.f90:51-58 (lambda, total, q bond/angle/torsion/improper, qx el/vdw,
+ * qq el/vdw, qp el/vdw, qw el/vdw, restraint) -- the 15 doubles of an .en
+ * record. */
+#define QMI_N_Q_ENERGIES 15
+
+/* pair-list kinds for qmi_get_pairlist (md.f90:5175 nbpp, 6684 nbpw,
+ * 8233 nbww, 7875 nbqp, 8066 nbqw) */
+enum { QMI_LIST_PP = 0, QMI_LIST_PW = 1, QMI_LIST_WW = 2, QMI_LIST_QP = 3, QMI_LIST_QW = 4 };
+
+/* Version of this interface; bumped on any layout change. No reference
+ * counterpart. */
+int qmi_abi_version(void);
+
+/* Create an engine on HIP device `hip_device`.  Replaces the process start of
+ * qdyn (qdyn.f90:64-90). */
+int qmi_engine_create(int hip_device, qmi_engine** out);
+
+/* Add one window: reads `inp_path` as qdyn reads its last argument
+ * (md.f90:2528-3262); file names inside it are relative to its directory.
+ * `seed_override` >= 0 replaces [MD] random_seed.  The window's outputs go
+ * where the .inp names them.  Replaces initialize/topology/prep_coord/
+ * get_fep/prep_sim/init_shake/make_nbqqlist/shrink_topology and the
+ * Maxwell start (qdyn.f90:94-116). */
+int qmi_window_add(qmi_engine* e, const char* inp_path, long long seed_override, int* window_id);
+
+/* Upload all windows to the GPU, build the static Q-atom lists, pick tiles
+ * and capture the step graph.  Replaces distribute_nonbonds
+ * (md.f90:1044-1361) and the set-up half of md_run (md.f90:3917-4000). */
+int qmi_prepare(qmi_engine* e);
+
+/* Advance every window by `nsteps` MD steps (each window stops at its own
+ * [MD] steps), writing its .en/.re/log at the intervals of its .inp; the
+ * step loop of md_run (md.f90:4003-4182).  With `finalize` != 0 the final
+ * pair lists, energies, summary and restart of md_run (md.f90:4207-4213)
+ * are produced for windows that have completed. */
+int qmi_run(qmi_engine* e, long long nsteps, int finalize);
+
+/* Evaluate pair lists (if `rebuild_lists`) and the potential energy and
+ * gradient at the current coordinates without moving atoms: make_pair_lists
+ * + pot_energy (md.f90:3716, 13278). */
+int qmi_evaluate(qmi_engine* e, int rebuild_lists);
+
+/* Energies of window w from the last energy evaluation (QMI_N_ENERGIES
+ * doubles) -- E of md.f90 (type energies, nrgy.f90:43-49). */
+int qmi_get_energies(qmi_engine* e, int w, double* out, int cap);
+
+/* Per-state Q energies, nstates x QMI_N_Q_ENERGIES doubles -- EQ(:)
+ * (qatom.f90:150, nrgy.f90:51-58). Returns the number of states. */
+int qmi_get_q_energies(qmi_engine* e, int w, double* out, int cap);
+
+/* Coordinates / velocities / gradient of window w, 3N doubles in Fortran
+ * atom order (x, v, d of md.f90:222-225). */
+int qmi_get_coords(qmi_engine* e, int w, double* xyz, long long cap);
+int qmi_get_velocities(qmi_engine* e, int w, double* v, long long cap);
+int qmi_get_forces(qmi_engine* e, int w, double* d, long long cap);
+/* Replace coordinates / velocities of window w (3N doubles). */
+int qmi_set_state(qmi_engine* e, int w, const double* xyz, const double* v, long long n3);
+
+/* Pair list of window w from the last list build, as 1-based topology atom
+ * pairs (i, j) and LJ code (1, 2, 3): the entries of nbpp/nbpw/nbqp, the
+ * O-O molecule pairs of nbww (j = O atom), and the O atoms of nbqw (i = 0).
+ * Pairs are unordered; `*n` returns the total count even when it exceeds
+ * `cap`. */
+int qmi_get_pairlist(qmi_engine* e, int w, int kind, int32_t* i, int32_t* j, int16_t* ljcod,
+                     long long cap, long long* n);
+
+/* Size of window w: natom, nat_solute, nwat, nstates, nqat (5 ints). */
+int qmi_window_info(qmi_engine* e, int w, int32_t* info, int cap);
+
+/* Write window w's restart file (write_xfin, md.f90:15452-15470). */
+int qmi_write_restart(qmi_engine* e, int w, const char* path);
+
+/* Device time per kernel family of the last qmi_run, in microseconds per
+ * step, for the bench roofline (no reference counterpart).  Order:
+ * nonbonded tiles, LRF build, list build, bonded+restraints, integrate+SHAKE,
+ * total.  Returns the number of values written. */
+int qmi_timings(qmi_engine* e, double* out, int cap);
+
+/* Last error message of the calling thread. */
+const char* qmi_last_error(void);
+
+void qmi_engine_destroy(qmi_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -57,13 +57,14 @@ program qprobe
   call dump_vec('X0', x, 3*natom)
   call dump_vec('V0', v, 3*natom)
 
-  ! step 0: pair lists and potential energy
+  ! step 0 as md_run does it: temperature first (md.f90:3969-3976, it sets
+  ! Tfree used by restrain_solvent), then pair lists and potential energy
   istep = 0
-  call make_pair_lists
-  call pot_energy
   Ekinmax = 1000.0*Ndegf*boltz*Temp0/2.0/real(natom)
   call temperature(Temp, Tsolute, Tsolvent, Ekinmax)
   write(u,'(a,5es25.16e3)') 'TEMP0', Temp, Tfree, Tsolute, Tsolvent, E%kinetic
+  call make_pair_lists
+  call pot_energy
   call dump_energies('E0')
   call dump_vec('D0', d, 3*natom)
 

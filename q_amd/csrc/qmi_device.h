@@ -1,0 +1,154 @@
+// Device-side layout of the batched engine.  Every window's atoms, groups,
+// clusters and terms live in flat arrays with per-window offsets, so each
+// kernel launch covers all windows at once.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace qmi {
+
+constexpr int kMaxS = 4;            // states handled on the GPU (2 in FEP windows)
+constexpr double kFScale = 4294967296.0;        // 2^32: forces, fixed point
+constexpr double kEScale = 68719476736.0;       // 2^36: energies, fixed point
+constexpr int kCl = 8;               // cluster size (8 x 8 tile = one wave64)
+
+// energy slots per window (int64 fixed point), order used by the finaliser
+enum ESlot {
+  E_PP_EL, E_PP_VDW, E_PW_EL, E_PW_VDW, E_WW_EL, E_WW_VDW, E_LRF,
+  E_P_BOND, E_P_ANG, E_P_TOR, E_P_IMP, E_W_BOND, E_W_ANG, E_W_TOR, E_W_IMP,
+  E_FIX, E_SHELL, E_RAD, E_WPOL,
+  E_NBASE,
+  // per state s: E_Q0 + s*8 + {qbond, qangle, qtors, qimp, qq_el, qq_vdw, qp_el, qp_vdw}
+  // and E_QW0 + s*2 + {qw_el, qw_vdw}
+  E_Q0 = E_NBASE,
+  E_QW0 = E_Q0 + 8 * kMaxS,
+  E_NSLOT = E_QW0 + 2 * kMaxS
+};
+
+struct WinDev {
+  int ao, natom, nat_solute, nwat;
+  int nq, S, qvdw, natyps;
+  int q_off, t_off;                 // first global q-atom / first entry of the type tables
+  int qs_off, qsc_off;              // first entry of the [q][state] and [q][type][state] tables
+  int g_off, ngrp, ngrp_solute;     // charge groups
+  int wc_off, nwc, sc_off, nsc, qc_off, nqc;
+  int sh_off, nshell;               // water polarisation shells
+  int nsteps, nbcycle, iene, iout;
+  int use_lrf, wpol, separate_scaling, iac_ow;   // iac_ow: 0-based type of water O
+  int ndegf, ndegfree, ndegf_solute, ndegfree_solute, ndegf_solvent, ndegfree_solvent;
+  float el14, crg_ow, crg_hw, aO1, bO1;
+  double dt, tau_t, temp0;
+  double rcpp2, rcpw2, rcww2, rclrf2;
+  double fk_pshell, fk_wsphere, fkwpol, dwmz, awmz, rwat;
+  double xwcent[3], xpcent[3];
+  double lambda[kMaxS];
+};
+
+struct WinState {
+  int istep;          // current step (counts from 0 at the start of the run)
+  int stop_at;        // the current qmi_run stops this window at this step
+  int list_now, energy_now, active;
+  double tscale_solute, tscale_solvent;
+  double temp, tfree, tfree_solute, tfree_solvent, temp_solute, temp_solvent, ekin;
+  double ekin_parts[4];  // solute all, solute free, solvent all, solvent free
+  int shake_fail;
+  long long E[E_NSLOT];
+};
+
+struct ShellDev { float rout, dr, cstb, theta_corr, avtheta, avn_insh; int n_insh; int pad; };
+
+// tile types (i-centric): WW, W<-P, P<-W, P<-P, Q<-P, P<-Q, Q<-W, W<-Q
+enum TileType { T_WW = 0, T_WP, T_PW, T_PP, T_QP, T_PQ, T_QW, T_WQ, T_NTYPES };
+
+struct TileList {
+  int* start;      // per i-cluster: first entry (fixed-capacity layout)
+  int* count;      // per i-cluster: number of entries
+  int* jc;         // j-cluster (global index)
+  uint2* mask;     // 64-bit lane mask: bit (a*8 + b)
+  uint2* m14;      // 1-4 mask (P<-P, Q<-P, P<-Q)
+  int nicl;        // number of i-clusters (all windows)
+  int cap_total;
+};
+
+struct Dev {
+  int nwin, ntot;
+  const WinDev* win;
+  WinState* ws;
+  // atoms
+  double *x, *v, *xx, *xtop;
+  float4* x32;          // x, y, z, scaled charge
+  float4* ljp;          // avdw1, bvdw1, avdw3, bvdw3
+  long long* f;         // gradient, 3 per atom, fixed point 2^-32
+  const float* winv;
+  const double* mass;
+  const int* wid;       // window of atom
+  const int* iac;       // 0-based window-local atom type
+  const int* grp;       // charge group (global) of atom
+  const uint8_t* aflag; // bit0 excl, bit1 shell, bit2 heavy, bit3 q-atom
+  // charge groups
+  const int* g_sw;      // switch atom (global); water groups: O
+  const int* g_first;   // into g_atoms
+  const int* g_n;
+  const int* g_atoms;   // global atoms
+  const int* g_win;
+  double* g_cent;       // 3 per group
+  double* g_lrf;        // 20 per group: phi0, phi1(3), phi2(6: xx xy xz yy yz zz), phi3(10)
+  // special partners of solute atoms (exclusions / 1-4), CSR
+  const int* sp_start;  // per atom (ntot+1)
+  const int* sp_j;
+  const uint8_t* sp_t;  // 1 = excluded, 2 = 1-4
+  // clusters
+  const int* wc_o;      // 8 per water cluster: O atom or -1
+  const int* sc_a;      // 8 per solute cluster: atom or -1
+  const int* qc_q;      // 8 per q cluster: window-local q index or -1
+  const int* wc_win; const int* sc_win; const int* qc_win;
+  double* wc_bs;        // bounding sphere per water cluster (x, y, z, r)
+  double* sc_bs;        // bounding sphere of the switch atoms of the members
+  // q tables (per window, offset q_off)
+  const int* q_atom;     // global atom
+  const float* q_crg;    // [q*S + s]
+  const float4* q_lj;    // [q*S + s]: qavdw1, qbvdw1, qavdw3, qbvdw3 (or topology params)
+  const double* q_ljd;   // same in double, for the energy path
+  const float* q_sc;     // [(q*natyps + t)*S + s]
+  const double* q_scd;
+  // topology LJ params in double per window type (energy path)
+  const double* t_lj;    // [(t_off + t)*4]
+  // tiles
+  TileList tl[T_NTYPES];
+  // q-q pairs (static)
+  int nqq;
+  const int4* qq_ij;     // iq (window-local), j (global), jq (1-based q or 0), ljcod
+  const float* qq_es;    // el_scale
+  const int* qq_st;      // state
+  const int* qq_win;
+  const float* qq_sc;    // soft-core term of the pair
+  // bonded terms
+  int nbond, nang, ntor, nimp;
+  const int2* b_ij; const double2* b_par; const int* b_win; const uint8_t* b_solv;
+  const int4* a_ijk; const double4* a_par; const int* a_win; const uint8_t* a_solv;  // fk, ang0, ureyfk, ureyr0
+  const int4* t_ijkl; const float4* t_par; const int* t_win; const uint8_t* t_solv;  // fk, rmult, deltor, paths
+  const int4* i_ijkl; const float2* i_par; const int* i_win; const uint8_t* i_solv;
+  const int* i_kind;     // 1 harmonic (improper), 2 periodic (improper2) per window
+  const int* urey;       // per window: CHARMM Urey-Bradley on
+  // q bonded (one entry per term and state)
+  int nqb, nqa, nqt, nqi;
+  const int2* qb_ij; const double4* qb_par; const int2* qb_ws;   // Dmz, amz, r0, fk; (window, state)
+  const int4* qa_ijk; const double2* qa_par; const int2* qa_ws;
+  const int4* qt_ijkl; const double4* qt_par; const int2* qt_ws; // fk, rmult, deltor
+  const int4* qi_ijkl; const double2* qi_par; const int2* qi_ws; // fk, imp0
+  // shake units (independent constraint components)
+  int nsu;
+  const int* su_start; const int* su_n; const int* su_win;
+  const int* sc_i; const int* sc_j; const double* sc_d2;
+  // water shells
+  ShellDev* shells;
+  double* wtheta;        // per water (global water index), theta
+  int* wshell;           // shell index (-1 if none)
+  const int* w_off;      // per window: first global water index
+  // energy snapshots for output: [ring slot][window][E_NSLOT] + temperature
+  long long* e_ring;
+  double* t_ring;        // [slot][window][4]: temp, tfree, ekin, istep
+  int ring_n;
+};
+
+}  // namespace qmi

@@ -1,0 +1,1183 @@
+// Engine host code and the extern "C" boundary declared in include/qmi.h.
+//
+// An engine owns one HIP stream and all device memory of its windows.  At
+// qmi_prepare every window is flattened into the batched device layout of
+// qmi_device.h (per-window offsets), the static Q-atom tile lists are built
+// and the initial thermostat state is computed; qmi_run then launches, per
+// MD step and for all windows at once: list build (at list-update steps),
+// nonbonded tiles, LRF, bonded and restraint kernels, leapfrog, SHAKE and
+// the thermostat.  Energy snapshots go to a device ring that the host drains
+// after each chunk of steps to write .en records and the log.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/qmi.h"
+#include "qmi_device.h"
+#include "qmi_kernels.h"
+#include "qmi_types.h"
+
+using namespace qmi;
+
+static thread_local std::string g_err;
+static int set_err(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+
+#define HIP_OK(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) return set_err(-100, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #call); \
+  } while (0)
+
+namespace {
+
+struct HostWin {
+  std::unique_ptr<Window> w;
+  std::string inp_path, log_path;
+  FILE* en = nullptr;
+  FILE* log = nullptr;
+  // static lists kept for qmi_get_pairlist
+  std::vector<int> qw_waters;                 // O atoms (window-local) of nbqw
+  std::vector<std::array<int, 3>> qp_pairs;   // iq (0-based), j (local), ljcod
+  // last energies (host)
+  double E[QMI_N_ENERGIES] = {0};
+  std::vector<double> EQ;                     // S x 15
+  double temp = 0, tfree = 0, ekin = 0;
+  double rad = 0, wpol = 0;                   // solvent radial / polarisation restraint energies
+  int istep = 0;                              // host mirror of the device step
+  bool done = false;
+};
+
+}  // namespace
+
+struct qmi_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<HostWin> wins;
+  bool prepared = false;
+  Dev d{};
+  std::vector<void*> bufs;
+  int ngrp = 0, nwc = 0, nsc = 0, nqc = 0, ntot = 0, nwat_total = 0;
+  int shake_any = 0;
+  // host-side tables for pair-list export
+  std::vector<int> h_wc_o, h_sc_a, h_qc_q, h_atom_win;
+  std::vector<double> timings;
+  ~qmi_engine();
+};
+
+template <class T>
+static int dev_upload(qmi_engine* e, T** dst, const std::vector<T>& src, size_t min_n = 1) {
+  size_t n = std::max(src.size(), min_n);
+  void* p = nullptr;
+  HIP_OK(hipMalloc(&p, n * sizeof(T)));
+  e->bufs.push_back(p);
+  if (!src.empty()) HIP_OK(hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  else HIP_OK(hipMemset(p, 0, n * sizeof(T)));
+  *dst = (T*)p;
+  return 0;
+}
+template <class T>
+static int dev_alloc(qmi_engine* e, T** dst, size_t n) {
+  void* p = nullptr;
+  HIP_OK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  HIP_OK(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  e->bufs.push_back(p);
+  *dst = (T*)p;
+  return 0;
+}
+
+qmi_engine::~qmi_engine() {
+  for (void* p : bufs) (void)hipFree(p);
+  if (stream) (void)hipStreamDestroy(stream);
+  for (auto& h : wins) {
+    if (h.en) fclose(h.en);
+    if (h.log) fclose(h.log);
+  }
+}
+
+// ---------------------------------------------------------- prepare --
+static uint64_t morton(double x, double y, double z) {
+  auto sp = [](uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+  };
+  auto q = [](double c) { return (uint64_t)std::max(0.0, std::floor((c + 1000.0) / 2.5)); };
+  return sp(q(x)) | (sp(q(y)) << 1) | (sp(q(z)) << 2);
+}
+
+static int build_device(qmi_engine* e) {
+  const int NW = (int)e->wins.size();
+  std::vector<WinDev> wd(NW);
+  std::vector<WinState> wst(NW);
+  // atoms
+  std::vector<double> x, v, xtop, mass;
+  std::vector<float4> x32, ljp;
+  std::vector<float> winv;
+  std::vector<int> wid, iac, grp;
+  std::vector<uint8_t> aflag;
+  // groups
+  std::vector<int> g_sw, g_first, g_n, g_atoms, g_win;
+  // special partners
+  std::vector<int> sp_start, sp_j;
+  std::vector<uint8_t> sp_t;
+  // clusters
+  std::vector<int> wc_o, sc_a, qc_q, wc_win, sc_win, qc_win;
+  // q tables
+  std::vector<int> q_atom;
+  std::vector<float> q_crg, q_sc;
+  std::vector<float4> q_lj;
+  std::vector<double> q_ljd, q_scd, t_lj;
+  // q-q
+  std::vector<int4> qq_ij;
+  std::vector<float> qq_es, qq_sc;
+  std::vector<int> qq_st, qq_win;
+  // bonded
+  std::vector<int2> b_ij; std::vector<double2> b_par; std::vector<int> b_win; std::vector<uint8_t> b_solv;
+  std::vector<int4> a_ijk; std::vector<double4> a_par; std::vector<int> a_win; std::vector<uint8_t> a_solv;
+  std::vector<int4> t_ijkl; std::vector<float4> t_par; std::vector<int> t_win; std::vector<uint8_t> t_solv;
+  std::vector<int4> i_ijkl; std::vector<float2> i_par; std::vector<int> i_win; std::vector<uint8_t> i_solv;
+  std::vector<int> i_kind(NW), urey(NW);
+  std::vector<int2> qb_ij, qb_ws, qa_ws, qt_ws, qi_ws;
+  std::vector<double4> qb_par, qt_par;
+  std::vector<int4> qa_ijk, qt_ijkl, qi_ijkl;
+  std::vector<double2> qa_par, qi_par;
+  // shake
+  std::vector<int> su_start, su_n, su_win, sc_i, sc_j;
+  std::vector<double> sc_d2;
+  // shells
+  std::vector<ShellDev> shells;
+  std::vector<int> w_off(NW);
+  // static Q tiles (host-built), per type: per i-cluster lists
+  struct HT { std::vector<std::vector<std::pair<int, uint2>>> l; std::vector<std::vector<uint2>> m14; };
+  HT qt[4];  // T_QP, T_PQ, T_QW, T_WQ
+  int ao = 0, goff = 0, woff = 0, qoff = 0, toff = 0;
+  e->shake_any = 0;
+  for (int wi = 0; wi < NW; ++wi) {
+    Window& W = *e->wins[wi].w;
+    const Topology& t = W.top;
+    const Fep& fp = W.fep;
+    const InputParams& p = W.inp;
+    const int N = W.natom, NS = t.nat_solute, S = p.nstates > 0 ? p.nstates : 0;
+    if (S > kMaxS) return set_err(-10, "more than 4 FEP states are not supported on the GPU");
+    if (!t.lj2.empty()) return set_err(-10, "LJ type-2 pairs are not supported");
+    WinDev& D = wd[wi];
+    std::memset(&D, 0, sizeof(D));
+    D.ao = ao; D.natom = N; D.nat_solute = NS; D.nwat = W.nwat;
+    D.nq = fp.nqat; D.S = S; D.qvdw = fp.qvdw_flag ? 1 : 0; D.natyps = t.natyps;
+    D.q_off = qoff; D.t_off = toff;
+    D.qs_off = (int)q_crg.size(); D.qsc_off = (int)q_sc.size(); D.g_off = goff; D.ngrp = (int)t.cgp.size(); D.ngrp_solute = t.ncgp_solute;
+    D.nsteps = p.nsteps; D.nbcycle = p.nbcycle > 0 ? p.nbcycle : 1;
+    D.iene = p.iene_cycle > 0 ? p.iene_cycle : 0; D.iout = p.iout_cycle > 0 ? p.iout_cycle : 0;
+    D.use_lrf = p.use_lrf; D.wpol = p.wpol_restr && W.nwat > 0; D.separate_scaling = p.separate_scaling;
+    D.iac_ow = W.nwat > 0 ? t.iac[NS] - 1 : 0;
+    D.ndegf = W.ndegf; D.ndegfree = W.ndegfree; D.ndegf_solute = W.ndegf_solute;
+    D.ndegfree_solute = W.ndegfree_solute; D.ndegf_solvent = W.ndegf_solvent; D.ndegfree_solvent = W.ndegfree_solvent;
+    D.el14 = (float)t.el14_scale; D.crg_ow = W.crg_ow; D.crg_hw = W.crg_hw;
+    if (W.nwat > 0) {
+      D.aO1 = (float)t.iaclib[t.iac[NS]].avdw[0];
+      D.bO1 = (float)t.iaclib[t.iac[NS]].bvdw[0];
+      const AtomType& h = t.iaclib[t.iac[NS + 1]];
+      if (h.avdw[0] != 0.0 || h.bvdw[0] != 0.0 || h.avdw[2] != 0.0)
+        return set_err(-10, "SPC water hydrogens must have zero LJ parameters");
+    }
+    D.dt = p.dt; D.tau_t = p.tau_t; D.temp0 = p.temp0;
+    D.rcpp2 = p.rcpp * p.rcpp; D.rcpw2 = p.rcpw * p.rcpw; D.rcww2 = p.rcww * p.rcww; D.rclrf2 = p.rclrf * p.rclrf;
+    D.fk_pshell = p.fk_pshell; D.fk_wsphere = W.fk_wsphere; D.fkwpol = W.fkwpol; D.dwmz = W.dwmz; D.awmz = W.awmz;
+    D.rwat = W.rwat;
+    for (int k = 0; k < 3; ++k) { D.xwcent[k] = t.xwcent[k]; D.xpcent[k] = t.xpcent[k]; }
+    for (int s = 0; s < S; ++s) D.lambda[s] = p.lambda[s];
+    if (fp.nqat > 0 && p.rcq <= t.rexcl_o)
+      return set_err(-10, "q_atom cut-off must exceed the exclusion radius (static Q-atom lists)");
+
+    // --- atoms
+    for (int i = 0; i < N; ++i) {
+      for (int c = 0; c < 3; ++c) { x.push_back(W.x[3 * i + c]); v.push_back(W.v[3 * i + c]); xtop.push_back(t.xtop[3 * i + c]); }
+      x32.push_back(make_float4((float)W.x[3 * i], (float)W.x[3 * i + 1], (float)W.x[3 * i + 2], t.crg[i]));
+      const AtomType& at = t.iaclib[t.iac[i]];
+      ljp.push_back(make_float4((float)at.avdw[0], (float)at.bvdw[0], (float)at.avdw[2], (float)at.bvdw[2]));
+      winv.push_back(W.winv[i]);
+      mass.push_back(at.mass);
+      wid.push_back(wi);
+      iac.push_back(t.iac[i] - 1);
+      grp.push_back(goff + W.iwhich_cgp[i]);
+      uint8_t fl = 0;
+      if (t.excl[i]) fl |= 1;
+      if (W.shell[i]) fl |= 2;
+      if (t.heavy[i]) fl |= 4;
+      if (W.iqatom[i]) fl |= 8;
+      aflag.push_back(fl);
+      e->h_atom_win.push_back(wi);
+    }
+    for (int ty = 1; ty <= t.natyps; ++ty) {
+      const AtomType& at = t.iaclib[ty];
+      t_lj.push_back(at.avdw[0]); t_lj.push_back(at.bvdw[0]); t_lj.push_back(at.avdw[2]); t_lj.push_back(at.bvdw[2]);
+    }
+    // --- charge groups (water groups use their oxygen as switch atom)
+    for (size_t g = 0; g < t.cgp.size(); ++g) {
+      const Cgp& c = t.cgp[g];
+      int sw = c.iswitch;
+      if ((int)g >= t.ncgp_solute) {
+        int first = t.cgpatom[c.first];
+        sw = NS + 3 * ((first - NS) / 3);
+      }
+      g_sw.push_back(ao + sw);
+      g_first.push_back((int)g_atoms.size());
+      g_n.push_back(c.last - c.first + 1);
+      for (int k = c.first; k <= c.last; ++k) g_atoms.push_back(ao + t.cgpatom[k]);
+      g_win.push_back(wi);
+    }
+    // --- special partners (solute): listex / list14 and the long lists
+    {
+      std::vector<std::vector<std::pair<int, int>>> sp(N);
+      auto addp = [&](int i, int j, int ty) {
+        for (auto& q : sp[i]) if (q.first == j) { q.second = std::min(q.second, ty); return; }
+        sp[i].push_back({j, ty});
+      };
+      for (int i = 0; i < NS; ++i)
+        for (int k = 1; k <= kNbrRange && i + k < NS; ++k) {
+          if (t.listex[i * kNbrRange + k - 1]) { addp(i, i + k, 1); addp(i + k, i, 1); }
+          else if (t.list14[i * kNbrRange + k - 1]) { addp(i, i + k, 2); addp(i + k, i, 2); }
+        }
+      for (auto& pr : t.listexlong) { addp(pr.first, pr.second, 1); addp(pr.second, pr.first, 1); }
+      for (auto& pr : t.list14long) {
+        bool ex = false;
+        for (auto& q : sp[pr.first]) if (q.first == pr.second && q.second == 1) ex = true;
+        if (!ex) { addp(pr.first, pr.second, 2); addp(pr.second, pr.first, 2); }
+      }
+      for (int i = 0; i < N; ++i) {
+        sp_start.push_back((int)sp_j.size());
+        for (auto& q : sp[i]) { sp_j.push_back(ao + q.first); sp_t.push_back((uint8_t)q.second); }
+      }
+    }
+    // --- clusters
+    D.wc_off = (int)wc_win.size();
+    {
+      std::vector<int> wl(W.nwat);
+      std::iota(wl.begin(), wl.end(), 0);
+      std::vector<uint64_t> key(W.nwat);
+      for (int k = 0; k < W.nwat; ++k) {
+        int o = NS + 3 * k;
+        key[k] = morton(W.x[3 * o], W.x[3 * o + 1], W.x[3 * o + 2]);
+      }
+      std::stable_sort(wl.begin(), wl.end(), [&](int a, int b) { return key[a] < key[b]; });
+      for (int k = 0; k < W.nwat; k += kCl) {
+        for (int m = 0; m < kCl; ++m) wc_o.push_back(k + m < W.nwat ? ao + NS + 3 * wl[k + m] : -1);
+        wc_win.push_back(wi);
+      }
+    }
+    D.nwc = (int)wc_win.size() - D.wc_off;
+    D.sc_off = (int)sc_win.size();
+    {
+      std::vector<int> sl;
+      for (int i = 0; i < NS; ++i) if (!W.iqatom[i]) sl.push_back(i);
+      for (size_t k = 0; k < sl.size(); k += kCl) {
+        for (int m = 0; m < kCl; ++m) sc_a.push_back(k + m < sl.size() ? ao + sl[k + m] : -1);
+        sc_win.push_back(wi);
+      }
+    }
+    D.nsc = (int)sc_win.size() - D.sc_off;
+    D.qc_off = (int)qc_win.size();
+    for (int k = 0; k < fp.nqat; k += kCl) {
+      for (int m = 0; m < kCl; ++m) qc_q.push_back(k + m < fp.nqat ? k + m : -1);
+      qc_win.push_back(wi);
+    }
+    D.nqc = (int)qc_win.size() - D.qc_off;
+    // --- q tables
+    for (int q = 0; q < fp.nqat; ++q) {
+      q_atom.push_back(ao + fp.iqseq[q]);
+      for (int s = 0; s < S; ++s) {
+        q_crg.push_back(fp.qcrg[q * S + s]);
+        double la1, lb1, la3, lb3;
+        if (fp.qvdw_flag) {
+          int qt_ = fp.qiac[q * S + s];
+          la1 = fp.qavdw[qt_ * 3 + 0]; lb1 = fp.qbvdw[qt_ * 3 + 0];
+          la3 = fp.qavdw[qt_ * 3 + 2]; lb3 = fp.qbvdw[qt_ * 3 + 2];
+        } else {
+          const AtomType& at = t.iaclib[t.iac[fp.iqseq[q]]];
+          la1 = at.avdw[0]; lb1 = at.bvdw[0]; la3 = at.avdw[2]; lb3 = at.bvdw[2];
+        }
+        q_lj.push_back(make_float4((float)la1, (float)lb1, (float)la3, (float)lb3));
+        q_ljd.push_back(la1); q_ljd.push_back(lb1); q_ljd.push_back(la3); q_ljd.push_back(lb3);
+      }
+    }
+    const int ncol = t.natyps + fp.nqat;
+    for (int q = 0; q < fp.nqat; ++q)
+      for (int ty = 0; ty < t.natyps; ++ty)
+        for (int s = 0; s < S; ++s) {
+          double scv = fp.sc_lookup.empty() ? 0.0 : fp.sc_lookup[((size_t)q * ncol + ty) * S + s];
+          q_sc.push_back((float)scv);
+          q_scd.push_back(scv);
+        }
+    // --- q-q pairs
+    for (int s = 0; s < S; ++s)
+      for (auto& pq : W.nbqq[s]) {
+        qq_ij.push_back(make_int4(pq.iq, ao + pq.j, pq.jq, pq.ljcod));
+        qq_es.push_back((float)pq.el_scale);
+        qq_st.push_back(s);
+        qq_win.push_back(wi);
+        int col = t.natyps + pq.jq - 1;  // sc_lookup(iq, natyps+jq, s), jq = 0 gives column natyps
+        qq_sc.push_back((float)fp.sc_lookup[((size_t)pq.iq * ncol + col) * S + s]);
+      }
+    // --- static Q tiles: nbqplist (md.f90:7875-7964) and nbqwlist (8066-8118)
+    if (fp.nqat > 0) {
+      const double rcq2 = p.rcq * p.rcq;
+      auto r2c = [&](int a) {
+        double dx = W.x[3 * a] - t.xpcent[0], dy = W.x[3 * a + 1] - t.xpcent[1], dz = W.x[3 * a + 2] - t.xpcent[2];
+        return dx * dx + dy * dy + dz * dz;
+      };
+      auto& hw = e->wins[wi];
+      hw.qw_waters.clear();
+      hw.qp_pairs.clear();
+      std::vector<char> wlisted(W.nwat, 0);
+      for (int k = 0; k < W.nwat; ++k) {
+        int o = NS + 3 * k;
+        if (t.excl[o]) continue;
+        if (r2c(o) <= rcq2) { wlisted[k] = 1; hw.qw_waters.push_back(o); }
+      }
+      std::vector<char> plisted(NS, 0);
+      const int nq = fp.nqat;
+      auto qc = [&](int s, int a, int q) { return (int)W.qconn[((size_t)s * NS + a) * nq + q]; };
+      for (int g = 0; g < t.ncgp_solute; ++g) {
+        const Cgp& c = t.cgp[g];
+        if (t.excl[c.iswitch]) continue;
+        if (r2c(c.iswitch) > rcq2) continue;
+        for (int k = c.first; k <= c.last; ++k) {
+          int i = t.cgpatom[k];
+          bool conn = false;
+          for (int s = 0; s < S && !conn; ++s)
+            for (int q = 0; q < nq; ++q) if (qc(s, i, q) <= 3) { conn = true; break; }
+          if (conn) continue;
+          plisted[i] = 1;
+          for (int q = 0; q < nq; ++q) {
+            int lj = W.ljcod[(t.iac[i] - 1) * t.natyps + (t.iac[fp.iqseq[q]] - 1)];
+            if (qc(0, i, q) == 4) lj = 3;
+            hw.qp_pairs.push_back({q, i, lj});
+          }
+        }
+      }
+      // tiles: QW / WQ
+      auto setbit = [](uint2& m, int bit) { if (bit < 32) m.x |= 1u << bit; else m.y |= 1u << (bit - 32); };
+      for (int qcl = 0; qcl < D.nqc; ++qcl) {
+        std::vector<std::pair<int, uint2>> lqw, lqp;
+        std::vector<uint2> m14qp;
+        for (int wc = 0; wc < D.nwc; ++wc) {
+          uint2 m = {0, 0};
+          for (int a = 0; a < kCl; ++a) {
+            int q = qc_q[(D.qc_off + qcl) * kCl + a];
+            if (q < 0) continue;
+            for (int b = 0; b < kCl; ++b) {
+              int o = wc_o[(D.wc_off + wc) * kCl + b];
+              if (o < 0) continue;
+              int k = (o - ao - NS) / 3;
+              if (wlisted[k]) setbit(m, a * 8 + b);
+            }
+          }
+          if (m.x | m.y) lqw.push_back({D.wc_off + wc, m});
+        }
+        for (int sc = 0; sc < D.nsc; ++sc) {
+          uint2 m = {0, 0}, m14 = {0, 0};
+          for (int a = 0; a < kCl; ++a) {
+            int q = qc_q[(D.qc_off + qcl) * kCl + a];
+            if (q < 0) continue;
+            for (int b = 0; b < kCl; ++b) {
+              int at = sc_a[(D.sc_off + sc) * kCl + b];
+              if (at < 0) continue;
+              int li = at - ao;
+              if (!plisted[li]) continue;
+              setbit(m, a * 8 + b);
+              if (qc(0, li, q) == 4) setbit(m14, a * 8 + b);
+            }
+          }
+          if (m.x | m.y) { lqp.push_back({D.sc_off + sc, m}); m14qp.push_back(m14); }
+        }
+        qt[0].l.push_back(lqp); qt[0].m14.push_back(m14qp);   // T_QP
+        qt[2].l.push_back(lqw); qt[2].m14.push_back({});      // T_QW
+      }
+      // transposed lists (i = solute / water cluster, j = q cluster)
+      auto transpose = [](uint2 m) {
+        uint2 r = {0, 0};
+        for (int bit = 0; bit < 64; ++bit) {
+          bool on = bit < 32 ? (m.x >> bit) & 1 : (m.y >> (bit - 32)) & 1;
+          if (!on) continue;
+          int a = bit >> 3, b = bit & 7, nb = b * 8 + a;
+          if (nb < 32) r.x |= 1u << nb; else r.y |= 1u << (nb - 32);
+        }
+        return r;
+      };
+      std::vector<std::vector<std::pair<int, uint2>>> pq(D.nsc), wq(D.nwc);
+      std::vector<std::vector<uint2>> pq14(D.nsc);
+      for (int qcl = 0; qcl < D.nqc; ++qcl) {
+        size_t base = qt[0].l.size() - D.nqc + qcl;
+        for (size_t k = 0; k < qt[0].l[base].size(); ++k) {
+          int sc = qt[0].l[base][k].first - D.sc_off;
+          pq[sc].push_back({D.qc_off + qcl, transpose(qt[0].l[base][k].second)});
+          pq14[sc].push_back(transpose(qt[0].m14[base][k]));
+        }
+        for (auto& pr : qt[2].l[base]) wq[pr.first - D.wc_off].push_back({D.qc_off + qcl, transpose(pr.second)});
+      }
+      for (int sc = 0; sc < D.nsc; ++sc) { qt[1].l.push_back(pq[sc]); qt[1].m14.push_back(pq14[sc]); }
+      for (int wc = 0; wc < D.nwc; ++wc) { qt[3].l.push_back(wq[wc]); qt[3].m14.push_back({}); }
+    } else {
+      for (int sc = 0; sc < D.nsc; ++sc) { qt[1].l.push_back({}); qt[1].m14.push_back({}); }
+      for (int wc = 0; wc < D.nwc; ++wc) { qt[3].l.push_back({}); qt[3].m14.push_back({}); }
+    }
+    // --- bonded terms (after shrink_topology)
+    for (size_t k = 0; k < t.bnd.size(); ++k) {
+      const Bond& b = t.bnd[k];
+      b_ij.push_back(make_int2(ao + b.i, ao + b.j));
+      b_par.push_back(make_double2(t.bondlib[b.cod].fk, t.bondlib[b.cod].b0));
+      b_win.push_back(wi);
+      b_solv.push_back((int)k >= t.nbonds_solute);
+    }
+    for (size_t k = 0; k < t.ang.size(); ++k) {
+      const Angle& a = t.ang[k];
+      const AngLib& l = t.anglib[a.cod];
+      a_ijk.push_back(make_int4(ao + a.i, ao + a.j, ao + a.k, 0));
+      a_par.push_back(make_double4(l.fk, l.ang0, l.ureyfk, l.ureyr0));
+      a_win.push_back(wi);
+      a_solv.push_back((int)k >= t.nangles_solute);
+    }
+    for (size_t k = 0; k < t.tor.size(); ++k) {
+      const Tors& q = t.tor[k];
+      const TorLib& l = t.torlib[q.cod];
+      t_ijkl.push_back(make_int4(ao + q.i, ao + q.j, ao + q.k, ao + q.l));
+      t_par.push_back(make_float4(l.fk, l.rmult, l.deltor, l.paths));
+      t_win.push_back(wi);
+      t_solv.push_back((int)k >= t.ntors_solute);
+    }
+    for (size_t k = 0; k < t.imp.size(); ++k) {
+      const Tors& q = t.imp[k];
+      const ImpLib& l = t.implib[q.cod];
+      i_ijkl.push_back(make_int4(ao + q.i, ao + q.j, ao + q.k, ao + q.l));
+      i_par.push_back(make_float2(l.fk, l.imp0));
+      i_win.push_back(wi);
+      i_solv.push_back((int)k >= t.nimps_solute);
+    }
+    i_kind[wi] = (t.ff_type == 2) ? 2 : 1;
+    urey[wi] = (t.ff_type == 3) ? 1 : 0;
+    for (auto& qb : fp.qbnd)
+      for (int s = 0; s < S; ++s)
+        if (qb.cod[s] > 0) {
+          const QBondLib& l = fp.qbondlib[qb.cod[s]];
+          qb_ij.push_back(make_int2(ao + qb.i, ao + qb.j));
+          qb_par.push_back(make_double4(l.Dmz, l.amz, l.r0, l.fk));
+          qb_ws.push_back(make_int2(wi, s));
+        }
+    for (auto& qa : fp.qang)
+      for (int s = 0; s < S; ++s)
+        if (qa.cod[s] > 0) {
+          const AngLib& l = fp.qanglib[qa.cod[s]];
+          qa_ijk.push_back(make_int4(ao + qa.i, ao + qa.j, ao + qa.k, 0));
+          qa_par.push_back(make_double2(l.fk, l.ang0));
+          qa_ws.push_back(make_int2(wi, s));
+        }
+    for (auto& q : fp.qtor)
+      for (int s = 0; s < S; ++s)
+        if (q.cod[s] > 0) {
+          qt_ijkl.push_back(make_int4(ao + q.i, ao + q.j, ao + q.k, ao + q.l));
+          qt_par.push_back(make_double4(fp.qfktor[q.cod[s]], fp.qrmult[q.cod[s]], fp.qdeltor[q.cod[s]], 0.0));
+          qt_ws.push_back(make_int2(wi, s));
+        }
+    for (auto& q : fp.qimp)
+      for (int s = 0; s < S; ++s)
+        if (q.cod[s] > 0) {
+          qi_ijkl.push_back(make_int4(ao + q.i, ao + q.j, ao + q.k, ao + q.l));
+          qi_par.push_back(make_double2(fp.qfkimp[q.cod[s]], fp.qimp0[q.cod[s]]));
+          qi_ws.push_back(make_int2(wi, s));
+        }
+    // --- SHAKE: split each molecule into connected components
+    for (auto& m : W.shake_mol) {
+      const int nc = (int)m.i.size();
+      std::vector<int> parent(nc);
+      std::iota(parent.begin(), parent.end(), 0);
+      std::function<int(int)> find = [&](int a) { return parent[a] == a ? a : parent[a] = find(parent[a]); };
+      for (int a = 0; a < nc; ++a)
+        for (int b = a + 1; b < nc; ++b)
+          if (m.i[a] == m.i[b] || m.i[a] == m.j[b] || m.j[a] == m.i[b] || m.j[a] == m.j[b]) parent[find(a)] = find(b);
+      std::vector<int> roots;
+      for (int a = 0; a < nc; ++a) if (find(a) == a) roots.push_back(a);
+      for (int r : roots) {
+        su_start.push_back((int)sc_i.size());
+        int cnt = 0;
+        for (int a = 0; a < nc; ++a)
+          if (find(a) == r) { sc_i.push_back(ao + m.i[a]); sc_j.push_back(ao + m.j[a]); sc_d2.push_back(m.dist2[a]); ++cnt; }
+        if (cnt > 64) return set_err(-10, "a SHAKE component with more than 64 coupled constraints is not supported");
+        su_n.push_back(cnt);
+        su_win.push_back(wi);
+      }
+      e->shake_any = 1;
+    }
+    // --- water shells
+    D.sh_off = (int)shells.size();
+    D.nshell = (int)W.wshell.size();
+    if (D.nshell > 8) return set_err(-10, "more than 8 water polarisation shells");
+    for (auto& sh : W.wshell) {
+      ShellDev s{};
+      s.rout = sh.rout; s.dr = sh.dr; s.cstb = sh.cstb; s.theta_corr = sh.theta_corr;
+      s.avtheta = 0.f; s.avn_insh = 0.f; s.n_insh = 0;
+      shells.push_back(s);
+    }
+    w_off[wi] = woff;
+    // --- state
+    WinState& st = wst[wi];
+    std::memset(&st, 0, sizeof(st));
+    st.istep = 0;
+    st.stop_at = 0;
+    st.active = 1;
+    st.tscale_solute = 0.0;
+    st.tscale_solvent = 0.0;
+    ao += N;
+    goff += (int)t.cgp.size();
+    woff += W.nwat;
+    qoff += fp.nqat;
+    toff += t.natyps;
+  }
+  sp_start.push_back((int)sp_j.size());
+  e->ntot = ao;
+  e->ngrp = goff;
+  e->nwc = (int)wc_win.size();
+  e->nsc = (int)sc_win.size();
+  e->nqc = (int)qc_win.size();
+  e->nwat_total = woff;
+  e->h_wc_o = wc_o;
+  e->h_sc_a = sc_a;
+  e->h_qc_q = qc_q;
+
+  Dev& d = e->d;
+  d.nwin = NW;
+  d.ntot = ao;
+  int rc = 0;
+#define UP(dst, src) if ((rc = dev_upload(e, &dst, src)) != 0) return rc
+  WinDev* wdp; UP(wdp, wd); d.win = wdp;
+  UP(d.ws, wst);
+  UP(d.x, x); UP(d.v, v); UP(d.xtop, xtop);
+  if ((rc = dev_alloc(e, &d.xx, x.size())) != 0) return rc;
+  UP(d.x32, x32); UP(d.ljp, ljp);
+  if ((rc = dev_alloc(e, &d.f, x.size())) != 0) return rc;
+  float* fwinv; UP(fwinv, winv); d.winv = fwinv;
+  double* fmass; UP(fmass, mass); d.mass = fmass;
+  int* ip;
+  UP(ip, wid); d.wid = ip;
+  UP(ip, iac); d.iac = ip;
+  UP(ip, grp); d.grp = ip;
+  uint8_t* up;
+  UP(up, aflag); d.aflag = up;
+  UP(ip, g_sw); d.g_sw = ip;
+  UP(ip, g_first); d.g_first = ip;
+  UP(ip, g_n); d.g_n = ip;
+  UP(ip, g_atoms); d.g_atoms = ip;
+  UP(ip, g_win); d.g_win = ip;
+  if ((rc = dev_alloc(e, &d.g_cent, 3 * (size_t)goff)) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.g_lrf, 20 * (size_t)goff)) != 0) return rc;
+  UP(ip, sp_start); d.sp_start = ip;
+  UP(ip, sp_j); d.sp_j = ip;
+  UP(up, sp_t); d.sp_t = up;
+  UP(ip, wc_o); d.wc_o = ip;
+  UP(ip, sc_a); d.sc_a = ip;
+  UP(ip, qc_q); d.qc_q = ip;
+  UP(ip, wc_win); d.wc_win = ip;
+  UP(ip, sc_win); d.sc_win = ip;
+  UP(ip, qc_win); d.qc_win = ip;
+  if ((rc = dev_alloc(e, &d.wc_bs, 4 * (size_t)e->nwc)) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.sc_bs, 4 * (size_t)e->nsc)) != 0) return rc;
+  UP(ip, q_atom); d.q_atom = ip;
+  float* fp_;
+  UP(fp_, q_crg); d.q_crg = fp_;
+  float4* f4;
+  UP(f4, q_lj); d.q_lj = f4;
+  double* dp;
+  UP(dp, q_ljd); d.q_ljd = dp;
+  UP(fp_, q_sc); d.q_sc = fp_;
+  UP(dp, q_scd); d.q_scd = dp;
+  UP(dp, t_lj); d.t_lj = dp;
+  d.nqq = (int)qq_ij.size();
+  int4* i4;
+  UP(i4, qq_ij); d.qq_ij = i4;
+  UP(fp_, qq_es); d.qq_es = fp_;
+  UP(ip, qq_st); d.qq_st = ip;
+  UP(ip, qq_win); d.qq_win = ip;
+  UP(fp_, qq_sc); d.qq_sc = fp_;
+  d.nbond = (int)b_ij.size(); d.nang = (int)a_ijk.size(); d.ntor = (int)t_ijkl.size(); d.nimp = (int)i_ijkl.size();
+  int2* i2; double2* d2; double4* d4; float2* f2;
+  UP(i2, b_ij); d.b_ij = i2; UP(d2, b_par); d.b_par = d2; UP(ip, b_win); d.b_win = ip; UP(up, b_solv); d.b_solv = up;
+  UP(i4, a_ijk); d.a_ijk = i4; UP(d4, a_par); d.a_par = d4; UP(ip, a_win); d.a_win = ip; UP(up, a_solv); d.a_solv = up;
+  UP(i4, t_ijkl); d.t_ijkl = i4; UP(f4, t_par); d.t_par = f4; UP(ip, t_win); d.t_win = ip; UP(up, t_solv); d.t_solv = up;
+  UP(i4, i_ijkl); d.i_ijkl = i4; UP(f2, i_par); d.i_par = f2; UP(ip, i_win); d.i_win = ip; UP(up, i_solv); d.i_solv = up;
+  UP(ip, i_kind); d.i_kind = ip;
+  UP(ip, urey); d.urey = ip;
+  d.nqb = (int)qb_ij.size(); d.nqa = (int)qa_ijk.size(); d.nqt = (int)qt_ijkl.size(); d.nqi = (int)qi_ijkl.size();
+  UP(i2, qb_ij); d.qb_ij = i2; UP(d4, qb_par); d.qb_par = d4; UP(i2, qb_ws); d.qb_ws = i2;
+  UP(i4, qa_ijk); d.qa_ijk = i4; UP(d2, qa_par); d.qa_par = d2; UP(i2, qa_ws); d.qa_ws = i2;
+  UP(i4, qt_ijkl); d.qt_ijkl = i4; UP(d4, qt_par); d.qt_par = d4; UP(i2, qt_ws); d.qt_ws = i2;
+  UP(i4, qi_ijkl); d.qi_ijkl = i4; UP(d2, qi_par); d.qi_par = d2; UP(i2, qi_ws); d.qi_ws = i2;
+  d.nsu = (int)su_n.size();
+  UP(ip, su_start); d.su_start = ip; UP(ip, su_n); d.su_n = ip; UP(ip, su_win); d.su_win = ip;
+  UP(ip, sc_i); d.sc_i = ip; UP(ip, sc_j); d.sc_j = ip;
+  UP(dp, sc_d2); d.sc_d2 = dp;
+  ShellDev* sd;
+  UP(sd, shells); d.shells = sd;
+  if ((rc = dev_alloc(e, &d.wtheta, (size_t)e->nwat_total)) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.wshell, (size_t)e->nwat_total)) != 0) return rc;
+  UP(ip, w_off); d.w_off = ip;
+  d.ring_n = 64;
+  if ((rc = dev_alloc(e, &d.e_ring, (size_t)d.ring_n * NW * E_NSLOT)) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.t_ring, (size_t)d.ring_n * NW * 4)) != 0) return rc;
+
+  // --- tile lists: dynamic types get a fixed capacity (all j-clusters of the window)
+  auto make_dyn = [&](TileType tt, const std::vector<int>& iwin, bool jwater, bool m14) -> int {
+    TileList& L = d.tl[tt];
+    std::vector<int> start(iwin.size()), count(iwin.size(), 0);
+    int total = 0;
+    for (size_t ic = 0; ic < iwin.size(); ++ic) {
+      start[ic] = total;
+      total += jwater ? wd[iwin[ic]].nwc : wd[iwin[ic]].nsc;
+    }
+    L.nicl = (int)iwin.size();
+    L.cap_total = total;
+    int r;
+    if ((r = dev_upload(e, &L.start, start)) != 0) return r;
+    if ((r = dev_upload(e, &L.count, count)) != 0) return r;
+    if ((r = dev_alloc(e, &L.jc, (size_t)total)) != 0) return r;
+    if ((r = dev_alloc(e, &L.mask, (size_t)total)) != 0) return r;
+    if (m14) { if ((r = dev_alloc(e, &L.m14, (size_t)total)) != 0) return r; }
+    else L.m14 = nullptr;
+    return 0;
+  };
+  if ((rc = make_dyn(T_WW, wc_win, true, false)) != 0) return rc;
+  if ((rc = make_dyn(T_WP, wc_win, false, false)) != 0) return rc;
+  if ((rc = make_dyn(T_PW, sc_win, true, false)) != 0) return rc;
+  if ((rc = make_dyn(T_PP, sc_win, false, true)) != 0) return rc;
+  auto make_static = [&](TileType tt, const HT& h) -> int {
+    TileList& L = d.tl[tt];
+    std::vector<int> start, count, jc;
+    std::vector<uint2> mask, m14;
+    for (size_t ic = 0; ic < h.l.size(); ++ic) {
+      start.push_back((int)jc.size());
+      count.push_back((int)h.l[ic].size());
+      for (size_t k = 0; k < h.l[ic].size(); ++k) {
+        jc.push_back(h.l[ic][k].first);
+        mask.push_back(h.l[ic][k].second);
+        m14.push_back(k < h.m14[ic].size() ? h.m14[ic][k] : make_uint2(0, 0));
+      }
+    }
+    L.nicl = (int)h.l.size();
+    L.cap_total = (int)jc.size();
+    int r;
+    if ((r = dev_upload(e, &L.start, start)) != 0) return r;
+    if ((r = dev_upload(e, &L.count, count)) != 0) return r;
+    if ((r = dev_upload(e, &L.jc, jc)) != 0) return r;
+    if ((r = dev_upload(e, &L.mask, mask)) != 0) return r;
+    if ((r = dev_upload(e, &L.m14, m14)) != 0) return r;
+    return 0;
+  };
+  if ((rc = make_static(T_QP, qt[0])) != 0) return rc;
+  if ((rc = make_static(T_PQ, qt[1])) != 0) return rc;
+  if ((rc = make_static(T_QW, qt[2])) != 0) return rc;
+  if ((rc = make_static(T_WQ, qt[3])) != 0) return rc;
+  return 0;
+}
+
+// ----------------------------------------------------------- output --
+static void assemble_energies(HostWin& h, const long long* Es) {
+  Window& W = *h.w;
+  const int S = W.inp.nstates;
+  auto ev = [&](int k) { return (double)Es[k] / kEScale; };
+  double* E = h.E;
+  std::memset(E, 0, sizeof(h.E));
+  // layout of nrgy.f90 type energies
+  double pbond = ev(E_P_BOND), pang = ev(E_P_ANG), ptor = ev(E_P_TOR), pimp = ev(E_P_IMP);
+  double wbond = ev(E_W_BOND), wang = ev(E_W_ANG), wtor = ev(E_W_TOR), wimp = ev(E_W_IMP);
+  h.EQ.assign((size_t)S * QMI_N_Q_ENERGIES, 0.0);
+  double qb = 0, qa = 0, qt = 0, qi = 0, qxel = 0, qxvdw = 0, rprot = 0;
+  for (int s = 0; s < S; ++s) {
+    double* q = &h.EQ[(size_t)s * QMI_N_Q_ENERGIES];
+    const int b = E_Q0 + 8 * s;
+    q[0] = W.inp.lambda[s];
+    q[2] = ev(b + 0); q[3] = ev(b + 1); q[4] = ev(b + 2); q[5] = ev(b + 3);
+    q[8] = ev(b + 4); q[9] = ev(b + 5);
+    q[10] = ev(b + 6); q[11] = ev(b + 7);
+    q[12] = ev(E_QW0 + 2 * s); q[13] = ev(E_QW0 + 2 * s + 1);
+    q[14] = 0.0;
+    q[6] = q[8] + q[10] + q[12];
+    q[7] = q[9] + q[11] + q[13];
+    q[1] = q[2] + q[3] + q[4] + q[5] + q[6] + q[7] + q[14];
+    const double l = q[0];
+    qb += q[2] * l; qa += q[3] * l; qt += q[4] * l; qi += q[5] * l;
+    qxel += q[6] * l; qxvdw += q[7] * l; rprot += q[14] * l;
+  }
+  E[2] = ev(E_LRF);
+  E[3] = pbond; E[4] = pang; E[5] = ptor; E[6] = pimp;
+  E[7] = wbond; E[8] = wang; E[9] = wtor; E[10] = wimp;
+  E[11] = qb; E[12] = qa; E[13] = qt; E[14] = qi;
+  E[15] = ev(E_PP_EL); E[16] = ev(E_PP_VDW);
+  E[17] = ev(E_PW_EL); E[18] = ev(E_PW_VDW);
+  E[19] = ev(E_WW_EL); E[20] = ev(E_WW_VDW);
+  E[21] = qxel; E[22] = qxvdw;
+  E[23] = 0.0;
+  E[24] = ev(E_FIX); E[25] = ev(E_SHELL); E[26] = rprot;
+  double rad = ev(E_RAD), wpol = ev(E_WPOL);
+  E[27] = rad;
+  E[28] = wpol;
+  // restraint total (fix + shell + protein + solvent_radial + water_pol)
+  E[23] = E[24] + E[25] + E[26] + rad + wpol;
+  // potential (pot_energy, md.f90:13466-13470)
+  E[0] = E[3] + E[7] + E[4] + E[8] + E[5] + E[6] + E[15] + E[16] + E[17] + E[18] + E[19] + E[20] + E[11] + E[12] +
+         E[13] + E[14] + E[21] + E[22] + E[23] + E[2];
+  h.rad = rad;
+  h.wpol = wpol;
+}
+
+static void write_en(HostWin& h) {
+  if (!h.en) return;
+  const int S = h.w->inp.nstates;
+  for (int s = 0; s < S; ++s) {
+    int32_t len = 4 + 8 * QMI_N_Q_ENERGIES, st = s + 1;
+    fwrite(&len, 4, 1, h.en);
+    fwrite(&st, 4, 1, h.en);
+    fwrite(&h.EQ[(size_t)s * QMI_N_Q_ENERGIES], 8, QMI_N_Q_ENERGIES, h.en);
+    fwrite(&len, 4, 1, h.en);
+  }
+  int32_t z = 0;
+  fwrite(&z, 4, 1, h.en);
+  fwrite(&z, 4, 1, h.en);
+  fflush(h.en);
+}
+
+static void write_summary(HostWin& h, int istep, bool final_) {
+  if (!h.log) return;
+  Window& W = *h.w;
+  FILE* f = h.log;
+  const double* E = h.E;
+  if (final_) fprintf(f, "=========================== FINAL %15s =============================\n", "Energy summary");
+  else fprintf(f, "======================= %15s at step %6d ========================\n", "Energy summary", istep);
+  fprintf(f, "                %10s%10s%10s%10s%10s%10s\n", "el", "vdW", "bond", "angle", "torsion", "improper");
+  fprintf(f, "%-16s%10.2f%10.2f%10.2f%10.2f%10.2f%10.2f\n", "solute", E[15], E[16], E[3], E[4], E[5], E[6]);
+  if (W.nwat > 0) fprintf(f, "%-16s%10.2f%10.2f%10.2f%10.2f%10.2f%10.2f\n", "solvent", E[19], E[20], E[7], E[8], E[9], E[10]);
+  fprintf(f, "%-16s%10.2f%10.2f\n", "solute-solvent", E[17], E[18]);
+  if (W.inp.use_lrf) fprintf(f, "%-16s%10.2f\n", "LRF", E[2]);
+  if (W.fep.nqat > 0)
+    fprintf(f, "%-16s%10.2f%10.2f%10.2f%10.2f%10.2f%10.2f\n", "Q-atom", E[21], E[22], E[11], E[12], E[13], E[14]);
+  fprintf(f, "\n                %10s%10s%10s%10s%10s%10s\n", "total", "fix", "slvnt_rad", "slvnt_pol", "shell", "solute");
+  fprintf(f, "%-16s%10.2f%10.2f%10.2f%10.2f%10.2f%10.2f\n\n", "restraints", E[23], E[24], h.rad, h.wpol, E[25], E[26]);
+  fprintf(f, "                %10s%10s%10s\n", "total", "potential", "kinetic");
+  fprintf(f, "%-16s%10.2f%10.2f%10.2f\n", "SUM", E[0] + E[1], E[0], E[1]);
+  const int S = W.inp.nstates;
+  if (S > 0) {
+    if (final_) fprintf(f, "=========================== FINAL %15s =============================\n", "Q-atom energies");
+    else fprintf(f, "======================= %15s at step %6d ========================\n", "Q-atom energies", istep);
+    fprintf(f, "type   st lambda%10s%10s%10s%10s%10s%10s\n", "el", "vdW", "bond", "angle", "torsion", "improper");
+    auto q = [&](int s) { return &h.EQ[(size_t)s * QMI_N_Q_ENERGIES]; };
+    for (int s = 0; s < S; ++s) fprintf(f, "%-7s%2d%7.4f%10.2f%10.2f\n", "Q-Q", s + 1, q(s)[0], q(s)[8], q(s)[9]);
+    fprintf(f, "\n");
+    if (W.top.nat_solute > W.fep.nqat) {
+      for (int s = 0; s < S; ++s) fprintf(f, "%-7s%2d%7.4f%10.2f%10.2f\n", "Q-prot", s + 1, q(s)[0], q(s)[10], q(s)[11]);
+      fprintf(f, "\n");
+    }
+    if (W.nwat > 0) {
+      for (int s = 0; s < S; ++s) fprintf(f, "%-7s%2d%7.4f%10.2f%10.2f\n", "Q-wat", s + 1, q(s)[0], q(s)[12], q(s)[13]);
+      fprintf(f, "\n");
+    }
+    for (int s = 0; s < S; ++s)
+      fprintf(f, "%-7s%2d%7.4f%10.2f%10.2f\n", "Q-surr.", s + 1, q(s)[0], q(s)[10] + q(s)[12], q(s)[11] + q(s)[13]);
+    fprintf(f, "\n");
+    for (int s = 0; s < S; ++s)
+      fprintf(f, "%-7s%2d%7.4f%10.2f%10.2f%10.2f%10.2f%10.2f%10.2f\n", "Q-any", s + 1, q(s)[0], q(s)[6], q(s)[7], q(s)[2],
+              q(s)[3], q(s)[4], q(s)[5]);
+    fprintf(f, "\ntype   st lambda%10s%10s\n", "total", "restraint");
+    for (int s = 0; s < S; ++s) fprintf(f, "%-7s%2d%7.4f%10.2f%10.2f\n", "Q-SUM", s + 1, q(s)[0], q(s)[1], q(s)[14]);
+  }
+  fprintf(f, "===============================================================================\n");
+  fflush(f);
+}
+
+// ------------------------------------------------------------ C-ABI --
+extern "C" {
+
+int qmi_abi_version(void) { return QMI_ABI_VERSION; }
+
+const char* qmi_last_error(void) { return g_err.c_str(); }
+
+int qmi_engine_create(int hip_device, qmi_engine** out) {
+  // no device work here: windows can be read and checked on a host without
+  // a GPU; the device is opened by qmi_prepare
+  if (!out) return set_err(-1, "null output pointer");
+  if (hip_device < 0) return set_err(-1, "invalid HIP device");
+  auto* e = new qmi_engine();
+  e->device = hip_device;
+  *out = e;
+  return 0;
+}
+
+int qmi_window_add(qmi_engine* e, const char* inp_path, long long seed_override, int* window_id) {
+  if (!e || !inp_path) return set_err(-1, "null argument");
+  if (e->prepared) return set_err(-2, "windows must be added before qmi_prepare");
+  HostWin h;
+  h.w.reset(new Window());
+  Error err{0, ""};
+  if (!load_input(inp_path, &h.w->inp, &err)) return set_err(err.code, err.msg);
+  if (!prepare_window(h.w.get(), seed_override, &err)) return set_err(err.code, err.msg);
+  h.inp_path = inp_path;
+  std::string base = h.inp_path;
+  size_t dot = base.find_last_of('.');
+  size_t sl = base.find_last_of('/');
+  if (dot != std::string::npos && (sl == std::string::npos || dot > sl)) base = base.substr(0, dot);
+  h.log_path = base + ".log";
+  e->wins.push_back(std::move(h));
+  if (window_id) *window_id = (int)e->wins.size() - 1;
+  return 0;
+}
+
+int qmi_prepare(qmi_engine* e) {
+  if (!e) return set_err(-1, "null engine");
+  if (e->prepared) return 0;
+  if (e->wins.empty()) return set_err(-2, "no windows");
+  int ndev = 0;
+  HIP_OK(hipGetDeviceCount(&ndev));
+  if (e->device >= ndev) return set_err(-1, "invalid HIP device");
+  HIP_OK(hipSetDevice(e->device));
+  if (!e->stream) HIP_OK(hipStreamCreate(&e->stream));
+  int rc = build_device(e);
+  if (rc) return rc;
+  for (auto& h : e->wins) {
+    Window& W = *h.w;
+    if (W.inp.iene_cycle > 0 && !W.inp.ene_file.empty()) {
+      h.en = fopen((W.inp.dir + "/" + W.inp.ene_file).c_str(), "wb");
+      if (!h.en) return set_err(-42, "error opening energy output file");
+    }
+    h.log = fopen(h.log_path.c_str(), "w");
+    if (h.log) fputs(W.log.c_str(), h.log);
+  }
+  // initial temperature and scaling factors (md_run, md.f90:3969-3976)
+  launch_temperature(e->d, e->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(e->stream));
+  e->prepared = true;
+  return 0;
+}
+
+static int drain(qmi_engine* e, int first_step, int nsteps_done) {
+  // copy the energy / temperature ring and write .en records and summaries
+  const int NW = (int)e->wins.size();
+  const int R = e->d.ring_n;
+  std::vector<long long> er((size_t)R * NW * E_NSLOT);
+  std::vector<double> tr((size_t)R * NW * 4);
+  HIP_OK(hipMemcpyAsync(er.data(), e->d.e_ring, er.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipMemcpyAsync(tr.data(), e->d.t_ring, tr.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  for (int wi = 0; wi < NW; ++wi) {
+    HostWin& h = e->wins[wi];
+    Window& W = *h.w;
+    for (int k = 0; k < nsteps_done; ++k) {
+      const int istep = first_step + k;
+      if (istep >= W.inp.nsteps || h.done) continue;
+      const int slot = istep % R;
+      const double* t = &tr[((size_t)slot * NW + wi) * 4];
+      if ((int)t[3] != istep) continue;
+      h.temp = t[0]; h.tfree = t[1]; h.ekin = t[2];
+      const bool en_step = W.inp.iene_cycle > 0 && istep % W.inp.iene_cycle == 0 && istep > 0;
+      const bool out_step = W.inp.iout_cycle > 0 && istep % W.inp.iout_cycle == 0;
+      if (en_step || out_step) {
+        assemble_energies(h, &er[((size_t)slot * NW + wi) * E_NSLOT]);
+        h.E[1] = h.ekin;
+        if (en_step) write_en(h);
+        if (out_step) write_summary(h, istep, false);
+      }
+      if (W.inp.itemp_cycle > 0 && istep % W.inp.itemp_cycle == 0 && istep > 0 && h.log)
+        fprintf(h.log, "Temperature at step%8d:         T_tot=%10.1f         T_free=%10.1f\n", istep, h.temp, h.tfree);
+    }
+  }
+  return 0;
+}
+
+static int sync_state_to_host(qmi_engine* e) {
+  const int NW = (int)e->wins.size();
+  std::vector<double> x((size_t)3 * e->ntot), v((size_t)3 * e->ntot);
+  HIP_OK(hipMemcpyAsync(x.data(), e->d.x, x.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipMemcpyAsync(v.data(), e->d.v, v.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  std::vector<ShellDev> sh;
+  size_t nsh = 0;
+  for (auto& h : e->wins) nsh += h.w->wshell.size();
+  sh.resize(nsh);
+  if (nsh) HIP_OK(hipMemcpyAsync(sh.data(), e->d.shells, nsh * sizeof(ShellDev), hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  size_t ao = 0, so = 0;
+  for (int wi = 0; wi < NW; ++wi) {
+    Window& W = *e->wins[wi].w;
+    std::copy(x.begin() + 3 * ao, x.begin() + 3 * (ao + W.natom), W.x.begin());
+    std::copy(v.begin() + 3 * ao, v.begin() + 3 * (ao + W.natom), W.v.begin());
+    for (auto& s : W.wshell) { s.theta_corr = sh[so].theta_corr; ++so; }
+    ao += W.natom;
+  }
+  return 0;
+}
+
+int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
+  if (!e || !e->prepared) return set_err(-2, "engine not prepared");
+  HIP_OK(hipSetDevice(e->device));
+  const int NW = (int)e->wins.size();
+  // stop_at per window
+  std::vector<WinState> ws(NW);
+  HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
+  int first = ws[0].istep;
+  for (int wi = 0; wi < NW; ++wi) {
+    ws[wi].stop_at = (int)std::min<long long>(e->wins[wi].w->inp.nsteps, (long long)ws[wi].istep + nsteps);
+    HIP_OK(hipMemcpy(&e->d.ws[wi].stop_at, &ws[wi].stop_at, sizeof(int), hipMemcpyHostToDevice));
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  long long done = 0;
+  const int R = e->d.ring_n;
+  while (done < nsteps) {
+    int chunk = (int)std::min<long long>(R, nsteps - done);
+    for (int k = 0; k < chunk; ++k) {
+      launch_begin(e->d, e->stream, 0, 0, 0);
+      launch_zero(e->d, e->stream);
+      launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
+      launch_forces(e->d, e->stream);
+      launch_integrate(e->d, e->stream, e->shake_any);
+    }
+    HIP_OK(hipGetLastError());
+    int rc = drain(e, first + (int)done, chunk);
+    if (rc) return rc;
+    done += chunk;
+  }
+  HIP_OK(hipStreamSynchronize(e->stream));
+  auto t1 = std::chrono::steady_clock::now();
+  double us = std::chrono::duration<double, std::micro>(t1 - t0).count();
+  e->timings.assign(6, 0.0);
+  e->timings[5] = nsteps > 0 ? us / (double)nsteps : 0.0;
+  HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
+  for (int wi = 0; wi < NW; ++wi) {
+    if (ws[wi].shake_fail) return set_err(-60, "shake failure in window " + std::to_string(wi));
+    e->wins[wi].istep = ws[wi].istep;
+  }
+  int rc = sync_state_to_host(e);
+  if (rc) return rc;
+  if (finalize) {
+    // final pair lists, energies, summary and restart (md.f90:4207-4213)
+    rc = qmi_evaluate(e, 1);
+    if (rc) return rc;
+    for (int wi = 0; wi < NW; ++wi) {
+      HostWin& h = e->wins[wi];
+      if (h.istep < h.w->inp.nsteps) continue;
+      h.E[1] = h.ekin;
+      write_summary(h, h.istep, true);
+      Error err{0, ""};
+      if (!write_restart(h.w->inp.dir + "/" + h.w->inp.xfin_file, *h.w, &err)) return set_err(err.code, err.msg);
+      h.done = true;
+    }
+  }
+  return 0;
+}
+
+int qmi_evaluate(qmi_engine* e, int rebuild_lists) {
+  if (!e || !e->prepared) return set_err(-2, "engine not prepared");
+  HIP_OK(hipSetDevice(e->device));
+  launch_begin(e->d, e->stream, 1, 1, rebuild_lists ? 1 : 0);
+  launch_zero(e->d, e->stream);
+  launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
+  launch_forces(e->d, e->stream);
+  HIP_OK(hipGetLastError());
+  const int NW = (int)e->wins.size();
+  std::vector<WinState> ws(NW);
+  HIP_OK(hipMemcpyAsync(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  for (int wi = 0; wi < NW; ++wi) {
+    assemble_energies(e->wins[wi], ws[wi].E);
+    e->wins[wi].E[1] = ws[wi].ekin;
+  }
+  return 0;
+}
+
+static HostWin* getw(qmi_engine* e, int w) {
+  if (!e || w < 0 || w >= (int)e->wins.size()) { set_err(-1, "invalid window"); return nullptr; }
+  return &e->wins[w];
+}
+
+int qmi_get_energies(qmi_engine* e, int w, double* out, int cap) {
+  HostWin* h = getw(e, w);
+  if (!h) return -1;
+  if (cap < QMI_N_ENERGIES) return set_err(-3, "buffer too small");
+  std::memcpy(out, h->E, sizeof(h->E));
+  return QMI_N_ENERGIES;
+}
+
+int qmi_get_q_energies(qmi_engine* e, int w, double* out, int cap) {
+  HostWin* h = getw(e, w);
+  if (!h) return -1;
+  const int S = h->w->inp.nstates;
+  if (cap < S * QMI_N_Q_ENERGIES) return set_err(-3, "buffer too small");
+  if (!h->EQ.empty()) std::memcpy(out, h->EQ.data(), sizeof(double) * S * QMI_N_Q_ENERGIES);
+  return S;
+}
+
+static int get_atom_vec(qmi_engine* e, int w, double* out, long long cap, int which) {
+  HostWin* h = getw(e, w);
+  if (!h) return -1;
+  if (!e->prepared) {
+    const std::vector<double>& src = which == 0 ? h->w->x : h->w->v;
+    if (which == 2) return set_err(-2, "no forces before qmi_prepare");
+    if (cap < (long long)src.size()) return set_err(-3, "buffer too small");
+    std::memcpy(out, src.data(), src.size() * 8);
+    return 0;
+  }
+  long long ao = 0;
+  for (int k = 0; k < w; ++k) ao += e->wins[k].w->natom;
+  const long long n3 = 3LL * h->w->natom;
+  if (cap < n3) return set_err(-3, "buffer too small");
+  if (which == 2) {
+    std::vector<long long> f(n3);
+    HIP_OK(hipMemcpy(f.data(), e->d.f + 3 * ao, n3 * 8, hipMemcpyDeviceToHost));
+    for (long long k = 0; k < n3; ++k) out[k] = (double)f[k] / kFScale;
+  } else {
+    HIP_OK(hipMemcpy(out, (which == 0 ? e->d.x : e->d.v) + 3 * ao, n3 * 8, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int qmi_get_coords(qmi_engine* e, int w, double* xyz, long long cap) { return get_atom_vec(e, w, xyz, cap, 0); }
+int qmi_get_velocities(qmi_engine* e, int w, double* v, long long cap) { return get_atom_vec(e, w, v, cap, 1); }
+int qmi_get_forces(qmi_engine* e, int w, double* d, long long cap) { return get_atom_vec(e, w, d, cap, 2); }
+
+int qmi_set_state(qmi_engine* e, int w, const double* xyz, const double* v, long long n3) {
+  HostWin* h = getw(e, w);
+  if (!h) return -1;
+  if (n3 != 3LL * h->w->natom) return set_err(-3, "wrong number of coordinates");
+  if (xyz) std::memcpy(h->w->x.data(), xyz, n3 * 8);
+  if (v) std::memcpy(h->w->v.data(), v, n3 * 8);
+  if (e->prepared) {
+    long long ao = 0;
+    for (int k = 0; k < w; ++k) ao += e->wins[k].w->natom;
+    if (xyz) HIP_OK(hipMemcpy(e->d.x + 3 * ao, xyz, n3 * 8, hipMemcpyHostToDevice));
+    if (v) HIP_OK(hipMemcpy(e->d.v + 3 * ao, v, n3 * 8, hipMemcpyHostToDevice));
+    // refresh the FP32 copy of the coordinates
+    launch_begin(e->d, e->stream, 0, 1, 0);
+    launch_refresh32(e->d, e->stream);
+    HIP_OK(hipStreamSynchronize(e->stream));
+  }
+  return 0;
+}
+
+int qmi_window_info(qmi_engine* e, int w, int32_t* info, int cap) {
+  HostWin* h = getw(e, w);
+  if (!h) return -1;
+  if (cap < 5) return set_err(-3, "buffer too small");
+  info[0] = h->w->natom; info[1] = h->w->top.nat_solute; info[2] = h->w->nwat;
+  info[3] = h->w->inp.nstates; info[4] = h->w->fep.nqat;
+  return 5;
+}
+
+int qmi_get_pairlist(qmi_engine* e, int w, int kind, int32_t* pi, int32_t* pj, int16_t* pl, long long cap,
+                     long long* n) {
+  HostWin* h = getw(e, w);
+  if (!h) return -1;
+  if (!e->prepared) return set_err(-2, "engine not prepared");
+  Window& W = *h->w;
+  const Topology& t = W.top;
+  long long ao = 0;
+  for (int k = 0; k < w; ++k) ao += e->wins[k].w->natom;
+  std::vector<std::array<int, 3>> out;
+  auto zero_pair = [&](int i, int j, int lj) {
+    // nbpplist/nbpwlist skip pairs whose charge and LJ products are all zero
+    float qq = t.crg[i] * t.crg[j];
+    const AtomType& a = t.iaclib[t.iac[i]];
+    const AtomType& b = t.iaclib[t.iac[j]];
+    return qq == 0.f && a.avdw[lj - 1] * b.avdw[lj - 1] == 0. && a.bvdw[lj - 1] * b.bvdw[lj - 1] == 0.;
+  };
+  auto ljc = [&](int i, int j) { return W.ljcod[(t.iac[i] - 1) * t.natyps + (t.iac[j] - 1)]; };
+  if (kind == QMI_LIST_QW) {
+    for (int o : h->qw_waters) out.push_back({0, o + 1, 0});
+  } else if (kind == QMI_LIST_QP) {
+    for (auto& p : h->qp_pairs) out.push_back({p[0] + 1, p[1] + 1, p[2]});
+  } else {
+    TileType tt = kind == QMI_LIST_WW ? T_WW : kind == QMI_LIST_PW ? T_PW : kind == QMI_LIST_PP ? T_PP : T_NTYPES;
+    if (tt == T_NTYPES) return set_err(-3, "invalid list kind");
+    TileList& L = e->d.tl[tt];
+    std::vector<int> start(L.nicl), count(L.nicl), jc(L.cap_total);
+    std::vector<uint2> mask(L.cap_total), m14;
+    HIP_OK(hipMemcpy(start.data(), L.start, L.nicl * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(count.data(), L.count, L.nicl * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(jc.data(), L.jc, (size_t)L.cap_total * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(mask.data(), L.mask, (size_t)L.cap_total * 8, hipMemcpyDeviceToHost));
+    if (tt == T_PP) { m14.resize(L.cap_total); HIP_OK(hipMemcpy(m14.data(), L.m14, (size_t)L.cap_total * 8, hipMemcpyDeviceToHost)); }
+    const bool iw = tt == T_WW;
+    const std::vector<int>& icl = iw ? e->h_wc_o : e->h_sc_a;
+    const std::vector<int>& jcl = (tt == T_PP) ? e->h_sc_a : e->h_wc_o;
+    for (int ic = 0; ic < L.nicl; ++ic) {
+      int a0 = -1;
+      for (int k = 0; k < kCl && a0 < 0; ++k) a0 = icl[ic * kCl + k];
+      if (a0 < 0 || e->h_atom_win[a0] != w) continue;
+      for (int k = 0; k < count[ic]; ++k) {
+        const int q = start[ic] + k;
+        const uint2 m = mask[q];
+        for (int bit = 0; bit < 64; ++bit) {
+          bool on = bit < 32 ? (m.x >> bit) & 1 : (m.y >> (bit - 32)) & 1;
+          if (!on) continue;
+          int a = bit >> 3, b = bit & 7;
+          int ea = icl[ic * kCl + a] - (int)ao, eb = jcl[jc[q] * kCl + b] - (int)ao;
+          if (tt == T_WW) {
+            if (ic < jc[q] || (ic == jc[q] && a < b)) out.push_back({ea + 1, eb + 1, 1});
+          } else if (tt == T_PP) {
+            if (!(ic < jc[q] || (ic == jc[q] && a < b))) continue;
+            bool is14 = bit < 32 ? (m14[q].x >> bit) & 1 : (m14[q].y >> (bit - 32)) & 1;
+            int lj = is14 ? 3 : ljc(ea, eb);
+            if (zero_pair(ea, eb, ljc(ea, eb))) continue;
+            out.push_back({ea + 1, eb + 1, lj});
+          } else {  // PW: solute atom ea, water O eb -> three atom pairs
+            for (int s = 0; s < 3; ++s) {
+              int lj = ljc(ea, eb + s);
+              if (zero_pair(ea, eb + s, lj)) continue;
+              out.push_back({ea + 1, eb + s + 1, lj});
+            }
+          }
+        }
+      }
+    }
+  }
+  *n = (long long)out.size();
+  for (long long k = 0; k < std::min<long long>(cap, (long long)out.size()); ++k) {
+    if (pi) pi[k] = out[k][0];
+    if (pj) pj[k] = out[k][1];
+    if (pl) pl[k] = (int16_t)out[k][2];
+  }
+  return 0;
+}
+
+int qmi_write_restart(qmi_engine* e, int w, const char* path) {
+  HostWin* h = getw(e, w);
+  if (!h || !path) return set_err(-1, "invalid argument");
+  if (e->prepared) {
+    int rc = sync_state_to_host(e);
+    if (rc) return rc;
+  }
+  Error err{0, ""};
+  if (!write_restart(path, *h->w, &err)) return set_err(err.code, err.msg);
+  return 0;
+}
+
+int qmi_timings(qmi_engine* e, double* out, int cap) {
+  if (!e) return set_err(-1, "null engine");
+  int n = std::min<int>(cap, (int)e->timings.size());
+  for (int k = 0; k < n; ++k) out[k] = e->timings[k];
+  return n;
+}
+
+void qmi_engine_destroy(qmi_engine* e) { delete e; }
+
+}  // extern "C"

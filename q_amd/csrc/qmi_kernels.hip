@@ -1,0 +1,1504 @@
+// HIP kernels of the MI355X qdyn step (gfx950, wave64).
+//
+// Nonbonded work is organised as 8x8 cluster tiles, one wave per i-cluster:
+// lane l handles the pair (i-member l/8, j-member l%8) of every tile of its
+// i-cluster and accumulates the gradient of its i-member only ("owner
+// computes": each unordered pair is evaluated from both sides), so no
+// per-pair atomics are needed; the 8 lanes sharing an i-member are reduced
+// with cross-lane shuffles and added once into the int64 fixed-point
+// gradient.  Tile masks are built with the reference's FP64 group cut-off
+// tests, so the set of interacting pairs equals the reference pair lists.
+// Pair arithmetic is FP32; Q-atom energies on energy steps use double-float
+// (two-FP32) arithmetic; LRF, bonded terms, restraints, leapfrog and SHAKE
+// run in FP64.
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+#include "qmi_device.h"
+#include "qmi_kernels.h"
+
+namespace qmi {
+
+// ------------------------------------------------------------ helpers --
+__device__ __forceinline__ long long fix(double v, double scale) { return __double2ll_rn(v * scale); }
+
+__device__ __forceinline__ void add_grad(long long* f, int atom, double gx, double gy, double gz) {
+  atomicAdd((unsigned long long*)&f[3 * atom + 0], (unsigned long long)fix(gx, kFScale));
+  atomicAdd((unsigned long long*)&f[3 * atom + 1], (unsigned long long)fix(gy, kFScale));
+  atomicAdd((unsigned long long*)&f[3 * atom + 2], (unsigned long long)fix(gz, kFScale));
+}
+
+__device__ __forceinline__ void add_energy(WinState* ws, int w, int slot, double e) {
+  if (e != 0.0) atomicAdd((unsigned long long*)&ws[w].E[slot], (unsigned long long)fix(e, kEScale));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// sum over the 8 lanes that share lane>>3
+__device__ __forceinline__ double row8_sum(double v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  return v;
+}
+// sum over the 8 lanes that share lane&7
+__device__ __forceinline__ double col8_sum(double v) {
+  v += __shfl_xor(v, 8, 64);
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+__device__ __forceinline__ bool lane_bit(uint2 m, int lane) {
+  return lane < 32 ? ((m.x >> lane) & 1u) : ((m.y >> (lane - 32)) & 1u);
+}
+
+// squared distance in FP64 evaluated as the reference does, left to right
+// and without fused multiply-add: (dx*dx + dy*dy) + dz*dz
+__device__ __forceinline__ double r2_exact(const double* x, int i, int j) {
+#pragma clang fp contract(off)
+  double dx = __dsub_rn(x[3 * i + 0], x[3 * j + 0]);
+  double dy = __dsub_rn(x[3 * i + 1], x[3 * j + 1]);
+  double dz = __dsub_rn(x[3 * i + 2], x[3 * j + 2]);
+  return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+// ---------------------------------------------------- double-float (df32)
+// Error-free transformations (Dekker/Knuth two-sum, fma two-product) and the
+// double-float add/mul/div/sqrt built on them (Bailey's double-double scheme
+// carried out in FP32): about 44 significant bits.
+struct df { float h, l; };
+__device__ __forceinline__ df df_q2s(float a, float b) { float s = a + b; return {s, b - (s - a)}; }
+__device__ __forceinline__ df df_2s(float a, float b) {
+  float s = a + b; float bb = s - a; return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ df df_2p(float a, float b) { float p = a * b; return {p, fmaf(a, b, -p)}; }
+__device__ __forceinline__ df df_from(double x) { float h = (float)x; return {h, (float)(x - (double)h)}; }
+__device__ __forceinline__ double df_to(df a) { return (double)a.h + (double)a.l; }
+__device__ __forceinline__ df df_add(df a, df b) {
+  df s = df_2s(a.h, b.h);
+  df t = df_2s(a.l, b.l);
+  s.l += t.h;
+  s = df_q2s(s.h, s.l);
+  s.l += t.l;
+  return df_q2s(s.h, s.l);
+}
+__device__ __forceinline__ df df_neg(df a) { return {-a.h, -a.l}; }
+__device__ __forceinline__ df df_mul(df a, df b) {
+  df p = df_2p(a.h, b.h);
+  p.l += a.h * b.l + a.l * b.h;
+  return df_q2s(p.h, p.l);
+}
+__device__ __forceinline__ df df_mulf(df a, float b) {
+  df p = df_2p(a.h, b);
+  p.l += a.l * b;
+  return df_q2s(p.h, p.l);
+}
+__device__ __forceinline__ df df_div(df a, df b) {
+  float q1 = a.h / b.h;
+  df r = df_add(a, df_neg(df_mulf(b, q1)));
+  float q2 = r.h / b.h;
+  r = df_add(r, df_neg(df_mulf(b, q2)));
+  float q3 = r.h / b.h;
+  df q = df_q2s(q1, q2);
+  return df_add(q, df{q3, 0.f});
+}
+__device__ __forceinline__ df df_sqrt(df a) {
+  if (a.h <= 0.f) return {0.f, 0.f};
+  float x = sqrtf(a.h);
+  df r = df_add(a, df_neg(df_2p(x, x)));
+  return df_q2s(x, r.h * (0.5f / x));
+}
+// difference of two FP64 coordinates as a df
+__device__ __forceinline__ df df_dx(double xj, double xi) { return df_from(xj - xi); }
+
+// ------------------------------------------------------ step control --
+__global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= d.nwin) return;
+  WinState& s = d.ws[w];
+  const WinDev& W = d.win[w];
+  if (eval_only) {
+    s.active = 1;
+    s.list_now = rebuild;
+    s.energy_now = 1;
+  } else {
+    s.active = s.istep < s.stop_at ? 1 : 0;
+    s.list_now = (s.istep % W.nbcycle) == 0;
+    bool e = (W.iene > 0 && s.istep % W.iene == 0 && s.istep > 0) || (W.iout > 0 && s.istep % W.iout == 0) ||
+             force_energy;
+    s.energy_now = e ? 1 : 0;
+  }
+  if (s.energy_now)
+    for (int k = 0; k < E_NSLOT; ++k) s.E[k] = 0;
+}
+
+// charge-group centres (cgp_centers, md.f90:1000-1019) and cluster bounding
+// spheres, at list-update steps
+__global__ void k_centres(Dev d, int ngrp_total) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrp_total) return;
+  int w = d.g_win[g];
+  if (!d.ws[w].active || !d.ws[w].list_now) return;
+  double c[3] = {0, 0, 0};
+  int n = d.g_n[g], f0 = d.g_first[g];
+  for (int k = 0; k < n; ++k) {
+    int a = d.g_atoms[f0 + k];
+    c[0] += d.x[3 * a]; c[1] += d.x[3 * a + 1]; c[2] += d.x[3 * a + 2];
+  }
+  for (int k = 0; k < 3; ++k) d.g_cent[3 * g + k] = c[k] / (double)(float)n;
+  for (int k = 0; k < 20; ++k) d.g_lrf[20 * g + k] = 0.0;
+}
+
+__global__ void k_bounds(Dev d, int nwc_total, int nsc_total) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nwc_total + nsc_total) return;
+  bool water = c < nwc_total;
+  int cl = water ? c : c - nwc_total;
+  int w = water ? d.wc_win[cl] : d.sc_win[cl];
+  if (!d.ws[w].active || !d.ws[w].list_now) return;
+  double p[8][3];
+  int n = 0;
+  for (int k = 0; k < kCl; ++k) {
+    int a = water ? d.wc_o[cl * kCl + k] : d.sc_a[cl * kCl + k];
+    if (a < 0) continue;
+    int sw = water ? a : d.g_sw[d.grp[a]];
+    p[n][0] = d.x[3 * sw]; p[n][1] = d.x[3 * sw + 1]; p[n][2] = d.x[3 * sw + 2];
+    ++n;
+  }
+  double cx = 0, cy = 0, cz = 0;
+  for (int k = 0; k < n; ++k) { cx += p[k][0]; cy += p[k][1]; cz += p[k][2]; }
+  if (n > 0) { cx /= n; cy /= n; cz /= n; }
+  double r = 0;
+  for (int k = 0; k < n; ++k) {
+    double dx = p[k][0] - cx, dy = p[k][1] - cy, dz = p[k][2] - cz;
+    r = fmax(r, sqrt(dx * dx + dy * dy + dz * dz));
+  }
+  double* bs = water ? &d.wc_bs[4 * cl] : &d.sc_bs[4 * cl];
+  bs[0] = cx; bs[1] = cy; bs[2] = cz; bs[3] = n > 0 ? r : -1.0;
+}
+
+// ------------------------------------------------------- tile builder --
+// One wave per i-cluster scans the j-clusters of its window in order; each
+// lane tests one candidate j-cluster (bounding-sphere prune, then the exact
+// FP64 group tests of the reference for all 64 member pairs); surviving
+// tiles are compacted in j order (ballot + popcount), so lists are
+// deterministic.
+__device__ __forceinline__ int special_type(const Dev& d, int i, int j) {
+  // 1 = excluded pair, 2 = 1-4 pair, 0 = normal (listex/list14 and the long
+  // lists of topo.f90, as nbpplist checks them, md.f90:5256-5294)
+  int s0 = d.sp_start[i], s1 = d.sp_start[i + 1];
+  for (int k = s0; k < s1; ++k)
+    if (d.sp_j[k] == j) return d.sp_t[k];
+  return 0;
+}
+
+template <int TT>
+__global__ void k_build_tiles(Dev d) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  TileList L = d.tl[TT];
+  if (wave >= L.nicl) return;
+  const int ic = wave;
+  const bool iw = (TT == T_WW || TT == T_WP);
+  const int w = iw ? d.wc_win[ic] : d.sc_win[ic];
+  const WinState& st = d.ws[w];
+  if (!st.active || !st.list_now) return;
+  const WinDev& W = d.win[w];
+  const bool jw = (TT == T_WW || TT == T_PW);
+  const int j0 = jw ? W.wc_off : W.sc_off;
+  const int nj = jw ? W.nwc : W.nsc;
+  const double rc2 = (TT == T_WW) ? W.rcww2 : (TT == T_PP) ? W.rcpp2 : W.rcpw2;
+  const double rc = sqrt(rc2);
+  const double* bi = iw ? &d.wc_bs[4 * ic] : &d.sc_bs[4 * ic];
+  int out = L.start[ic];
+  int cnt = 0;
+  for (int base = 0; base < nj; base += 64) {
+    const int jc = j0 + base + lane;
+    uint2 m = {0u, 0u}, m14 = {0u, 0u};
+    if (base + lane < nj && bi[3] >= 0.0) {
+      const double* bj = jw ? &d.wc_bs[4 * jc] : &d.sc_bs[4 * jc];
+      double dx = bi[0] - bj[0], dy = bi[1] - bj[1], dz = bi[2] - bj[2];
+      double dc = sqrt(dx * dx + dy * dy + dz * dz);
+      if (bj[3] >= 0.0 && dc - bi[3] - bj[3] <= rc + 1e-3) {
+        for (int a = 0; a < kCl; ++a) {
+          int ea = iw ? d.wc_o[ic * kCl + a] : d.sc_a[ic * kCl + a];
+          if (ea < 0) continue;
+          int swa = iw ? ea : d.g_sw[d.grp[ea]];
+          if (d.aflag[swa] & 1) continue;
+          for (int b = 0; b < kCl; ++b) {
+            int eb = jw ? d.wc_o[jc * kCl + b] : d.sc_a[jc * kCl + b];
+            if (eb < 0) continue;
+            if (TT == T_WW && ea == eb) continue;
+            int swb = jw ? eb : d.g_sw[d.grp[eb]];
+            if (d.aflag[swb] & 1) continue;
+            int sp = 0;
+            if (TT == T_PP) {
+              if (ea == eb) continue;
+              sp = special_type(d, ea, eb);
+              if (sp == 1) continue;
+            }
+            if (r2_exact(d.x, swa, swb) > rc2) continue;
+            int bit = a * 8 + b;
+            if (bit < 32) m.x |= 1u << bit; else m.y |= 1u << (bit - 32);
+            if (sp == 2) { if (bit < 32) m14.x |= 1u << bit; else m14.y |= 1u << (bit - 32); }
+          }
+        }
+      }
+    }
+    bool keep = (m.x | m.y) != 0u;
+    unsigned long long bal = __ballot(keep);
+    if (keep) {
+      int pos = __popcll(bal & ((1ull << lane) - 1ull));
+      L.jc[out + cnt + pos] = jc;
+      L.mask[out + cnt + pos] = m;
+      if (TT == T_PP) L.m14[out + cnt + pos] = m14;
+    }
+    cnt += __popcll(bal);
+  }
+  if (lane == 0) L.count[ic] = cnt;
+}
+
+// --------------------------------------------------- nonbonded tiles --
+// water-water: 9 interactions per molecule pair (nonbond_ww_spc, md.f90:11823-12047)
+__global__ void __launch_bounds__(256) k_ww(Dev d) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const TileList L = d.tl[T_WW];
+  if (wave >= L.nicl) return;
+  const int ic = wave;
+  const int w = d.wc_win[ic];
+  const WinState& st = d.ws[w];
+  if (!st.active) return;
+  const WinDev& W = d.win[w];
+  const int a = lane >> 3, b = lane & 7;
+  const int oa = d.wc_o[ic * kCl + a];
+  float3 pi[3];
+  for (int k = 0; k < 3; ++k) {
+    float4 q = oa >= 0 ? d.x32[oa + k] : make_float4(0, 0, 0, 0);
+    pi[k] = make_float3(q.x, q.y, q.z);
+  }
+  const float qo = W.crg_ow, qh = W.crg_hw;
+  const float qq[3][3] = {{qo * qo, qo * qh, qo * qh}, {qh * qo, qh * qh, qh * qh}, {qh * qo, qh * qh, qh * qh}};
+  const float A = W.aO1 * W.aO1, B = W.bO1 * W.bO1;
+  const bool en = st.energy_now;
+  float g[9];
+  for (int k = 0; k < 9; ++k) g[k] = 0.f;
+  double eel = 0.0, evdw = 0.0;
+  const int s0 = L.start[ic], n = L.count[ic];
+  for (int t = 0; t < n; ++t) {
+    const int jc = L.jc[s0 + t];
+    const uint2 m = L.mask[s0 + t];
+    if (!lane_bit(m, lane)) continue;
+    const int ob = d.wc_o[jc * kCl + b];
+    float3 pj[3];
+    for (int k = 0; k < 3; ++k) {
+      float4 q = d.x32[ob + k];
+      pj[k] = make_float3(q.x, q.y, q.z);
+    }
+    const bool cnt = en && (ic < jc || (ic == jc && a < b));
+    for (int p = 0; p < 3; ++p) {
+      for (int q = 0; q < 3; ++q) {
+        float dx = pj[q].x - pi[p].x, dy = pj[q].y - pi[p].y, dz = pj[q].z - pi[p].z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float rinv = rsqrtf(r2);
+        float r2inv = rinv * rinv;
+        float vel = qq[p][q] * rinv;
+        float dv;
+        if (p == 0 && q == 0) {
+          float r6 = r2inv * r2inv * r2inv;
+          float va = A * r6 * r6, vb = B * r6;
+          dv = r2inv * (-vel - 12.f * va + 6.f * vb);
+          if (cnt) evdw += (double)(va - vb);
+        } else {
+          dv = r2inv * (-vel);
+        }
+        if (cnt) eel += (double)vel;
+        g[3 * p + 0] -= dv * dx;
+        g[3 * p + 1] -= dv * dy;
+        g[3 * p + 2] -= dv * dz;
+      }
+    }
+  }
+  double gs[9];
+  for (int k = 0; k < 9; ++k) gs[k] = row8_sum((double)g[k]);
+  if (b == 0 && oa >= 0)
+    for (int p = 0; p < 3; ++p) add_grad(d.f, oa + p, gs[3 * p], gs[3 * p + 1], gs[3 * p + 2]);
+  if (en) {
+    eel = wave_sum(eel);
+    evdw = wave_sum(evdw);
+    if (lane == 0) { add_energy(d.ws, w, E_WW_EL, eel); add_energy(d.ws, w, E_WW_VDW, evdw); }
+  }
+}
+
+// solute atom - water molecule pair (nonbond_pw, md.f90:10631-10682), both
+// directions: WATER_SIDE accumulates the water's gradient, otherwise the
+// solute atom's; the energy is counted on the solute side only
+template <bool WATER_SIDE>
+__global__ void __launch_bounds__(256) k_pw(Dev d) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const TileList L = d.tl[WATER_SIDE ? T_WP : T_PW];
+  if (wave >= L.nicl) return;
+  const int ic = wave;
+  const int w = WATER_SIDE ? d.wc_win[ic] : d.sc_win[ic];
+  const WinState& st = d.ws[w];
+  if (!st.active) return;
+  const WinDev& W = d.win[w];
+  const int a = lane >> 3, b = lane & 7;
+  const int ea = WATER_SIDE ? d.wc_o[ic * kCl + a] : d.sc_a[ic * kCl + a];
+  float4 pi[3];
+  float4 ljs = make_float4(0, 0, 0, 0);
+  if (WATER_SIDE) {
+    for (int k = 0; k < 3; ++k) pi[k] = ea >= 0 ? d.x32[ea + k] : make_float4(0, 0, 0, 0);
+  } else {
+    pi[0] = ea >= 0 ? d.x32[ea] : make_float4(0, 0, 0, 0);
+    ljs = ea >= 0 ? d.ljp[ea] : ljs;
+  }
+  const bool en = st.energy_now && !WATER_SIDE;
+  float g[9];
+  for (int k = 0; k < 9; ++k) g[k] = 0.f;
+  double eel = 0.0, evdw = 0.0;
+  const int s0 = L.start[ic], n = L.count[ic];
+  for (int t = 0; t < n; ++t) {
+    const int jc = L.jc[s0 + t];
+    const uint2 m = L.mask[s0 + t];
+    if (!lane_bit(m, lane)) continue;
+    const int eb = WATER_SIDE ? d.sc_a[jc * kCl + b] : d.wc_o[jc * kCl + b];
+    float4 ps, pw[3];
+    float4 lj;
+    if (WATER_SIDE) {
+      ps = d.x32[eb]; lj = d.ljp[eb];
+      pw[0] = pi[0]; pw[1] = pi[1]; pw[2] = pi[2];
+    } else {
+      ps = pi[0]; lj = ljs;
+      pw[0] = d.x32[eb]; pw[1] = d.x32[eb + 1]; pw[2] = d.x32[eb + 2];
+    }
+    for (int k = 0; k < 3; ++k) {
+      // dx = x(water atom) - x(solute atom), as nonbond_pw with i solute, j water
+      float dx = pw[k].x - ps.x, dy = pw[k].y - ps.y, dz = pw[k].z - ps.z;
+      float r2 = dx * dx + dy * dy + dz * dz;
+      float rinv = rsqrtf(r2);
+      float r2inv = rinv * rinv;
+      float vel = (ps.w * pw[k].w) * rinv;
+      float dv;
+      if (k == 0) {
+        float r6 = r2inv * r2inv * r2inv;
+        float va = (lj.x * W.aO1) * r6 * r6, vb = (lj.y * W.bO1) * r6;
+        dv = r2inv * (-vel - 12.f * va + 6.f * vb);
+        if (en) evdw += (double)(va - vb);
+      } else {
+        dv = r2inv * (-vel);
+      }
+      if (en) eel += (double)vel;
+      if (WATER_SIDE) {
+        g[3 * k + 0] += dv * dx; g[3 * k + 1] += dv * dy; g[3 * k + 2] += dv * dz;
+      } else {
+        g[0] -= dv * dx; g[1] -= dv * dy; g[2] -= dv * dz;
+      }
+    }
+  }
+  const int nat = WATER_SIDE ? 3 : 1;
+  double gs[9];
+  for (int k = 0; k < 3 * nat; ++k) gs[k] = row8_sum((double)g[k]);
+  if (b == 0 && ea >= 0)
+    for (int p = 0; p < nat; ++p) add_grad(d.f, ea + p, gs[3 * p], gs[3 * p + 1], gs[3 * p + 2]);
+  if (en) {
+    eel = wave_sum(eel);
+    evdw = wave_sum(evdw);
+    if (lane == 0) { add_energy(d.ws, w, E_PW_EL, eel); add_energy(d.ws, w, E_PW_VDW, evdw); }
+  }
+}
+
+// solute-solute (nonbond_pp, md.f90:10387-10491); 1-4 pairs use the 1-4 LJ
+// parameters and el14_scale
+__global__ void __launch_bounds__(256) k_pp(Dev d) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const TileList L = d.tl[T_PP];
+  if (wave >= L.nicl) return;
+  const int ic = wave;
+  const int w = d.sc_win[ic];
+  const WinState& st = d.ws[w];
+  if (!st.active) return;
+  const WinDev& W = d.win[w];
+  const int a = lane >> 3, b = lane & 7;
+  const int ea = d.sc_a[ic * kCl + a];
+  const float4 pi = ea >= 0 ? d.x32[ea] : make_float4(0, 0, 0, 0);
+  const float4 li = ea >= 0 ? d.ljp[ea] : make_float4(0, 0, 0, 0);
+  const bool en = st.energy_now;
+  float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+  double eel = 0.0, evdw = 0.0;
+  const int s0 = L.start[ic], n = L.count[ic];
+  for (int t = 0; t < n; ++t) {
+    const int jc = L.jc[s0 + t];
+    const uint2 m = L.mask[s0 + t];
+    if (!lane_bit(m, lane)) continue;
+    const bool is14 = lane_bit(L.m14[s0 + t], lane);
+    const int eb = d.sc_a[jc * kCl + b];
+    const float4 pj = d.x32[eb];
+    const float4 lj = d.ljp[eb];
+    float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float rinv = rsqrtf(r2);
+    float r2inv = rinv * rinv;
+    float r6 = r2inv * r2inv * r2inv;
+    float vel = (pi.w * pj.w) * rinv;
+    float aa, bb;
+    if (is14) { vel *= W.el14; aa = li.z * lj.z; bb = li.w * lj.w; }
+    else { aa = li.x * lj.x; bb = li.y * lj.y; }
+    float va = r6 * r6 * aa, vb = r6 * bb;
+    float dv = r2inv * (-vel - 12.f * va + 6.f * vb);
+    g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz;
+    if (en && (ic < jc || (ic == jc && a < b))) { eel += (double)vel; evdw += (double)(va - vb); }
+  }
+  double s0x = row8_sum((double)g0), s0y = row8_sum((double)g1), s0z = row8_sum((double)g2);
+  if (b == 0 && ea >= 0) add_grad(d.f, ea, s0x, s0y, s0z);
+  if (en) {
+    eel = wave_sum(eel);
+    evdw = wave_sum(evdw);
+    if (lane == 0) { add_energy(d.ws, w, E_PP_EL, eel); add_energy(d.ws, w, E_PP_VDW, evdw); }
+  }
+}
+
+// ---- Q-atom tiles: q-atom vs water (nonbond_qw_spc, md.f90:11302-11411) and
+// q-atom vs solute atom (nonbond_qp_qvdw / nonbond_qp, md.f90:11138-11204,
+// 10980-11045), all states; gradients are lambda-weighted over states.
+
+// df32 energies of one q-atom / water pair in state s (energy steps only)
+__device__ void qw_energy_df(const Dev& d, const WinDev& W, int iq, int s, int qa, int ow, double* el,
+                             double* vdw) {
+  const double* x = d.x;
+  const int S = W.S;
+  float qc = d.q_crg[W.qs_off + iq * S + s];
+  df r2[3];
+  for (int k = 0; k < 3; ++k) {
+    df dx = df_dx(x[3 * (ow + k)], x[3 * qa]);
+    df dy = df_dx(x[3 * (ow + k) + 1], x[3 * qa + 1]);
+    df dz = df_dx(x[3 * (ow + k) + 2], x[3 * qa + 2]);
+    r2[k] = df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz));
+  }
+  df one = {1.f, 0.f};
+  double e = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    float qq = (k == 0 ? W.crg_ow : W.crg_hw) * qc;   // real(4) product as in the reference
+    df rinv = df_div(one, df_sqrt(r2[k]));
+    e += df_to(df_mulf(rinv, qq));
+  }
+  *el = e;
+  // LJ on the oxygen
+  df r6 = df_mul(df_mul(r2[0], r2[0]), r2[0]);
+  double aLJ, bLJ, sc = 0.0;
+  const double* tl = &d.t_lj[(W.t_off + W.iac_ow) * 4];
+  if (W.qvdw) {
+    const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
+    aLJ = ql[0] * tl[0];
+    bLJ = ql[1] * tl[1];
+    sc = d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s];
+  } else {
+    const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
+    aLJ = ql[0] * tl[0];
+    bLJ = ql[1] * tl[1];
+  }
+  df r6sc = df_add(r6, df_from(sc));
+  df va = df_div(df_from(aLJ), df_mul(r6sc, r6sc));
+  df vb = df_div(df_from(bLJ), r6sc);
+  *vdw = df_to(df_add(va, df_neg(vb)));
+}
+
+// df32 energies of one q-atom / solute-atom pair in state s
+__device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int s, int qa, int j, bool is14, double* el,
+                             double* vdw) {
+  const double* x = d.x;
+  const int S = W.S;
+  df dx = df_dx(x[3 * j], x[3 * qa]);
+  df dy = df_dx(x[3 * j + 1], x[3 * qa + 1]);
+  df dz = df_dx(x[3 * j + 2], x[3 * qa + 2]);
+  df r2 = df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz));
+  df one = {1.f, 0.f};
+  float qq = d.q_crg[W.qs_off + iq * S + s] * d.x32[j].w;
+  df vel = df_mulf(df_div(one, df_sqrt(r2)), qq);
+  if (is14) vel = df_mul(vel, df_from((double)W.el14));
+  *el = df_to(vel);
+  const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
+  const double* tl = &d.t_lj[(W.t_off + d.iac[j]) * 4];
+  int k = is14 ? 2 : 0;
+  double aLJ = ql[k] * tl[k], bLJ = ql[k + 1] * tl[k + 1];
+  double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + d.iac[j]) * S + s] : 0.0;
+  df r6 = df_mul(df_mul(r2, r2), r2);
+  df r6sc = df_add(r6, df_from(sc));
+  df va = df_div(df_from(aLJ), df_mul(r6sc, r6sc));
+  df vb = df_div(df_from(bLJ), r6sc);
+  *vdw = df_to(df_add(va, df_neg(vb)));
+}
+
+// Q-atom side of q-water tiles: one lane = (q-atom a, water b)
+template <bool Q_SIDE>
+__global__ void __launch_bounds__(256) k_qw(Dev d) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const TileList L = d.tl[Q_SIDE ? T_QW : T_WQ];
+  if (wave >= L.nicl) return;
+  const int ic = wave;
+  const int w = Q_SIDE ? d.qc_win[ic] : d.wc_win[ic];
+  const WinState& st = d.ws[w];
+  if (!st.active) return;
+  const WinDev& W = d.win[w];
+  const int S = W.S;
+  const int a = lane >> 3, b = lane & 7;
+  int iq_l = -1, qa = -1, ow = -1;
+  if (Q_SIDE) {
+    iq_l = d.qc_q[ic * kCl + a];
+    if (iq_l >= 0) qa = d.q_atom[W.q_off + iq_l];
+  } else {
+    ow = d.wc_o[ic * kCl + a];
+  }
+  const bool en = st.energy_now && Q_SIDE;
+  float g[9];
+  for (int k = 0; k < 9; ++k) g[k] = 0.f;
+  double eel[kMaxS], evdw[kMaxS];
+  for (int s = 0; s < kMaxS; ++s) { eel[s] = 0.0; evdw[s] = 0.0; }
+  const float aO = W.aO1, bO = W.bO1;
+  const int s0 = L.start[ic], n = L.count[ic];
+  for (int t = 0; t < n; ++t) {
+    const int jc = L.jc[s0 + t];
+    const uint2 m = L.mask[s0 + t];
+    if (!lane_bit(m, lane)) continue;
+    int iq = iq_l, q_at = qa, o = ow;
+    if (Q_SIDE) o = d.wc_o[jc * kCl + b];
+    else { iq = d.qc_q[jc * kCl + b]; q_at = d.q_atom[W.q_off + iq]; }
+    const float4 pq = d.x32[q_at];
+    float dxv[3], dyv[3], dzv[3], r2v[3];
+    for (int k = 0; k < 3; ++k) {
+      float4 pw = d.x32[o + k];
+      dxv[k] = pw.x - pq.x; dyv[k] = pw.y - pq.y; dzv[k] = pw.z - pq.z;
+      r2v[k] = dxv[k] * dxv[k] + dyv[k] * dyv[k] + dzv[k] * dzv[k];
+    }
+    float rO = rsqrtf(r2v[0]), r2O = rO * rO;
+    float rH1 = rsqrtf(r2v[1]), r2H1 = rH1 * rH1;
+    float rH2 = rsqrtf(r2v[2]), r2H2 = rH2 * rH2;
+    float r6hc = r2v[0] * r2v[0] * r2v[0];
+    float dvO = 0.f, dvH1 = 0.f, dvH2 = 0.f;
+    for (int s = 0; s < S; ++s) {
+      const float lam = (float)W.lambda[s];
+      const float4 ql = d.q_lj[W.qs_off + iq * S + s];
+      float va, vb, ratio = 1.f;
+      if (W.qvdw) {
+        float r6sc = r6hc + d.q_sc[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s];
+        float inv = 1.f / r6sc;
+        va = ql.x * aO * inv * inv;
+        vb = ql.y * bO * inv;
+        ratio = r6hc * inv;
+      } else {
+        float r6 = 1.f / r6hc;
+        va = ql.x * aO * r6 * r6;
+        vb = ql.y * bO * r6;
+      }
+      float qc = d.q_crg[W.qs_off + iq * S + s];
+      float velO = (W.crg_ow * qc) * rO, velH1 = (W.crg_hw * qc) * rH1, velH2 = (W.crg_hw * qc) * rH2;
+      dvO += r2O * (-velO - (12.f * va - 6.f * vb) * ratio) * lam;
+      dvH1 -= r2H1 * velH1 * lam;
+      dvH2 -= r2H2 * velH2 * lam;
+      if (en) {
+        double e_el, e_vdw;
+        qw_energy_df(d, W, iq, s, q_at, o, &e_el, &e_vdw);
+        eel[s] += e_el;
+        evdw[s] += e_vdw;
+      }
+    }
+    if (Q_SIDE) {
+      g[0] -= dvO * dxv[0] + dvH1 * dxv[1] + dvH2 * dxv[2];
+      g[1] -= dvO * dyv[0] + dvH1 * dyv[1] + dvH2 * dyv[2];
+      g[2] -= dvO * dzv[0] + dvH1 * dzv[1] + dvH2 * dzv[2];
+    } else {
+      g[0] += dvO * dxv[0]; g[1] += dvO * dyv[0]; g[2] += dvO * dzv[0];
+      g[3] += dvH1 * dxv[1]; g[4] += dvH1 * dyv[1]; g[5] += dvH1 * dzv[1];
+      g[6] += dvH2 * dxv[2]; g[7] += dvH2 * dyv[2]; g[8] += dvH2 * dzv[2];
+    }
+  }
+  const int nat = Q_SIDE ? 1 : 3;
+  const int own = Q_SIDE ? qa : ow;
+  double gs[9];
+  for (int k = 0; k < 3 * nat; ++k) gs[k] = row8_sum((double)g[k]);
+  if (b == 0 && own >= 0)
+    for (int p = 0; p < nat; ++p) add_grad(d.f, own + p, gs[3 * p], gs[3 * p + 1], gs[3 * p + 2]);
+  if (en) {
+    for (int s = 0; s < S; ++s) {
+      double e1 = wave_sum(eel[s]), e2 = wave_sum(evdw[s]);
+      if (lane == 0) { add_energy(d.ws, w, E_QW0 + 2 * s, e1); add_energy(d.ws, w, E_QW0 + 2 * s + 1, e2); }
+    }
+  }
+}
+
+// q-atom vs solute atom tiles
+template <bool Q_SIDE>
+__global__ void __launch_bounds__(256) k_qp(Dev d) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const TileList L = d.tl[Q_SIDE ? T_QP : T_PQ];
+  if (wave >= L.nicl) return;
+  const int ic = wave;
+  const int w = Q_SIDE ? d.qc_win[ic] : d.sc_win[ic];
+  const WinState& st = d.ws[w];
+  if (!st.active) return;
+  const WinDev& W = d.win[w];
+  const int S = W.S;
+  const int a = lane >> 3, b = lane & 7;
+  int iq_l = -1, qa = -1, pa = -1;
+  if (Q_SIDE) {
+    iq_l = d.qc_q[ic * kCl + a];
+    if (iq_l >= 0) qa = d.q_atom[W.q_off + iq_l];
+  } else {
+    pa = d.sc_a[ic * kCl + a];
+  }
+  const bool en = st.energy_now && Q_SIDE;
+  float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+  double eel[kMaxS], evdw[kMaxS];
+  for (int s = 0; s < kMaxS; ++s) { eel[s] = 0.0; evdw[s] = 0.0; }
+  const int s0 = L.start[ic], n = L.count[ic];
+  for (int t = 0; t < n; ++t) {
+    const int jc = L.jc[s0 + t];
+    const uint2 m = L.mask[s0 + t];
+    if (!lane_bit(m, lane)) continue;
+    const bool is14 = lane_bit(L.m14[s0 + t], lane);
+    int iq = iq_l, q_at = qa, j = pa;
+    if (Q_SIDE) j = d.sc_a[jc * kCl + b];
+    else { iq = d.qc_q[jc * kCl + b]; q_at = d.q_atom[W.q_off + iq]; }
+    const float4 pq = d.x32[q_at];
+    const float4 pj = d.x32[j];
+    const float4 lj = d.ljp[j];
+    float dx = pj.x - pq.x, dy = pj.y - pq.y, dz = pj.z - pq.z;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float rinv = rsqrtf(r2), r2inv = rinv * rinv;
+    float r6hc = r2 * r2 * r2;
+    float dv = 0.f;
+    for (int s = 0; s < S; ++s) {
+      const float lam = (float)W.lambda[s];
+      const float4 ql = d.q_lj[W.qs_off + iq * S + s];
+      float aLJ = is14 ? ql.z * lj.z : ql.x * lj.x;
+      float bLJ = is14 ? ql.w * lj.w : ql.y * lj.y;
+      float vel = (d.q_crg[W.qs_off + iq * S + s] * pj.w) * rinv;
+      if (is14) vel *= W.el14;
+      float va, vb, ratio = 1.f;
+      if (W.qvdw) {
+        float r6sc = r6hc + d.q_sc[W.qsc_off + ((size_t)iq * W.natyps + d.iac[j]) * S + s];
+        float inv = 1.f / r6sc;
+        va = aLJ * inv * inv;
+        vb = bLJ * inv;
+        ratio = r6hc * inv;
+      } else {
+        float r6 = r2inv * r2inv * r2inv;
+        va = aLJ * r6 * r6;
+        vb = bLJ * r6;
+      }
+      dv += r2inv * (-vel - (12.f * va - 6.f * vb) * ratio) * lam;
+      if (en) {
+        double e_el, e_vdw;
+        qp_energy_df(d, W, iq, s, q_at, j, is14, &e_el, &e_vdw);
+        eel[s] += e_el;
+        evdw[s] += e_vdw;
+      }
+    }
+    if (Q_SIDE) { g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz; }
+    else { g0 += dv * dx; g1 += dv * dy; g2 += dv * dz; }
+  }
+  const int own = Q_SIDE ? qa : pa;
+  double sx = row8_sum((double)g0), sy = row8_sum((double)g1), sz = row8_sum((double)g2);
+  if (b == 0 && own >= 0) add_grad(d.f, own, sx, sy, sz);
+  if (en) {
+    for (int s = 0; s < S; ++s) {
+      double e1 = wave_sum(eel[s]), e2 = wave_sum(evdw[s]);
+      if (lane == 0) { add_energy(d.ws, w, E_Q0 + 8 * s + 6, e1); add_energy(d.ws, w, E_Q0 + 8 * s + 7, e2); }
+    }
+  }
+}
+
+// q-q and q-(bonded neighbour) pairs (nonbond_qq, md.f90:10763-10868), one
+// thread per (pair, state)
+__global__ void k_qq(Dev d) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nqq) return;
+  const int w = d.qq_win[k];
+  const WinState& st = d.ws[w];
+  if (!st.active) return;
+  const WinDev& W = d.win[w];
+  const int S = W.S;
+  const int4 e = d.qq_ij[k];
+  const int s = d.qq_st[k];
+  const int i = d.q_atom[W.q_off + e.x], j = e.y, jq = e.z, iLJ = e.w;
+  const float es = d.qq_es[k];
+  // parameters (double, as the reference)
+  double qi = d.q_crg[W.qs_off + e.x * S + s];
+  double qj = jq ? (double)d.q_crg[W.qs_off + (jq - 1) * S + s] : (double)d.x32[j].w;
+  double aLJ, bLJ;
+  const double* qli = &d.q_ljd[(size_t)(W.qs_off + e.x * S + s) * 4];
+  const int c = (iLJ == 3) ? 2 : 0;
+  if (!W.qvdw) {
+    const double* ti = &d.t_lj[(W.t_off + d.iac[i]) * 4];
+    const double* tj = &d.t_lj[(W.t_off + d.iac[j]) * 4];
+    aLJ = ti[c] * tj[c]; bLJ = ti[c + 1] * tj[c + 1];
+  } else if (jq) {
+    const double* qlj = &d.q_ljd[(size_t)(W.qs_off + (jq - 1) * S + s) * 4];
+    aLJ = qli[c] * qlj[c]; bLJ = qli[c + 1] * qlj[c + 1];
+  } else {
+    const double* tj = &d.t_lj[(W.t_off + d.iac[j]) * 4];
+    aLJ = qli[c] * tj[c]; bLJ = qli[c + 1] * tj[c + 1];
+  }
+  const double sc = d.qq_sc[k];
+  const bool expo = W.qvdw && jq && iLJ == 2;
+  // gradient in FP32 from the FP32 coordinates
+  float4 pi = d.x32[i], pj = d.x32[j];
+  float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+  float r2 = dx * dx + dy * dy + dz * dz;
+  float r6hc = r2 * r2 * r2;
+  float r6 = 1.f / (r6hc + (float)sc);
+  float r12 = r6 * r6;
+  float r2inv = 1.f / r2, r = sqrtf(r2inv);
+  float vel = (float)(qi * qj) * r * es;
+  if (iLJ == 3) vel *= W.el14;
+  float dv;
+  if (expo) {
+    float va = (float)aLJ * expf(-(float)bLJ / r);
+    dv = r2inv * (-vel - (float)bLJ * va / r) * (float)W.lambda[s];
+  } else {
+    float va = (float)aLJ * r12, vb = (float)bLJ * r6;
+    dv = r2inv * (-vel - (12.f * va - 6.f * vb) * r6hc * r6) * (float)W.lambda[s];
+  }
+  add_grad(d.f, i, -(double)(dv * dx), -(double)(dv * dy), -(double)(dv * dz));
+  add_grad(d.f, j, (double)(dv * dx), (double)(dv * dy), (double)(dv * dz));
+  if (st.energy_now) {
+    df ddx = df_dx(d.x[3 * j], d.x[3 * i]), ddy = df_dx(d.x[3 * j + 1], d.x[3 * i + 1]),
+       ddz = df_dx(d.x[3 * j + 2], d.x[3 * i + 2]);
+    df rr2 = df_add(df_add(df_mul(ddx, ddx), df_mul(ddy, ddy)), df_mul(ddz, ddz));
+    df one = {1.f, 0.f};
+    df rinv = df_div(one, df_sqrt(rr2));
+    df qq = df_2p((float)qi, (float)qj);
+    df v = df_mul(df_mul(qq, rinv), df_from((double)es));
+    if (iLJ == 3) v = df_mul(v, df_from((double)W.el14));
+    double e_el = df_to(v), e_vdw;
+    if (expo) {
+      e_vdw = aLJ * exp(-bLJ * sqrt(df_to(rr2)));
+    } else {
+      df r6s = df_add(df_mul(df_mul(rr2, rr2), rr2), df_from(sc));
+      df va = df_div(df_from(aLJ), df_mul(r6s, r6s));
+      df vb = df_div(df_from(bLJ), r6s);
+      e_vdw = df_to(df_add(va, df_neg(vb)));
+    }
+    int base = jq ? 4 : 6;  // qq or qp slot
+    add_energy(d.ws, w, E_Q0 + 8 * s + base, e_el);
+    add_energy(d.ws, w, E_Q0 + 8 * s + base + 1, e_vdw);
+  }
+}
+
+// ------------------------------------------------------------- LRF --
+// Build: one block per target charge group accumulates the Taylor
+// expansion of every source atom of every group beyond the explicit cut-off
+// and within RcLRF (nbpplist_lrf 5448-5700, nbpwlist_lrf 6864-7068,
+// nbwwlist_lrf 8370-8556); FP64, fixed summation order per block.
+__global__ void __launch_bounds__(256) k_lrf_build(Dev d, int ngrp_total) {
+  const int g = blockIdx.x;
+  if (g >= ngrp_total) return;
+  const int w = d.g_win[g];
+  const WinState& st = d.ws[w];
+  const WinDev& W = d.win[w];
+  if (!st.active || !st.list_now || !W.use_lrf) return;
+  const int swg = d.g_sw[g];
+  if (d.aflag[swg] & 1) return;
+  const bool gw = (g - W.g_off) >= W.ngrp_solute;
+  const double cx = d.g_cent[3 * g], cy = d.g_cent[3 * g + 1], cz = d.g_cent[3 * g + 2];
+  double acc[20];
+  for (int k = 0; k < 20; ++k) acc[k] = 0.0;
+  for (int h = W.g_off + threadIdx.x; h < W.g_off + W.ngrp; h += blockDim.x) {
+    if (h == g) continue;
+    const int swh = d.g_sw[h];
+    if (d.aflag[swh] & 1) continue;
+    const bool hw = (h - W.g_off) >= W.ngrp_solute;
+    const double rc2 = (gw && hw) ? W.rcww2 : ((gw || hw) ? W.rcpw2 : W.rcpp2);
+    const double r2g = r2_exact(d.x, swg, swh);
+    if (r2g <= rc2 || r2g > W.rclrf2) continue;
+    const int n = d.g_n[h], f0 = d.g_first[h];
+    for (int k = 0; k < n; ++k) {
+      const int at = d.g_atoms[f0 + k];
+      if (d.aflag[at] & 8) continue;  // q-atom
+      const double q = (double)d.x32[at].w;
+      const double dx = d.x[3 * at] - cx, dy = d.x[3 * at + 1] - cy, dz = d.x[3 * at + 2] - cz;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double f0v = q / (r2 * sqrt(r2));
+      const double f1 = 3.0 * f0v / r2;
+      const double f2 = -f1 / r2;
+      acc[0] += f0v * r2;
+      acc[1] -= f0v * dx; acc[2] -= f0v * dy; acc[3] -= f0v * dz;
+      acc[4] += f1 * dx * dx - f0v; acc[5] += f1 * dx * dy; acc[6] += f1 * dx * dz;
+      acc[7] += f1 * dy * dy - f0v; acc[8] += f1 * dy * dz; acc[9] += f1 * dz * dz - f0v;
+      acc[10] += f2 * (5.0 * dx * dx * dx - r2 * 3.0 * dx);
+      acc[11] += f2 * (5.0 * dx * dx * dy - r2 * dy);
+      acc[12] += f2 * (5.0 * dx * dx * dz - r2 * dz);
+      acc[13] += f2 * (5.0 * dx * dy * dy - r2 * dx);
+      acc[14] += f2 * (5.0 * dx * dy * dz);
+      acc[15] += f2 * (5.0 * dx * dz * dz - r2 * dx);
+      acc[16] += f2 * (5.0 * dy * dy * dy - r2 * 3.0 * dy);
+      acc[17] += f2 * (5.0 * dy * dy * dz - r2 * dz);
+      acc[18] += f2 * (5.0 * dy * dz * dz - r2 * dy);
+      acc[19] += f2 * (5.0 * dz * dz * dz - r2 * 3.0 * dz);
+    }
+  }
+  __shared__ double red[4][20];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = 0; k < 20; ++k) {
+    double v = wave_sum(acc[k]);
+    if (lane == 0) red[wv][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 20) {
+    double v = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) v += red[k][threadIdx.x];
+    d.g_lrf[20 * g + threadIdx.x] = v;
+  }
+}
+
+// Evaluation per atom (lrf_taylor, md.f90:3646-3712), FP64.
+__global__ void k_lrf_taylor(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.ntot) return;
+  const int w = d.wid[i];
+  const WinState& st = d.ws[w];
+  const WinDev& W = d.win[w];
+  if (!st.active || !W.use_lrf) return;
+  if (d.aflag[i] & (1 | 8)) return;  // excluded or q-atom
+  const int g = d.grp[i];
+  const double* p = &d.g_lrf[20 * g];
+  const double r0 = d.g_cent[3 * g] - d.x[3 * i], r1 = d.g_cent[3 * g + 1] - d.x[3 * i + 1],
+               r2 = d.g_cent[3 * g + 2] - d.x[3 * i + 2];
+  const double xx = p[4], xy = p[5], xz = p[6], yy = p[7], yz = p[8], zz = p[9];
+  const double t_xxx = p[10], t_xxy = p[11], t_xxz = p[12], t_xyy = p[13], t_xyz = p[14], t_xzz = p[15],
+               t_yyy = p[16], t_yyz = p[17], t_yzz = p[18], t_zzz = p[19];
+  double vij = p[0] + p[1] * r0 + p[2] * r1 + p[3] * r2 + 0.5 * (xx * r0 + xy * r1 + xz * r2) * r0 +
+               0.5 * (xy * r0 + yy * r1 + yz * r2) * r1 + 0.5 * (xz * r0 + yz * r1 + zz * r2) * r2;
+  double f0 = p[1] + xx * r0 + xy * r1 + xz * r2 + 0.5 * (t_xxx * r0 + t_xxy * r1 + t_xxz * r2) * r0 +
+              0.5 * (t_xxy * r0 + t_xyy * r1 + t_xyz * r2) * r1 + 0.5 * (t_xxz * r0 + t_xyz * r1 + t_xzz * r2) * r2;
+  double f1 = p[2] + xy * r0 + yy * r1 + yz * r2 + 0.5 * (t_xxy * r0 + t_xyy * r1 + t_xyz * r2) * r0 +
+              0.5 * (t_xyy * r0 + t_yyy * r1 + t_yyz * r2) * r1 + 0.5 * (t_xyz * r0 + t_yyz * r1 + t_yzz * r2) * r2;
+  double f2 = p[3] + xz * r0 + yz * r1 + zz * r2 + 0.5 * (t_xxz * r0 + t_xyz * r1 + t_xzz * r2) * r0 +
+              0.5 * (t_xyz * r0 + t_yyz * r1 + t_yzz * r2) * r1 + 0.5 * (t_xzz * r0 + t_yzz * r1 + t_zzz * r2) * r2;
+  const double q = (double)d.x32[i].w;
+  add_grad(d.f, i, -q * f0, -q * f1, -q * f2);
+  if (st.energy_now) add_energy(d.ws, w, E_LRF, 0.5 * q * vij);
+}
+
+// ---------------------------------------------------------- bonded --
+__device__ __forceinline__ void torsion_geom(const double* x, int i, int j, int k, int l, double* phi,
+                                             double* scp_out, double dp[12], bool inv_style) {
+  double rji[3], rjk[3], rkl[3], rnj[3], rnk[3];
+  for (int c = 0; c < 3; ++c) {
+    rji[c] = x[3 * i + c] - x[3 * j + c];
+    rjk[c] = x[3 * k + c] - x[3 * j + c];
+    rkl[c] = x[3 * l + c] - x[3 * k + c];
+  }
+  rnj[0] = rji[1] * rjk[2] - rji[2] * rjk[1];
+  rnj[1] = rji[2] * rjk[0] - rji[0] * rjk[2];
+  rnj[2] = rji[0] * rjk[1] - rji[1] * rjk[0];
+  rnk[0] = -rjk[1] * rkl[2] + rjk[2] * rkl[1];
+  rnk[1] = -rjk[2] * rkl[0] + rjk[0] * rkl[2];
+  rnk[2] = -rjk[0] * rkl[1] + rjk[1] * rkl[0];
+  double bj2inv = 1.0 / (rnj[0] * rnj[0] + rnj[1] * rnj[1] + rnj[2] * rnj[2]);
+  double bk2inv = 1.0 / (rnk[0] * rnk[0] + rnk[1] * rnk[1] + rnk[2] * rnk[2]);
+  double bjinv = sqrt(bj2inv), bkinv = sqrt(bk2inv);
+  double scp = (rnj[0] * rnk[0] + rnj[1] * rnk[1] + rnj[2] * rnk[2]) * (bjinv * bkinv);
+  if (scp > 1.0) scp = 1.0;
+  else if (scp < -1.0) scp = -1.0;
+  double ph = acos(scp);
+  double sg = rjk[0] * (rnj[1] * rnk[2] - rnj[2] * rnk[1]) + rjk[1] * (rnj[2] * rnk[0] - rnj[0] * rnk[2]) +
+              rjk[2] * (rnj[0] * rnk[1] - rnj[1] * rnk[0]);
+  if (sg < 0) ph = -ph;
+  double f1 = sin(ph);
+  if (fabs(f1) < 1.e-12) f1 = 1.e-12;
+  f1 = -1.0 / f1;
+  double di[3], dl[3];
+  (void)inv_style;
+  for (int c = 0; c < 3; ++c) {
+    di[c] = f1 * (rnk[c] * (bjinv * bkinv) - scp * rnj[c] * bj2inv);
+    dl[c] = f1 * (rnj[c] * (bjinv * bkinv) - scp * rnk[c] * bk2inv);
+  }
+  double rki[3], rlj[3];
+  for (int c = 0; c < 3; ++c) { rki[c] = rji[c] - rjk[c]; rlj[c] = -rjk[c] - rkl[c]; }
+  dp[0] = rjk[1] * di[2] - rjk[2] * di[1];
+  dp[1] = rjk[2] * di[0] - rjk[0] * di[2];
+  dp[2] = rjk[0] * di[1] - rjk[1] * di[0];
+  dp[3] = rki[1] * di[2] - rki[2] * di[1] + rkl[1] * dl[2] - rkl[2] * dl[1];
+  dp[4] = rki[2] * di[0] - rki[0] * di[2] + rkl[2] * dl[0] - rkl[0] * dl[2];
+  dp[5] = rki[0] * di[1] - rki[1] * di[0] + rkl[0] * dl[1] - rkl[1] * dl[0];
+  dp[6] = rlj[1] * dl[2] - rlj[2] * dl[1] - rji[1] * di[2] + rji[2] * di[1];
+  dp[7] = rlj[2] * dl[0] - rlj[0] * dl[2] - rji[2] * di[0] + rji[0] * di[2];
+  dp[8] = rlj[0] * dl[1] - rlj[1] * dl[0] - rji[0] * di[1] + rji[1] * di[0];
+  dp[9] = rjk[1] * dl[2] - rjk[2] * dl[1];
+  dp[10] = rjk[2] * dl[0] - rjk[0] * dl[2];
+  dp[11] = rjk[0] * dl[1] - rjk[1] * dl[0];
+  *phi = ph;
+  *scp_out = scp;
+}
+
+__device__ __forceinline__ void apply4(long long* f, int i, int j, int k, int l, double dv, const double dp[12]) {
+  add_grad(f, i, dv * dp[0], dv * dp[1], dv * dp[2]);
+  add_grad(f, j, dv * dp[3], dv * dp[4], dv * dp[5]);
+  add_grad(f, k, dv * dp[6], dv * dp[7], dv * dp[8]);
+  add_grad(f, l, dv * dp[9], dv * dp[10], dv * dp[11]);
+}
+
+__device__ void angle_term(const double* x, long long* f, int i, int j, int k, double fk, double ang0, double lam,
+                           double* e) {
+  // angle (md.f90:814-901) with gradient scaled by lam (1 for topology angles)
+  double rji[3], rjk[3];
+  for (int c = 0; c < 3; ++c) { rji[c] = x[3 * i + c] - x[3 * j + c]; rjk[c] = x[3 * k + c] - x[3 * j + c]; }
+  double bji2inv = 1.0 / (rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+  double bjk2inv = 1.0 / (rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+  double bjiinv = sqrt(bji2inv), bjkinv = sqrt(bjk2inv);
+  double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]) * bjiinv * bjkinv;
+  if (scp > 1.0) scp = 1.0;
+  else if (scp < -1.0) scp = -1.0;
+  double angv = acos(scp);
+  double da = angv - ang0;
+  *e = 0.5 * fk * da * da;
+  double dv = fk * da * lam;
+  double f1 = sin(angv);
+  if (fabs(f1) < 1.e-12) f1 = -1.e12;
+  else f1 = -1.0 / f1;
+  double di[3], dk[3];
+  for (int c = 0; c < 3; ++c) {
+    di[c] = f1 * (rjk[c] * bjiinv * bjkinv - scp * rji[c] * bji2inv);
+    dk[c] = f1 * (rji[c] * bjiinv * bjkinv - scp * rjk[c] * bjk2inv);
+  }
+  add_grad(f, i, dv * di[0], dv * di[1], dv * di[2]);
+  add_grad(f, k, dv * dk[0], dv * dk[1], dv * dk[2]);
+  add_grad(f, j, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
+}
+
+__global__ void k_bonded(Dev d) {
+  const int n1 = d.nbond, n2 = n1 + d.nang, n3 = n2 + d.ntor, n4 = n3 + d.nimp;
+  const int n5 = n4 + d.nqb, n6 = n5 + d.nqa, n7 = n6 + d.nqt, n8 = n7 + d.nqi;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n8) return;
+  const double* x = d.x;
+  if (t < n1) {  // bond (md.f90:954-997)
+    const int w = d.b_win[t];
+    if (!d.ws[w].active) return;
+    int2 ij = d.b_ij[t];
+    double2 p = d.b_par[t];
+    double r[3];
+    for (int c = 0; c < 3; ++c) r[c] = x[3 * ij.y + c] - x[3 * ij.x + c];
+    double bl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double db = bl - p.y;
+    double dv = p.x * db / bl;
+    add_grad(d.f, ij.y, r[0] * dv, r[1] * dv, r[2] * dv);
+    add_grad(d.f, ij.x, -r[0] * dv, -r[1] * dv, -r[2] * dv);
+    if (d.ws[w].energy_now) add_energy(d.ws, w, d.b_solv[t] ? E_W_BOND : E_P_BOND, 0.5 * p.x * db * db);
+  } else if (t < n2) {  // angle (+ Urey-Bradley for CHARMM, md.f90:904-951)
+    const int a = t - n1;
+    const int w = d.a_win[a];
+    if (!d.ws[w].active) return;
+    int4 ijk = d.a_ijk[a];
+    double4 p = d.a_par[a];
+    double e;
+    angle_term(x, d.f, ijk.x, ijk.y, ijk.z, p.x, p.y, 1.0, &e);
+    if (d.urey[w] && p.z > 0.0) {
+      double r[3];
+      for (int c = 0; c < 3; ++c) r[c] = x[3 * ijk.z + c] - x[3 * ijk.x + c];
+      double dik = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      double ru = dik - p.w;
+      e += p.z * ru * ru;
+      double du = 2 * p.z * ru / dik;
+      add_grad(d.f, ijk.z, du * r[0], du * r[1], du * r[2]);
+      add_grad(d.f, ijk.x, -du * r[0], -du * r[1], -du * r[2]);
+    }
+    if (d.ws[w].energy_now) add_energy(d.ws, w, d.a_solv[a] ? E_W_ANG : E_P_ANG, e);
+  } else if (t < n3) {  // torsion (md.f90:14793-14907)
+    const int a = t - n2;
+    const int w = d.t_win[a];
+    if (!d.ws[w].active) return;
+    int4 q = d.t_ijkl[a];
+    float4 p = d.t_par[a];  // fk, rmult, deltor, paths (1/paths)
+    double phi, scp, dp[12];
+    torsion_geom(x, q.x, q.y, q.z, q.w, &phi, &scp, dp, false);
+    double arg = (double)p.y * phi - (double)p.z;
+    double e = (double)p.x * (1.0 + cos(arg)) * (double)p.w;
+    double dv = -(double)p.y * (double)p.x * sin(arg) * (double)p.w;
+    apply4(d.f, q.x, q.y, q.z, q.w, dv, dp);
+    if (d.ws[w].energy_now) add_energy(d.ws, w, d.t_solv[a] ? E_W_TOR : E_P_TOR, e);
+  } else if (t < n4) {  // improper: harmonic (1677-1786) or periodic (1789-1897)
+    const int a = t - n3;
+    const int w = d.i_win[a];
+    if (!d.ws[w].active) return;
+    int4 q = d.i_ijkl[a];
+    float2 p = d.i_par[a];
+    double phi, scp, dp[12];
+    torsion_geom(x, q.x, q.y, q.z, q.w, &phi, &scp, dp, false);
+    double e, dv;
+    if (d.i_kind[w] == 2) {
+      double arg = 2 * phi - (double)p.y;
+      e = (double)p.x * (1 + cos(arg));
+      dv = -2 * (double)p.x * sin(arg);
+    } else {
+      const double pi = (double)(4.0f * atanf(1.0f));
+      double arg = phi - (double)p.y;
+      arg = arg - 2. * pi * round(arg / (2. * pi));
+      dv = (double)p.x * arg;
+      e = 0.5 * dv * arg;
+    }
+    apply4(d.f, q.x, q.y, q.z, q.w, dv, dp);
+    if (d.ws[w].energy_now) add_energy(d.ws, w, d.i_solv[a] ? E_W_IMP : E_P_IMP, e);
+  } else if (t < n5) {  // q-bond, Morse + harmonic (md.f90:14060-14114)
+    const int a = t - n4;
+    int2 ws2 = d.qb_ws[a];
+    const int w = ws2.x, s = ws2.y;
+    if (!d.ws[w].active) return;
+    const double lam = d.win[w].lambda[s];
+    int2 ij = d.qb_ij[a];
+    double4 p = d.qb_par[a];  // Dmz, amz, r0, fk
+    double r[3];
+    for (int c = 0; c < 3; ++c) r[c] = x[3 * ij.y + c] - x[3 * ij.x + c];
+    double bl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double db = bl - p.z;
+    double fexp = exp(-p.y * db);
+    double be = p.x * (fexp * fexp - 2. * fexp) + 0.5 * p.w * db * db;
+    double dv = (2. * p.x * p.y * (fexp - fexp * fexp) + p.w * db) * lam / bl;
+    add_grad(d.f, ij.x, -dv * r[0], -dv * r[1], -dv * r[2]);
+    add_grad(d.f, ij.y, dv * r[0], dv * r[1], dv * r[2]);
+    if (d.ws[w].energy_now) add_energy(d.ws, w, E_Q0 + 8 * s + 0, be);
+  } else if (t < n6) {  // q-angle (md.f90:13896-13988)
+    const int a = t - n5;
+    int2 ws2 = d.qa_ws[a];
+    const int w = ws2.x, s = ws2.y;
+    if (!d.ws[w].active) return;
+    int4 ijk = d.qa_ijk[a];
+    double2 p = d.qa_par[a];
+    const int i = ijk.x, j = ijk.y, k = ijk.z;
+    double rji[3], rjk[3];
+    for (int c = 0; c < 3; ++c) { rji[c] = x[3 * i + c] - x[3 * j + c]; rjk[c] = x[3 * k + c] - x[3 * j + c]; }
+    double bji = sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+    double bjk = sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+    double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]) / (bji * bjk);
+    if (scp > 1.0) scp = 1.0;
+    if (scp < -1.0) scp = -1.0;
+    double ang = acos(scp);
+    double da = ang - p.y;
+    double ae = 0.5 * p.x * da * da;
+    double dv = p.x * da * d.win[w].lambda[s];
+    double f1 = sin(ang);
+    if (fabs(f1) < 1.e-12) f1 = 1.e-12;
+    f1 = -1.0 / f1;
+    double di[3], dk[3];
+    for (int c = 0; c < 3; ++c) {
+      di[c] = f1 * (rjk[c] / (bji * bjk) - scp * rji[c] / (bji * bji));
+      dk[c] = f1 * (rji[c] / (bji * bjk) - scp * rjk[c] / (bjk * bjk));
+    }
+    add_grad(d.f, i, dv * di[0], dv * di[1], dv * di[2]);
+    add_grad(d.f, k, dv * dk[0], dv * dk[1], dv * dk[2]);
+    add_grad(d.f, j, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
+    if (d.ws[w].energy_now) add_energy(d.ws, w, E_Q0 + 8 * s + 1, ae);
+  } else {  // q-torsion (14262-14403) / q-improper (14117-14258)
+    const bool tors = t < n7;
+    const int a = tors ? t - n6 : t - n7;
+    int2 ws2 = tors ? d.qt_ws[a] : d.qi_ws[a];
+    const int w = ws2.x, s = ws2.y;
+    if (!d.ws[w].active) return;
+    int4 q = tors ? d.qt_ijkl[a] : d.qi_ijkl[a];
+    // the q-terms use bj, bk and divisions (same values as the topology form)
+    double phi, scp, dp[12];
+    torsion_geom(x, q.x, q.y, q.z, q.w, &phi, &scp, dp, true);
+    const double lam = d.win[w].lambda[s];
+    double pe, dv;
+    if (tors) {
+      double4 p = d.qt_par[a];  // fk, rmult, deltor
+      double arg = p.y * phi - p.z;
+      pe = p.x * (1.0 + cos(arg));
+      dv = -p.y * p.x * sin(arg) * lam;
+    } else {
+      const double pi = (double)(4.0f * atanf(1.0f));
+      double2 p = d.qi_par[a];  // fk, imp0
+      double arg = phi - p.y;
+      arg = arg - 2. * pi * round(arg / (2. * pi));
+      dv = p.x * arg;
+      pe = 0.5 * dv * arg;
+      dv = dv * lam;
+    }
+    apply4(d.f, q.x, q.y, q.z, q.w, dv, dp);
+    if (d.ws[w].energy_now) add_energy(d.ws, w, E_Q0 + 8 * s + (tors ? 2 : 3), pe);
+  }
+}
+
+// ------------------------------------------------------ restraints --
+// fix_shell (md.f90:1414-1450) per atom and the radial water restraint
+// (restrain_solvent, md.f90:14910-14957) per water oxygen
+__global__ void k_restr(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.ntot) return;
+  const int w = d.wid[i];
+  const WinState& st = d.ws[w];
+  if (!st.active) return;
+  const WinDev& W = d.win[w];
+  const uint8_t fl = d.aflag[i];
+  if (fl & 3) {
+    const double fk = (fl & 1) ? 200.0 : W.fk_pshell;
+    double dr[3];
+    for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * i + c] - d.xtop[3 * i + c];
+    add_grad(d.f, i, fk * dr[0], fk * dr[1], fk * dr[2]);
+    if (st.energy_now) {
+      double e = 0.5 * fk * (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+      if (fl & 1) add_energy(d.ws, w, E_FIX, e);
+      if (fl & 2) add_energy(d.ws, w, E_SHELL, e);
+    }
+  }
+  const int li = i - W.ao;
+  if (li >= W.nat_solute && ((li - W.nat_solute) % 3) == 0 && !(fl & 1)) {
+    const double shift = (W.fk_wsphere != 0.) ? sqrt((double)0.001986f * st.tfree / W.fk_wsphere) : 0.0;
+    double dr[3];
+    for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * i + c] - W.xwcent[c];
+    double b = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+    double db = b - (W.rwat - shift);
+    double erst, dv;
+    if (db > 0) {
+      erst = 0.5 * W.fk_wsphere * db * db - W.dwmz;
+      dv = W.fk_wsphere * db / b;
+    } else if (b > 0.0) {
+      double fexp = exp(W.awmz * db);
+      erst = W.dwmz * (fexp * fexp - 2. * fexp);
+      dv = -2. * W.dwmz * W.awmz * (fexp - fexp * fexp) / b;
+    } else {
+      dv = 0; erst = 0;
+    }
+    add_grad(d.f, i, dv * dr[0], dv * dr[1], dv * dr[2]);
+    if (st.energy_now) add_energy(d.ws, w, E_RAD, erst);
+  }
+}
+
+// water polarisation restraint (watpol, md.f90:15122-15288): one block per
+// window; the per-shell selection sort of the reference becomes a rank
+// count (theta ascending, ties by water index, as the first-minimum rule)
+__global__ void __launch_bounds__(256) k_watpol(Dev d) {
+  const int w = blockIdx.x;
+  if (w >= d.nwin) return;
+  const WinState& st = d.ws[w];
+  const WinDev& W = d.win[w];
+  if (!st.active || !W.wpol || W.nwat == 0 || W.nshell == 0) return;
+  ShellDev* sh = d.shells + W.sh_off;
+  const int nsh = W.nshell;
+  const double pi = (double)(4.0f * atanf(1.0f));
+  const int wo = d.w_off[w];
+  __shared__ int cnt[8];
+  __shared__ double avt[8][4];
+  if (threadIdx.x == 0) {
+    if (st.istep != 0 && st.istep % 100 == 0) {
+      for (int is = 0; is < nsh; ++is) {
+        sh[is].avtheta = sh[is].avtheta / 100.0f;
+        sh[is].avn_insh = sh[is].avn_insh / 100.0f;
+        sh[is].theta_corr = sh[is].theta_corr + sh[is].avtheta - acosf(sh[is].cstb);
+        sh[is].avtheta = 0.0f;
+        sh[is].avn_insh = 0.0f;
+      }
+    }
+    for (int is = 0; is < 8; ++is) cnt[is] = 0;
+  }
+  __syncthreads();
+  for (int iw = threadIdx.x; iw < W.nwat; iw += blockDim.x) {
+    const int i = W.ao + W.nat_solute + 3 * iw;
+    d.wshell[wo + iw] = -1;
+    d.wtheta[wo + iw] = 0.0;
+    if (d.aflag[i] & 1) continue;
+    const double* x = d.x;
+    double rmu[3], rcu[3];
+    for (int c = 0; c < 3; ++c) rmu[c] = x[3 * (i + 1) + c] + x[3 * (i + 2) + c] - 2. * x[3 * i + c];
+    double rm = sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
+    for (int c = 0; c < 3; ++c) rmu[c] /= rm;
+    for (int c = 0; c < 3; ++c) rcu[c] = x[3 * i + c] - W.xwcent[c];
+    double rc = sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
+    for (int c = 0; c < 3; ++c) rcu[c] /= rc;
+    double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+    if (scp > 1.0) scp = 1.0;
+    if (scp < -1.0) scp = -1.0;
+    d.wtheta[wo + iw] = acos(scp);
+    if (rc > (double)sh[nsh - 1].rout - (double)sh[nsh - 1].dr) {
+      int is = nsh - 1;
+      for (; is >= 1; --is)
+        if (rc <= (double)sh[is].rout) break;
+      d.wshell[wo + iw] = is;
+      atomicAdd(&cnt[is], 1);
+    }
+  }
+  __syncthreads();
+  double eacc = 0.0;
+  double tsum[8];
+  for (int k = 0; k < 8; ++k) tsum[k] = 0.0;
+  for (int iw = threadIdx.x; iw < W.nwat; iw += blockDim.x) {
+    const int is = d.wshell[wo + iw];
+    if (is < 0) continue;
+    const double th = d.wtheta[wo + iw];
+    int rank = 0;
+    for (int jw = 0; jw < W.nwat; ++jw) {
+      if (d.wshell[wo + jw] != is) continue;
+      double tj = d.wtheta[wo + jw];
+      if (tj < th || (tj == th && jw < iw)) ++rank;
+    }
+    const int il = rank + 1;
+    const int n = cnt[is];
+    float argf = 1.0f + (1.0f - 2.0f * (float)il) / (float)n;
+    double th0 = acos((double)argf);
+    th0 = th0 - 3. * sin(th0) * (double)sh[is].cstb / 2.;
+    if (th0 < 0.0) th0 = 0.0;
+    if (th0 > pi) th0 = pi;
+    tsum[is] += th;
+    const double tc = (double)sh[is].theta_corr;
+    const double dth = th - th0 + tc;
+    eacc += 0.5 * W.fkwpol * dth * dth;
+    const double dv = W.fkwpol * dth;
+    const int i = W.ao + W.nat_solute + 3 * iw;
+    const double* x = d.x;
+    double rmu[3], rcu[3];
+    for (int c = 0; c < 3; ++c) rmu[c] = x[3 * (i + 1) + c] + x[3 * (i + 2) + c] - 2. * x[3 * i + c];
+    double rm = sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
+    for (int c = 0; c < 3; ++c) rmu[c] /= rm;
+    for (int c = 0; c < 3; ++c) rcu[c] = x[3 * i + c] - W.xwcent[c];
+    double rc = sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
+    for (int c = 0; c < 3; ++c) rcu[c] /= rc;
+    double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+    if (scp > 1.0) scp = 1.0;
+    if (scp < -1.0) scp = -1.0;
+    double f0 = sin(acos(scp));
+    if (fabs(f0) < 1.e-12) f0 = 1.e-12;
+    f0 = -1.0 / f0;
+    f0 = dv * f0;
+    double f1[3], f2[3];
+    for (int c = 0; c < 3; ++c) {
+      f1[c] = (rcu[c] - rmu[c] * scp) / rm;
+      f2[c] = (rmu[c] - rcu[c] * scp) / rc;
+    }
+    add_grad(d.f, i, f0 * (-2. * f1[0] + f2[0]), f0 * (-2. * f1[1] + f2[1]), f0 * (-2. * f1[2] + f2[2]));
+    add_grad(d.f, i + 1, f0 * f1[0], f0 * f1[1], f0 * f1[2]);
+    add_grad(d.f, i + 2, f0 * f1[0], f0 * f1[1], f0 * f1[2]);
+  }
+  // per-shell theta sums (deterministic: fixed per-thread partition + tree)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int is = 0; is < nsh; ++is) {
+    double v = wave_sum(tsum[is]);
+    if (lane == 0) avt[is][wv] = v;
+  }
+  double esum = wave_sum(eacc);
+  __shared__ double ered[4];
+  if (lane == 0) ered[wv] = esum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int is = 0; is < nsh; ++is) {
+      if (cnt[is] == 0) continue;
+      double avtdum = 0.0;
+      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) avtdum += avt[is][k];
+      sh[is].avtheta = sh[is].avtheta + (float)(avtdum / (double)(float)cnt[is]);
+      sh[is].avn_insh = sh[is].avn_insh + (float)cnt[is];
+      sh[is].n_insh = cnt[is];
+    }
+    if (st.energy_now) {
+      double e = 0.0;
+      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) e += ered[k];
+      add_energy(d.ws, w, E_WPOL, e);
+    }
+  }
+}
+
+// ------------------------------------------------------- integrator --
+// leapfrog (md.f90:4081-4111): v = (v - d*winv*dt)*Tscale; xx = x; x += v*dt
+__global__ void k_leapfrog(Dev d) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.ntot) return;
+  const int w = d.wid[i];
+  const WinState& st = d.ws[w];
+  if (!st.active) return;
+  const WinDev& W = d.win[w];
+  const double ts = (i - W.ao) < W.nat_solute ? st.tscale_solute : st.tscale_solvent;
+  const double wi = (double)d.winv[i];
+  for (int c = 0; c < 3; ++c) {
+    const int k = 3 * i + c;
+    const double g = (double)d.f[k] * (1.0 / kFScale);
+    double v = (d.v[k] - g * wi * W.dt) * ts;
+    d.v[k] = v;
+    d.xx[k] = d.x[k];
+    d.x[k] = d.x[k] + v * W.dt;
+  }
+}
+
+// SHAKE (md.f90:14445-14566): one thread per connected component of a
+// molecule's constraint graph; components share no atoms, so iterating them
+// separately gives the reference's result for the whole molecule
+__global__ void k_shake(Dev d) {
+#pragma clang fp contract(off)
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= d.nsu) return;
+  const int w = d.su_win[u];
+  WinState& st = d.ws[w];
+  if (!st.active) return;
+  const int c0 = d.su_start[u], nc = d.su_n[u];
+  const double tol = (double)0.0001f;
+  unsigned long long ready = 0ull;  // up to 64 constraints per component
+  const unsigned long long all = nc >= 64 ? ~0ull : ((1ull << nc) - 1ull);
+  double* x = d.x;
+  const double* xx = d.xx;
+  int nits = 0;
+  while (true) {
+    for (int c = 0; c < nc; ++c) {
+      if (ready & (1ull << c)) continue;
+      const int i = d.sc_i[c0 + c], j = d.sc_j[c0 + c];
+      double xij[3], xxij[3];
+      for (int k = 0; k < 3; ++k) xij[k] = x[3 * i + k] - x[3 * j + k];
+      const double xij2 = xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2];
+      const double d2 = d.sc_d2[c0 + c];
+      const double diff = d2 - xij2;
+      if (fabs(diff) < tol * d2) ready |= 1ull << c;
+      for (int k = 0; k < 3; ++k) xxij[k] = xx[3 * i + k] - xx[3 * j + k];
+      const double scp = xij[0] * xxij[0] + xij[1] * xxij[1] + xij[2] * xxij[2];
+      const float wsum = d.winv[i] + d.winv[j];
+      const double corr = diff / (2. * scp * (double)wsum);
+      const double wi = (double)d.winv[i], wj = (double)d.winv[j];
+      for (int k = 0; k < 3; ++k) {
+        x[3 * i + k] = x[3 * i + k] + xxij[k] * corr * wi;
+        x[3 * j + k] = x[3 * j + k] - xxij[k] * corr * wj;
+      }
+    }
+    ++nits;
+    if ((ready & all) == all) break;
+    if (nits >= 1000) { st.shake_fail = 1; break; }
+  }
+}
+
+// after SHAKE: v = (x - xx)/dt for every atom (md.f90:4117-4120), refresh the
+// FP32 coordinates and clear the gradient for the next step
+__global__ void k_post(Dev d, int shake_any) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.ntot) return;
+  const int w = d.wid[i];
+  const WinState& st = d.ws[w];
+  if (!st.active && shake_any >= 0) return;
+  if (shake_any < 0) shake_any = 0;
+  const WinDev& W = d.win[w];
+  for (int c = 0; c < 3; ++c) {
+    const int k = 3 * i + c;
+    if (shake_any) d.v[k] = (d.x[k] - d.xx[k]) / W.dt;
+  }
+  float4 q = d.x32[i];
+  d.x32[i] = make_float4((float)d.x[3 * i], (float)d.x[3 * i + 1], (float)d.x[3 * i + 2], q.w);
+}
+
+// temperature and Berendsen scaling factors (md.f90:3813-3914): one block per
+// window, fixed-order reduction; advances the window's step counter
+__global__ void __launch_bounds__(256) k_temperature(Dev d, int advance) {
+  const int w = blockIdx.x;
+  if (w >= d.nwin) return;
+  WinState& st = d.ws[w];
+  const WinDev& W = d.win[w];
+  if (!st.active && advance) return;
+  double acc[4] = {0, 0, 0, 0};  // solute, solute free, solvent, solvent free
+  for (int li = threadIdx.x; li < W.natom; li += blockDim.x) {
+    const int i = W.ao + li;
+    const double* v = &d.v[3 * i];
+    const double ek = 0.5 * d.mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const bool solute = li < W.nat_solute;
+    const bool freeat = !(d.aflag[i] & 1);
+    acc[solute ? 0 : 2] += ek;
+    if (freeat) acc[solute ? 1 : 3] += ek;
+  }
+  __shared__ double red[4][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = 0; k < 4; ++k) {
+    double s = wave_sum(acc[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k)
+      for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t[k] += red[q][k];
+    const double boltz = (double)0.001986f;
+    const double tfree_e = t[1] + t[3];
+    const double temp_e = t[0] + t[2];
+    st.ekin = temp_e;
+    st.temp = 2.0 * temp_e / boltz / (double)(float)W.ndegf;
+    st.tfree = 2.0 * tfree_e / boltz / (double)(float)W.ndegfree;
+    st.tfree_solute = W.ndegfree_solute ? 2.0 * t[1] / boltz / (double)(float)W.ndegfree_solute : 0.0;
+    st.tfree_solvent = W.ndegfree_solvent ? 2.0 * t[3] / boltz / (double)(float)W.ndegfree_solvent : 0.0;
+    if (W.separate_scaling) {
+      if (st.tfree_solvent != 0) st.tscale_solvent = W.temp0 / st.tfree_solvent - 1.0;
+      st.tscale_solvent = sqrt(1 + W.dt / W.tau_t * st.tscale_solvent);
+      if (st.tfree_solute != 0) st.tscale_solute = W.temp0 / st.tfree_solute - 1.0;
+      st.tscale_solute = sqrt(1 + W.dt / W.tau_t * st.tscale_solute);
+    } else {
+      if (st.tfree != 0) st.tscale_solvent = W.temp0 / st.tfree - 1.0;
+      st.tscale_solvent = sqrt(1 + W.dt / W.tau_t * st.tscale_solvent);
+      st.tscale_solute = st.tscale_solvent;
+    }
+    if (advance) {
+      const int slot = st.istep % d.ring_n;
+      double* tr = &d.t_ring[((size_t)slot * d.nwin + w) * 4];
+      tr[0] = st.temp; tr[1] = st.tfree; tr[2] = st.ekin; tr[3] = (double)st.istep;
+      if (st.energy_now) {
+        long long* er = &d.e_ring[((size_t)slot * d.nwin + w) * E_NSLOT];
+        for (int k = 0; k < E_NSLOT; ++k) er[k] = st.E[k];
+      }
+      st.istep += 1;
+    }
+  }
+}
+
+// ------------------------------------------------------ launchers --
+static inline int blocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+void launch_begin(const Dev& d, hipStream_t s, int force_energy, int eval_only, int rebuild) {
+  k_begin<<<blocks(d.nwin, 64), 64, 0, s>>>(d, force_energy, eval_only, rebuild);
+}
+
+void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
+  k_centres<<<blocks(ngrp, 128), 128, 0, s>>>(d, ngrp);
+  k_bounds<<<blocks(nwc + nsc, 128), 128, 0, s>>>(d, nwc, nsc);
+  if (d.tl[T_WW].nicl) k_build_tiles<T_WW><<<blocks((long long)d.tl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_WP].nicl) k_build_tiles<T_WP><<<blocks((long long)d.tl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_PW].nicl) k_build_tiles<T_PW><<<blocks((long long)d.tl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_PP].nicl) k_build_tiles<T_PP><<<blocks((long long)d.tl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
+  if (ngrp) k_lrf_build<<<ngrp, 256, 0, s>>>(d, ngrp);
+}
+
+void launch_forces(const Dev& d, hipStream_t s) {
+  if (d.tl[T_WW].nicl) k_ww<<<blocks((long long)d.tl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_WP].nicl) k_pw<true><<<blocks((long long)d.tl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_PW].nicl) k_pw<false><<<blocks((long long)d.tl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_PP].nicl) k_pp<<<blocks((long long)d.tl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_QW].nicl) k_qw<true><<<blocks((long long)d.tl[T_QW].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_WQ].nicl) k_qw<false><<<blocks((long long)d.tl[T_WQ].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_QP].nicl) k_qp<true><<<blocks((long long)d.tl[T_QP].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_PQ].nicl) k_qp<false><<<blocks((long long)d.tl[T_PQ].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
+  k_lrf_taylor<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+  int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
+  if (nb) k_bonded<<<blocks(nb, 128), 128, 0, s>>>(d);
+  k_restr<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+  k_watpol<<<d.nwin, 256, 0, s>>>(d);
+}
+
+void launch_integrate(const Dev& d, hipStream_t s, int shake_any) {
+  k_leapfrog<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+  if (d.nsu) k_shake<<<blocks(d.nsu, 64), 64, 0, s>>>(d);
+  k_post<<<blocks(d.ntot, 256), 256, 0, s>>>(d, shake_any);
+  k_temperature<<<d.nwin, 256, 0, s>>>(d, 1);
+}
+
+void launch_temperature(const Dev& d, hipStream_t s) { k_temperature<<<d.nwin, 256, 0, s>>>(d, 0); }
+
+void launch_refresh32(const Dev& d, hipStream_t s) { k_post<<<blocks(d.ntot, 256), 256, 0, s>>>(d, -1); }
+
+// gradient accumulators of active windows start each evaluation at zero
+__global__ void k_zero(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.ntot) return;
+  if (!d.ws[d.wid[i]].active) return;
+  d.f[3 * i] = 0; d.f[3 * i + 1] = 0; d.f[3 * i + 2] = 0;
+}
+
+void launch_zero(const Dev& d, hipStream_t s) { k_zero<<<blocks(d.ntot, 256), 256, 0, s>>>(d); }
+
+}  // namespace qmi

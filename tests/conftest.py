@@ -1,0 +1,29 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
+
+
+@pytest.fixture(scope="session")
+def systems(tmp_path_factory):
+    """Seeded synthetic windows (topology/fep/inp) shared by the tests."""
+    from q_amd import synth
+    base = tmp_path_factory.mktemp("systems")
+    cache = {}
+
+    def get(config, seed=0, steps=20, **kw):
+        key = (config, seed, steps, tuple(sorted(kw.items())))
+        if key not in cache:
+            d = base / f"{config}_{seed}_{steps}_{len(cache)}"
+            cache[key] = synth.make_window(config, str(d), seed=seed, steps=steps, **kw)
+        return cache[key]
+
+    return get
