@@ -106,10 +106,15 @@ int qmi_window_info(qmi_engine* e, int w, int32_t* info, int cap);
 /* Write window w's restart file (write_xfin, md.f90:15452-15470). */
 int qmi_write_restart(qmi_engine* e, int w, const char* path);
 
+/* Turn per-step HIP-event timing of the kernel families on or off for the
+ * following qmi_run calls (no reference counterpart; for measurement). */
+int qmi_set_timing(qmi_engine* e, int on);
+
 /* Device time per kernel family of the last qmi_run, in microseconds per
- * step, for the bench roofline (no reference counterpart).  Order:
- * nonbonded tiles, LRF build, list build, bonded+restraints, integrate+SHAKE,
- * total.  Returns the number of values written. */
+ * step (no reference counterpart).  Order: pair-list + LRF build,
+ * nonbonded tiles, LRF evaluation + bonded + restraints, leapfrog + SHAKE +
+ * thermostat, (unused), wall-clock total.  Families are zero unless
+ * qmi_set_timing was on.  Returns the number of values written. */
 int qmi_timings(qmi_engine* e, double* out, int cap);
 
 /* Last error message of the calling thread. */

@@ -74,6 +74,8 @@ struct qmi_engine {
   // host-side tables for pair-list export
   std::vector<int> h_wc_o, h_sc_a, h_qc_q, h_atom_win;
   std::vector<double> timings;
+  bool timing_on = false;
+  hipEvent_t ev[8] = {};
   ~qmi_engine();
 };
 
@@ -99,6 +101,7 @@ static int dev_alloc(qmi_engine* e, T** dst, size_t n) {
 }
 
 qmi_engine::~qmi_engine() {
+  for (auto& x : ev) if (x) (void)hipEventDestroy(x);
   for (void* p : bufs) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
   for (auto& h : wins) {
@@ -940,6 +943,10 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     ws[wi].stop_at = (int)std::min<long long>(e->wins[wi].w->inp.nsteps, (long long)ws[wi].istep + nsteps);
     HIP_OK(hipMemcpy(&e->d.ws[wi].stop_at, &ws[wi].stop_at, sizeof(int), hipMemcpyHostToDevice));
   }
+  if (e->timing_on && !e->ev[0])
+    for (auto& x : e->ev) HIP_OK(hipEventCreate(&x));
+  // per family: 0 lists+LRF build, 1 nonbonded tiles, 2 LRF/bonded/restraints, 3 integrate
+  double fam[4] = {0, 0, 0, 0};
   auto t0 = std::chrono::steady_clock::now();
   long long done = 0;
   const int R = e->d.ring_n;
@@ -948,9 +955,23 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     for (int k = 0; k < chunk; ++k) {
       launch_begin(e->d, e->stream, 0, 0, 0);
       launch_zero(e->d, e->stream);
+      if (e->timing_on) HIP_OK(hipEventRecord(e->ev[0], e->stream));
       launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
-      launch_forces(e->d, e->stream);
+      if (e->timing_on) HIP_OK(hipEventRecord(e->ev[1], e->stream));
+      launch_nonbonded(e->d, e->stream);
+      if (e->timing_on) HIP_OK(hipEventRecord(e->ev[2], e->stream));
+      launch_other_forces(e->d, e->stream);
+      if (e->timing_on) HIP_OK(hipEventRecord(e->ev[3], e->stream));
       launch_integrate(e->d, e->stream, e->shake_any);
+      if (e->timing_on) {
+        HIP_OK(hipEventRecord(e->ev[4], e->stream));
+        HIP_OK(hipEventSynchronize(e->ev[4]));
+        for (int f = 0; f < 4; ++f) {
+          float ms = 0.f;
+          HIP_OK(hipEventElapsedTime(&ms, e->ev[f], e->ev[f + 1]));
+          fam[f] += ms * 1000.0;
+        }
+      }
     }
     HIP_OK(hipGetLastError());
     int rc = drain(e, first + (int)done, chunk);
@@ -961,6 +982,7 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   auto t1 = std::chrono::steady_clock::now();
   double us = std::chrono::duration<double, std::micro>(t1 - t0).count();
   e->timings.assign(6, 0.0);
+  for (int f = 0; f < 4; ++f) e->timings[f] = nsteps > 0 ? fam[f] / (double)nsteps : 0.0;
   e->timings[5] = nsteps > 0 ? us / (double)nsteps : 0.0;
   HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
   for (int wi = 0; wi < NW; ++wi) {
@@ -992,7 +1014,8 @@ int qmi_evaluate(qmi_engine* e, int rebuild_lists) {
   launch_begin(e->d, e->stream, 1, 1, rebuild_lists ? 1 : 0);
   launch_zero(e->d, e->stream);
   launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
-  launch_forces(e->d, e->stream);
+  launch_nonbonded(e->d, e->stream);
+  launch_other_forces(e->d, e->stream);
   HIP_OK(hipGetLastError());
   const int NW = (int)e->wins.size();
   std::vector<WinState> ws(NW);
@@ -1168,6 +1191,12 @@ int qmi_write_restart(qmi_engine* e, int w, const char* path) {
   }
   Error err{0, ""};
   if (!write_restart(path, *h->w, &err)) return set_err(err.code, err.msg);
+  return 0;
+}
+
+int qmi_set_timing(qmi_engine* e, int on) {
+  if (!e) return set_err(-1, "null engine");
+  e->timing_on = on != 0;
   return 0;
 }
 

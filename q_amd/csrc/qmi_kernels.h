@@ -8,7 +8,8 @@ namespace qmi {
 void launch_begin(const Dev& d, hipStream_t s, int force_energy, int eval_only, int rebuild);
 void launch_zero(const Dev& d, hipStream_t s);
 void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc);
-void launch_forces(const Dev& d, hipStream_t s);
+void launch_nonbonded(const Dev& d, hipStream_t s);
+void launch_other_forces(const Dev& d, hipStream_t s);
 void launch_integrate(const Dev& d, hipStream_t s, int shake_any);
 void launch_temperature(const Dev& d, hipStream_t s);
 void launch_refresh32(const Dev& d, hipStream_t s);

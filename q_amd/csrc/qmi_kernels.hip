@@ -70,14 +70,26 @@ __device__ __forceinline__ double r2_exact(const double* x, int i, int j) {
 // double-float add/mul/div/sqrt built on them (Bailey's double-double scheme
 // carried out in FP32): about 44 significant bits.
 struct df { float h, l; };
-__device__ __forceinline__ df df_q2s(float a, float b) { float s = a + b; return {s, b - (s - a)}; }
+__device__ __forceinline__ df df_q2s(float a, float b) {
+#pragma clang fp contract(off)
+ float s = a + b; return {s, b - (s - a)}; }
 __device__ __forceinline__ df df_2s(float a, float b) {
+#pragma clang fp contract(off)
+
   float s = a + b; float bb = s - a; return {s, (a - (s - bb)) + (b - bb)};
 }
-__device__ __forceinline__ df df_2p(float a, float b) { float p = a * b; return {p, fmaf(a, b, -p)}; }
-__device__ __forceinline__ df df_from(double x) { float h = (float)x; return {h, (float)(x - (double)h)}; }
-__device__ __forceinline__ double df_to(df a) { return (double)a.h + (double)a.l; }
+__device__ __forceinline__ df df_2p(float a, float b) {
+#pragma clang fp contract(off)
+ float p = a * b; return {p, fmaf(a, b, -p)}; }
+__device__ __forceinline__ df df_from(double x) {
+#pragma clang fp contract(off)
+ float h = (float)x; return {h, (float)(x - (double)h)}; }
+__device__ __forceinline__ double df_to(df a) {
+#pragma clang fp contract(off)
+ return (double)a.h + (double)a.l; }
 __device__ __forceinline__ df df_add(df a, df b) {
+#pragma clang fp contract(off)
+
   df s = df_2s(a.h, b.h);
   df t = df_2s(a.l, b.l);
   s.l += t.h;
@@ -85,18 +97,26 @@ __device__ __forceinline__ df df_add(df a, df b) {
   s.l += t.l;
   return df_q2s(s.h, s.l);
 }
-__device__ __forceinline__ df df_neg(df a) { return {-a.h, -a.l}; }
+__device__ __forceinline__ df df_neg(df a) {
+#pragma clang fp contract(off)
+ return {-a.h, -a.l}; }
 __device__ __forceinline__ df df_mul(df a, df b) {
+#pragma clang fp contract(off)
+
   df p = df_2p(a.h, b.h);
   p.l += a.h * b.l + a.l * b.h;
   return df_q2s(p.h, p.l);
 }
 __device__ __forceinline__ df df_mulf(df a, float b) {
+#pragma clang fp contract(off)
+
   df p = df_2p(a.h, b);
   p.l += a.l * b;
   return df_q2s(p.h, p.l);
 }
 __device__ __forceinline__ df df_div(df a, df b) {
+#pragma clang fp contract(off)
+
   float q1 = a.h / b.h;
   df r = df_add(a, df_neg(df_mulf(b, q1)));
   float q2 = r.h / b.h;
@@ -106,13 +126,17 @@ __device__ __forceinline__ df df_div(df a, df b) {
   return df_add(q, df{q3, 0.f});
 }
 __device__ __forceinline__ df df_sqrt(df a) {
+#pragma clang fp contract(off)
+
   if (a.h <= 0.f) return {0.f, 0.f};
   float x = sqrtf(a.h);
   df r = df_add(a, df_neg(df_2p(x, x)));
   return df_q2s(x, r.h * (0.5f / x));
 }
 // difference of two FP64 coordinates as a df
-__device__ __forceinline__ df df_dx(double xj, double xi) { return df_from(xj - xi); }
+__device__ __forceinline__ df df_dx(double xj, double xi) {
+#pragma clang fp contract(off)
+ return df_from(xj - xi); }
 
 // ------------------------------------------------------ step control --
 __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
@@ -1463,7 +1487,7 @@ void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
   if (ngrp) k_lrf_build<<<ngrp, 256, 0, s>>>(d, ngrp);
 }
 
-void launch_forces(const Dev& d, hipStream_t s) {
+void launch_nonbonded(const Dev& d, hipStream_t s) {
   if (d.tl[T_WW].nicl) k_ww<<<blocks((long long)d.tl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_WP].nicl) k_pw<true><<<blocks((long long)d.tl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PW].nicl) k_pw<false><<<blocks((long long)d.tl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
@@ -1473,6 +1497,9 @@ void launch_forces(const Dev& d, hipStream_t s) {
   if (d.tl[T_QP].nicl) k_qp<true><<<blocks((long long)d.tl[T_QP].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PQ].nicl) k_qp<false><<<blocks((long long)d.tl[T_PQ].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
+}
+
+void launch_other_forces(const Dev& d, hipStream_t s) {
   k_lrf_taylor<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
   if (nb) k_bonded<<<blocks(nb, 128), 128, 0, s>>>(d);

@@ -70,6 +70,7 @@ def load_library(path: str = LIB_PATH):
         "qmi_window_info": (i32, [p, i32, C.POINTER(C.c_int32), i32]),
         "qmi_write_restart": (i32, [p, i32, C.c_char_p]),
         "qmi_timings": (i32, [p, dp, i32]),
+        "qmi_set_timing": (i32, [p, i32]),
         "qmi_last_error": (C.c_char_p, []),
         "qmi_engine_destroy": (None, [p]),
     }
@@ -85,7 +86,7 @@ EXPORTED = [
     "qmi_abi_version", "qmi_engine_create", "qmi_window_add", "qmi_prepare", "qmi_run", "qmi_evaluate",
     "qmi_get_energies", "qmi_get_q_energies", "qmi_get_coords", "qmi_get_velocities", "qmi_get_forces",
     "qmi_set_state", "qmi_get_pairlist", "qmi_window_info", "qmi_write_restart", "qmi_timings",
-    "qmi_last_error", "qmi_engine_destroy",
+    "qmi_set_timing", "qmi_last_error", "qmi_engine_destroy",
 ]
 
 
@@ -185,6 +186,9 @@ class Engine:
 
     def write_restart(self, w: int, path: str):
         _check(self.lib.qmi_write_restart(self._h, w, os.fspath(path).encode()))
+
+    def set_timing(self, on: bool):
+        _check(self.lib.qmi_set_timing(self._h, 1 if on else 0))
 
     def timings(self) -> list:
         a = np.zeros(8)

@@ -88,6 +88,10 @@ class System:
     qangles: list = field(default_factory=list)    # (i, j, k, code1, code2)
     qtorsions: list = field(default_factory=list)  # (i, j, k, l, code1, code2)
     qimpropers: list = field(default_factory=list)
+    bond_lib: list = field(default_factory=lambda: list(BOND_LIB))
+    angle_lib: list = field(default_factory=lambda: list(ANGLE_LIB))
+    qbond_lib: list = field(default_factory=list)    # (fk, r0) per q-bond type
+    qangle_lib: list = field(default_factory=list)   # (fk, theta0 deg) per q-angle type
 
     @property
     def natom(self) -> int:
@@ -357,6 +361,26 @@ def _add_water_lattice(rng, sysm: System, radius: float, grid: _Grid | None, spa
         sysm.molecules.append(base)
 
 
+def _strain_free(s: System) -> None:
+    """Give every solute bond and angle its own equilibrium parameter taken
+    from the built geometry, so the bonded terms start unstrained."""
+    x = np.asarray(s.x, float)
+    nb = []
+    for (i, j, c) in s.bonds[: s.nbonds_solute]:
+        r = float(np.linalg.norm(x[j - 1] - x[i - 1]))
+        s.bond_lib.append((BOND_LIB[c - 1][0], round(r, 5)))
+        nb.append((i, j, len(s.bond_lib)))
+    s.bonds = nb + s.bonds[s.nbonds_solute:]
+    na = []
+    for (i, j, k, c) in s.angles[: s.nangles_solute]:
+        u = x[i - 1] - x[j - 1]
+        w = x[k - 1] - x[j - 1]
+        th = math.degrees(math.acos(max(-1.0, min(1.0, float(np.dot(u, w) / np.linalg.norm(u) / np.linalg.norm(w))))))
+        s.angle_lib.append((ANGLE_LIB[c - 1][0], round(th, 3)))
+        na.append((i, j, k, len(s.angle_lib)))
+    s.angles = na + s.angles[s.nangles_solute:]
+
+
 def _neutral_charges(rng, n: int, sd: float = 0.3) -> list:
     q = rng.normal(0.0, sd, size=n)
     q = np.round(q - q.mean(), 4)
@@ -369,13 +393,19 @@ def _ligand_fep(rng, sysm: System, first: int, n_lig: int, types: list, n_pert: 
     """All ligand atoms become q-atoms; ``n_pert`` of them change charge and
     LJ type between state 1 and state 2 and get soft-core alpha."""
     pert = sorted(int(v) for v in rng.choice(n_lig, size=n_pert, replace=False))
+    # at least one perturbed heavy atom carries the neutrality correction
+    if not any(heavy[k] for k in pert):
+        pert[-1] = int(rng.choice([k for k in range(n_lig) if heavy[k] and k not in pert]))
+        pert = sorted(pert)
     q1 = [sysm.crg[first - 1 + k] for k in range(n_lig)]
     q2 = list(q1)
     for k in pert:
-        q2[k] = round(q1[k] + float(rng.normal(0.0, 0.3)), 4)
-    # keep state 2 neutral: spread the residual over the perturbed atoms
+        # hydrogens vanish (dummy type, no LJ) in state 2: their charge goes to 0
+        q2[k] = round(q1[k] + float(rng.normal(0.0, 0.3)), 4) if heavy[k] else 0.0
+    # keep state 2 neutral: put the residual on a perturbed heavy atom
     resid = round(sum(q2) - sum(q1), 4)
-    q2[pert[-1]] = round(q2[pert[-1]] - resid, 4)
+    kh = [k for k in pert if heavy[k]][-1]
+    q2[kh] = round(q2[kh] - resid, 4)
     names = {t: LJ_TYPES[t - 1][0] for t in range(1, len(LJ_TYPES) + 1)}
     sysm.qatoms = [first + k for k in range(n_lig)]
     sysm.qcharges = [(q1[k], q2[k]) for k in range(n_lig)]
@@ -390,13 +420,20 @@ def _ligand_fep(rng, sysm: System, first: int, n_lig: int, types: list, n_pert: 
         else:
             sysm.qtypes.append((names[t1], names[t1]))
             sysm.qalpha.append((0.0, 0.0))
-    # redefine the bonded terms touching perturbed atoms as state-dependent q-terms
+    # redefine the bonded terms touching perturbed atoms as state-dependent
+    # q-terms: state 1 keeps the topology parameters, state 2 shifts them
     pset = {first + k for k in pert}
     for (i, j, c) in [b for b in sysm.bonds if b[0] in pset or b[1] in pset]:
         if i in sysm.qatoms and j in sysm.qatoms:
-            sysm.qbonds.append((i, j, c, 1 if c == B_HEAVY else 2))
+            fk, r0 = sysm.bond_lib[c - 1]
+            sysm.qbond_lib += [(fk, r0), (fk, round(r0 + 0.03, 5))]
+            n = len(sysm.qbond_lib)
+            sysm.qbonds.append((i, j, n - 1, n))
     for (i, j, k, c) in [a for a in sysm.angles if a[1] in pset]:
-        sysm.qangles.append((i, j, k, c, 1 if c == A_HEAVY else 2))
+        fk, t0 = sysm.angle_lib[c - 1]
+        sysm.qangle_lib += [(fk, t0), (fk, round(t0 + 2.0, 3))]
+        n = len(sysm.qangle_lib)
+        sysm.qangles.append((i, j, k, n - 1, n))
     seen = set()
     for (i, j, k, l, c) in sysm.torsions:
         if (j in pset or k in pset) and (i, j, k, l) not in seen and c == 1:
@@ -442,6 +479,7 @@ def ligand_in_water(radius: float = 20.0, seed: int = 0, n_heavy: int = 30, n_h:
     s.nbonds_solute = len(s.bonds)
     s.nangles_solute = len(s.angles)
     s.ncgp_solute = len(s.cgps)
+    _strain_free(s)
     _ligand_fep(rng, s, 1, n, types, n_pert, alpha, nb, heavy)
     _add_water_lattice(rng, s, radius, grid)
     return s
@@ -534,7 +572,7 @@ def protein_leg(radius: float = 25.0, seed: int = 2, n_units: int = 160, n_lig: 
                 pc = chain_xyz[prev_c]
                 a0 = chain_xyz[chain_nb[prev_c][0]]
                 b0 = chain_xyz[chain_nb[chain_nb[prev_c][0]][0]] if chain_nb[chain_nb[prev_c][0]] else a0 + 1
-                n0 = _place(b0, a0, pc, 1.34, 116.0, prev_tor_o + 180.0)
+                n0 = _place(b0, a0, pc, 1.46, 116.0, prev_tor_o + 180.0)
             hp.append(n0)
             ref_a = chain_xyz[prev_c] if prev_c is not None else n0 + rng.normal(size=3)
             ref_b = (chain_xyz[chain_nb[prev_c][0]] if prev_c is not None else ref_a + rng.normal(size=3))
@@ -542,7 +580,7 @@ def protein_leg(radius: float = 25.0, seed: int = 2, n_units: int = 160, n_lig: 
             c2 = _place(ref_a, n0, c1, 1.53, 111.0, rng.choice([60.0, 180.0, 300.0]) + rng.normal(0, 20))
             c3 = _place(n0, c1, c2, 1.52, 112.0, rng.choice([60.0, 180.0, 300.0]) + rng.normal(0, 20))
             tor_o = rng.uniform(0, 360)
-            o4 = _place(c1, c2, c3, 1.23, 121.0, tor_o)
+            o4 = _place(c1, c2, c3, 1.46, 121.0, tor_o)
             c5 = _place(c2, n0, c1, 1.53, 110.0, 120.0 + rng.normal(0, 10))
             c6 = _place(n0, c1, c5, 1.53, 112.0, 180.0 + rng.normal(0, 20))
             hp += [c1, c2, c3, o4, c5, c6]
@@ -564,7 +602,7 @@ def protein_leg(radius: float = 25.0, seed: int = 2, n_units: int = 160, n_lig: 
                 nbp = [hp[v] for v in _UNIT_NB[par]]
                 if par == 0 and prev_c is not None:
                     nbp.append(chain_xyz[prev_c])
-                bl = 1.01 if ht == T_HN else 1.09
+                bl = 1.09
                 for d in _tetra_dirs(hp[par], nbp, nh, rng):
                     hlist.append((par, ht, hp[par] + bl * d))
             hxyz = [h[2] for h in hlist]
@@ -641,6 +679,7 @@ def protein_leg(radius: float = 25.0, seed: int = 2, n_units: int = 160, n_lig: 
     s.nbonds_solute = len(s.bonds)
     s.nangles_solute = len(s.angles)
     s.ncgp_solute = len(s.cgps)
+    _strain_free(s)
     _ligand_fep(rng, s, 1, lig_n, ltypes, n_pert, alpha, lnb, lheavy)
     _add_water_lattice(rng, s, radius, grid, heavy_grid=hgrid)
     return s
@@ -711,27 +750,27 @@ def write_top(s: System, path: str) -> None:
         w("".join(f"{v:5d}" for v in s.iac[k:k + 16]))
     w(f"{len(s.bonds):8d}{s.nbonds_solute:8d} = No. of bonds, no. of solute bonds. i - j - icode: (5 per line)")
     for k in range(0, len(s.bonds), 5):
-        w("".join(f"{b[0]:6d}{b[1]:6d}{b[2]:4d}" for b in s.bonds[k:k + 5]))
-    w(f"{len(BOND_LIB):8d} = No. of bond codes. Parameters:")
-    for c, (fk, r0) in enumerate(BOND_LIB, 1):
-        w(f"{c:5d}{fk:10.3f}{r0:10.5f}")
+        w("".join(f" {b[0]:6d} {b[1]:6d} {b[2]:5d}" for b in s.bonds[k:k + 5]))
+    w(f"{len(s.bond_lib):8d} = No. of bond codes. Parameters:")
+    for c, (fk, r0) in enumerate(s.bond_lib, 1):
+        w(f"{c:6d} {fk:10.3f} {r0:10.5f}")
     w(f"{len(s.angles):8d}{s.nangles_solute:8d} = No. of angles, no. of solute angles. i - j - k - icode: (3 per line)")
     for k in range(0, len(s.angles), 3):
-        w("".join(f"{a[0]:6d}{a[1]:6d}{a[2]:6d}{a[3]:4d}" for a in s.angles[k:k + 3]))
-    w(f"{len(ANGLE_LIB):8d} = No. of angle codes. Parameters:")
-    for c, (fk, t0) in enumerate(ANGLE_LIB, 1):
-        w(f"{c:5d}{fk:10.3f}{t0:10.3f}   0.000   0.00000")
+        w("".join(f" {a[0]:6d} {a[1]:6d} {a[2]:6d} {a[3]:5d}" for a in s.angles[k:k + 3]))
+    w(f"{len(s.angle_lib):8d} = No. of angle codes. Parameters:")
+    for c, (fk, t0) in enumerate(s.angle_lib, 1):
+        w(f"{c:6d} {fk:10.3f} {t0:10.3f}   0.000   0.00000")
     ntor = len(s.torsions)
     w(f"{ntor:8d}{ntor:8d} = No. of torsions, solute torsions. i - j - k - l - icode: (2 per line)")
     for k in range(0, ntor, 2):
-        w("".join(f"{t[0]:6d}{t[1]:6d}{t[2]:6d}{t[3]:6d}{t[4]:4d}" for t in s.torsions[k:k + 2]))
+        w("".join(f" {t[0]:6d} {t[1]:6d} {t[2]:6d} {t[3]:6d} {t[4]:5d}" for t in s.torsions[k:k + 2]))
     w(f"{len(TORSION_LIB):8d} = No. of torsion codes. Parameters:")
     for c, (fk, m, dl, p) in enumerate(TORSION_LIB, 1):
         w(f"{c:5d}{fk:10.4f}{m:8.3f}{dl:10.3f}{p:8.3f}")
     nimp = len(s.impropers)
     w(f"{nimp:8d}{nimp:8d} = No. of impropers, solute impr. i - j - k - l - icode: (2 per line)")
     for k in range(0, nimp, 2):
-        w("".join(f"{t[0]:6d}{t[1]:6d}{t[2]:6d}{t[3]:6d}{t[4]:4d}" for t in s.impropers[k:k + 2]))
+        w("".join(f" {t[0]:6d} {t[1]:6d} {t[2]:6d} {t[3]:6d} {t[4]:5d}" for t in s.impropers[k:k + 2]))
     w(f"{len(IMPROPER_LIB):8d}       2 = No. of improper codes, type (1=harmonic,2=periodic). Parameters:")
     for c, (fk, i0) in enumerate(IMPROPER_LIB, 1):
         w(f"{c:5d}{fk:10.4f}{i0:10.3f}")
@@ -823,18 +862,18 @@ def write_fep(s: System, path: str) -> None:
         out.append(f"{k:5d}{a1:8.2f}{a2:8.2f}")
     if s.qbonds:
         out += ["", "[bond_types]"]
-        for c, (fk, r0) in enumerate(BOND_LIB[:2], 1):
-            out.append(f"{c:4d}{fk:10.2f}{r0 + 0.02 * c:10.4f}")
+        for c, (fk, r0) in enumerate(s.qbond_lib, 1):
+            out.append(f"{c:6d} {fk:10.2f} {r0:10.5f}")
         out += ["", "[change_bonds]"]
         for (i, j, c1, c2) in s.qbonds:
-            out.append(f"{i:6d}{j:6d}{c1:4d}{c2:4d}")
+            out.append(f"{i:6d} {j:6d} {c1:5d} {c2:5d}")
     if s.qangles:
         out += ["", "[angle_types]"]
-        for c, (fk, t0) in enumerate(ANGLE_LIB[:2], 1):
-            out.append(f"{c:4d}{fk:10.2f}{t0 + 2.0 * c:10.3f}")
+        for c, (fk, t0) in enumerate(s.qangle_lib, 1):
+            out.append(f"{c:6d} {fk:10.2f} {t0:10.3f}")
         out += ["", "[change_angles]"]
         for (i, j, k, c1, c2) in s.qangles:
-            out.append(f"{i:6d}{j:6d}{k:6d}{c1:4d}{c2:4d}")
+            out.append(f"{i:6d} {j:6d} {k:6d} {c1:5d} {c2:5d}")
     if s.qtorsions:
         out += ["", "[torsion_types]"]
         out.append(f"{1:4d}{0.60:10.3f}{1.0:8.2f}{0.0:10.3f}")

@@ -19,11 +19,17 @@ def systems(tmp_path_factory):
     base = tmp_path_factory.mktemp("systems")
     cache = {}
 
+    systems_cache = {}
+
     def get(config, seed=0, steps=20, **kw):
         key = (config, seed, steps, tuple(sorted(kw.items())))
         if key not in cache:
             d = base / f"{config}_{seed}_{steps}_{len(cache)}"
-            cache[key] = synth.make_window(config, str(d), seed=seed, steps=steps, **kw)
+            if (config, seed) not in systems_cache:
+                builder, bkw, _ = synth.CONFIGS[config]
+                systems_cache[(config, seed)] = builder(seed=seed, **bkw)
+            cache[key] = synth.make_window(config, str(d), seed=seed, steps=steps,
+                                           system=systems_cache[(config, seed)], **kw)
         return cache[key]
 
     return get

@@ -1,0 +1,98 @@
+"""GPU dynamics: 100 MD steps against the reference qdyn run on the same
+inputs (md_run, src/q6/md.f90:3917-4263), the .en file layout
+(nrgy.f90:94-119), and batching invariance (a window gives bitwise the same
+trajectory alone or batched with others)."""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from oracle import qprobe_dump as Q
+
+pytestmark = pytest.mark.gpu
+
+
+def read_en(path, nstates):
+    """Records of a Fortran unformatted .en file: per write, nstates records
+    of (int32 state, 15 f64) and one off-diagonal record."""
+    raw = open(path, "rb").read()
+    out, k = [], 0
+    while k < len(raw):
+        block = []
+        for s in range(nstates):
+            n = np.frombuffer(raw, np.int32, 1, k)[0]
+            assert n == 124
+            st = np.frombuffer(raw, np.int32, 1, k + 4)[0]
+            assert st == s + 1
+            block.append(np.frombuffer(raw, np.float64, 15, k + 8).copy())
+            assert np.frombuffer(raw, np.int32, 1, k + 4 + n)[0] == n
+            k += 8 + n
+        n = np.frombuffer(raw, np.int32, 1, k)[0]
+        assert n == 0  # no off-diagonal elements
+        k += 8
+        out.append(np.array(block))
+    return out
+
+
+@pytest.mark.parametrize("cfg,seed", [("ligwat", 0), ("protein", 2)])
+def test_100_steps(systems, cfg, seed):
+    from q_amd.engine import Engine
+    inp = systems(cfg, seed=seed, steps=100, energy=10, output=10)
+    d = os.path.dirname(inp)
+    tag = os.path.splitext(os.path.basename(inp))[0]
+    print(f"[gpu] reference run {cfg}", flush=True)
+    dump = Q.run_qprobe(inp, out_name="dyn.dump")
+    shutil.move(os.path.join(d, tag + ".en"), os.path.join(d, tag + ".ref.en"))
+    e = Engine(0)
+    w = e.add_window(inp)
+    e.prepare()
+    print(f"[gpu] engine run {cfg}", flush=True)
+    e.run(100, finalize=True)
+    x = e.coords(w)
+    dx = np.abs(x - dump["XN"]).max()
+    print(cfg, "max |x - x_ref| after 100 steps", dx)
+    assert dx < 0.05
+    eq = e.q_energies(w)
+    ref = dump["EQN"]
+    rel = np.abs(eq - ref) / np.maximum(np.abs(ref), 1.0)
+    print(cfg, "final EQ max rel", rel.max(), "per field", rel.max(axis=0))
+    assert rel.max() < 1e-4
+    S = eq.shape[0]
+    en_ref = read_en(os.path.join(d, tag + ".ref.en"), S)
+    en = read_en(os.path.join(d, tag + ".en"), S)
+    assert len(en) == len(en_ref) == 9     # steps 10, 20, ..., 90
+    for k, (a, b) in enumerate(zip(en, en_ref)):
+        r = np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+        print(f"  .en record {k} max rel {r.max():.2e}")
+        assert r.max() < 1e-4
+
+
+def test_batched_equals_single(systems):
+    from q_amd.engine import Engine
+    inps = [systems("ligwat", seed=0, steps=40, lam=0.25), systems("protein", seed=2, steps=40, lam=0.75)]
+    eb = Engine(0)
+    for p in inps:
+        eb.add_window(p)
+    eb.prepare()
+    eb.run(30, finalize=False)
+    xb = [eb.coords(w) for w in range(2)]
+    for w, p in enumerate(inps):
+        es = Engine(0)
+        es.add_window(p)
+        es.prepare()
+        es.run(30, finalize=False)
+        assert np.array_equal(es.coords(0), xb[w]), p
+
+
+def test_deterministic(systems):
+    from q_amd.engine import Engine
+    inp = systems("ligwat", seed=0, steps=40, lam=0.5)
+    xs = []
+    for _ in range(2):
+        e = Engine(0)
+        e.add_window(inp)
+        e.prepare()
+        e.run(25, finalize=False)
+        xs.append(e.coords(0))
+    assert np.array_equal(xs[0], xs[1])
