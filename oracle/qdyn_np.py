@@ -110,7 +110,22 @@ def read_inp(path):
     lam = []
     for l in s.get("LAMBDAS", []):
         lam += [float(v) for v in l.split()]
+    if not lam and p["fep"]:
+        lam = [1.0]          # one state at lambda 1 without [lambdas] (md.f90:3093-3098)
     p["lambda"] = np.array(lam)
+    # restraint sections (md.f90:3119-3259); atoms 0-based
+    p["rst_seq"] = []
+    for l in s.get("SEQUENCE_RESTRAINTS", []):
+        v = (l.split() + ["0", "0", "0", "0", "0"])[:5]
+        p["rst_seq"].append((int(v[0]) - 1, int(v[1]) - 1, float(v[2]), int(v[3]), int(v[4])))
+    p["rst_pos"] = [(int(v[0]) - 1, [float(t) for t in v[1:4]], [float(t) for t in v[4:7]], int(v[7]))
+                    for v in (l.split() for l in s.get("ATOM_RESTRAINTS", []))]
+    p["rst_dis"] = [(int(v[0]) - 1, int(v[1]) - 1, float(v[2]), float(v[3]), float(v[4]), int(v[5]))
+                    for v in (l.split() for l in s.get("DISTANCE_RESTRAINTS", []))]
+    p["rst_ang"] = [(int(v[0]) - 1, int(v[1]) - 1, int(v[2]) - 1, float(v[3]), float(v[4]), int(v[5]))
+                    for v in (l.split() for l in s.get("ANGLE_RESTRAINTS", []))]
+    p["rst_wal"] = [(int(v[0]) - 1, int(v[1]) - 1, float(v[2]), float(v[3]), float(v[4]), float(v[5]), int(v[6]))
+                    for v in (l.split() for l in s.get("WALL_RESTRAINTS", []))]
     return p
 
 
@@ -294,6 +309,9 @@ def read_fep(path, nq_states, nat_solute, natyps):
     fep = s.get("FEP", [])
     if _onoff(fep, "softcore_use_max_potential", False):
         raise ValueError("softcore_use_max_potential is not restated")
+    for sec in ("SOFT_PAIRS", "EL_SCALE", "EXCLUDED_PAIRS", "OFF_DIAGONALS", "COUPLINGS"):
+        if s.get(sec):
+            raise ValueError(f"[{sec.lower()}] is not restated")
 
     def lib(sec, ncol):
         out = {}
@@ -463,7 +481,8 @@ class Window:
                 dr = float(F32(0.5) * F32(k + 1))
                 rin = rout - dr
                 rsh = (0.5 * (rout ** 3 + rin ** 3)) ** float(F32(1.0) / F32(3.0))
-                cstb = F32(self.crgqtot * float(F32(0.98750)) / (RHO_WAT * float(mu) * 4. * PI * rsh ** 2))
+                # rho_wat*mu_w*4. multiplies two real(4) values (md.f90:15110)
+                cstb = F32(self.crgqtot * float(F32(0.98750)) / (float(F32(RHO_WAT) * mu * F32(4.0)) * PI * rsh ** 2))
                 self.shells.append(dict(rout=F32(rout), dr=F32(dr), cstb=cstb, theta_corr=F32(0.0)))
                 rout = rout - dr
         self.crg = (t["crg"].astype(np.float64) * sq).astype(F32)
@@ -705,6 +724,7 @@ class Window:
         self._bonded(d, E, EQ)
         # restraints
         self._restraints(d, E, tfree)
+        self._p_restrain(d, E, EQ)
         # totals (pot_energy, md.f90:13442-13471)
         for s in range(S):
             EQ[s, 6] = EQ[s, 8] + EQ[s, 10] + EQ[s, 12]
@@ -713,7 +733,7 @@ class Window:
         lam = p["lambda"] if S else np.zeros(0)
         for k, nm in ((2, "q.bond"), (3, "q.angle"), (4, "q.torsion"), (5, "q.improper"), (6, "qx.el"), (7, "qx.vdw")):
             E[nm] = float((EQ[:, k] * lam).sum()) if S else 0.0
-        E["restraint.protein"] = float((EQ[:, 14] * lam).sum()) if S else 0.0
+        E["restraint.protein"] = E["p_restrain"] + (float((EQ[:, 14] * lam).sum()) if S else 0.0)
         E["restraint.total"] = (E["restraint.fix"] + E["restraint.shell"] + E["restraint.protein"] +
                                 E["restraint.solvent_radial"] + E["restraint.water_pol"])
         E["potential"] = sum(E[k] for k in ("p.bond", "w.bond", "p.angle", "w.angle", "p.torsion", "p.improper",
@@ -740,7 +760,7 @@ class Window:
             r6 = r2h ** 3
             r2 = 1.0 / r2h
             r = np.sqrt(r2)
-            col = np.where(c == 3, 2, 0)
+            col = c - 1
             dv = np.zeros(len(q))
             for s in range(S):
                 if qv:
@@ -820,19 +840,23 @@ class Window:
                         pairs.append((a, j, 0, 3 if qc[s, j, a] == 4 else 1))
             for (a, j, jq, lj) in pairs:
                 i = iq_at[a]
-                col = 2 if lj == 3 else 0
-                qt = f["qiac"][a, s]
-                aLJ = f["qavdw"][qt, col]
-                bLJ = f["qbvdw"][qt, col]
-                if jq:
-                    qj = float(self.qcrg[jq - 1, s])
-                    qtj = f["qiac"][jq - 1, s]
-                    aLJ *= f["qavdw"][qtj, col]
-                    bLJ *= f["qbvdw"][qtj, col]
+                col = lj - 1
+                qj = float(self.qcrg[jq - 1, s]) if jq else float(self.crg[j])
+                if not qv:
+                    # topology types for both atoms (nonbond_qq, md.f90:10786-10796)
+                    aLJ = A[iac[i], col] * A[iac[j], col]
+                    bLJ = B[iac[i], col] * B[iac[j], col]
                 else:
-                    qj = float(self.crg[j])
-                    aLJ *= A[iac[j], col]
-                    bLJ *= B[iac[j], col]
+                    qt = f["qiac"][a, s]
+                    aLJ = f["qavdw"][qt, col]
+                    bLJ = f["qbvdw"][qt, col]
+                    if jq:
+                        qtj = f["qiac"][jq - 1, s]
+                        aLJ *= f["qavdw"][qtj, col]
+                        bLJ *= f["qbvdw"][qtj, col]
+                    else:
+                        aLJ *= A[iac[j], col]
+                        bLJ *= B[iac[j], col]
                 dx = x[j] - x[i]
                 r2 = (dx ** 2).sum()
                 r6hc = r2 ** 3
@@ -1019,6 +1043,79 @@ class Window:
                              np.cross(rlj, dl) - np.cross(rji, di), np.cross(rjk, dl)], axis=1)
         return phi, dp
 
+    def _p_restrain(self, d, E, EQ):
+        """Sequence, position, distance, angle and wall restraints
+        (p_restrain, md.f90:12940-13274)."""
+        t, x, p = self.t, self.x, self.p
+        mass = t["mass"][t["iac"] - 1]
+        lam = p["lambda"]
+        e_prot = 0.0
+
+        def take(a, ih):
+            return t["heavy"][a] or ih == 1
+
+        for (i0, j0, fk, ih, tc) in p["rst_seq"]:
+            at = [a for a in range(i0, j0 + 1) if take(a, ih)]
+            if tc in (1, 2):
+                if not at:
+                    continue
+                m = mass[at] if tc == 2 else np.ones(len(at))
+                dr = ((x[at] - t["xtop"][at]) * m[:, None]).sum(0) / m.sum()
+                e_prot += 0.5 * fk * (dr @ dr)
+                for a in at:
+                    d[a] += fk * dr * (mass[a] / float(F32(12.010)) if tc == 1 else 1.0)
+            else:
+                for a in at:
+                    dr = x[a] - t["xtop"][a]
+                    e_prot += 0.5 * fk * (dr @ dr)
+                    d[a] += fk * dr
+
+        def state_energy(ipsi, e):
+            nonlocal e_prot
+            if ipsi == 0:
+                EQ[:, 14] += e
+                if self.S == 0:
+                    e_prot += e
+            else:
+                EQ[ipsi - 1, 14] += e
+
+        for (i, x0, fk, ipsi) in p["rst_pos"]:
+            wgt = lam[ipsi - 1] if ipsi else 1.0
+            dr = x[i] - np.array(x0)
+            d[i] += np.array(fk) * dr * wgt
+            state_energy(ipsi, float((0.5 * np.array(fk) * dr * dr).sum()))
+        for (i, j, d1, d2, fk, ipsi) in p["rst_dis"]:
+            wgt = lam[ipsi - 1] if ipsi else 1.0
+            dr = x[j] - x[i]
+            b = math.sqrt(dr @ dr)
+            if d1 <= b <= d2:
+                continue
+            db = b - (d1 if b < d1 else d2)
+            dv = wgt * fk * db / b
+            d[j] += dr * dv
+            d[i] -= dr * dv
+            state_energy(ipsi, 0.5 * fk * db * db)
+        for (i, j, k, ang, fk, ipsi) in p["rst_ang"]:
+            wgt = lam[ipsi - 1] if ipsi else 1.0
+            e = self._angle(d, i, j, k, fk, ang * DEG2RAD, wgt)
+            state_energy(ipsi, e)
+        for (i0, j0, dw, fk, aM, dM, ih) in p["rst_wal"]:
+            for a in range(i0, j0 + 1):
+                if not take(a, ih):
+                    continue
+                dr = x[a] - t["xwcent"]
+                b = math.sqrt(dr @ dr)
+                db = b - dw
+                if db > 0:
+                    e_prot += 0.5 * fk * db * db - dM
+                    dv = fk * db / b
+                else:
+                    fe = math.exp(aM * db)
+                    e_prot += dM * (fe * fe - 2. * fe)
+                    dv = -2. * dM * aM * (fe - fe * fe) / b
+                d[a] += dv * dr
+        E["p_restrain"] = e_prot
+
     def _restraints(self, d, E, tfree):
         t, x, p = self.t, self.x, self.p
         E["restraint.fix"] = E["restraint.shell"] = 0.0
@@ -1066,7 +1163,7 @@ class Window:
             rc = math.sqrt(rc_v @ rc_v)
             scp = min(1.0, max(-1.0, mu @ (rc_v / rc)))
             theta[w] = math.acos(scp)
-            if rc > float(sh[-1]["rout"]) - float(sh[-1]["dr"]):
+            if rc > float(F32(sh[-1]["rout"]) - F32(sh[-1]["dr"])):   # real(4) difference
                 s = nsh - 1
                 while s >= 1 and not rc <= float(sh[s]["rout"]):
                     s -= 1

@@ -17,13 +17,18 @@ enum ESlot {
   E_PP_EL, E_PP_VDW, E_PW_EL, E_PW_VDW, E_WW_EL, E_WW_VDW, E_LRF,
   E_P_BOND, E_P_ANG, E_P_TOR, E_P_IMP, E_W_BOND, E_W_ANG, E_W_TOR, E_W_IMP,
   E_FIX, E_SHELL, E_RAD, E_WPOL,
+  E_PROT,             // state-independent part of restraint%protein (sequence, wall)
   E_NBASE,
-  // per state s: E_Q0 + s*8 + {qbond, qangle, qtors, qimp, qq_el, qq_vdw, qp_el, qp_vdw}
-  // and E_QW0 + s*2 + {qw_el, qw_vdw}
+  // per state s: E_Q0 + s*8 + {qbond, qangle, qtors, qimp, qq_el, qq_vdw, qp_el, qp_vdw},
+  // E_QW0 + s*2 + {qw_el, qw_vdw} and E_QR0 + s (state restraints)
   E_Q0 = E_NBASE,
   E_QW0 = E_Q0 + 8 * kMaxS,
-  E_NSLOT = E_QW0 + 2 * kMaxS
+  E_QR0 = E_QW0 + 2 * kMaxS,
+  E_NSLOT = E_QR0 + kMaxS
 };
+
+// restraint records of p_restrain (md.f90:12940-13274)
+enum RstKind { R_SEQ_ATOMS = 0, R_SEQ_GEOM, R_SEQ_MASS, R_POS, R_DIST, R_ANGLE, R_WALL };
 
 struct WinDev {
   int ao, natom, nat_solute, nwat;
@@ -136,6 +141,10 @@ struct Dev {
   const int4* qa_ijk; const double2* qa_par; const int2* qa_ws;
   const int4* qt_ijkl; const double4* qt_par; const int2* qt_ws; // fk, rmult, deltor
   const int4* qi_ijkl; const double2* qi_par; const int2* qi_ws; // fk, imp0
+  // restraints: (kind, window, i, j) global atoms; (k, flag) with flag = ih
+  // (sequence, wall) or ipsi (position, distance, angle); 8 parameters each
+  int nrst;
+  const int4* rs_kwij; const int2* rs_kf; const double* rs_par;
   // shake units (independent constraint components)
   int nsu;
   const int* su_start; const int* su_n; const int* su_win;

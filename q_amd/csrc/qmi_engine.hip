@@ -161,6 +161,8 @@ static int build_device(qmi_engine* e) {
   std::vector<double4> qb_par, qt_par;
   std::vector<int4> qa_ijk, qt_ijkl, qi_ijkl;
   std::vector<double2> qa_par, qi_par;
+  // restraints
+  std::vector<int4> rs_kwij; std::vector<int2> rs_kf; std::vector<double> rs_par;
   // shake
   std::vector<int> su_start, su_n, su_win, sc_i, sc_j;
   std::vector<double> sc_d2;
@@ -441,6 +443,24 @@ static int build_device(qmi_engine* e) {
       for (int sc = 0; sc < D.nsc; ++sc) { qt[1].l.push_back({}); qt[1].m14.push_back({}); }
       for (int wc = 0; wc < D.nwc; ++wc) { qt[3].l.push_back({}); qt[3].m14.push_back({}); }
     }
+    // --- restraints of the .inp (p_restrain); angles converted to radians
+    {
+      auto rec = [&](int kind, int i, int j, int k, int flag, std::initializer_list<double> par) {
+        rs_kwij.push_back(make_int4(kind, wi, ao + i, ao + j));
+        rs_kf.push_back(make_int2(ao + k, flag));
+        size_t b = rs_par.size();
+        rs_par.resize(b + 8, 0.0);
+        int c = 0;
+        for (double v : par) rs_par[b + c++] = v;
+      };
+      for (auto& r : p.rst_seq)
+        rec(r.to_center == 1 ? R_SEQ_GEOM : r.to_center == 2 ? R_SEQ_MASS : R_SEQ_ATOMS, r.i, r.j, 0, r.ih, {r.fk});
+      for (auto& r : p.rst_pos)
+        rec(R_POS, r.i, r.i, 0, r.ipsi, {r.x[0], r.x[1], r.x[2], r.fk[0], r.fk[1], r.fk[2]});
+      for (auto& r : p.rst_dis) rec(R_DIST, r.i, r.j, 0, r.ipsi, {r.d1, r.d2, r.fk});
+      for (auto& r : p.rst_ang) rec(R_ANGLE, r.i, r.j, r.k, r.ipsi, {r.ang_rad, r.fk});
+      for (auto& r : p.rst_wal) rec(R_WALL, r.i, r.j, 0, r.ih, {r.d, r.fk, r.aMorse, r.dMorse});
+    }
     // --- bonded terms (after shrink_topology)
     for (size_t k = 0; k < t.bnd.size(); ++k) {
       const Bond& b = t.bnd[k];
@@ -630,6 +650,8 @@ static int build_device(qmi_engine* e) {
   UP(i4, qa_ijk); d.qa_ijk = i4; UP(d2, qa_par); d.qa_par = d2; UP(i2, qa_ws); d.qa_ws = i2;
   UP(i4, qt_ijkl); d.qt_ijkl = i4; UP(d4, qt_par); d.qt_par = d4; UP(i2, qt_ws); d.qt_ws = i2;
   UP(i4, qi_ijkl); d.qi_ijkl = i4; UP(d2, qi_par); d.qi_par = d2; UP(i2, qi_ws); d.qi_ws = i2;
+  d.nrst = (int)rs_kwij.size();
+  UP(i4, rs_kwij); d.rs_kwij = i4; UP(i2, rs_kf); d.rs_kf = i2; UP(dp, rs_par); d.rs_par = dp;
   d.nsu = (int)su_n.size();
   UP(ip, su_start); d.su_start = ip; UP(ip, su_n); d.su_n = ip; UP(ip, su_win); d.su_win = ip;
   UP(ip, sc_i); d.sc_i = ip; UP(ip, sc_j); d.sc_j = ip;
@@ -717,7 +739,7 @@ static void assemble_energies(HostWin& h, const long long* Es) {
     q[8] = ev(b + 4); q[9] = ev(b + 5);
     q[10] = ev(b + 6); q[11] = ev(b + 7);
     q[12] = ev(E_QW0 + 2 * s); q[13] = ev(E_QW0 + 2 * s + 1);
-    q[14] = 0.0;
+    q[14] = ev(E_QR0 + s);
     q[6] = q[8] + q[10] + q[12];
     q[7] = q[9] + q[11] + q[13];
     q[1] = q[2] + q[3] + q[4] + q[5] + q[6] + q[7] + q[14];
@@ -734,7 +756,7 @@ static void assemble_energies(HostWin& h, const long long* Es) {
   E[19] = ev(E_WW_EL); E[20] = ev(E_WW_VDW);
   E[21] = qxel; E[22] = qxvdw;
   E[23] = 0.0;
-  E[24] = ev(E_FIX); E[25] = ev(E_SHELL); E[26] = rprot;
+  E[24] = ev(E_FIX); E[25] = ev(E_SHELL); E[26] = ev(E_PROT) + rprot;
   double rad = ev(E_RAD), wpol = ev(E_WPOL);
   E[27] = rad;
   E[28] = wpol;

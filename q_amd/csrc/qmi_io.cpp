@@ -46,7 +46,7 @@ static std::vector<std::string> split_ws(const std::string& s) {
 }
 
 // A Fortran list-directed number: accepts d/D exponents.
-static bool parse_num(const std::string& tok, double* v) {
+bool parse_num(const std::string& tok, double* v) {
   if (tok.empty()) return false;
   std::string t = tok;
   for (auto& c : t) if (c == 'd' || c == 'D') c = 'e';
@@ -302,13 +302,98 @@ bool load_input(const std::string& inp_path, InputParams* p, Error* err) {
   if (p->nstates == 0 && !p->fep_file.empty()) { p->nstates = 1; p->lambda[0] = 1.0; }
   if (p->nstates > 0 && p->fep_file.empty()) return fail(err, -11, "FEP file required to use lambdas");
 
-  p->n_restraints = 0;
-  for (const char* sec : {"SEQUENCE_RESTRAINTS", "ATOM_RESTRAINTS", "DISTANCE_RESTRAINTS", "ANGLE_RESTRAINTS",
-                          "WALL_RESTRAINTS"}) {
-    if (const PrmSection* s = f.find(sec)) p->n_restraints += (int)s->lines.size();
+  // restraint sections (md.f90:3119-3259), read list-directed
+  auto fields = [](const std::string& ln) {
+    std::string t = ln;
+    size_t sl = t.find('/');
+    if (sl != std::string::npos) t = t.substr(0, sl);
+    for (auto& c : t)
+      if (c == ',') c = ' ';
+    return split_ws(t);
+  };
+  auto num = [](const std::vector<std::string>& v, size_t k, double* out) {
+    return k < v.size() && parse_num(v[k], out);
+  };
+  if (const PrmSection* s = f.find("SEQUENCE_RESTRAINTS")) {
+    for (auto& ln : s->lines) {
+      // read(text,*,end=,err=) rstseq(i): fields after the first bad one keep
+      // their preset values (to_center 0)
+      auto v = fields(ln);
+      RstSeq r;
+      double a;
+      if (num(v, 0, &a)) { r.i = (int)a - 1;
+        if (num(v, 1, &a)) { r.j = (int)a - 1;
+          if (num(v, 2, &a)) { r.fk = a;
+            if (num(v, 3, &a)) { r.ih = (int)a;
+              if (num(v, 4, &a)) r.to_center = (int)a; } } } }
+      p->rst_seq.push_back(r);
+    }
   }
-  if (p->n_restraints > 0)
-    return fail(err, -10, "sequence/atom/distance/angle/wall restraints are not supported by this engine");
+  if (const PrmSection* s = f.find("ATOM_RESTRAINTS")) {
+    for (auto& ln : s->lines) {
+      auto v = fields(ln);
+      double a[8];
+      for (int k = 0; k < 8; ++k)
+        if (!num(v, k, &a[k])) return fail(err, -11, "Invalid atom restraint data.");
+      RstPos r;
+      r.i = (int)a[0] - 1;
+      for (int k = 0; k < 3; ++k) { r.x[k] = a[1 + k]; r.fk[k] = a[4 + k]; }
+      r.ipsi = (int)a[7];
+      p->rst_pos.push_back(r);
+    }
+  }
+  if (const PrmSection* s = f.find("DISTANCE_RESTRAINTS")) {
+    for (auto& ln : s->lines) {
+      auto v = fields(ln);
+      RstDis r;
+      double a[4];
+      if (v.size() < 6) return fail(err, -11, "Invalid distance restraint data.");
+      for (int k = 0; k < 4; ++k)
+        if (!num(v, 2 + k, &a[k])) return fail(err, -11, "Invalid distance restraint data.");
+      if (ln.find(':') != std::string::npos) {
+        // residue:atom forms are resolved against the topology (prep_coord)
+        r.itext = v[0];
+        r.jtext = v[1];
+      } else {
+        double ai, aj;
+        if (!num(v, 0, &ai) || !num(v, 1, &aj)) return fail(err, -11, "Invalid distance restraint data.");
+        r.i = (int)ai - 1;
+        r.j = (int)aj - 1;
+      }
+      r.d1 = a[0]; r.d2 = a[1]; r.fk = a[2]; r.ipsi = (int)a[3];
+      p->rst_dis.push_back(r);
+    }
+  }
+  if (const PrmSection* s = f.find("ANGLE_RESTRAINTS")) {
+    for (auto& ln : s->lines) {
+      auto v = fields(ln);
+      double a[6];
+      for (int k = 0; k < 6; ++k)
+        if (!num(v, k, &a[k])) return fail(err, -11, "Invalid angle restraint data.");
+      RstAng r;
+      r.i = (int)a[0] - 1; r.j = (int)a[1] - 1; r.k = (int)a[2] - 1;
+      r.ang = a[3]; r.fk = a[4]; r.ipsi = (int)a[5];
+      p->rst_ang.push_back(r);
+    }
+  }
+  if (const PrmSection* s = f.find("WALL_RESTRAINTS")) {
+    for (auto& ln : s->lines) {
+      auto v = fields(ln);
+      double a[7];
+      for (int k = 0; k < 7; ++k)
+        if (!num(v, k, &a[k])) return fail(err, -11, "Invalid wall restraint data.");
+      RstWal r;
+      r.i = (int)a[0] - 1; r.j = (int)a[1] - 1;
+      r.d = a[2]; r.fk = a[3]; r.aMorse = a[4]; r.dMorse = a[5]; r.ih = (int)a[6];
+      p->rst_wal.push_back(r);
+    }
+  }
+  for (auto& r : p->rst_pos)
+    if (r.ipsi < 0 || r.ipsi > p->nstates) return fail(err, -11, "atom restraint state out of range");
+  for (auto& r : p->rst_dis)
+    if (r.ipsi < 0 || r.ipsi > p->nstates) return fail(err, -11, "distance restraint state out of range");
+  for (auto& r : p->rst_ang)
+    if (r.ipsi < 0 || r.ipsi > p->nstates) return fail(err, -11, "angle restraint state out of range");
   return true;
 }
 
@@ -565,6 +650,8 @@ bool load_topology(const std::string& path, Topology* t, Error* err) {
   nn = lead_nums(line);
   int nres = (int)nn[0];
   if (nres > 0 && !r.values(nres, &tmp)) return bad("residue starts");
+  t->res_start.clear();
+  for (int k = 0; k < nres; ++k) t->res_start.push_back(tmp[k] - 1);
   r.pos++;  // "Sequence:"
   r.pos += (nres + 15) / 16;
   if (!r.values(1, &tmp)) return bad("molecules");

@@ -1196,6 +1196,143 @@ __global__ void k_restr(Dev d) {
   }
 }
 
+// p_restrain (md.f90:12940-13274): one 64-lane block per restraint record.
+// Sequence and wall records walk their atom range across the lanes; the
+// centre forms first reduce the mean displacement over the block.
+// State-dependent records (position, distance, angle) add their energy to
+// every state (ipsi 0) or to state ipsi, whose lambda weights the force.
+__device__ __forceinline__ void state_restraint_energy(Dev& d, int w, const WinDev& W, int ipsi, double e) {
+  if (ipsi == 0) {
+    for (int s = 0; s < W.S; ++s) add_energy(d.ws, w, E_QR0 + s, e);
+    if (W.S == 0) add_energy(d.ws, w, E_PROT, e);
+  } else {
+    add_energy(d.ws, w, E_QR0 + ipsi - 1, e);
+  }
+}
+
+__global__ void __launch_bounds__(64) k_prestr(Dev d) {
+  const int r = blockIdx.x;
+  if (r >= d.nrst) return;
+  const int4 kwij = d.rs_kwij[r];
+  const int kind = kwij.x, w = kwij.y, i0 = kwij.z, j0 = kwij.w;
+  if (!d.ws[w].active) return;
+  const bool en = d.ws[w].energy_now;
+  const WinDev& W = d.win[w];
+  const int2 kf = d.rs_kf[r];
+  const double* P = &d.rs_par[8 * r];
+  const int lane = threadIdx.x;
+  if (kind == R_SEQ_ATOMS || kind == R_WALL) {
+    double e = 0.0;
+    for (int a = i0 + lane; a <= j0; a += 64) {
+      if (!((d.aflag[a] & 4) || kf.y == 1)) continue;   // heavy atoms, or all with the H flag
+      if (kind == R_SEQ_ATOMS) {
+        const double fk = P[0];
+        double dr[3];
+        for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * a + c] - d.xtop[3 * a + c];
+        e += 0.5 * fk * (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+        add_grad(d.f, a, fk * dr[0], fk * dr[1], fk * dr[2]);
+      } else {
+        const double fk = P[1], aM = P[2], dM = P[3];
+        double dr[3];
+        for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * a + c] - W.xwcent[c];
+        const double b = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+        const double db = b - P[0];
+        double dv;
+        if (db > 0.) {
+          e += 0.5 * fk * db * db - dM;
+          dv = fk * db / b;
+        } else {
+          const double fexp = exp(aM * db);
+          e += dM * (fexp * fexp - 2. * fexp);
+          dv = -2. * dM * aM * (fexp - fexp * fexp) / b;
+        }
+        add_grad(d.f, a, dv * dr[0], dv * dr[1], dv * dr[2]);
+      }
+    }
+    if (en) {
+      e = wave_sum(e);
+      if (lane == 0) add_energy(d.ws, w, E_PROT, e);
+    }
+    return;
+  }
+  if (kind == R_SEQ_GEOM || kind == R_SEQ_MASS) {
+    const bool mw = kind == R_SEQ_MASS;
+    double sx = 0, sy = 0, sz = 0, sm = 0;
+    for (int a = i0 + lane; a <= j0; a += 64) {
+      if (!((d.aflag[a] & 4) || kf.y == 1)) continue;
+      const double m = mw ? d.mass[a] : 1.0;
+      sx += (d.x[3 * a] - d.xtop[3 * a]) * m;
+      sy += (d.x[3 * a + 1] - d.xtop[3 * a + 1]) * m;
+      sz += (d.x[3 * a + 2] - d.xtop[3 * a + 2]) * m;
+      sm += m;
+    }
+    sx = wave_sum(sx); sy = wave_sum(sy); sz = wave_sum(sz); sm = wave_sum(sm);
+    if (!(sm > 0.0)) return;
+    const double fk = P[0];
+    const double dr[3] = {sx / sm, sy / sm, sz / sm};
+    if (en && lane == 0) add_energy(d.ws, w, E_PROT, 0.5 * fk * (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]));
+    for (int a = i0 + lane; a <= j0; a += 64) {
+      if (!((d.aflag[a] & 4) || kf.y == 1)) continue;
+      if (mw) {
+        add_grad(d.f, a, fk * dr[0], fk * dr[1], fk * dr[2]);
+      } else {
+        // geometric centre: force scaled by mass / 12.010 (a default-real literal)
+        const double sc = d.mass[a] / (double)12.010f;
+        add_grad(d.f, a, fk * dr[0] * sc, fk * dr[1] * sc, fk * dr[2] * sc);
+      }
+    }
+    return;
+  }
+  if (lane != 0) return;
+  const int ipsi = kf.y;
+  const double wgt = ipsi != 0 ? W.lambda[ipsi - 1] : 1.0;
+  if (kind == R_POS) {
+    double dr[3];
+    for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * i0 + c] - P[c];
+    const double e = 0.5 * P[3] * dr[0] * dr[0] + 0.5 * P[4] * dr[1] * dr[1] + 0.5 * P[5] * dr[2] * dr[2];
+    add_grad(d.f, i0, P[3] * dr[0] * wgt, P[4] * dr[1] * wgt, P[5] * dr[2] * wgt);
+    if (en) state_restraint_energy(d, w, W, ipsi, e);
+  } else if (kind == R_DIST) {
+    double dr[3];
+    for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * j0 + c] - d.x[3 * i0 + c];
+    const double b = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+    double db;
+    if (b < P[0]) db = b - P[0];
+    else if (b > P[1]) db = b - P[1];
+    else return;
+    const double dv = wgt * P[2] * db / b;
+    add_grad(d.f, j0, dr[0] * dv, dr[1] * dv, dr[2] * dv);
+    add_grad(d.f, i0, -dr[0] * dv, -dr[1] * dv, -dr[2] * dv);
+    if (en) state_restraint_energy(d, w, W, ipsi, 0.5 * P[2] * db * db);
+  } else if (kind == R_ANGLE) {
+    const int k0 = kf.x;
+    double a[3], c[3];
+    for (int q = 0; q < 3; ++q) {
+      a[q] = d.x[3 * i0 + q] - d.x[3 * j0 + q];
+      c[q] = d.x[3 * k0 + q] - d.x[3 * j0 + q];
+    }
+    const double r2ij = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    const double r2jk = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    const double rij = sqrt(r2ij), rjk = sqrt(r2jk);
+    double scp = (a[0] * c[0] + a[1] * c[1] + a[2] * c[2]) / (rij * rjk);
+    scp = fmin(1.0, fmax(-1.0, scp));
+    const double theta = acos(scp);
+    const double db = theta - P[0];
+    const double dv = wgt * P[1] * db;
+    double f1 = sin(theta);
+    f1 = fabs(f1) < 1.e-12 ? -1.e12 : -1.0 / f1;
+    double di[3], dk[3];
+    for (int q = 0; q < 3; ++q) {
+      di[q] = f1 * (c[q] / (rij * rjk) - scp * a[q] / r2ij);
+      dk[q] = f1 * (a[q] / (rij * rjk) - scp * c[q] / r2jk);
+    }
+    add_grad(d.f, i0, dv * di[0], dv * di[1], dv * di[2]);
+    add_grad(d.f, k0, dv * dk[0], dv * dk[1], dv * dk[2]);
+    add_grad(d.f, j0, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
+    if (en) state_restraint_energy(d, w, W, ipsi, 0.5 * P[1] * db * db);
+  }
+}
+
 // water polarisation restraint (watpol, md.f90:15122-15288): one block per
 // window; the per-shell selection sort of the reference becomes a rank
 // count (theta ascending, ties by water index, as the first-minimum rule)
@@ -1504,6 +1641,7 @@ void launch_other_forces(const Dev& d, hipStream_t s) {
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
   if (nb) k_bonded<<<blocks(nb, 128), 128, 0, s>>>(d);
   k_restr<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+  if (d.nrst) k_prestr<<<d.nrst, 64, 0, s>>>(d);
   k_watpol<<<d.nwin, 256, 0, s>>>(d);
 }
 

@@ -22,7 +22,6 @@ static bool fail(Error* e, int code, const std::string& m) {
 static const double kPi = (double)(4.0f * std::atan(1.0f));   // md.f90:383
 static const double kDeg2Rad = kPi / (double)180.0f;
 static const double kBoltz = (double)0.001986f;               // md.f90:41
-static const double kRhoWat = (double)0.0335f;                // md.f90:40
 
 double Randm::next(int* ig) {
   // md.f90:14408-14442: multiply irand by 31415821 modulo 1e8 using
@@ -105,7 +104,46 @@ bool prepare_window(Window* w, long long seed_override, Error* err) {
     w->ljcod[(pr.second - 1) * t.natyps + (pr.first - 1)] = 2;
   }
 
-  // --- prep_coord(): restrained shell from topology coordinates
+  // --- prep_coord(): residue:atom restraint atoms (md.f90:13609-13623,
+  // get_atom_from_resnum_atnum 16059-16095; the last residue ends at the
+  // last solute atom), then the restrained shell from topology coordinates
+  auto res_atom = [&](const std::string& txt, int* out) -> bool {
+    size_t c = txt.find(':');
+    if (c == std::string::npos) {
+      double v;
+      if (!parse_num(txt, &v)) return false;
+      *out = (int)v - 1;
+      return true;
+    }
+    double rn, an;
+    if (c < 1 || c + 1 >= txt.size() || !parse_num(txt.substr(0, c), &rn) || !parse_num(txt.substr(c + 1), &an))
+      return false;
+    int r = (int)rn, a = (int)an;
+    if (r < 1 || r > (int)t.res_start.size()) return false;
+    int end = r < (int)t.res_start.size() ? t.res_start[r] : NS;
+    if (a < 1 || a > end - t.res_start[r - 1]) return false;
+    *out = t.res_start[r - 1] + a - 1;
+    return true;
+  };
+  for (auto& r : p.rst_dis) {
+    if (r.itext.empty()) continue;
+    if (!res_atom(r.itext, &r.i) || !res_atom(r.jtext, &r.j))
+      return fail(err, -50, "error in finding atom number from resnum:atnum.");
+  }
+  auto in_range = [&](int a) { return a >= 0 && a < N; };
+  for (auto& r : p.rst_seq)
+    if (!in_range(r.i) || !in_range(r.j) || r.j < r.i) return fail(err, -50, "sequence restraint atoms out of range");
+  for (auto& r : p.rst_pos)
+    if (!in_range(r.i)) return fail(err, -50, "atom restraint atom out of range");
+  for (auto& r : p.rst_dis)
+    if (!in_range(r.i) || !in_range(r.j)) return fail(err, -50, "distance restraint atoms out of range");
+  for (auto& r : p.rst_ang) {
+    if (!in_range(r.i) || !in_range(r.j) || !in_range(r.k)) return fail(err, -50, "angle restraint atoms out of range");
+    r.ang_rad = r.ang * kDeg2Rad;   // (rstang%ang)*deg2rad, md.f90:13194
+  }
+  for (auto& r : p.rst_wal)
+    if (!in_range(r.i) || !in_range(r.j) || r.j < r.i) return fail(err, -50, "wall restraint atoms out of range");
+
   double rexcl_i = p.rexcl_i;
   if (rexcl_i > t.rexcl_o) return fail(err, -50, "inner radius of restrained shell must be < exclusion radius");
   w->shell.assign(N, 0);
@@ -226,8 +264,10 @@ bool prepare_window(Window* w, long long seed_override, Error* err) {
         double ri = rout - dr;
         sh.dr = (float)dr;
         double rshell = std::pow(0.5 * (rout * rout * rout + ri * ri * ri), (double)(1.0f / 3.0f));
-        sh.cstb = (float)(w->crgqtot * (double)0.98750f /
-                          (kRhoWat * (double)w->mu_w * 4. * kPi * rshell * rshell));
+        // rho_wat*mu_w*4. is a default-real product (both operands real(4)),
+        // promoted to real(8) at pi
+        const float rmu4 = (0.0335f * w->mu_w) * 4.0f;     // rho_wat, md.f90:40
+        sh.cstb = (float)(w->crgqtot * (double)0.98750f / ((double)rmu4 * kPi * rshell * rshell));
         rout = rout - dr;
       }
     }

@@ -39,6 +39,7 @@ struct Cgp { int iswitch, first, last; };            // first/last index into cg
 struct AtomType { double mass, avdw[kNljtyp], bvdw[kNljtyp]; };
 
 struct Topology {
+  std::vector<int> res_start;        // first atom (0-based) of each residue
   double version = 0;
   int ff_type = 1;
   int nat_pro = 0, nat_solute = 0;
@@ -117,6 +118,13 @@ struct Fep {
   std::vector<double> sc_lookup;     // [(iq*(natyps+nqat) + t)*S + s], t 0-based column
 };
 
+// restraints of the .inp (md.f90:174-209 rstr*_type); atoms 0-based
+struct RstSeq { int i = 0, j = 0; double fk = 0; int ih = 0, to_center = 0; };
+struct RstPos { int i = 0, ipsi = 0; double fk[3] = {0, 0, 0}, x[3] = {0, 0, 0}; };
+struct RstDis { int i = 0, j = 0, ipsi = 0; double fk = 0, d1 = 0, d2 = 0; std::string itext, jtext; };
+struct RstAng { int i = 0, j = 0, k = 0, ipsi = 0; double fk = 0, ang = 0, ang_rad = 0; };
+struct RstWal { int i = 0, j = 0; double d = 0, fk = 0, aMorse = 0, dMorse = 0; int ih = 0; };
+
 struct InputParams {
   std::string dir;                   // directory of the .inp (file names are relative to it)
   int nsteps = 0;
@@ -137,7 +145,11 @@ struct InputParams {
   bool restart = false;
   int nstates = 0;
   double lambda[kMaxStates] = {0};
-  int n_restraints = 0;              // sequence/atom/distance/angle/wall restraints (unsupported)
+  std::vector<RstSeq> rst_seq;
+  std::vector<RstPos> rst_pos;
+  std::vector<RstDis> rst_dis;
+  std::vector<RstAng> rst_ang;
+  std::vector<RstWal> rst_wal;
   bool box = false;
 };
 
@@ -179,6 +191,8 @@ struct Window {
 };
 
 // entry points of the host side (qmi_io.cpp / qmi_setup.cpp)
+// number token as Fortran list-directed input reads it (d/D exponents)
+bool parse_num(const std::string& tok, double* v);
 bool read_prm(const std::string& path, PrmFile* out, Error* err);
 bool load_input(const std::string& inp_path, InputParams* p, Error* err);
 bool load_topology(const std::string& path, Topology* t, Error* err);

@@ -33,3 +33,21 @@ def systems(tmp_path_factory):
         return cache[key]
 
     return get
+
+
+@pytest.fixture(scope="session")
+def golden_inp(systems, tmp_path_factory):
+    """Input file of the window a golden vector set of tests/golden/ was made
+    from (same generator arguments as tests/golden/make_golden.py)."""
+    from tests.golden import make_golden as M
+    cache = {}
+
+    def get(cfg, seed):
+        if (cfg, seed) not in cache:
+            if cfg == "kat":
+                cache[(cfg, seed)] = M.kat_window(str(tmp_path_factory.mktemp("kat")))
+            else:
+                cache[(cfg, seed)] = systems(cfg, seed=seed, steps=M.STEPS, energy=M.ENERGY, output=M.OUTPUT)
+        return cache[(cfg, seed)]
+
+    return get

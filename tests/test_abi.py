@@ -12,7 +12,7 @@ import re
 import numpy as np
 import pytest
 
-from tests.golden.make_golden import file_sha
+from tests.golden.make_golden import file_sha, input_files
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = os.path.join(ROOT, "tests", "golden")
@@ -55,16 +55,12 @@ def test_errors_are_reported_not_fatal(tmp_path):
     e.close()
 
 
-@pytest.mark.parametrize("cfg,seed", [("water", 0), ("ligwat", 0), ("protein", 2)])
-def test_host_setup_matches_reference_start(systems, cfg, seed):
+@pytest.mark.parametrize("cfg,seed", [("water", 0), ("ligwat", 0), ("protein", 2), ("kat", 0)])
+def test_host_setup_matches_reference_start(golden_inp, cfg, seed):
     from q_amd.engine import Engine
     g = dict(np.load(os.path.join(GOLD, f"{cfg}_s{seed}.npz")))
-    inp = systems(cfg, seed=seed, steps=100, energy=10, output=10)
-    d = os.path.dirname(inp)
-    files = [os.path.join(d, f"{cfg}_s{seed}.top")]
-    if os.path.exists(os.path.join(d, f"{cfg}_s{seed}.fep")):
-        files.append(os.path.join(d, f"{cfg}_s{seed}.fep"))
-    assert file_sha(files) == str(g["input_sha"])
+    inp = golden_inp(cfg, seed)
+    assert file_sha(input_files(cfg, seed, os.path.dirname(inp))) == str(g["input_sha"])
     e = Engine(0)
     w = e.add_window(inp)
     info = e.info(w)

@@ -1,14 +1,13 @@
-"""GPU dynamics: 100 MD steps against the reference qdyn run on the same
-inputs (md_run, src/q6/md.f90:3917-4263), the .en file layout
-(nrgy.f90:94-119), and batching invariance (a window gives bitwise the same
-trajectory alone or batched with others)."""
+"""GPU dynamics: 100 MD steps against the reference md_run on the same
+inputs (src/q6/md.f90:3917-4263; golden vectors of tests/golden/), the .en
+file layout (nrgy.f90:94-119), and batching invariance (a window gives
+bitwise the same trajectory alone or batched with others)."""
 import os
-import shutil
 
 import numpy as np
 import pytest
 
-from oracle import qprobe_dump as Q
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -35,31 +34,29 @@ def read_en(path, nstates):
     return out
 
 
-@pytest.mark.parametrize("cfg,seed", [("ligwat", 0), ("protein", 2)])
-def test_100_steps(systems, cfg, seed):
+@pytest.mark.parametrize("cfg,seed", [("ligwat", 0), ("protein", 2), ("kat", 0)])
+def test_100_steps(golden_inp, cfg, seed):
     from q_amd.engine import Engine
-    inp = systems(cfg, seed=seed, steps=100, energy=10, output=10)
+    g = dict(np.load(os.path.join(GOLD, f"{cfg}_s{seed}.npz")))
+    inp = golden_inp(cfg, seed)
     d = os.path.dirname(inp)
     tag = os.path.splitext(os.path.basename(inp))[0]
-    print(f"[gpu] reference run {cfg}", flush=True)
-    dump = Q.run_qprobe(inp, out_name="dyn.dump")
-    shutil.move(os.path.join(d, tag + ".en"), os.path.join(d, tag + ".ref.en"))
     e = Engine(0)
     w = e.add_window(inp)
     e.prepare()
     print(f"[gpu] engine run {cfg}", flush=True)
     e.run(100, finalize=True)
     x = e.coords(w)
-    dx = np.abs(x - dump["XN"]).max()
+    dx = np.abs(x - g["XN"]).max()
     print(cfg, "max |x - x_ref| after 100 steps", dx)
     assert dx < 0.05
     eq = e.q_energies(w)
-    ref = dump["EQN"]
+    ref = g["EQN"]
     rel = np.abs(eq - ref) / np.maximum(np.abs(ref), 1.0)
     print(cfg, "final EQ max rel", rel.max(), "per field", rel.max(axis=0))
     assert rel.max() < 1e-4
     S = eq.shape[0]
-    en_ref = read_en(os.path.join(d, tag + ".ref.en"), S)
+    en_ref = g["en"]
     en = read_en(os.path.join(d, tag + ".en"), S)
     assert len(en) == len(en_ref) == 9     # steps 10, 20, ..., 90
     for k, (a, b) in enumerate(zip(en, en_ref)):

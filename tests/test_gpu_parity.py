@@ -1,16 +1,25 @@
 """GPU parity of the HIP engine against the reference Q6 routines.
 
-The reference side is oracle/_ref/qprobe: the unmodified Q6 modules
-(src/q6/md.f90 ...) driven by this project's harness, run on the same
-input files.  Step-0 per-state Q energies must agree within 1e-6 relative,
+The reference side is the golden vectors of tests/golden/ (written by
+tests/golden/make_golden.py from the unmodified Q6 modules, src/q6/md.f90
+make_pair_lists + pot_energy at step 0) for the same seeded input files.
+Step-0 per-state Q energies must agree within 1e-6 relative, FP64 terms
+within 1e-6, bulk FP32 nonbonded sums within 2e-6 relative + 2e-2 kcal/mol,
 pair lists exactly (as sets of unordered atom pairs).
 """
+import os
+
 import numpy as np
 import pytest
 
-from oracle import qprobe_dump as Q
+from tests.golden.make_golden import E_NAMES, canonical_lists, sha
 
 pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+# three seeded synthetic windows and the reference's own test system (KAT
+# ligand in water, one state, sequence restraint)
+CASES = {"water": 0, "ligwat": 0, "protein": 2, "kat": 0}
 
 
 def _rel(a, b, floor=1.0):
@@ -18,13 +27,14 @@ def _rel(a, b, floor=1.0):
 
 
 @pytest.fixture(scope="module")
-def evaluated(systems):
+def evaluated(golden_inp):
     from q_amd.engine import Engine
     out = {}
-    for cfg, seed in (("water", 0), ("ligwat", 0), ("protein", 2)):
-        inp = systems(cfg, seed=seed, steps=10)
-        print(f"[gpu] qprobe {cfg} {inp}", flush=True)
-        dump = Q.run_qprobe(inp)
+    for cfg, seed in CASES.items():
+        inp = golden_inp(cfg, seed)
+        g = dict(np.load(os.path.join(GOLD, f"{cfg}_s{seed}.npz")))
+        ref = dict(zip(E_NAMES, g["E0"].tolist()))
+        dump = {"E0": ref, "EQ0": g["EQ0"], "D0": g["D0"], "gold": g}
         e = Engine(0)
         w = e.add_window(inp)
         e.prepare()
@@ -34,7 +44,7 @@ def evaluated(systems):
     return out
 
 
-@pytest.mark.parametrize("cfg", ["water", "ligwat", "protein"])
+@pytest.mark.parametrize("cfg", list(CASES))
 def test_step0_energies(evaluated, cfg):
     e, w, dump = evaluated[cfg]
     E = e.energies(w)
@@ -51,11 +61,12 @@ def test_step0_energies(evaluated, cfg):
         assert abs(E[k] - ref[k]) < 2e-6 * abs(ref[k]) + 2e-2, k
     # FP64 terms
     for k in ["LRF", "p.bond", "p.angle", "p.torsion", "p.improper", "q.bond", "q.angle", "q.torsion", "q.improper",
-              "restraint.total", "restraint.solvent_radial", "restraint.water_pol", "restraint.shell"]:
+              "restraint.total", "restraint.solvent_radial", "restraint.water_pol", "restraint.shell",
+              "restraint.protein"]:
         assert _rel(E[k], ref[k]) < 1e-6, k
 
 
-@pytest.mark.parametrize("cfg", ["ligwat", "protein"])
+@pytest.mark.parametrize("cfg", ["ligwat", "protein", "kat"])
 def test_step0_q_energies(evaluated, cfg):
     e, w, dump = evaluated[cfg]
     EQ = e.q_energies(w)
@@ -70,7 +81,7 @@ def test_step0_q_energies(evaluated, cfg):
             assert _rel(EQ[s, k], ref[s, k]) < 1e-6, (s, names[k])
 
 
-@pytest.mark.parametrize("cfg", ["water", "ligwat", "protein"])
+@pytest.mark.parametrize("cfg", list(CASES))
 def test_step0_gradient(evaluated, cfg):
     e, w, dump = evaluated[cfg]
     d = e.forces(w).reshape(-1, 3)
@@ -82,21 +93,23 @@ def test_step0_gradient(evaluated, cfg):
     assert err.max() < 1e-3 * max(1.0, np.sqrt((scale ** 2).mean()))
 
 
-@pytest.mark.parametrize("cfg", ["water", "ligwat", "protein"])
+@pytest.mark.parametrize("cfg", list(CASES))
 def test_pairlists(evaluated, cfg):
     from q_amd import engine as EN
     e, w, dump = evaluated[cfg]
-    i, j, _ = e.pairlist(w, EN.LIST_WW)
-    assert Q.unordered(i, j) == Q.unordered(dump["NBWW"][:, 0], dump["NBWW"][:, 1])
+    g = dump["gold"]
     i, j, lj = e.pairlist(w, EN.LIST_PP)
-    ref = dump["NBPP"]
-    assert {(min(a, b), max(a, b), c) for a, b, c in zip(i, j, lj)} == \
-        {(min(a, b), max(a, b), c) for a, b, c in ref.tolist()}
-    i, j, lj = e.pairlist(w, EN.LIST_PW)
-    ref = dump["NBPW"]
-    assert set(zip(i.tolist(), j.tolist(), lj.tolist())) == set(map(tuple, ref.tolist()))
-    i, j, lj = e.pairlist(w, EN.LIST_QP)
-    ref = dump["NBQP"]
-    assert set(zip(i.tolist(), j.tolist(), lj.tolist())) == set(map(tuple, ref[:, :3].tolist()))
-    _, j, _ = e.pairlist(w, EN.LIST_QW)
-    assert set(j.tolist()) == set(dump["NBQW"].tolist())
+    pp = np.stack([i, j, lj], 1) if len(i) else np.zeros((0, 3))
+    pi, pj, plj = e.pairlist(w, EN.LIST_PW)
+    pw = np.stack([pi, pj, plj], 1) if len(pi) else np.zeros((0, 3))
+    wi, wj, _ = e.pairlist(w, EN.LIST_WW)
+    ww = np.stack([wi, wj], 1) if len(wi) else np.zeros((0, 2))
+    qi, qj, qlj = e.pairlist(w, EN.LIST_QP)
+    qp = np.stack([qi, qj, qlj], 1) if len(qi) else np.zeros((0, 3))
+    _, ow, _ = e.pairlist(w, EN.LIST_QW)
+    C = canonical_lists(pp, pw, ww, qp, ow)
+    for k, a in C.items():
+        print(f"  {cfg} {k}: gpu {len(a)} ref {int(g['n_' + k])}")
+        assert len(a) == int(g[f"n_{k}"]), k
+        assert len(np.unique(a, axis=0)) == len(a), k
+        assert sha(a) == str(g[f"sha_{k}"]), k

@@ -14,34 +14,27 @@ import numpy as np
 import pytest
 
 from oracle import qdyn_np as O
-from tests.golden.make_golden import E_NAMES, canonical_lists, file_sha, sha
+from tests.golden.make_golden import E_NAMES, canonical_lists, file_sha, input_files, sha
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = [("water", 0), ("ligwat", 0), ("protein", 2)]
+# three seeded synthetic windows and the reference's own test system
+CASES = [("water", 0), ("ligwat", 0), ("protein", 2), ("kat", 0)]
 
 
 def load_gold(cfg, seed):
     return dict(np.load(os.path.join(GOLD, f"{cfg}_s{seed}.npz")))
 
 
-def window_inp(systems, cfg, seed):
-    # same inp parameters as tests/golden/make_golden.py
-    return systems(cfg, seed=seed, steps=100, energy=10, output=10)
-
-
 @pytest.fixture(scope="module")
-def oracle_eval(systems):
+def oracle_eval(golden_inp):
     cache = {}
 
     def get(cfg, seed):
         if (cfg, seed) not in cache:
             g = load_gold(cfg, seed)
-            inp = window_inp(systems, cfg, seed)
-            d = os.path.dirname(inp)
-            files = [os.path.join(d, f"{cfg}_s{seed}.top")]
-            if os.path.exists(os.path.join(d, f"{cfg}_s{seed}.fep")):
-                files.append(os.path.join(d, f"{cfg}_s{seed}.fep"))
-            assert file_sha(files) == str(g["input_sha"]), "synthetic generator output changed: regenerate golden"
+            inp = golden_inp(cfg, seed)
+            files = input_files(cfg, seed, os.path.dirname(inp))
+            assert file_sha(files) == str(g["input_sha"]), "input files changed: regenerate golden"
             E, EQ, dgrad, L = O.evaluate(inp, g["X0"], float(g["temp0"][1]))
             cache[(cfg, seed)] = (g, E, EQ, dgrad, L, inp)
         return cache[(cfg, seed)]
@@ -143,13 +136,13 @@ def _energy(w, x, tfree):
     return E["potential"], d
 
 
-@pytest.mark.parametrize("cfg,seed,npick", [("ligwat", 0, 4), ("protein", 2, 2)])
-def test_gradient_is_derivative_of_energy(systems, cfg, seed, npick):
+@pytest.mark.parametrize("cfg,seed,npick", [("ligwat", 0, 4), ("protein", 2, 2), ("kat", 0, 3)])
+def test_gradient_is_derivative_of_energy(golden_inp, cfg, seed, npick):
     """Central differences of the potential (lists frozen, LRF off: the LRF
     force is a Taylor field, not the derivative of its energy) against the
     analytic gradient at q-atoms, solute atoms and water atoms."""
     g = load_gold(cfg, seed)
-    inp = window_inp(systems, cfg, seed)
+    inp = golden_inp(cfg, seed)
     w = O.Window(inp, g["X0"])
     w.p["lrf"] = False
     w.pair_lists()
