@@ -88,6 +88,32 @@ def cpu_reference(inp_water, inp_protein, n_water_steps, n_protein_steps):
     return out
 
 
+def traffic_per_launch(kernel: str):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), as recorded
+    in profiles/traffic.txt ("kernel bytes" lines); None when not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        t = line.split()
+        if len(t) >= 2 and t[0] == kernel and not line.startswith("#"):
+            return float(t[1])
+    return None
+
+
+def copy_probe(nbytes: int):
+    """A plain device copy of nbytes (torch clone), the calibration kernel
+    for rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    x.fill_(1)
+    torch.cuda.synchronize()
+    y = x.clone()
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "ok": bool(y[0].item() == 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +122,11 @@ def main():
     ap.add_argument("--windows", type=int, default=64, help="total windows (half per leg)")
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run a plain device copy of this many bytes")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs shard windows and are not "
                                    "part of this build", "n_gpus": args.gpus}))
@@ -104,7 +134,9 @@ def main():
 
     work = tempfile.mkdtemp(prefix="qmi_bench_")
     nw = args.windows // 2
-    steps_inp = args.warmup + args.steps + 10
+    n_timing = max(20, min(args.steps, 100))
+    # every window must still be running through the timing pass
+    steps_inp = args.warmup + args.steps + n_timing + 10
     t = time.perf_counter()
     inps = make_ladder(work, nw, args.windows - nw, steps_inp)
     t_gen = time.perf_counter() - t
@@ -120,8 +152,7 @@ def main():
 
     # live per-family device timing over a second timed region of the same engine
     e.set_timing(True)
-    nt = max(20, min(args.steps, 100))
-    e.run(nt, finalize=False)
+    e.run(n_timing, finalize=False)
     fam = e.timings()
     e.close()
 
@@ -135,11 +166,11 @@ def main():
     n_ww = [len(e2.pairlist(0, LIST_WW)[0]), len(e2.pairlist(1, LIST_WW)[0])]
     e2.close()
     n_ww_total = nw * n_ww[0] + (len(inps) - nw) * n_ww[1]
-    # nonbonded tile family: algorithmic FLOPs of the water-water part only
-    # (200 FP32 FLOP per molecule pair) over the family's measured time is a
-    # lower bound of its achieved rate
-    t_nb = fam[1] * 1e-6 if len(fam) > 1 and fam[1] > 0 else None
-    achieved = (200.0 * n_ww_total / t_nb / 1e12) if t_nb else None
+    # water-water tile kernel: 200 FP32 FLOP per listed molecule pair
+    # (1 LJ + Coulomb and 8 Coulomb-only site pairs) over the kernel's own
+    # device time, measured with HIP events on the engine's stream
+    t_ww = fam[4] * 1e-6 if len(fam) > 4 and fam[4] > 0 else None
+    achieved = (200.0 * n_ww_total / t_ww / 1e12) if t_ww else None
 
     result = {
         "metric": "aggregate ns/day, 64 batched FEP windows on 1 GPU",
@@ -167,13 +198,16 @@ def main():
             "nonbonded_tiles": round(fam[1], 2) if fam else None,
             "lrf_bonded_restraints": round(fam[2], 2) if fam else None,
             "integrate_shake_thermostat": round(fam[3], 2) if fam else None,
-            "wall_per_step": round(fam[5], 2) if fam else None,
+            "ww_kernel": round(fam[4], 2) if fam else None,
+            "wall_per_step_timed_pass": round(fam[5], 2) if fam else None,
         },
         "roofline": {
-            "bound": "valu", "kernel": "nonbonded tile family (water-water FLOPs only)",
+            "bound": "mfma", "kernel": "k_ww (water-water tiles, FP32 vector ALU; compute roof = FP32 peak)",
             "achieved": round(achieved, 4) if achieved else None, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 5) if achieved else None, "traffic": None,
-            "work": {"ww_molecule_pairs_per_step": n_ww_total, "flop_per_pair": 200},
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 5) if achieved else None,
+            "traffic": traffic_per_launch("k_ww"),
+            "work": {"ww_molecule_pairs_per_step": n_ww_total, "flop_per_pair": 200,
+                     "kernel_us_per_launch": round(fam[4], 2) if fam else None},
         },
     }
 

@@ -59,6 +59,12 @@ int qmi_engine_create(int hip_device, qmi_engine** out);
  * Maxwell start (qdyn.f90:94-116). */
 int qmi_window_add(qmi_engine* e, const char* inp_path, long long seed_override, int* window_id);
 
+/* Where window w writes its run log (the stdout of qdyn, write_out
+ * md.f90:15292-15438): a file path, "-" for this process's stdout, or NULL
+ * for none.  Default: the .inp path with the extension .log.  Call before
+ * qmi_prepare. */
+int qmi_window_set_log(qmi_engine* e, int w, const char* path);
+
 /* Upload all windows to the GPU, build the static Q-atom lists, pick tiles
  * and capture the step graph.  Replaces distribute_nonbonds
  * (md.f90:1044-1361) and the set-up half of md_run (md.f90:3917-4000). */
@@ -113,8 +119,9 @@ int qmi_set_timing(qmi_engine* e, int on);
 /* Device time per kernel family of the last qmi_run, in microseconds per
  * step (no reference counterpart).  Order: pair-list + LRF build,
  * nonbonded tiles, LRF evaluation + bonded + restraints, leapfrog + SHAKE +
- * thermostat, (unused), wall-clock total.  Families are zero unless
- * qmi_set_timing was on.  Returns the number of values written. */
+ * thermostat, the water-water tile kernel alone, wall-clock total.  All but
+ * the last are zero unless qmi_set_timing was on.  Returns the number of
+ * values written. */
 int qmi_timings(qmi_engine* e, double* out, int cap);
 
 /* Last error message of the calling thread. */

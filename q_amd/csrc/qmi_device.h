@@ -73,7 +73,13 @@ struct TileList {
   uint2* m14;      // 1-4 mask (P<-P, Q<-P, P<-Q)
   int nicl;        // number of i-clusters (all windows)
   int cap_total;
+  // static lists are cut into work items of at most kTileChunk tiles, one
+  // wave each: (i-cluster, first entry, number of entries); null for
+  // dynamic lists, which run one wave per i-cluster
+  const int4* work;
+  int nwork;
 };
+constexpr int kTileChunk = 16;
 
 struct Dev {
   int nwin, ntot;
@@ -127,6 +133,7 @@ struct Dev {
   const int* qq_st;      // state
   const int* qq_win;
   const float* qq_sc;    // soft-core term of the pair
+  const double2* qq_ab;  // A, B of the pair in its state
   // bonded terms
   int nbond, nang, ntor, nimp;
   const int2* b_ij; const double2* b_par; const int* b_win; const uint8_t* b_solv;
@@ -141,6 +148,11 @@ struct Dev {
   const int4* qa_ijk; const double2* qa_par; const int2* qa_ws;
   const int4* qt_ijkl; const double4* qt_par; const int2* qt_ws; // fk, rmult, deltor
   const int4* qi_ijkl; const double2* qi_par; const int2* qi_ws; // fk, imp0
+  // per-term gradient slots of the bonded terms (3 int64 each) and, per
+  // atom with bonded terms, its slots in CSR form
+  long long* bslot;
+  int nbg;
+  const int* bg_atom; const int* bg_start; const int* bg_slot;
   // restraints: (kind, window, i, j) global atoms; (k, flag) with flag = ih
   // (sequence, wall) or ipsi (position, distance, angle); 8 parameters each
   int nrst;
@@ -154,6 +166,8 @@ struct Dev {
   double* wtheta;        // per water (global water index), theta
   int* wshell;           // shell index (-1 if none)
   const int* w_off;      // per window: first global water index
+  const int* wat_win;    // per global water: its window
+  int nwat_total;
   // energy snapshots for output: [ring slot][window][E_NSLOT] + temperature
   long long* e_ring;
   double* t_ring;        // [slot][window][4]: temp, tfree, ekin, istep

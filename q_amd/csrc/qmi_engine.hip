@@ -106,7 +106,7 @@ qmi_engine::~qmi_engine() {
   if (stream) (void)hipStreamDestroy(stream);
   for (auto& h : wins) {
     if (h.en) fclose(h.en);
-    if (h.log) fclose(h.log);
+    if (h.log && h.log != stdout) fclose(h.log);
   }
 }
 
@@ -149,6 +149,7 @@ static int build_device(qmi_engine* e) {
   std::vector<double> q_ljd, q_scd, t_lj;
   // q-q
   std::vector<int4> qq_ij;
+  std::vector<double2> qq_ab;
   std::vector<float> qq_es, qq_sc;
   std::vector<int> qq_st, qq_win;
   // bonded
@@ -168,7 +169,7 @@ static int build_device(qmi_engine* e) {
   std::vector<double> sc_d2;
   // shells
   std::vector<ShellDev> shells;
-  std::vector<int> w_off(NW);
+  std::vector<int> w_off(NW), wat_win;
   // static Q tiles (host-built), per type: per i-cluster lists
   struct HT { std::vector<std::vector<std::pair<int, uint2>>> l; std::vector<std::vector<uint2>> m14; };
   HT qt[4];  // T_QP, T_PQ, T_QW, T_WQ
@@ -334,6 +335,33 @@ static int build_device(qmi_engine* e) {
     for (int s = 0; s < S; ++s)
       for (auto& pq : W.nbqq[s]) {
         qq_ij.push_back(make_int4(pq.iq, ao + pq.j, pq.jq, pq.ljcod));
+        {
+          // A and B of the pair (nonbond_qq, md.f90:10786-10818); LJ code c
+          // selects column c of the parameter tables
+          const int c = pq.ljcod - 1, i_at = fp.iqseq[pq.iq];
+          double aLJ, bLJ;
+          if (!fp.qvdw_flag) {
+            const AtomType& ti = t.iaclib[t.iac[i_at]];
+            const AtomType& tj = t.iaclib[t.iac[pq.j]];
+            aLJ = ti.avdw[c] * tj.avdw[c];
+            bLJ = ti.bvdw[c] * tj.bvdw[c];
+          } else {
+            const int qi = fp.qiac[pq.iq * S + s];
+            aLJ = fp.qavdw[qi * 3 + c];
+            bLJ = fp.qbvdw[qi * 3 + c];
+            if (pq.jq) {
+              const int qj = fp.qiac[(pq.jq - 1) * S + s];
+              aLJ *= fp.qavdw[qj * 3 + c];
+              bLJ *= fp.qbvdw[qj * 3 + c];
+            } else {
+              if (pq.ljcod == 2) { aLJ = fp.qavdw[qi * 3]; bLJ = fp.qbvdw[qi * 3]; }
+              const AtomType& tj = t.iaclib[t.iac[pq.j]];
+              aLJ *= tj.avdw[c];
+              bLJ *= tj.bvdw[c];
+            }
+          }
+          qq_ab.push_back(make_double2(aLJ, bLJ));
+        }
         qq_es.push_back((float)pq.el_scale);
         qq_st.push_back(s);
         qq_win.push_back(wi);
@@ -568,6 +596,7 @@ static int build_device(qmi_engine* e) {
     st.tscale_solvent = 0.0;
     ao += N;
     goff += (int)t.cgp.size();
+    for (int iw = 0; iw < W.nwat; ++iw) wat_win.push_back(wi);
     woff += W.nwat;
     qoff += fp.nqat;
     toff += t.natyps;
@@ -637,6 +666,7 @@ static int build_device(qmi_engine* e) {
   UP(ip, qq_st); d.qq_st = ip;
   UP(ip, qq_win); d.qq_win = ip;
   UP(fp_, qq_sc); d.qq_sc = fp_;
+  double2* qab; UP(qab, qq_ab); d.qq_ab = qab;
   d.nbond = (int)b_ij.size(); d.nang = (int)a_ijk.size(); d.ntor = (int)t_ijkl.size(); d.nimp = (int)i_ijkl.size();
   int2* i2; double2* d2; double4* d4; float2* f2;
   UP(i2, b_ij); d.b_ij = i2; UP(d2, b_par); d.b_par = d2; UP(ip, b_win); d.b_win = ip; UP(up, b_solv); d.b_solv = up;
@@ -650,6 +680,35 @@ static int build_device(qmi_engine* e) {
   UP(i4, qa_ijk); d.qa_ijk = i4; UP(d2, qa_par); d.qa_par = d2; UP(i2, qa_ws); d.qa_ws = i2;
   UP(i4, qt_ijkl); d.qt_ijkl = i4; UP(d4, qt_par); d.qt_par = d4; UP(i2, qt_ws); d.qt_ws = i2;
   UP(i4, qi_ijkl); d.qi_ijkl = i4; UP(d2, qi_par); d.qi_par = d2; UP(i2, qi_ws); d.qi_ws = i2;
+  // bonded gradient slots, in the order k_bonded writes them
+  {
+    std::vector<int> slot_atom;
+    for (auto& v : b_ij) { slot_atom.push_back(v.x); slot_atom.push_back(v.y); }
+    for (auto& v : a_ijk) { slot_atom.push_back(v.x); slot_atom.push_back(v.y); slot_atom.push_back(v.z); }
+    for (auto& v : t_ijkl) for (int a : {v.x, v.y, v.z, v.w}) slot_atom.push_back(a);
+    for (auto& v : i_ijkl) for (int a : {v.x, v.y, v.z, v.w}) slot_atom.push_back(a);
+    for (auto& v : qb_ij) { slot_atom.push_back(v.x); slot_atom.push_back(v.y); }
+    for (auto& v : qa_ijk) { slot_atom.push_back(v.x); slot_atom.push_back(v.y); slot_atom.push_back(v.z); }
+    for (auto& v : qt_ijkl) for (int a : {v.x, v.y, v.z, v.w}) slot_atom.push_back(a);
+    for (auto& v : qi_ijkl) for (int a : {v.x, v.y, v.z, v.w}) slot_atom.push_back(a);
+    std::vector<int> per_atom(ao, 0);
+    for (int a : slot_atom) per_atom[a]++;
+    std::vector<int> bg_atom, bg_start, bg_slot(slot_atom.size()), pos(ao, -1);
+    for (int a = 0; a < ao; ++a)
+      if (per_atom[a]) { pos[a] = (int)bg_atom.size(); bg_atom.push_back(a); }
+    bg_start.assign(bg_atom.size() + 1, 0);
+    for (size_t k = 0; k < bg_atom.size(); ++k) bg_start[k + 1] = bg_start[k] + per_atom[bg_atom[k]];
+    std::vector<int> fill(bg_atom.size(), 0);
+    for (size_t sidx = 0; sidx < slot_atom.size(); ++sidx) {
+      const int k = pos[slot_atom[sidx]];
+      bg_slot[bg_start[k] + fill[k]++] = (int)sidx;
+    }
+    d.nbg = (int)bg_atom.size();
+    UP(ip, bg_atom); d.bg_atom = ip;
+    UP(ip, bg_start); d.bg_start = ip;
+    UP(ip, bg_slot); d.bg_slot = ip;
+    if ((rc = dev_alloc(e, &d.bslot, 3 * std::max<size_t>(1, slot_atom.size()))) != 0) return rc;
+  }
   d.nrst = (int)rs_kwij.size();
   UP(i4, rs_kwij); d.rs_kwij = i4; UP(i2, rs_kf); d.rs_kf = i2; UP(dp, rs_par); d.rs_par = dp;
   d.nsu = (int)su_n.size();
@@ -661,6 +720,8 @@ static int build_device(qmi_engine* e) {
   if ((rc = dev_alloc(e, &d.wtheta, (size_t)e->nwat_total)) != 0) return rc;
   if ((rc = dev_alloc(e, &d.wshell, (size_t)e->nwat_total)) != 0) return rc;
   UP(ip, w_off); d.w_off = ip;
+  UP(ip, wat_win); d.wat_win = ip;
+  d.nwat_total = e->nwat_total;
   d.ring_n = 64;
   if ((rc = dev_alloc(e, &d.e_ring, (size_t)d.ring_n * NW * E_NSLOT)) != 0) return rc;
   if ((rc = dev_alloc(e, &d.t_ring, (size_t)d.ring_n * NW * 4)) != 0) return rc;
@@ -676,6 +737,8 @@ static int build_device(qmi_engine* e) {
     }
     L.nicl = (int)iwin.size();
     L.cap_total = total;
+    L.work = nullptr;
+    L.nwork = 0;
     int r;
     if ((r = dev_upload(e, &L.start, start)) != 0) return r;
     if ((r = dev_upload(e, &L.count, count)) != 0) return r;
@@ -704,7 +767,15 @@ static int build_device(qmi_engine* e) {
     }
     L.nicl = (int)h.l.size();
     L.cap_total = (int)jc.size();
+    std::vector<int4> work;
+    for (size_t ic = 0; ic < h.l.size(); ++ic)
+      for (int k = 0; k < count[ic]; k += kTileChunk)
+        work.push_back(make_int4((int)ic, start[ic] + k, std::min(kTileChunk, count[ic] - k), 0));
+    L.nwork = (int)work.size();
     int r;
+    int4* wp = nullptr;
+    if ((r = dev_upload(e, &wp, work)) != 0) return r;
+    L.work = wp;
     if ((r = dev_upload(e, &L.start, start)) != 0) return r;
     if ((r = dev_upload(e, &L.count, count)) != 0) return r;
     if ((r = dev_upload(e, &L.jc, jc)) != 0) return r;
@@ -886,7 +957,8 @@ int qmi_prepare(qmi_engine* e) {
       h.en = fopen((W.inp.dir + "/" + W.inp.ene_file).c_str(), "wb");
       if (!h.en) return set_err(-42, "error opening energy output file");
     }
-    h.log = fopen(h.log_path.c_str(), "w");
+    if (h.log_path == "-") h.log = stdout;
+    else if (!h.log_path.empty()) h.log = fopen(h.log_path.c_str(), "w");
     if (h.log) fputs(W.log.c_str(), h.log);
   }
   // initial temperature and scaling factors (md_run, md.f90:3969-3976)
@@ -961,6 +1033,11 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   std::vector<WinState> ws(NW);
   HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
   int first = ws[0].istep;
+  // no window runs past its own [MD] steps: stop when the last one is done
+  long long remaining = 0;
+  for (int wi = 0; wi < NW; ++wi)
+    remaining = std::max<long long>(remaining, (long long)e->wins[wi].w->inp.nsteps - ws[wi].istep);
+  nsteps = std::max<long long>(0, std::min(nsteps, remaining));
   for (int wi = 0; wi < NW; ++wi) {
     ws[wi].stop_at = (int)std::min<long long>(e->wins[wi].w->inp.nsteps, (long long)ws[wi].istep + nsteps);
     HIP_OK(hipMemcpy(&e->d.ws[wi].stop_at, &ws[wi].stop_at, sizeof(int), hipMemcpyHostToDevice));
@@ -968,7 +1045,7 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   if (e->timing_on && !e->ev[0])
     for (auto& x : e->ev) HIP_OK(hipEventCreate(&x));
   // per family: 0 lists+LRF build, 1 nonbonded tiles, 2 LRF/bonded/restraints, 3 integrate
-  double fam[4] = {0, 0, 0, 0};
+  double fam[5] = {0, 0, 0, 0, 0};
   auto t0 = std::chrono::steady_clock::now();
   long long done = 0;
   const int R = e->d.ring_n;
@@ -980,19 +1057,23 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[0], e->stream));
       launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[1], e->stream));
-      launch_nonbonded(e->d, e->stream);
+      launch_ww(e->d, e->stream);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[2], e->stream));
-      launch_other_forces(e->d, e->stream);
+      launch_nonbonded(e->d, e->stream, false);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[3], e->stream));
+      launch_other_forces(e->d, e->stream);
+      if (e->timing_on) HIP_OK(hipEventRecord(e->ev[4], e->stream));
       launch_integrate(e->d, e->stream, e->shake_any);
       if (e->timing_on) {
-        HIP_OK(hipEventRecord(e->ev[4], e->stream));
-        HIP_OK(hipEventSynchronize(e->ev[4]));
-        for (int f = 0; f < 4; ++f) {
-          float ms = 0.f;
-          HIP_OK(hipEventElapsedTime(&ms, e->ev[f], e->ev[f + 1]));
-          fam[f] += ms * 1000.0;
-        }
+        HIP_OK(hipEventRecord(e->ev[5], e->stream));
+        HIP_OK(hipEventSynchronize(e->ev[5]));
+        float ms[5];
+        for (int f = 0; f < 5; ++f) HIP_OK(hipEventElapsedTime(&ms[f], e->ev[f], e->ev[f + 1]));
+        fam[0] += ms[0] * 1000.0;                 // lists + LRF build
+        fam[1] += (ms[1] + ms[2]) * 1000.0;       // all nonbonded tiles
+        fam[2] += ms[3] * 1000.0;                 // LRF evaluation, bonded, restraints
+        fam[3] += ms[4] * 1000.0;                 // leapfrog, SHAKE, thermostat
+        fam[4] += ms[1] * 1000.0;                 // water-water tiles alone
       }
     }
     HIP_OK(hipGetLastError());
@@ -1004,7 +1085,7 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   auto t1 = std::chrono::steady_clock::now();
   double us = std::chrono::duration<double, std::micro>(t1 - t0).count();
   e->timings.assign(6, 0.0);
-  for (int f = 0; f < 4; ++f) e->timings[f] = nsteps > 0 ? fam[f] / (double)nsteps : 0.0;
+  for (int f = 0; f < 5; ++f) e->timings[f] = nsteps > 0 ? fam[f] / (double)nsteps : 0.0;
   e->timings[5] = nsteps > 0 ? us / (double)nsteps : 0.0;
   HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
   for (int wi = 0; wi < NW; ++wi) {
@@ -1054,6 +1135,15 @@ static HostWin* getw(qmi_engine* e, int w) {
   if (!e || w < 0 || w >= (int)e->wins.size()) { set_err(-1, "invalid window"); return nullptr; }
   return &e->wins[w];
 }
+
+int qmi_window_set_log(qmi_engine* e, int w, const char* path) {
+  HostWin* h = getw(e, w);
+  if (!h) return -1;
+  if (e->prepared) return set_err(-2, "the log must be chosen before qmi_prepare");
+  h->log_path = path ? path : "";
+  return 0;
+}
+
 
 int qmi_get_energies(qmi_engine* e, int w, double* out, int cap) {
   HostWin* h = getw(e, w);

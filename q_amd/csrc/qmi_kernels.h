@@ -8,7 +8,9 @@ namespace qmi {
 void launch_begin(const Dev& d, hipStream_t s, int force_energy, int eval_only, int rebuild);
 void launch_zero(const Dev& d, hipStream_t s);
 void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc);
-void launch_nonbonded(const Dev& d, hipStream_t s);
+// water-water tiles alone, and all nonbonded tile kernels (with or without them)
+void launch_ww(const Dev& d, hipStream_t s);
+void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww = true);
 void launch_other_forces(const Dev& d, hipStream_t s);
 void launch_integrate(const Dev& d, hipStream_t s, int shake_any);
 void launch_temperature(const Dev& d, hipStream_t s);

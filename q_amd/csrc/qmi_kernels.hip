@@ -564,8 +564,9 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const TileList L = d.tl[Q_SIDE ? T_QW : T_WQ];
-  if (wave >= L.nicl) return;
-  const int ic = wave;
+  if (wave >= L.nwork) return;
+  const int4 wk = L.work[wave];
+  const int ic = wk.x;
   const int w = Q_SIDE ? d.qc_win[ic] : d.wc_win[ic];
   const WinState& st = d.ws[w];
   if (!st.active) return;
@@ -585,7 +586,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
   double eel[kMaxS], evdw[kMaxS];
   for (int s = 0; s < kMaxS; ++s) { eel[s] = 0.0; evdw[s] = 0.0; }
   const float aO = W.aO1, bO = W.bO1;
-  const int s0 = L.start[ic], n = L.count[ic];
+  const int s0 = wk.y, n = wk.z;
   for (int t = 0; t < n; ++t) {
     const int jc = L.jc[s0 + t];
     const uint2 m = L.mask[s0 + t];
@@ -662,8 +663,9 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const TileList L = d.tl[Q_SIDE ? T_QP : T_PQ];
-  if (wave >= L.nicl) return;
-  const int ic = wave;
+  if (wave >= L.nwork) return;
+  const int4 wk = L.work[wave];
+  const int ic = wk.x;
   const int w = Q_SIDE ? d.qc_win[ic] : d.sc_win[ic];
   const WinState& st = d.ws[w];
   if (!st.active) return;
@@ -681,7 +683,7 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   double eel[kMaxS], evdw[kMaxS];
   for (int s = 0; s < kMaxS; ++s) { eel[s] = 0.0; evdw[s] = 0.0; }
-  const int s0 = L.start[ic], n = L.count[ic];
+  const int s0 = wk.y, n = wk.z;
   for (int t = 0; t < n; ++t) {
     const int jc = L.jc[s0 + t];
     const uint2 m = L.mask[s0 + t];
@@ -756,20 +758,8 @@ __global__ void k_qq(Dev d) {
   // parameters (double, as the reference)
   double qi = d.q_crg[W.qs_off + e.x * S + s];
   double qj = jq ? (double)d.q_crg[W.qs_off + (jq - 1) * S + s] : (double)d.x32[j].w;
-  double aLJ, bLJ;
-  const double* qli = &d.q_ljd[(size_t)(W.qs_off + e.x * S + s) * 4];
-  const int c = (iLJ == 3) ? 2 : 0;
-  if (!W.qvdw) {
-    const double* ti = &d.t_lj[(W.t_off + d.iac[i]) * 4];
-    const double* tj = &d.t_lj[(W.t_off + d.iac[j]) * 4];
-    aLJ = ti[c] * tj[c]; bLJ = ti[c + 1] * tj[c + 1];
-  } else if (jq) {
-    const double* qlj = &d.q_ljd[(size_t)(W.qs_off + (jq - 1) * S + s) * 4];
-    aLJ = qli[c] * qlj[c]; bLJ = qli[c + 1] * qlj[c + 1];
-  } else {
-    const double* tj = &d.t_lj[(W.t_off + d.iac[j]) * 4];
-    aLJ = qli[c] * tj[c]; bLJ = qli[c + 1] * tj[c + 1];
-  }
+  const double2 ab = d.qq_ab[k];
+  const double aLJ = ab.x, bLJ = ab.y;
   const double sc = d.qq_sc[k];
   const bool expo = W.qvdw && jq && iLJ == 2;
   // gradient in FP32 from the FP32 coordinates
@@ -963,197 +953,267 @@ __device__ __forceinline__ void torsion_geom(const double* x, int i, int j, int 
   *scp_out = scp;
 }
 
-__device__ __forceinline__ void apply4(long long* f, int i, int j, int k, int l, double dv, const double dp[12]) {
-  add_grad(f, i, dv * dp[0], dv * dp[1], dv * dp[2]);
-  add_grad(f, j, dv * dp[3], dv * dp[4], dv * dp[5]);
-  add_grad(f, k, dv * dp[6], dv * dp[7], dv * dp[8]);
-  add_grad(f, l, dv * dp[9], dv * dp[10], dv * dp[11]);
+// Bonded terms write the fixed-point gradient of each of their atoms into a
+// slot of their own (no atomics); k_bonded_gather then sums every atom's
+// slots in a fixed order.  Integer sums make the result independent of the
+// order, so it equals the atomic form bit for bit.
+__device__ __forceinline__ void put_slot(long long* sl, int k, double gx, double gy, double gz) {
+  sl[3 * k + 0] = fix(gx, kFScale);
+  sl[3 * k + 1] = fix(gy, kFScale);
+  sl[3 * k + 2] = fix(gz, kFScale);
 }
 
-__device__ void angle_term(const double* x, long long* f, int i, int j, int k, double fk, double ang0, double lam,
-                           double* e) {
-  // angle (md.f90:814-901) with gradient scaled by lam (1 for topology angles)
+__device__ __forceinline__ void put4(long long* sl, int k, double dv, const double dp[12]) {
+  put_slot(sl, k + 0, dv * dp[0], dv * dp[1], dv * dp[2]);
+  put_slot(sl, k + 1, dv * dp[3], dv * dp[4], dv * dp[5]);
+  put_slot(sl, k + 2, dv * dp[6], dv * dp[7], dv * dp[8]);
+  put_slot(sl, k + 3, dv * dp[9], dv * dp[10], dv * dp[11]);
+}
+
+// angle (md.f90:814-901): energy and the gradient factors di, dk of the end
+// atoms (the middle atom takes -(di + dk)); dv = fk * (theta - theta0)
+__device__ __forceinline__ void angle_geom(const double* x, int i, int j, int k, double fk, double ang0, double* e,
+                                           double* dv, double di[3], double dk[3]) {
   double rji[3], rjk[3];
   for (int c = 0; c < 3; ++c) { rji[c] = x[3 * i + c] - x[3 * j + c]; rjk[c] = x[3 * k + c] - x[3 * j + c]; }
-  double bji2inv = 1.0 / (rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
-  double bjk2inv = 1.0 / (rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
-  double bjiinv = sqrt(bji2inv), bjkinv = sqrt(bjk2inv);
+  const double bji2inv = 1.0 / (rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+  const double bjk2inv = 1.0 / (rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+  const double bjiinv = sqrt(bji2inv), bjkinv = sqrt(bjk2inv);
   double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]) * bjiinv * bjkinv;
   if (scp > 1.0) scp = 1.0;
   else if (scp < -1.0) scp = -1.0;
-  double angv = acos(scp);
-  double da = angv - ang0;
+  const double angv = acos(scp);
+  const double da = angv - ang0;
   *e = 0.5 * fk * da * da;
-  double dv = fk * da * lam;
+  *dv = fk * da;
   double f1 = sin(angv);
   if (fabs(f1) < 1.e-12) f1 = -1.e12;
   else f1 = -1.0 / f1;
-  double di[3], dk[3];
   for (int c = 0; c < 3; ++c) {
     di[c] = f1 * (rjk[c] * bjiinv * bjkinv - scp * rji[c] * bji2inv);
     dk[c] = f1 * (rji[c] * bjiinv * bjkinv - scp * rjk[c] * bjk2inv);
   }
-  add_grad(f, i, dv * di[0], dv * di[1], dv * di[2]);
-  add_grad(f, k, dv * dk[0], dv * dk[1], dv * dk[2]);
-  add_grad(f, j, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
 }
 
-__global__ void k_bonded(Dev d) {
-  const int n1 = d.nbond, n2 = n1 + d.nang, n3 = n2 + d.ntor, n4 = n3 + d.nimp;
-  const int n5 = n4 + d.nqb, n6 = n5 + d.nqa, n7 = n6 + d.nqt, n8 = n7 + d.nqi;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n8) return;
-  const double* x = d.x;
-  if (t < n1) {  // bond (md.f90:954-997)
-    const int w = d.b_win[t];
-    if (!d.ws[w].active) return;
-    int2 ij = d.b_ij[t];
-    double2 p = d.b_par[t];
-    double r[3];
-    for (int c = 0; c < 3; ++c) r[c] = x[3 * ij.y + c] - x[3 * ij.x + c];
-    double bl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-    double db = bl - p.y;
-    double dv = p.x * db / bl;
-    add_grad(d.f, ij.y, r[0] * dv, r[1] * dv, r[2] * dv);
-    add_grad(d.f, ij.x, -r[0] * dv, -r[1] * dv, -r[2] * dv);
-    if (d.ws[w].energy_now) add_energy(d.ws, w, d.b_solv[t] ? E_W_BOND : E_P_BOND, 0.5 * p.x * db * db);
-  } else if (t < n2) {  // angle (+ Urey-Bradley for CHARMM, md.f90:904-951)
-    const int a = t - n1;
-    const int w = d.a_win[a];
-    if (!d.ws[w].active) return;
-    int4 ijk = d.a_ijk[a];
-    double4 p = d.a_par[a];
-    double e;
-    angle_term(x, d.f, ijk.x, ijk.y, ijk.z, p.x, p.y, 1.0, &e);
-    if (d.urey[w] && p.z > 0.0) {
-      double r[3];
-      for (int c = 0; c < 3; ++c) r[c] = x[3 * ijk.z + c] - x[3 * ijk.x + c];
-      double dik = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-      double ru = dik - p.w;
-      e += p.z * ru * ru;
-      double du = 2 * p.z * ru / dik;
-      add_grad(d.f, ijk.z, du * r[0], du * r[1], du * r[2]);
-      add_grad(d.f, ijk.x, -du * r[0], -du * r[1], -du * r[2]);
-    }
-    if (d.ws[w].energy_now) add_energy(d.ws, w, d.a_solv[a] ? E_W_ANG : E_P_ANG, e);
-  } else if (t < n3) {  // torsion (md.f90:14793-14907)
-    const int a = t - n2;
-    const int w = d.t_win[a];
-    if (!d.ws[w].active) return;
-    int4 q = d.t_ijkl[a];
-    float4 p = d.t_par[a];  // fk, rmult, deltor, paths (1/paths)
-    double phi, scp, dp[12];
-    torsion_geom(x, q.x, q.y, q.z, q.w, &phi, &scp, dp, false);
-    double arg = (double)p.y * phi - (double)p.z;
-    double e = (double)p.x * (1.0 + cos(arg)) * (double)p.w;
-    double dv = -(double)p.y * (double)p.x * sin(arg) * (double)p.w;
-    apply4(d.f, q.x, q.y, q.z, q.w, dv, dp);
-    if (d.ws[w].energy_now) add_energy(d.ws, w, d.t_solv[a] ? E_W_TOR : E_P_TOR, e);
-  } else if (t < n4) {  // improper: harmonic (1677-1786) or periodic (1789-1897)
-    const int a = t - n3;
-    const int w = d.i_win[a];
-    if (!d.ws[w].active) return;
-    int4 q = d.i_ijkl[a];
-    float2 p = d.i_par[a];
-    double phi, scp, dp[12];
-    torsion_geom(x, q.x, q.y, q.z, q.w, &phi, &scp, dp, false);
-    double e, dv;
-    if (d.i_kind[w] == 2) {
-      double arg = 2 * phi - (double)p.y;
-      e = (double)p.x * (1 + cos(arg));
-      dv = -2 * (double)p.x * sin(arg);
-    } else {
-      const double pi = (double)(4.0f * atanf(1.0f));
-      double arg = phi - (double)p.y;
-      arg = arg - 2. * pi * round(arg / (2. * pi));
-      dv = (double)p.x * arg;
-      e = 0.5 * dv * arg;
-    }
-    apply4(d.f, q.x, q.y, q.z, q.w, dv, dp);
-    if (d.ws[w].energy_now) add_energy(d.ws, w, d.i_solv[a] ? E_W_IMP : E_P_IMP, e);
-  } else if (t < n5) {  // q-bond, Morse + harmonic (md.f90:14060-14114)
-    const int a = t - n4;
-    int2 ws2 = d.qb_ws[a];
-    const int w = ws2.x, s = ws2.y;
-    if (!d.ws[w].active) return;
-    const double lam = d.win[w].lambda[s];
-    int2 ij = d.qb_ij[a];
-    double4 p = d.qb_par[a];  // Dmz, amz, r0, fk
-    double r[3];
-    for (int c = 0; c < 3; ++c) r[c] = x[3 * ij.y + c] - x[3 * ij.x + c];
-    double bl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-    double db = bl - p.z;
-    double fexp = exp(-p.y * db);
-    double be = p.x * (fexp * fexp - 2. * fexp) + 0.5 * p.w * db * db;
-    double dv = (2. * p.x * p.y * (fexp - fexp * fexp) + p.w * db) * lam / bl;
-    add_grad(d.f, ij.x, -dv * r[0], -dv * r[1], -dv * r[2]);
-    add_grad(d.f, ij.y, dv * r[0], dv * r[1], dv * r[2]);
-    if (d.ws[w].energy_now) add_energy(d.ws, w, E_Q0 + 8 * s + 0, be);
-  } else if (t < n6) {  // q-angle (md.f90:13896-13988)
-    const int a = t - n5;
-    int2 ws2 = d.qa_ws[a];
-    const int w = ws2.x, s = ws2.y;
-    if (!d.ws[w].active) return;
-    int4 ijk = d.qa_ijk[a];
-    double2 p = d.qa_par[a];
-    const int i = ijk.x, j = ijk.y, k = ijk.z;
-    double rji[3], rjk[3];
-    for (int c = 0; c < 3; ++c) { rji[c] = x[3 * i + c] - x[3 * j + c]; rjk[c] = x[3 * k + c] - x[3 * j + c]; }
-    double bji = sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
-    double bjk = sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
-    double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]) / (bji * bjk);
-    if (scp > 1.0) scp = 1.0;
-    if (scp < -1.0) scp = -1.0;
-    double ang = acos(scp);
-    double da = ang - p.y;
-    double ae = 0.5 * p.x * da * da;
-    double dv = p.x * da * d.win[w].lambda[s];
-    double f1 = sin(ang);
-    if (fabs(f1) < 1.e-12) f1 = 1.e-12;
-    f1 = -1.0 / f1;
-    double di[3], dk[3];
-    for (int c = 0; c < 3; ++c) {
-      di[c] = f1 * (rjk[c] / (bji * bjk) - scp * rji[c] / (bji * bji));
-      dk[c] = f1 * (rji[c] / (bji * bjk) - scp * rjk[c] / (bjk * bjk));
-    }
-    add_grad(d.f, i, dv * di[0], dv * di[1], dv * di[2]);
-    add_grad(d.f, k, dv * dk[0], dv * dk[1], dv * dk[2]);
-    add_grad(d.f, j, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
-    if (d.ws[w].energy_now) add_energy(d.ws, w, E_Q0 + 8 * s + 1, ae);
-  } else {  // q-torsion (14262-14403) / q-improper (14117-14258)
-    const bool tors = t < n7;
-    const int a = tors ? t - n6 : t - n7;
-    int2 ws2 = tors ? d.qt_ws[a] : d.qi_ws[a];
-    const int w = ws2.x, s = ws2.y;
-    if (!d.ws[w].active) return;
-    int4 q = tors ? d.qt_ijkl[a] : d.qi_ijkl[a];
-    // the q-terms use bj, bk and divisions (same values as the topology form)
-    double phi, scp, dp[12];
-    torsion_geom(x, q.x, q.y, q.z, q.w, &phi, &scp, dp, true);
-    const double lam = d.win[w].lambda[s];
-    double pe, dv;
-    if (tors) {
-      double4 p = d.qt_par[a];  // fk, rmult, deltor
-      double arg = p.y * phi - p.z;
-      pe = p.x * (1.0 + cos(arg));
-      dv = -p.y * p.x * sin(arg) * lam;
-    } else {
-      const double pi = (double)(4.0f * atanf(1.0f));
-      double2 p = d.qi_par[a];  // fk, imp0
-      double arg = phi - p.y;
-      arg = arg - 2. * pi * round(arg / (2. * pi));
-      dv = p.x * arg;
-      pe = 0.5 * dv * arg;
-      dv = dv * lam;
-    }
-    apply4(d.f, q.x, q.y, q.z, q.w, dv, dp);
-    if (d.ws[w].energy_now) add_energy(d.ws, w, E_Q0 + 8 * s + (tors ? 2 : 3), pe);
+// energy of a term into slot `slot` of window w: lanes of a wave that agree
+// on (w, slot) are summed first and added with one atomic
+__device__ __forceinline__ void wave_energy(WinState* ws, int w, int slot, double e, bool valid) {
+  const unsigned long long vm = __ballot(valid);
+  if (vm == 0ull) return;
+  const int first = __ffsll((long long)vm) - 1;
+  const int key = valid ? w * E_NSLOT + slot : -1;
+  const int key0 = __shfl(key, first, 64);
+  const bool uniform = __ballot(valid && key != key0) == 0ull;
+  if (uniform) {
+    const double sum = wave_sum(valid ? e : 0.0);
+    if ((int)(threadIdx.x & 63) == first) add_energy(ws, w, slot, sum);
+  } else if (valid) {
+    add_energy(ws, w, slot, e);
   }
 }
 
-// ------------------------------------------------------ restraints --
-// fix_shell (md.f90:1414-1450) per atom and the radial water restraint
-// (restrain_solvent, md.f90:14910-14957) per water oxygen
+__global__ void __launch_bounds__(256) k_bonded(Dev d) {
+  const int n1 = d.nbond, n2 = n1 + d.nang, n3 = n2 + d.ntor, n4 = n3 + d.nimp;
+  const int n5 = n4 + d.nqb, n6 = n5 + d.nqa, n7 = n6 + d.nqt, n8 = n7 + d.nqi;
+  // first slot of each kind (2, 3, 4, 4, 2, 3, 4, 4 atoms per term)
+  const int b1 = 0, b2 = b1 + 2 * d.nbond, b3 = b2 + 3 * d.nang, b4 = b3 + 4 * d.ntor;
+  const int b5 = b4 + 4 * d.nimp, b6 = b5 + 2 * d.nqb, b7 = b6 + 3 * d.nqa, b8 = b7 + 4 * d.nqt;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const double* x = d.x;
+  long long* sl = d.bslot;
+  bool valid = false;
+  int w = 0, eslot = 0;
+  double e = 0.0;
+  if (t < n1) {  // bond (md.f90:954-997)
+    w = d.b_win[t];
+    if (d.ws[w].active) {
+      const int2 ij = d.b_ij[t];
+      const double2 p = d.b_par[t];
+      double r[3];
+      for (int c = 0; c < 3; ++c) r[c] = x[3 * ij.y + c] - x[3 * ij.x + c];
+      const double bl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      const double db = bl - p.y;
+      const double dv = p.x * db / bl;
+      put_slot(sl, b1 + 2 * t, -r[0] * dv, -r[1] * dv, -r[2] * dv);
+      put_slot(sl, b1 + 2 * t + 1, r[0] * dv, r[1] * dv, r[2] * dv);
+      valid = true;
+      e = 0.5 * p.x * db * db;
+      eslot = d.b_solv[t] ? E_W_BOND : E_P_BOND;
+    }
+  } else if (t < n2) {  // angle (+ Urey-Bradley for CHARMM, md.f90:904-951)
+    const int a = t - n1;
+    w = d.a_win[a];
+    if (d.ws[w].active) {
+      const int4 ijk = d.a_ijk[a];
+      const double4 p = d.a_par[a];
+      double dv, di[3], dk[3];
+      angle_geom(x, ijk.x, ijk.y, ijk.z, p.x, p.y, &e, &dv, di, dk);
+      double gi[3], gk[3];
+      for (int c = 0; c < 3; ++c) { gi[c] = dv * di[c]; gk[c] = dv * dk[c]; }
+      double gj[3] = {-dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2])};
+      if (d.urey[w] && p.z > 0.0) {
+        double r[3];
+        for (int c = 0; c < 3; ++c) r[c] = x[3 * ijk.z + c] - x[3 * ijk.x + c];
+        const double dik = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const double ru = dik - p.w;
+        e += p.z * ru * ru;
+        const double du = 2 * p.z * ru / dik;
+        for (int c = 0; c < 3; ++c) { gk[c] += du * r[c]; gi[c] -= du * r[c]; }
+      }
+      put_slot(sl, b2 + 3 * a, gi[0], gi[1], gi[2]);
+      put_slot(sl, b2 + 3 * a + 1, gj[0], gj[1], gj[2]);
+      put_slot(sl, b2 + 3 * a + 2, gk[0], gk[1], gk[2]);
+      valid = true;
+      eslot = d.a_solv[a] ? E_W_ANG : E_P_ANG;
+    }
+  } else if (t < n3) {  // torsion (md.f90:14793-14907)
+    const int a = t - n2;
+    w = d.t_win[a];
+    if (d.ws[w].active) {
+      const int4 q = d.t_ijkl[a];
+      const float4 p = d.t_par[a];  // fk, rmult, deltor, paths (1/paths)
+      double phi, scp, dp[12];
+      torsion_geom(x, q.x, q.y, q.z, q.w, &phi, &scp, dp, false);
+      const double arg = (double)p.y * phi - (double)p.z;
+      e = (double)p.x * (1.0 + cos(arg)) * (double)p.w;
+      const double dv = -(double)p.y * (double)p.x * sin(arg) * (double)p.w;
+      put4(sl, b3 + 4 * a, dv, dp);
+      valid = true;
+      eslot = d.t_solv[a] ? E_W_TOR : E_P_TOR;
+    }
+  } else if (t < n4) {  // improper: harmonic (1677-1786) or periodic (1789-1897)
+    const int a = t - n3;
+    w = d.i_win[a];
+    if (d.ws[w].active) {
+      const int4 q = d.i_ijkl[a];
+      const float2 p = d.i_par[a];
+      double phi, scp, dp[12];
+      torsion_geom(x, q.x, q.y, q.z, q.w, &phi, &scp, dp, false);
+      double dv;
+      if (d.i_kind[w] == 2) {
+        const double arg = 2 * phi - (double)p.y;
+        e = (double)p.x * (1 + cos(arg));
+        dv = -2 * (double)p.x * sin(arg);
+      } else {
+        const double pi = (double)(4.0f * atanf(1.0f));
+        double arg = phi - (double)p.y;
+        arg = arg - 2. * pi * round(arg / (2. * pi));
+        dv = (double)p.x * arg;
+        e = 0.5 * dv * arg;
+      }
+      put4(sl, b4 + 4 * a, dv, dp);
+      valid = true;
+      eslot = d.i_solv[a] ? E_W_IMP : E_P_IMP;
+    }
+  } else if (t < n5) {  // q-bond, Morse + harmonic (md.f90:14060-14114)
+    const int a = t - n4;
+    const int2 ws2 = d.qb_ws[a];
+    w = ws2.x;
+    const int s = ws2.y;
+    if (d.ws[w].active) {
+      const double lam = d.win[w].lambda[s];
+      const int2 ij = d.qb_ij[a];
+      const double4 p = d.qb_par[a];  // Dmz, amz, r0, fk
+      double r[3];
+      for (int c = 0; c < 3; ++c) r[c] = x[3 * ij.y + c] - x[3 * ij.x + c];
+      const double bl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      const double db = bl - p.z;
+      const double fexp = exp(-p.y * db);
+      e = p.x * (fexp * fexp - 2. * fexp) + 0.5 * p.w * db * db;
+      const double dv = (2. * p.x * p.y * (fexp - fexp * fexp) + p.w * db) * lam / bl;
+      put_slot(sl, b5 + 2 * a, -dv * r[0], -dv * r[1], -dv * r[2]);
+      put_slot(sl, b5 + 2 * a + 1, dv * r[0], dv * r[1], dv * r[2]);
+      valid = true;
+      eslot = E_Q0 + 8 * s + 0;
+    }
+  } else if (t < n6) {  // q-angle (md.f90:13896-13988)
+    const int a = t - n5;
+    const int2 ws2 = d.qa_ws[a];
+    w = ws2.x;
+    const int s = ws2.y;
+    if (d.ws[w].active) {
+      const int4 ijk = d.qa_ijk[a];
+      const double2 p = d.qa_par[a];
+      const int i = ijk.x, j = ijk.y, k = ijk.z;
+      double rji[3], rjk[3];
+      for (int c = 0; c < 3; ++c) { rji[c] = x[3 * i + c] - x[3 * j + c]; rjk[c] = x[3 * k + c] - x[3 * j + c]; }
+      const double bji = sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+      const double bjk = sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+      double scp = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]) / (bji * bjk);
+      if (scp > 1.0) scp = 1.0;
+      if (scp < -1.0) scp = -1.0;
+      const double ang = acos(scp);
+      const double da = ang - p.y;
+      e = 0.5 * p.x * da * da;
+      const double dv = p.x * da * d.win[w].lambda[s];
+      double f1 = sin(ang);
+      if (fabs(f1) < 1.e-12) f1 = 1.e-12;
+      f1 = -1.0 / f1;
+      double di[3], dk[3];
+      for (int c = 0; c < 3; ++c) {
+        di[c] = f1 * (rjk[c] / (bji * bjk) - scp * rji[c] / (bji * bji));
+        dk[c] = f1 * (rji[c] / (bji * bjk) - scp * rjk[c] / (bjk * bjk));
+      }
+      put_slot(sl, b6 + 3 * a, dv * di[0], dv * di[1], dv * di[2]);
+      put_slot(sl, b6 + 3 * a + 1, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
+      put_slot(sl, b6 + 3 * a + 2, dv * dk[0], dv * dk[1], dv * dk[2]);
+      valid = true;
+      eslot = E_Q0 + 8 * s + 1;
+    }
+  } else if (t < n8) {  // q-torsion (14262-14403) / q-improper (14117-14258)
+    const bool tors = t < n7;
+    const int a = tors ? t - n6 : t - n7;
+    const int2 ws2 = tors ? d.qt_ws[a] : d.qi_ws[a];
+    w = ws2.x;
+    const int s = ws2.y;
+    if (d.ws[w].active) {
+      const int4 q = tors ? d.qt_ijkl[a] : d.qi_ijkl[a];
+      double phi, scp, dp[12];
+      torsion_geom(x, q.x, q.y, q.z, q.w, &phi, &scp, dp, true);
+      const double lam = d.win[w].lambda[s];
+      double dv;
+      if (tors) {
+        const double4 p = d.qt_par[a];  // fk, rmult, deltor
+        const double arg = p.y * phi - p.z;
+        e = p.x * (1.0 + cos(arg));
+        dv = -p.y * p.x * sin(arg) * lam;
+      } else {
+        const double pi = (double)(4.0f * atanf(1.0f));
+        const double2 p = d.qi_par[a];  // fk, imp0
+        double arg = phi - p.y;
+        arg = arg - 2. * pi * round(arg / (2. * pi));
+        dv = p.x * arg;
+        e = 0.5 * dv * arg;
+        dv = dv * lam;
+      }
+      put4(sl, tors ? b7 + 4 * a : b8 + 4 * a, dv, dp);
+      valid = true;
+      eslot = E_Q0 + 8 * s + (tors ? 2 : 3);
+    }
+  }
+  const bool en = valid && d.ws[w].energy_now;
+  wave_energy(d.ws, w, eslot, e, en);
+}
+
+// sum of each bonded atom's slots (CSR in slot order) into the gradient; one
+// thread per atom, so the add needs no atomic
+__global__ void __launch_bounds__(256) k_bonded_gather(Dev d) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nbg) return;
+  const int atom = d.bg_atom[k];
+  if (!d.ws[d.wid[atom]].active) return;
+  long long gx = 0, gy = 0, gz = 0;
+  for (int m = d.bg_start[k]; m < d.bg_start[k + 1]; ++m) {
+    const int s = d.bg_slot[m];
+    gx += d.bslot[3 * s];
+    gy += d.bslot[3 * s + 1];
+    gz += d.bslot[3 * s + 2];
+  }
+  d.f[3 * atom] += gx;
+  d.f[3 * atom + 1] += gy;
+  d.f[3 * atom + 2] += gz;
+}
+
 __global__ void k_restr(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.ntot) return;
@@ -1333,82 +1393,115 @@ __global__ void __launch_bounds__(64) k_prestr(Dev d) {
   }
 }
 
-// water polarisation restraint (watpol, md.f90:15122-15288): one block per
-// window; the per-shell selection sort of the reference becomes a rank
-// count (theta ascending, ties by water index, as the first-minimum rule)
-__global__ void __launch_bounds__(256) k_watpol(Dev d) {
-  const int w = blockIdx.x;
+// water polarisation restraint (watpol, md.f90:15122-15288) in two passes.
+// Pass 1, one thread per water: the angle theta between the dipole and the
+// radial vector, and the shell the oxygen lies in.
+__global__ void __launch_bounds__(256) k_watpol_theta(Dev d) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.nwat_total) return;
+  const int w = d.wat_win[g];
+  const WinDev& W = d.win[w];
+  d.wshell[g] = -1;
+  if (!d.ws[w].active || !W.wpol || W.nshell == 0) return;
+  const ShellDev* sh = d.shells + W.sh_off;
+  const int nsh = W.nshell;
+  const int iw = g - d.w_off[w];
+  const int i = W.ao + W.nat_solute + 3 * iw;
+  d.wtheta[g] = 0.0;
+  if (d.aflag[i] & 1) return;
+  const double* x = d.x;
+  double rmu[3], rcu[3];
+  for (int c = 0; c < 3; ++c) rmu[c] = x[3 * (i + 1) + c] + x[3 * (i + 2) + c] - 2. * x[3 * i + c];
+  const double rm = sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
+  for (int c = 0; c < 3; ++c) rmu[c] /= rm;
+  for (int c = 0; c < 3; ++c) rcu[c] = x[3 * i + c] - W.xwcent[c];
+  const double rc = sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
+  for (int c = 0; c < 3; ++c) rcu[c] /= rc;
+  double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+  scp = fmin(1.0, fmax(-1.0, scp));
+  d.wtheta[g] = acos(scp);
+  // shell bounds are real(4): rout - dr is a single-precision difference
+  if (rc > (double)(sh[nsh - 1].rout - sh[nsh - 1].dr)) {
+    int is = nsh - 1;
+    for (; is >= 1; --is)
+      if (rc <= (double)sh[is].rout) break;
+    d.wshell[g] = is;
+  }
+}
+
+// Pass 2, one block per (window, shell): the shell's waters are gathered in
+// LDS with their theta; a water's place in the theta order (ascending, ties
+// by water index: the first-minimum rule of the reference's selection sort)
+// is counted against the LDS copy, which gives its target angle theta0.
+constexpr int kWpolMaxShell = 8;
+constexpr int kWpolLds = 3072;       // waters of one shell held in LDS
+__global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
+  const int w = blockIdx.x / kWpolMaxShell, is = blockIdx.x % kWpolMaxShell;
   if (w >= d.nwin) return;
   const WinState& st = d.ws[w];
   const WinDev& W = d.win[w];
-  if (!st.active || !W.wpol || W.nwat == 0 || W.nshell == 0) return;
-  ShellDev* sh = d.shells + W.sh_off;
-  const int nsh = W.nshell;
+  if (!st.active || !W.wpol || W.nwat == 0 || is >= W.nshell) return;
+  ShellDev& sh = d.shells[W.sh_off + is];
   const double pi = (double)(4.0f * atanf(1.0f));
   const int wo = d.w_off[w];
-  __shared__ int cnt[8];
-  __shared__ double avt[8][4];
+  __shared__ double s_th[kWpolLds];
+  __shared__ int s_iw[kWpolLds];
+  __shared__ int s_n;
+  __shared__ double s_red[4][2];
+  // theta_corr follows the shell's mean theta every 100 steps
   if (threadIdx.x == 0) {
     if (st.istep != 0 && st.istep % 100 == 0) {
-      for (int is = 0; is < nsh; ++is) {
-        sh[is].avtheta = sh[is].avtheta / 100.0f;
-        sh[is].avn_insh = sh[is].avn_insh / 100.0f;
-        sh[is].theta_corr = sh[is].theta_corr + sh[is].avtheta - acosf(sh[is].cstb);
-        sh[is].avtheta = 0.0f;
-        sh[is].avn_insh = 0.0f;
+      sh.avtheta = sh.avtheta / 100.0f;
+      sh.avn_insh = sh.avn_insh / 100.0f;
+      sh.theta_corr = sh.theta_corr + sh.avtheta - acosf(sh.cstb);
+      sh.avtheta = 0.0f;
+      sh.avn_insh = 0.0f;
+    }
+    s_n = 0;
+  }
+  __syncthreads();
+  for (int iw = threadIdx.x; iw < W.nwat; iw += blockDim.x) {
+    if (d.wshell[wo + iw] != is) continue;
+    const int k = atomicAdd(&s_n, 1);
+    if (k < kWpolLds) { s_th[k] = d.wtheta[wo + iw]; s_iw[k] = iw; }
+  }
+  __syncthreads();
+  const int n = s_n;
+  const bool in_lds = n <= kWpolLds;
+  const double cstb = (double)sh.cstb, tc = (double)sh.theta_corr;
+  double eacc = 0.0, tsum = 0.0;
+  // members: from LDS, or (shell larger than LDS) straight from the window's waters
+  const int nloop = in_lds ? n : W.nwat;
+  for (int k = threadIdx.x; k < nloop; k += blockDim.x) {
+    int iw;
+    double th;
+    if (in_lds) {
+      iw = s_iw[k];
+      th = s_th[k];
+    } else {
+      iw = k;
+      if (d.wshell[wo + iw] != is) continue;
+      th = d.wtheta[wo + iw];
+    }
+    int rank = 0;
+    if (in_lds) {
+      for (int m = 0; m < n; ++m) {
+        const double tj = s_th[m];
+        rank += (tj < th || (tj == th && s_iw[m] < iw)) ? 1 : 0;
+      }
+    } else {
+      for (int jw = 0; jw < W.nwat; ++jw) {
+        if (d.wshell[wo + jw] != is) continue;
+        const double tj = d.wtheta[wo + jw];
+        rank += (tj < th || (tj == th && jw < iw)) ? 1 : 0;
       }
     }
-    for (int is = 0; is < 8; ++is) cnt[is] = 0;
-  }
-  __syncthreads();
-  for (int iw = threadIdx.x; iw < W.nwat; iw += blockDim.x) {
-    const int i = W.ao + W.nat_solute + 3 * iw;
-    d.wshell[wo + iw] = -1;
-    d.wtheta[wo + iw] = 0.0;
-    if (d.aflag[i] & 1) continue;
-    const double* x = d.x;
-    double rmu[3], rcu[3];
-    for (int c = 0; c < 3; ++c) rmu[c] = x[3 * (i + 1) + c] + x[3 * (i + 2) + c] - 2. * x[3 * i + c];
-    double rm = sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
-    for (int c = 0; c < 3; ++c) rmu[c] /= rm;
-    for (int c = 0; c < 3; ++c) rcu[c] = x[3 * i + c] - W.xwcent[c];
-    double rc = sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
-    for (int c = 0; c < 3; ++c) rcu[c] /= rc;
-    double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
-    if (scp > 1.0) scp = 1.0;
-    if (scp < -1.0) scp = -1.0;
-    d.wtheta[wo + iw] = acos(scp);
-    if (rc > (double)sh[nsh - 1].rout - (double)sh[nsh - 1].dr) {
-      int is = nsh - 1;
-      for (; is >= 1; --is)
-        if (rc <= (double)sh[is].rout) break;
-      d.wshell[wo + iw] = is;
-      atomicAdd(&cnt[is], 1);
-    }
-  }
-  __syncthreads();
-  double eacc = 0.0;
-  double tsum[8];
-  for (int k = 0; k < 8; ++k) tsum[k] = 0.0;
-  for (int iw = threadIdx.x; iw < W.nwat; iw += blockDim.x) {
-    const int is = d.wshell[wo + iw];
-    if (is < 0) continue;
-    const double th = d.wtheta[wo + iw];
-    int rank = 0;
-    for (int jw = 0; jw < W.nwat; ++jw) {
-      if (d.wshell[wo + jw] != is) continue;
-      double tj = d.wtheta[wo + jw];
-      if (tj < th || (tj == th && jw < iw)) ++rank;
-    }
     const int il = rank + 1;
-    const int n = cnt[is];
-    float argf = 1.0f + (1.0f - 2.0f * (float)il) / (float)n;
+    const float argf = 1.0f + (1.0f - 2.0f * (float)il) / (float)n;
     double th0 = acos((double)argf);
-    th0 = th0 - 3. * sin(th0) * (double)sh[is].cstb / 2.;
-    if (th0 < 0.0) th0 = 0.0;
-    if (th0 > pi) th0 = pi;
-    tsum[is] += th;
-    const double tc = (double)sh[is].theta_corr;
+    th0 = th0 - 3. * sin(th0) * cstb / 2.;
+    th0 = fmin(fmax(th0, 0.0), pi);
+    tsum += th;
     const double dth = th - th0 + tc;
     eacc += 0.5 * W.fkwpol * dth * dth;
     const double dv = W.fkwpol * dth;
@@ -1416,18 +1509,16 @@ __global__ void __launch_bounds__(256) k_watpol(Dev d) {
     const double* x = d.x;
     double rmu[3], rcu[3];
     for (int c = 0; c < 3; ++c) rmu[c] = x[3 * (i + 1) + c] + x[3 * (i + 2) + c] - 2. * x[3 * i + c];
-    double rm = sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
+    const double rm = sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
     for (int c = 0; c < 3; ++c) rmu[c] /= rm;
     for (int c = 0; c < 3; ++c) rcu[c] = x[3 * i + c] - W.xwcent[c];
-    double rc = sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
+    const double rc = sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
     for (int c = 0; c < 3; ++c) rcu[c] /= rc;
     double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
-    if (scp > 1.0) scp = 1.0;
-    if (scp < -1.0) scp = -1.0;
+    scp = fmin(1.0, fmax(-1.0, scp));
     double f0 = sin(acos(scp));
     if (fabs(f0) < 1.e-12) f0 = 1.e-12;
-    f0 = -1.0 / f0;
-    f0 = dv * f0;
+    f0 = dv * (-1.0 / f0);
     double f1[3], f2[3];
     for (int c = 0; c < 3; ++c) {
       f1[c] = (rcu[c] - rmu[c] * scp) / rm;
@@ -1437,30 +1528,20 @@ __global__ void __launch_bounds__(256) k_watpol(Dev d) {
     add_grad(d.f, i + 1, f0 * f1[0], f0 * f1[1], f0 * f1[2]);
     add_grad(d.f, i + 2, f0 * f1[0], f0 * f1[1], f0 * f1[2]);
   }
-  // per-shell theta sums (deterministic: fixed per-thread partition + tree)
+  // block sums in a fixed order (waves, then the 4 wave results)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int is = 0; is < nsh; ++is) {
-    double v = wave_sum(tsum[is]);
-    if (lane == 0) avt[is][wv] = v;
-  }
-  double esum = wave_sum(eacc);
-  __shared__ double ered[4];
-  if (lane == 0) ered[wv] = esum;
+  const double t_w = wave_sum(tsum), e_w = wave_sum(eacc);
+  if (lane == 0) { s_red[wv][0] = t_w; s_red[wv][1] = e_w; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int is = 0; is < nsh; ++is) {
-      if (cnt[is] == 0) continue;
-      double avtdum = 0.0;
-      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) avtdum += avt[is][k];
-      sh[is].avtheta = sh[is].avtheta + (float)(avtdum / (double)(float)cnt[is]);
-      sh[is].avn_insh = sh[is].avn_insh + (float)cnt[is];
-      sh[is].n_insh = cnt[is];
+    if (n > 0) {
+      const double avtdum = (s_red[0][0] + s_red[1][0]) + (s_red[2][0] + s_red[3][0]);
+      // real(4) + real(8) quotient, rounded once (md.f90:15285)
+      sh.avtheta = (float)((double)sh.avtheta + avtdum / (double)(float)n);
+      sh.avn_insh = sh.avn_insh + (float)n;
     }
-    if (st.energy_now) {
-      double e = 0.0;
-      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) e += ered[k];
-      add_energy(d.ws, w, E_WPOL, e);
-    }
+    sh.n_insh = n;
+    if (st.energy_now) add_energy(d.ws, w, E_WPOL, (s_red[0][1] + s_red[1][1]) + (s_red[2][1] + s_red[3][1]));
   }
 }
 
@@ -1624,25 +1705,35 @@ void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
   if (ngrp) k_lrf_build<<<ngrp, 256, 0, s>>>(d, ngrp);
 }
 
-void launch_nonbonded(const Dev& d, hipStream_t s) {
+void launch_ww(const Dev& d, hipStream_t s) {
   if (d.tl[T_WW].nicl) k_ww<<<blocks((long long)d.tl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
+}
+
+void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww) {
+  if (with_ww) launch_ww(d, s);
   if (d.tl[T_WP].nicl) k_pw<true><<<blocks((long long)d.tl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PW].nicl) k_pw<false><<<blocks((long long)d.tl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PP].nicl) k_pp<<<blocks((long long)d.tl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_QW].nicl) k_qw<true><<<blocks((long long)d.tl[T_QW].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_WQ].nicl) k_qw<false><<<blocks((long long)d.tl[T_WQ].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_QP].nicl) k_qp<true><<<blocks((long long)d.tl[T_QP].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_PQ].nicl) k_qp<false><<<blocks((long long)d.tl[T_PQ].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_QW].nwork) k_qw<true><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_WQ].nwork) k_qw<false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_PQ].nwork) k_qp<false><<<blocks((long long)d.tl[T_PQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
 }
 
 void launch_other_forces(const Dev& d, hipStream_t s) {
   k_lrf_taylor<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
-  if (nb) k_bonded<<<blocks(nb, 128), 128, 0, s>>>(d);
+  if (nb) {
+    k_bonded<<<blocks(nb, 256), 256, 0, s>>>(d);
+    k_bonded_gather<<<blocks(d.nbg, 256), 256, 0, s>>>(d);
+  }
   k_restr<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   if (d.nrst) k_prestr<<<d.nrst, 64, 0, s>>>(d);
-  k_watpol<<<d.nwin, 256, 0, s>>>(d);
+  if (d.nwat_total) {
+    k_watpol_theta<<<blocks(d.nwat_total, 256), 256, 0, s>>>(d);
+    k_watpol_shell<<<d.nwin * kWpolMaxShell, 256, 0, s>>>(d);
+  }
 }
 
 void launch_integrate(const Dev& d, hipStream_t s, int shake_any) {

@@ -104,6 +104,9 @@ bool prepare_window(Window* w, long long seed_override, Error* err) {
     w->ljcod[(pr.second - 1) * t.natyps + (pr.first - 1)] = 2;
   }
 
+  if (p.itrj_cycle > 0)
+    w->log += "NOTE: trajectory output (" + p.trj_file + ") is not written by this engine.\n";
+
   // --- prep_coord(): residue:atom restraint atoms (md.f90:13609-13623,
   // get_atom_from_resnum_atnum 16059-16095; the last residue ends at the
   // last solute atom), then the restrained shell from topology coordinates

@@ -57,6 +57,7 @@ def load_library(path: str = LIB_PATH):
         "qmi_engine_create": (i32, [i32, C.POINTER(p)]),
         "qmi_window_add": (i32, [p, C.c_char_p, i64, C.POINTER(i32)]),
         "qmi_prepare": (i32, [p]),
+        "qmi_window_set_log": (i32, [p, i32, C.c_char_p]),
         "qmi_run": (i32, [p, i64, i32]),
         "qmi_evaluate": (i32, [p, i32]),
         "qmi_get_energies": (i32, [p, i32, dp, i32]),
@@ -83,7 +84,8 @@ def load_library(path: str = LIB_PATH):
 
 
 EXPORTED = [
-    "qmi_abi_version", "qmi_engine_create", "qmi_window_add", "qmi_prepare", "qmi_run", "qmi_evaluate",
+    "qmi_abi_version", "qmi_engine_create", "qmi_window_add", "qmi_window_set_log", "qmi_prepare", "qmi_run",
+    "qmi_evaluate",
     "qmi_get_energies", "qmi_get_q_energies", "qmi_get_coords", "qmi_get_velocities", "qmi_get_forces",
     "qmi_set_state", "qmi_get_pairlist", "qmi_window_info", "qmi_write_restart", "qmi_timings",
     "qmi_set_timing", "qmi_last_error", "qmi_engine_destroy",
@@ -122,6 +124,10 @@ class Engine:
         _check(self.lib.qmi_window_add(self._h, os.fspath(inp_path).encode(), seed, C.byref(wid)))
         self.windows.append(inp_path)
         return wid.value
+
+    def set_log(self, w: int, path):
+        """Log destination of window w: a path, "-" for stdout, None for none."""
+        _check(self.lib.qmi_window_set_log(self._h, w, None if path is None else os.fspath(path).encode()))
 
     def prepare(self):
         _check(self.lib.qmi_prepare(self._h))
