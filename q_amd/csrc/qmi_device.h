@@ -57,6 +57,7 @@ struct WinState {
   double temp, tfree, tfree_solute, tfree_solvent, temp_solute, temp_solvent, ekin;
   double ekin_parts[4];  // solute all, solute free, solvent all, solvent free
   int shake_fail;
+  int list_overflow;  // a neighbour row exceeded its capacity
   long long E[E_NSLOT];
 };
 
@@ -81,6 +82,18 @@ struct TileList {
 };
 constexpr int kTileChunk = 16;
 
+// Neighbour rows of the dynamic pair lists (WW, W<-P, P<-W, P<-P): row
+// ic*8 + a belongs to member a of i-cluster ic and holds the j entries of
+// that member (water: O atom; solute: atom; P<-P entries carry the 1-4 flag
+// in bit 30), in a fixed capacity `cap` per row.
+struct RowList {
+  int* cnt;        // entries per row
+  int* ent;        // [row * cap + k]
+  int cap;
+  int nicl;        // i-clusters; rows = 8 * nicl
+};
+constexpr int kRow14 = 1 << 30;
+
 struct Dev {
   int nwin, ntot;
   const WinDev* win;
@@ -104,6 +117,9 @@ struct Dev {
   const int* g_win;
   double* g_cent;       // 3 per group
   double* g_lrf;        // 20 per group: phi0, phi1(3), phi2(6: xx xy xz yy yz zz), phi3(10)
+  int ngrp_all;         // groups of all windows
+  int nlrfw;            // LRF build work items (window, first target group, count)
+  const int4* lrf_work;
   // special partners of solute atoms (exclusions / 1-4), CSR
   const int* sp_start;  // per atom (ntot+1)
   const int* sp_j;
@@ -124,8 +140,9 @@ struct Dev {
   const double* q_scd;
   // topology LJ params in double per window type (energy path)
   const double* t_lj;    // [(t_off + t)*4]
-  // tiles
+  // tiles (static Q lists) and neighbour rows (dynamic lists, index T_WW..T_PP)
   TileList tl[T_NTYPES];
+  RowList rl[4];
   // q-q pairs (static)
   int nqq;
   const int4* qq_ij;     // iq (window-local), j (global), jq (1-based q or 0), ljcod
@@ -148,11 +165,14 @@ struct Dev {
   const int4* qa_ijk; const double2* qa_par; const int2* qa_ws;
   const int4* qt_ijkl; const double4* qt_par; const int2* qt_ws; // fk, rmult, deltor
   const int4* qi_ijkl; const double2* qi_par; const int2* qi_ws; // fk, imp0
-  // per-term gradient slots of the bonded terms (3 int64 each) and, per
-  // atom with bonded terms, its slots in CSR form
+  // per-term gradient slots of the bonded terms and q-q pairs (3 int64
+  // each), grouped by atom: atom bg_atom[k] owns slots bg_start[k] ..
+  // bg_start[k+1]-1
   long long* bslot;
+  int qq_slot0;          // first slot of the q-q pairs (2 per pair, after the bonded terms)
   int nbg;
-  const int* bg_atom; const int* bg_start; const int* bg_slot;
+  const int* bg_atom; const int* bg_start;
+  const int* slot_map;   // term atom (in term order) -> slot; an atom's slots are contiguous
   // restraints: (kind, window, i, j) global atoms; (k, flag) with flag = ih
   // (sequence, wall) or ipsi (position, distance, angle); 8 parameters each
   int nrst;
@@ -161,6 +181,8 @@ struct Dev {
   int nsu;
   const int* su_start; const int* su_n; const int* su_win;
   const int* sc_i; const int* sc_j; const double* sc_d2;
+  const int* su_atoms; const int* su_natom; const int* su_atom_list;   // atoms of each unit
+  const uint8_t* sc_loc;   // per constraint: local atom numbers of i and j within the unit
   // water shells
   ShellDev* shells;
   double* wtheta;        // per water (global water index), theta

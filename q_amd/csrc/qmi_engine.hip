@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -76,6 +77,10 @@ struct qmi_engine {
   std::vector<double> timings;
   bool timing_on = false;
   hipEvent_t ev[8] = {};
+  // the launch sequence of one MD step, captured once and replayed
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
+  bool graph_failed = false;
   ~qmi_engine();
 };
 
@@ -101,6 +106,8 @@ static int dev_alloc(qmi_engine* e, T** dst, size_t n) {
 }
 
 qmi_engine::~qmi_engine() {
+  if (step_exec) (void)hipGraphExecDestroy(step_exec);
+  if (step_graph) (void)hipGraphDestroy(step_graph);
   for (auto& x : ev) if (x) (void)hipEventDestroy(x);
   for (void* p : bufs) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
@@ -165,7 +172,8 @@ static int build_device(qmi_engine* e) {
   // restraints
   std::vector<int4> rs_kwij; std::vector<int2> rs_kf; std::vector<double> rs_par;
   // shake
-  std::vector<int> su_start, su_n, su_win, sc_i, sc_j;
+  std::vector<int> su_start, su_n, su_win, sc_i, sc_j, su_atoms, su_natom, su_atom_list;
+  std::vector<uint8_t> sc_loc;
   std::vector<double> sc_d2;
   // shells
   std::vector<ShellDev> shells;
@@ -567,8 +575,23 @@ static int build_device(qmi_engine* e) {
       for (int r : roots) {
         su_start.push_back((int)sc_i.size());
         int cnt = 0;
+        std::vector<int> atoms;   // the unit's atoms in first-use order
+        auto local = [&](int at) {
+          for (size_t k = 0; k < atoms.size(); ++k) if (atoms[k] == at) return (int)k;
+          atoms.push_back(at);
+          return (int)atoms.size() - 1;
+        };
         for (int a = 0; a < nc; ++a)
-          if (find(a) == r) { sc_i.push_back(ao + m.i[a]); sc_j.push_back(ao + m.j[a]); sc_d2.push_back(m.dist2[a]); ++cnt; }
+          if (find(a) == r) {
+            sc_i.push_back(ao + m.i[a]); sc_j.push_back(ao + m.j[a]); sc_d2.push_back(m.dist2[a]); ++cnt;
+            const int li = local(m.i[a]), lj = local(m.j[a]);
+            sc_loc.push_back((uint8_t)(li < 255 ? li : 255));
+            sc_loc.push_back((uint8_t)(lj < 255 ? lj : 255));
+          }
+        // units of up to kShakeRegAtoms atoms are iterated in registers
+        su_atoms.push_back((int)su_atom_list.size());
+        su_natom.push_back((int)atoms.size());
+        for (int at : atoms) su_atom_list.push_back(ao + at);
         if (cnt > 64) return set_err(-10, "a SHAKE component with more than 64 coupled constraints is not supported");
         su_n.push_back(cnt);
         su_win.push_back(wi);
@@ -638,6 +661,18 @@ static int build_device(qmi_engine* e) {
   UP(ip, g_win); d.g_win = ip;
   if ((rc = dev_alloc(e, &d.g_cent, 3 * (size_t)goff)) != 0) return rc;
   if ((rc = dev_alloc(e, &d.g_lrf, 20 * (size_t)goff)) != 0) return rc;
+  d.ngrp_all = goff;
+  {
+    // LRF build work: 16 target groups of one window per wave
+    std::vector<int4> lw;
+    for (int wi = 0; wi < NW; ++wi)
+      for (int g0 = 0; g0 < wd[wi].ngrp; g0 += 16)
+        lw.push_back(make_int4(wi, wd[wi].g_off + g0, std::min(16, wd[wi].ngrp - g0), 0));
+    d.nlrfw = (int)lw.size();
+    int4* lwp;
+    UP(lwp, lw);
+    d.lrf_work = lwp;
+  }
   UP(ip, sp_start); d.sp_start = ip;
   UP(ip, sp_j); d.sp_j = ip;
   UP(up, sp_t); d.sp_t = up;
@@ -691,22 +726,32 @@ static int build_device(qmi_engine* e) {
     for (auto& v : qa_ijk) { slot_atom.push_back(v.x); slot_atom.push_back(v.y); slot_atom.push_back(v.z); }
     for (auto& v : qt_ijkl) for (int a : {v.x, v.y, v.z, v.w}) slot_atom.push_back(a);
     for (auto& v : qi_ijkl) for (int a : {v.x, v.y, v.z, v.w}) slot_atom.push_back(a);
+    // q-q pairs: q-atom i, partner j
+    d.qq_slot0 = (int)slot_atom.size();
+    for (size_t k = 0; k < qq_ij.size(); ++k) {
+      const int wq = qq_win[k];
+      slot_atom.push_back(q_atom[wd[wq].q_off + qq_ij[k].x]);
+      slot_atom.push_back(qq_ij[k].y);
+    }
     std::vector<int> per_atom(ao, 0);
     for (int a : slot_atom) per_atom[a]++;
-    std::vector<int> bg_atom, bg_start, bg_slot(slot_atom.size()), pos(ao, -1);
+    std::vector<int> bg_atom, bg_start, pos(ao, -1);
     for (int a = 0; a < ao; ++a)
       if (per_atom[a]) { pos[a] = (int)bg_atom.size(); bg_atom.push_back(a); }
     bg_start.assign(bg_atom.size() + 1, 0);
     for (size_t k = 0; k < bg_atom.size(); ++k) bg_start[k + 1] = bg_start[k] + per_atom[bg_atom[k]];
+    // each term atom writes to the position of its atom's run, so that an
+    // atom's slots are contiguous for the gather
     std::vector<int> fill(bg_atom.size(), 0);
+    std::vector<int> slot_map(slot_atom.size());
     for (size_t sidx = 0; sidx < slot_atom.size(); ++sidx) {
       const int k = pos[slot_atom[sidx]];
-      bg_slot[bg_start[k] + fill[k]++] = (int)sidx;
+      slot_map[sidx] = bg_start[k] + fill[k]++;
     }
     d.nbg = (int)bg_atom.size();
     UP(ip, bg_atom); d.bg_atom = ip;
     UP(ip, bg_start); d.bg_start = ip;
-    UP(ip, bg_slot); d.bg_slot = ip;
+    UP(ip, slot_map); d.slot_map = ip;
     if ((rc = dev_alloc(e, &d.bslot, 3 * std::max<size_t>(1, slot_atom.size()))) != 0) return rc;
   }
   d.nrst = (int)rs_kwij.size();
@@ -714,6 +759,8 @@ static int build_device(qmi_engine* e) {
   d.nsu = (int)su_n.size();
   UP(ip, su_start); d.su_start = ip; UP(ip, su_n); d.su_n = ip; UP(ip, su_win); d.su_win = ip;
   UP(ip, sc_i); d.sc_i = ip; UP(ip, sc_j); d.sc_j = ip;
+  UP(ip, su_atoms); d.su_atoms = ip; UP(ip, su_natom); d.su_natom = ip; UP(ip, su_atom_list); d.su_atom_list = ip;
+  UP(up, sc_loc); d.sc_loc = up;
   UP(dp, sc_d2); d.sc_d2 = dp;
   ShellDev* sd;
   UP(sd, shells); d.shells = sd;
@@ -726,32 +773,82 @@ static int build_device(qmi_engine* e) {
   if ((rc = dev_alloc(e, &d.e_ring, (size_t)d.ring_n * NW * E_NSLOT)) != 0) return rc;
   if ((rc = dev_alloc(e, &d.t_ring, (size_t)d.ring_n * NW * 4)) != 0) return rc;
 
-  // --- tile lists: dynamic types get a fixed capacity (all j-clusters of the window)
-  auto make_dyn = [&](TileType tt, const std::vector<int>& iwin, bool jwater, bool m14) -> int {
-    TileList& L = d.tl[tt];
-    std::vector<int> start(iwin.size()), count(iwin.size(), 0);
-    int total = 0;
-    for (size_t ic = 0; ic < iwin.size(); ++ic) {
-      start[ic] = total;
-      total += jwater ? wd[iwin[ic]].nwc : wd[iwin[ic]].nsc;
+  // --- neighbour rows of the dynamic lists: capacity per row from the largest
+  // candidate count at the start (switching atoms within the cut-off plus
+  // 1 A), with room for density fluctuations; a row that overflows flags
+  // its window
+  auto max_rows = [&](TileType tt) -> int {
+    int best = 0;
+    for (int wi = 0; wi < NW; ++wi) {
+      const Window& W = *e->wins[wi].w;
+      const Topology& t = W.top;
+      const int NS = t.nat_solute;
+      const bool iw = (tt == T_WW || tt == T_WP), jw = (tt == T_WW || tt == T_PW);
+      const double rc = (tt == T_WW ? W.inp.rcww : tt == T_PP ? W.inp.rcpp : W.inp.rcpw) + 1.0;
+      auto pts = [&](bool water) {
+        std::vector<std::array<double, 3>> v;
+        if (water) {
+          for (int k = 0; k < W.nwat; ++k) {
+            const int o = NS + 3 * k;
+            v.push_back({W.x[3 * o], W.x[3 * o + 1], W.x[3 * o + 2]});
+          }
+        } else {
+          for (int i = 0; i < NS; ++i) {
+            if (W.iqatom[i]) continue;
+            const int sw = t.cgp[W.iwhich_cgp[i]].iswitch;
+            v.push_back({W.x[3 * sw], W.x[3 * sw + 1], W.x[3 * sw + 2]});
+          }
+        }
+        return v;
+      };
+      const auto I = pts(iw), J = pts(jw);
+      if (I.empty() || J.empty()) continue;
+      // cell grid over J
+      double lo[3] = {1e30, 1e30, 1e30};
+      for (auto& p : J) for (int c = 0; c < 3; ++c) lo[c] = std::min(lo[c], p[c]);
+      std::map<std::array<int, 3>, std::vector<int>> grid;
+      auto cell = [&](const std::array<double, 3>& p) {
+        return std::array<int, 3>{(int)std::floor((p[0] - lo[0]) / rc), (int)std::floor((p[1] - lo[1]) / rc),
+                                  (int)std::floor((p[2] - lo[2]) / rc)};
+      };
+      for (size_t k = 0; k < J.size(); ++k) grid[cell(J[k])].push_back((int)k);
+      for (auto& p : I) {
+        auto c = cell(p);
+        int n = 0;
+        for (int dx = -1; dx <= 1; ++dx)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+              auto it = grid.find({c[0] + dx, c[1] + dy, c[2] + dz});
+              if (it == grid.end()) continue;
+              for (int k : it->second) {
+                const double ex = J[k][0] - p[0], ey = J[k][1] - p[1], ez = J[k][2] - p[2];
+                if (ex * ex + ey * ey + ez * ez <= rc * rc) ++n;
+              }
+            }
+        best = std::max(best, n);
+      }
     }
+    return best;
+  };
+  auto make_rows = [&](TileType tt, const std::vector<int>& iwin) -> int {
+    RowList& L = d.rl[tt];
     L.nicl = (int)iwin.size();
-    L.cap_total = total;
-    L.work = nullptr;
-    L.nwork = 0;
+    int maxj = 0;
+    const bool jw = (tt == T_WW || tt == T_PW);
+    for (int wi = 0; wi < NW; ++wi) maxj = std::max(maxj, kCl * (jw ? wd[wi].nwc : wd[wi].nsc));
+    const int need = max_rows(tt);
+    int cap = std::min(maxj, ((int)(1.3 * need) + 24 + 7) / 8 * 8);
+    L.cap = std::max(cap, 8);
     int r;
-    if ((r = dev_upload(e, &L.start, start)) != 0) return r;
-    if ((r = dev_upload(e, &L.count, count)) != 0) return r;
-    if ((r = dev_alloc(e, &L.jc, (size_t)total)) != 0) return r;
-    if ((r = dev_alloc(e, &L.mask, (size_t)total)) != 0) return r;
-    if (m14) { if ((r = dev_alloc(e, &L.m14, (size_t)total)) != 0) return r; }
-    else L.m14 = nullptr;
+    if ((r = dev_alloc(e, &L.cnt, (size_t)kCl * std::max(1, L.nicl))) != 0) return r;
+    if ((r = dev_alloc(e, &L.ent, (size_t)kCl * std::max(1, L.nicl) * L.cap)) != 0) return r;
+    HIP_OK(hipMemset(L.cnt, 0, sizeof(int) * kCl * std::max(1, L.nicl)));
     return 0;
   };
-  if ((rc = make_dyn(T_WW, wc_win, true, false)) != 0) return rc;
-  if ((rc = make_dyn(T_WP, wc_win, false, false)) != 0) return rc;
-  if ((rc = make_dyn(T_PW, sc_win, true, false)) != 0) return rc;
-  if ((rc = make_dyn(T_PP, sc_win, false, true)) != 0) return rc;
+  if ((rc = make_rows(T_WW, wc_win)) != 0) return rc;
+  if ((rc = make_rows(T_WP, wc_win)) != 0) return rc;
+  if ((rc = make_rows(T_PW, sc_win)) != 0) return rc;
+  if ((rc = make_rows(T_PP, sc_win)) != 0) return rc;
   auto make_static = [&](TileType tt, const HT& h) -> int {
     TileList& L = d.tl[tt];
     std::vector<int> start, count, jc;
@@ -1049,9 +1146,42 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   auto t0 = std::chrono::steady_clock::now();
   long long done = 0;
   const int R = e->d.ring_n;
+  // one step = the same kernels with the same arguments every time (the
+  // kernels read per-window step state from device memory), so outside the
+  // timing pass it runs as one captured graph
+  auto launch_step = [&]() {
+    launch_begin(e->d, e->stream, 0, 0, 0);
+    launch_zero(e->d, e->stream);
+    launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
+    launch_ww(e->d, e->stream);
+    launch_nonbonded(e->d, e->stream, false);
+    launch_other_forces(e->d, e->stream);
+    launch_integrate(e->d, e->stream, e->shake_any);
+  };
+  if (!e->timing_on && !e->step_exec && !e->graph_failed) {
+    hipGraph_t g = nullptr;
+    bool ok = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+      launch_step();
+      ok = hipStreamEndCapture(e->stream, &g) == hipSuccess && g != nullptr;
+    }
+    if (ok) ok = hipGraphInstantiate(&e->step_exec, g, nullptr, nullptr, 0) == hipSuccess;
+    if (ok) {
+      e->step_graph = g;
+    } else {
+      // run with direct launches instead
+      if (g) (void)hipGraphDestroy(g);
+      e->step_exec = nullptr;
+      e->graph_failed = true;
+      (void)hipGetLastError();
+    }
+  }
   while (done < nsteps) {
     int chunk = (int)std::min<long long>(R, nsteps - done);
-    for (int k = 0; k < chunk; ++k) {
+    if (!e->timing_on && e->step_exec) {
+      for (int k = 0; k < chunk; ++k) HIP_OK(hipGraphLaunch(e->step_exec, e->stream));
+    }
+    for (int k = 0; k < chunk && (e->timing_on || !e->step_exec); ++k) {
       launch_begin(e->d, e->stream, 0, 0, 0);
       launch_zero(e->d, e->stream);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[0], e->stream));
@@ -1090,6 +1220,8 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
   for (int wi = 0; wi < NW; ++wi) {
     if (ws[wi].shake_fail) return set_err(-60, "shake failure in window " + std::to_string(wi));
+    if (ws[wi].list_overflow)
+      return set_err(-61, "neighbour list capacity exceeded in window " + std::to_string(wi));
     e->wins[wi].istep = ws[wi].istep;
   }
   int rc = sync_state_to_host(e);
@@ -1125,6 +1257,8 @@ int qmi_evaluate(qmi_engine* e, int rebuild_lists) {
   HIP_OK(hipMemcpyAsync(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost, e->stream));
   HIP_OK(hipStreamSynchronize(e->stream));
   for (int wi = 0; wi < NW; ++wi) {
+    if (ws[wi].list_overflow)
+      return set_err(-61, "neighbour list capacity exceeded in window " + std::to_string(wi));
     assemble_energies(e->wins[wi], ws[wi].E);
     e->wins[wi].E[1] = ws[wi].ekin;
   }
@@ -1243,43 +1377,33 @@ int qmi_get_pairlist(qmi_engine* e, int w, int kind, int32_t* pi, int32_t* pj, i
   } else {
     TileType tt = kind == QMI_LIST_WW ? T_WW : kind == QMI_LIST_PW ? T_PW : kind == QMI_LIST_PP ? T_PP : T_NTYPES;
     if (tt == T_NTYPES) return set_err(-3, "invalid list kind");
-    TileList& L = e->d.tl[tt];
-    std::vector<int> start(L.nicl), count(L.nicl), jc(L.cap_total);
-    std::vector<uint2> mask(L.cap_total), m14;
-    HIP_OK(hipMemcpy(start.data(), L.start, L.nicl * 4, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(count.data(), L.count, L.nicl * 4, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(jc.data(), L.jc, (size_t)L.cap_total * 4, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(mask.data(), L.mask, (size_t)L.cap_total * 8, hipMemcpyDeviceToHost));
-    if (tt == T_PP) { m14.resize(L.cap_total); HIP_OK(hipMemcpy(m14.data(), L.m14, (size_t)L.cap_total * 8, hipMemcpyDeviceToHost)); }
+    const RowList& L = e->d.rl[tt];
+    const size_t nrow = (size_t)kCl * L.nicl;
+    std::vector<int> cnt(nrow), ent(nrow * L.cap);
+    HIP_OK(hipMemcpy(cnt.data(), L.cnt, nrow * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ent.data(), L.ent, nrow * L.cap * 4, hipMemcpyDeviceToHost));
     const bool iw = tt == T_WW;
     const std::vector<int>& icl = iw ? e->h_wc_o : e->h_sc_a;
-    const std::vector<int>& jcl = (tt == T_PP) ? e->h_sc_a : e->h_wc_o;
-    for (int ic = 0; ic < L.nicl; ++ic) {
-      int a0 = -1;
-      for (int k = 0; k < kCl && a0 < 0; ++k) a0 = icl[ic * kCl + k];
-      if (a0 < 0 || e->h_atom_win[a0] != w) continue;
-      for (int k = 0; k < count[ic]; ++k) {
-        const int q = start[ic] + k;
-        const uint2 m = mask[q];
-        for (int bit = 0; bit < 64; ++bit) {
-          bool on = bit < 32 ? (m.x >> bit) & 1 : (m.y >> (bit - 32)) & 1;
-          if (!on) continue;
-          int a = bit >> 3, b = bit & 7;
-          int ea = icl[ic * kCl + a] - (int)ao, eb = jcl[jc[q] * kCl + b] - (int)ao;
-          if (tt == T_WW) {
-            if (ic < jc[q] || (ic == jc[q] && a < b)) out.push_back({ea + 1, eb + 1, 1});
-          } else if (tt == T_PP) {
-            if (!(ic < jc[q] || (ic == jc[q] && a < b))) continue;
-            bool is14 = bit < 32 ? (m14[q].x >> bit) & 1 : (m14[q].y >> (bit - 32)) & 1;
-            int lj = is14 ? 3 : ljc(ea, eb);
-            if (zero_pair(ea, eb, ljc(ea, eb))) continue;
-            out.push_back({ea + 1, eb + 1, lj});
-          } else {  // PW: solute atom ea, water O eb -> three atom pairs
-            for (int s = 0; s < 3; ++s) {
-              int lj = ljc(ea, eb + s);
-              if (zero_pair(ea, eb + s, lj)) continue;
-              out.push_back({ea + 1, eb + s + 1, lj});
-            }
+    for (size_t row = 0; row < nrow; ++row) {
+      const int ga = icl[row];
+      if (ga < 0 || e->h_atom_win[ga] != w) continue;
+      const int ea = ga - (int)ao;
+      for (int k = 0; k < cnt[row]; ++k) {
+        const int v = ent[row * L.cap + k];
+        const bool is14 = (tt == T_PP) && (v & kRow14);
+        const int eb = (v & ~kRow14) - (int)ao;
+        if (tt == T_WW) {
+          if (ea < eb) out.push_back({ea + 1, eb + 1, 1});
+        } else if (tt == T_PP) {
+          if (ea >= eb) continue;
+          const int lj = is14 ? 3 : ljc(ea, eb);
+          if (zero_pair(ea, eb, ljc(ea, eb))) continue;
+          out.push_back({ea + 1, eb + 1, lj});
+        } else {  // P<-W: solute atom ea, water O eb -> three atom pairs
+          for (int s = 0; s < 3; ++s) {
+            const int lj = ljc(ea, eb + s);
+            if (zero_pair(ea, eb + s, lj)) continue;
+            out.push_back({ea + 1, eb + s + 1, lj});
           }
         }
       }

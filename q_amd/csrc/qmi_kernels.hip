@@ -28,6 +28,14 @@ __device__ __forceinline__ void add_grad(long long* f, int atom, double gx, doub
   atomicAdd((unsigned long long*)&f[3 * atom + 2], (unsigned long long)fix(gz, kFScale));
 }
 
+// gradient of one term atom into its own slot (fixed point, no atomic)
+__device__ __forceinline__ void put_slot(const Dev& d, int k, double gx, double gy, double gz) {
+  const int m = d.slot_map[k];
+  d.bslot[3 * m + 0] = fix(gx, kFScale);
+  d.bslot[3 * m + 1] = fix(gy, kFScale);
+  d.bslot[3 * m + 2] = fix(gz, kFScale);
+}
+
 __device__ __forceinline__ void add_energy(WinState* ws, int w, int slot, double e) {
   if (e != 0.0) atomicAdd((unsigned long long*)&ws[w].E[slot], (unsigned long long)fix(e, kEScale));
 }
@@ -36,6 +44,23 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// energy of a term into slot `slot` of window w: lanes of a wave that agree
+// on (w, slot) are summed first and added with one atomic
+__device__ __forceinline__ void wave_energy(WinState* ws, int w, int slot, double e, bool valid) {
+  const unsigned long long vm = __ballot(valid);
+  if (vm == 0ull) return;
+  const int first = __ffsll((long long)vm) - 1;
+  const int key = valid ? w * E_NSLOT + slot : -1;
+  const int key0 = __shfl(key, first, 64);
+  const bool uniform = __ballot(valid && key != key0) == 0ull;
+  if (uniform) {
+    const double sum = wave_sum(valid ? e : 0.0);
+    if ((int)(threadIdx.x & 63) == first) add_energy(ws, w, slot, sum);
+  } else if (valid) {
+    add_energy(ws, w, slot, e);
+  }
+}
+
 // sum over the 8 lanes that share lane>>3
 __device__ __forceinline__ double row8_sum(double v) {
   v += __shfl_xor(v, 1, 64);
@@ -219,78 +244,94 @@ __device__ __forceinline__ int special_type(const Dev& d, int i, int j) {
   return 0;
 }
 
+// ------------------------------------------------- neighbour rows --
+// Pair-list build (nbwwlist/nbpwlist/nbpplist without LRF bookkeeping,
+// md.f90:8370-8556, 6864-7068, 5448-5700): one wave per i-cluster.  Candidate
+// j-clusters come from the bounding spheres of the switching atoms; for each
+// candidate, lane (a, b) applies the reference's exact FP64 group test to
+// member a of the i-cluster and member b of the j-cluster, and the passing
+// members are appended, in j order, to row a.
 template <int TT>
-__global__ void k_build_tiles(Dev d) {
+__global__ void __launch_bounds__(256) k_build_rows(Dev d) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  TileList L = d.tl[TT];
+  const RowList L = d.rl[TT];
   if (wave >= L.nicl) return;
   const int ic = wave;
-  const bool iw = (TT == T_WW || TT == T_WP);
+  constexpr bool iw = (TT == T_WW || TT == T_WP);
+  constexpr bool jw = (TT == T_WW || TT == T_PW);
   const int w = iw ? d.wc_win[ic] : d.sc_win[ic];
   const WinState& st = d.ws[w];
   if (!st.active || !st.list_now) return;
   const WinDev& W = d.win[w];
-  const bool jw = (TT == T_WW || TT == T_PW);
   const int j0 = jw ? W.wc_off : W.sc_off;
   const int nj = jw ? W.nwc : W.nsc;
   const double rc2 = (TT == T_WW) ? W.rcww2 : (TT == T_PP) ? W.rcpp2 : W.rcpw2;
   const double rc = sqrt(rc2);
   const double* bi = iw ? &d.wc_bs[4 * ic] : &d.sc_bs[4 * ic];
-  int out = L.start[ic];
-  int cnt = 0;
-  for (int base = 0; base < nj; base += 64) {
-    const int jc = j0 + base + lane;
-    uint2 m = {0u, 0u}, m14 = {0u, 0u};
-    if (base + lane < nj && bi[3] >= 0.0) {
-      const double* bj = jw ? &d.wc_bs[4 * jc] : &d.sc_bs[4 * jc];
-      double dx = bi[0] - bj[0], dy = bi[1] - bj[1], dz = bi[2] - bj[2];
-      double dc = sqrt(dx * dx + dy * dy + dz * dz);
-      if (bj[3] >= 0.0 && dc - bi[3] - bj[3] <= rc + 1e-3) {
-        for (int a = 0; a < kCl; ++a) {
-          int ea = iw ? d.wc_o[ic * kCl + a] : d.sc_a[ic * kCl + a];
-          if (ea < 0) continue;
-          int swa = iw ? ea : d.g_sw[d.grp[ea]];
-          if (d.aflag[swa] & 1) continue;
-          for (int b = 0; b < kCl; ++b) {
-            int eb = jw ? d.wc_o[jc * kCl + b] : d.sc_a[jc * kCl + b];
-            if (eb < 0) continue;
-            if (TT == T_WW && ea == eb) continue;
-            int swb = jw ? eb : d.g_sw[d.grp[eb]];
-            if (d.aflag[swb] & 1) continue;
-            int sp = 0;
-            if (TT == T_PP) {
-              if (ea == eb) continue;
-              sp = special_type(d, ea, eb);
-              if (sp == 1) continue;
-            }
-            if (r2_exact(d.x, swa, swb) > rc2) continue;
-            int bit = a * 8 + b;
-            if (bit < 32) m.x |= 1u << bit; else m.y |= 1u << (bit - 32);
-            if (sp == 2) { if (bit < 32) m14.x |= 1u << bit; else m14.y |= 1u << (bit - 32); }
-          }
+  const int a = lane >> 3, b = lane & 7;
+  const int row = ic * kCl + a;
+  const int ea = iw ? d.wc_o[ic * kCl + a] : d.sc_a[ic * kCl + a];
+  const int swa = ea < 0 ? -1 : (iw ? ea : d.g_sw[d.grp[ea]]);
+  const bool a_ok = ea >= 0 && !(d.aflag[swa] & 1);
+  int cnt = 0;  // entries of row a (the same in the 8 lanes of the row)
+  for (int base = 0; base < nj && bi[3] >= 0.0; base += 64) {
+    bool cand = false;
+    if (base + lane < nj) {
+      const double* bj = jw ? &d.wc_bs[4 * (j0 + base + lane)] : &d.sc_bs[4 * (j0 + base + lane)];
+      const double dx = bi[0] - bj[0], dy = bi[1] - bj[1], dz = bi[2] - bj[2];
+      const double dc = sqrt(dx * dx + dy * dy + dz * dz);
+      cand = bj[3] >= 0.0 && dc - bi[3] - bj[3] <= rc + 1e-3;
+    }
+    unsigned long long cb = __ballot(cand);
+    while (cb) {
+      const int k = __ffsll((long long)cb) - 1;
+      cb &= cb - 1ull;
+      const int jc = j0 + base + k;
+      const int eb = jw ? d.wc_o[jc * kCl + b] : d.sc_a[jc * kCl + b];
+      bool keep = false;
+      int sp = 0;
+      if (a_ok && eb >= 0 && !(TT == T_WW && ea == eb) && !(TT == T_PP && ea == eb)) {
+        const int swb = jw ? eb : d.g_sw[d.grp[eb]];
+        if (!(d.aflag[swb] & 1)) {
+          if (TT == T_PP) sp = special_type(d, ea, eb);
+          if (sp != 1) keep = r2_exact(d.x, swa, swb) <= rc2;
         }
       }
+      const unsigned long long kb = __ballot(keep);
+      const unsigned rowbits = (unsigned)((kb >> (8 * a)) & 0xffull);
+      if (keep) {
+        const int idx = cnt + __popc(rowbits & ((1u << b) - 1u));
+        if (idx < L.cap) L.ent[(size_t)row * L.cap + idx] = (TT == T_PP && sp == 2) ? (eb | kRow14) : eb;
+      }
+      cnt += __popc(rowbits);
     }
-    bool keep = (m.x | m.y) != 0u;
-    unsigned long long bal = __ballot(keep);
-    if (keep) {
-      int pos = __popcll(bal & ((1ull << lane) - 1ull));
-      L.jc[out + cnt + pos] = jc;
-      L.mask[out + cnt + pos] = m;
-      if (TT == T_PP) L.m14[out + cnt + pos] = m14;
-    }
-    cnt += __popcll(bal);
   }
-  if (lane == 0) L.count[ic] = cnt;
+  if (b == 0) {
+    L.cnt[row] = cnt < L.cap ? cnt : L.cap;
+    if (cnt > L.cap) d.ws[w].list_overflow = 1;
+  }
 }
 
-// --------------------------------------------------- nonbonded tiles --
+// the number of chunks of 8 entries the wave must walk: the longest row
+__device__ __forceinline__ int row_chunks(int n) {
+  int m = n;
+  m = max(m, __shfl_xor(m, 8, 64));
+  m = max(m, __shfl_xor(m, 16, 64));
+  m = max(m, __shfl_xor(m, 32, 64));
+  return (m + kCl - 1) / kCl;
+}
+
+// --------------------------------------------------- nonbonded rows --
+// Lane (a, b) of the wave for i-cluster ic evaluates entries b, b+8, ... of
+// row a and accumulates the gradient of its i-member only (each pair is
+// evaluated from both sides); the 8 lanes of a row are then reduced and
+// added once.
 // water-water: 9 interactions per molecule pair (nonbond_ww_spc, md.f90:11823-12047)
 __global__ void __launch_bounds__(256) k_ww(Dev d) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const TileList L = d.tl[T_WW];
+  const RowList L = d.rl[T_WW];
   if (wave >= L.nicl) return;
   const int ic = wave;
   const int w = d.wc_win[ic];
@@ -298,7 +339,10 @@ __global__ void __launch_bounds__(256) k_ww(Dev d) {
   if (!st.active) return;
   const WinDev& W = d.win[w];
   const int a = lane >> 3, b = lane & 7;
+  const int row = ic * kCl + a;
   const int oa = d.wc_o[ic * kCl + a];
+  const int n = oa >= 0 ? L.cnt[row] : 0;
+  const int nch = row_chunks(n);
   float3 pi[3];
   for (int k = 0; k < 3; ++k) {
     float4 q = oa >= 0 ? d.x32[oa + k] : make_float4(0, 0, 0, 0);
@@ -311,18 +355,17 @@ __global__ void __launch_bounds__(256) k_ww(Dev d) {
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   double eel = 0.0, evdw = 0.0;
-  const int s0 = L.start[ic], n = L.count[ic];
-  for (int t = 0; t < n; ++t) {
-    const int jc = L.jc[s0 + t];
-    const uint2 m = L.mask[s0 + t];
-    if (!lane_bit(m, lane)) continue;
-    const int ob = d.wc_o[jc * kCl + b];
+  const int* ent = L.ent + (size_t)row * L.cap;
+  for (int t = 0; t < nch; ++t) {
+    const int k = t * kCl + b;
+    if (k >= n) continue;
+    const int ob = ent[k];
     float3 pj[3];
-    for (int k = 0; k < 3; ++k) {
-      float4 q = d.x32[ob + k];
-      pj[k] = make_float3(q.x, q.y, q.z);
+    for (int s = 0; s < 3; ++s) {
+      float4 q = d.x32[ob + s];
+      pj[s] = make_float3(q.x, q.y, q.z);
     }
-    const bool cnt = en && (ic < jc || (ic == jc && a < b));
+    const bool cnt = en && oa < ob;   // each unordered pair counted once
     for (int p = 0; p < 3; ++p) {
       for (int q = 0; q < 3; ++q) {
         float dx = pj[q].x - pi[p].x, dy = pj[q].y - pi[p].y, dz = pj[q].z - pi[p].z;
@@ -357,14 +400,15 @@ __global__ void __launch_bounds__(256) k_ww(Dev d) {
   }
 }
 
-// solute atom - water molecule pair (nonbond_pw, md.f90:10631-10682), both
-// directions: WATER_SIDE accumulates the water's gradient, otherwise the
-// solute atom's; the energy is counted on the solute side only
+// solute atom - water: LJ on O, Coulomb on all three sites (nonbond_pw,
+// md.f90:10631-10682).  WATER_SIDE: rows are waters and entries solute atoms
+// (W<-P); otherwise rows are solute atoms and entries water O atoms (P<-W),
+// which also counts the energies.
 template <bool WATER_SIDE>
 __global__ void __launch_bounds__(256) k_pw(Dev d) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const TileList L = d.tl[WATER_SIDE ? T_WP : T_PW];
+  const RowList L = d.rl[WATER_SIDE ? T_WP : T_PW];
   if (wave >= L.nicl) return;
   const int ic = wave;
   const int w = WATER_SIDE ? d.wc_win[ic] : d.sc_win[ic];
@@ -372,7 +416,10 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
   if (!st.active) return;
   const WinDev& W = d.win[w];
   const int a = lane >> 3, b = lane & 7;
+  const int row = ic * kCl + a;
   const int ea = WATER_SIDE ? d.wc_o[ic * kCl + a] : d.sc_a[ic * kCl + a];
+  const int n = ea >= 0 ? L.cnt[row] : 0;
+  const int nch = row_chunks(n);
   float4 pi[3];
   float4 ljs = make_float4(0, 0, 0, 0);
   if (WATER_SIDE) {
@@ -385,12 +432,11 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   double eel = 0.0, evdw = 0.0;
-  const int s0 = L.start[ic], n = L.count[ic];
-  for (int t = 0; t < n; ++t) {
-    const int jc = L.jc[s0 + t];
-    const uint2 m = L.mask[s0 + t];
-    if (!lane_bit(m, lane)) continue;
-    const int eb = WATER_SIDE ? d.sc_a[jc * kCl + b] : d.wc_o[jc * kCl + b];
+  const int* ent = L.ent + (size_t)row * L.cap;
+  for (int t = 0; t < nch; ++t) {
+    const int kk = t * kCl + b;
+    if (kk >= n) continue;
+    const int eb = ent[kk];
     float4 ps, pw[3];
     float4 lj;
     if (WATER_SIDE) {
@@ -436,12 +482,12 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
   }
 }
 
-// solute-solute (nonbond_pp, md.f90:10387-10491); 1-4 pairs use the 1-4 LJ
-// parameters and el14_scale
+// solute-solute (nonbond_pp, md.f90:10387-10491): 1-4 pairs use the third
+// LJ column and the el14 scale
 __global__ void __launch_bounds__(256) k_pp(Dev d) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const TileList L = d.tl[T_PP];
+  const RowList L = d.rl[T_PP];
   if (wave >= L.nicl) return;
   const int ic = wave;
   const int w = d.sc_win[ic];
@@ -449,19 +495,22 @@ __global__ void __launch_bounds__(256) k_pp(Dev d) {
   if (!st.active) return;
   const WinDev& W = d.win[w];
   const int a = lane >> 3, b = lane & 7;
+  const int row = ic * kCl + a;
   const int ea = d.sc_a[ic * kCl + a];
+  const int n = ea >= 0 ? L.cnt[row] : 0;
+  const int nch = row_chunks(n);
   const float4 pi = ea >= 0 ? d.x32[ea] : make_float4(0, 0, 0, 0);
   const float4 li = ea >= 0 ? d.ljp[ea] : make_float4(0, 0, 0, 0);
   const bool en = st.energy_now;
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   double eel = 0.0, evdw = 0.0;
-  const int s0 = L.start[ic], n = L.count[ic];
-  for (int t = 0; t < n; ++t) {
-    const int jc = L.jc[s0 + t];
-    const uint2 m = L.mask[s0 + t];
-    if (!lane_bit(m, lane)) continue;
-    const bool is14 = lane_bit(L.m14[s0 + t], lane);
-    const int eb = d.sc_a[jc * kCl + b];
+  const int* ent = L.ent + (size_t)row * L.cap;
+  for (int t = 0; t < nch; ++t) {
+    const int kk = t * kCl + b;
+    if (kk >= n) continue;
+    const int e = ent[kk];
+    const bool is14 = (e & kRow14) != 0;
+    const int eb = e & ~kRow14;
     const float4 pj = d.x32[eb];
     const float4 lj = d.ljp[eb];
     float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
@@ -476,7 +525,7 @@ __global__ void __launch_bounds__(256) k_pp(Dev d) {
     float va = r6 * r6 * aa, vb = r6 * bb;
     float dv = r2inv * (-vel - 12.f * va + 6.f * vb);
     g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz;
-    if (en && (ic < jc || (ic == jc && a < b))) { eel += (double)vel; evdw += (double)(va - vb); }
+    if (en && ea < eb) { eel += (double)vel; evdw += (double)(va - vb); }
   }
   double s0x = row8_sum((double)g0), s0y = row8_sum((double)g1), s0z = row8_sum((double)g2);
   if (b == 0 && ea >= 0) add_grad(d.f, ea, s0x, s0y, s0z);
@@ -743,162 +792,187 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
 
 // q-q and q-(bonded neighbour) pairs (nonbond_qq, md.f90:10763-10868), one
 // thread per (pair, state)
-__global__ void k_qq(Dev d) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.nqq) return;
-  const int w = d.qq_win[k];
-  const WinState& st = d.ws[w];
-  if (!st.active) return;
-  const WinDev& W = d.win[w];
-  const int S = W.S;
-  const int4 e = d.qq_ij[k];
-  const int s = d.qq_st[k];
-  const int i = d.q_atom[W.q_off + e.x], j = e.y, jq = e.z, iLJ = e.w;
-  const float es = d.qq_es[k];
-  // parameters (double, as the reference)
-  double qi = d.q_crg[W.qs_off + e.x * S + s];
-  double qj = jq ? (double)d.q_crg[W.qs_off + (jq - 1) * S + s] : (double)d.x32[j].w;
-  const double2 ab = d.qq_ab[k];
-  const double aLJ = ab.x, bLJ = ab.y;
-  const double sc = d.qq_sc[k];
-  const bool expo = W.qvdw && jq && iLJ == 2;
-  // gradient in FP32 from the FP32 coordinates
-  float4 pi = d.x32[i], pj = d.x32[j];
-  float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-  float r2 = dx * dx + dy * dy + dz * dz;
-  float r6hc = r2 * r2 * r2;
-  float r6 = 1.f / (r6hc + (float)sc);
-  float r12 = r6 * r6;
-  float r2inv = 1.f / r2, r = sqrtf(r2inv);
-  float vel = (float)(qi * qj) * r * es;
-  if (iLJ == 3) vel *= W.el14;
-  float dv;
-  if (expo) {
-    float va = (float)aLJ * expf(-(float)bLJ / r);
-    dv = r2inv * (-vel - (float)bLJ * va / r) * (float)W.lambda[s];
-  } else {
-    float va = (float)aLJ * r12, vb = (float)bLJ * r6;
-    dv = r2inv * (-vel - (12.f * va - 6.f * vb) * r6hc * r6) * (float)W.lambda[s];
-  }
-  add_grad(d.f, i, -(double)(dv * dx), -(double)(dv * dy), -(double)(dv * dz));
-  add_grad(d.f, j, (double)(dv * dx), (double)(dv * dy), (double)(dv * dz));
-  if (st.energy_now) {
-    df ddx = df_dx(d.x[3 * j], d.x[3 * i]), ddy = df_dx(d.x[3 * j + 1], d.x[3 * i + 1]),
-       ddz = df_dx(d.x[3 * j + 2], d.x[3 * i + 2]);
-    df rr2 = df_add(df_add(df_mul(ddx, ddx), df_mul(ddy, ddy)), df_mul(ddz, ddz));
-    df one = {1.f, 0.f};
-    df rinv = df_div(one, df_sqrt(rr2));
-    df qq = df_2p((float)qi, (float)qj);
-    df v = df_mul(df_mul(qq, rinv), df_from((double)es));
-    if (iLJ == 3) v = df_mul(v, df_from((double)W.el14));
-    double e_el = df_to(v), e_vdw;
-    if (expo) {
-      e_vdw = aLJ * exp(-bLJ * sqrt(df_to(rr2)));
-    } else {
-      df r6s = df_add(df_mul(df_mul(rr2, rr2), rr2), df_from(sc));
-      df va = df_div(df_from(aLJ), df_mul(r6s, r6s));
-      df vb = df_div(df_from(bLJ), r6s);
-      e_vdw = df_to(df_add(va, df_neg(vb)));
+__global__ void __launch_bounds__(128) k_qq(Dev d) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = false, en = false;
+  int w = 0, s = 0, base = 6;
+  double e_el = 0.0, e_vdw = 0.0;
+  if (k < d.nqq) {
+    w = d.qq_win[k];
+    const WinState& st = d.ws[w];
+    if (st.active) {
+      valid = true;
+      en = st.energy_now;
+      const WinDev& W = d.win[w];
+      const int S = W.S;
+      const int4 e = d.qq_ij[k];
+      s = d.qq_st[k];
+      const int i = d.q_atom[W.q_off + e.x], j = e.y, jq = e.z, iLJ = e.w;
+      const float es = d.qq_es[k];
+      // parameters (double, as the reference)
+      const double qi = d.q_crg[W.qs_off + e.x * S + s];
+      const double qj = jq ? (double)d.q_crg[W.qs_off + (jq - 1) * S + s] : (double)d.x32[j].w;
+      const double2 ab = d.qq_ab[k];
+      const double aLJ = ab.x, bLJ = ab.y;
+      const double sc = d.qq_sc[k];
+      const bool expo = W.qvdw && jq && iLJ == 2;
+      // gradient in FP32 from the FP32 coordinates
+      const float4 pi = d.x32[i], pj = d.x32[j];
+      const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float r6hc = r2 * r2 * r2;
+      const float r6 = 1.f / (r6hc + (float)sc);
+      const float r12 = r6 * r6;
+      const float r2inv = 1.f / r2, r = sqrtf(r2inv);
+      float vel = (float)(qi * qj) * r * es;
+      if (iLJ == 3) vel *= W.el14;
+      float dv;
+      if (expo) {
+        const float va = (float)aLJ * expf(-(float)bLJ / r);
+        dv = r2inv * (-vel - (float)bLJ * va / r) * (float)W.lambda[s];
+      } else {
+        const float va = (float)aLJ * r12, vb = (float)bLJ * r6;
+        dv = r2inv * (-vel - (12.f * va - 6.f * vb) * r6hc * r6) * (float)W.lambda[s];
+      }
+      const int sl = d.qq_slot0 + 2 * k;
+      put_slot(d, sl, -(double)(dv * dx), -(double)(dv * dy), -(double)(dv * dz));
+      put_slot(d, sl + 1, (double)(dv * dx), (double)(dv * dy), (double)(dv * dz));
+      if (en) {
+        df ddx = df_dx(d.x[3 * j], d.x[3 * i]), ddy = df_dx(d.x[3 * j + 1], d.x[3 * i + 1]),
+           ddz = df_dx(d.x[3 * j + 2], d.x[3 * i + 2]);
+        df rr2 = df_add(df_add(df_mul(ddx, ddx), df_mul(ddy, ddy)), df_mul(ddz, ddz));
+        df one = {1.f, 0.f};
+        df rinv = df_div(one, df_sqrt(rr2));
+        df qq = df_2p((float)qi, (float)qj);
+        df v = df_mul(df_mul(qq, rinv), df_from((double)es));
+        if (iLJ == 3) v = df_mul(v, df_from((double)W.el14));
+        e_el = df_to(v);
+        if (expo) {
+          e_vdw = aLJ * exp(-bLJ * sqrt(df_to(rr2)));
+        } else {
+          df r6s = df_add(df_mul(df_mul(rr2, rr2), rr2), df_from(sc));
+          df va = df_div(df_from(aLJ), df_mul(r6s, r6s));
+          df vb = df_div(df_from(bLJ), r6s);
+          e_vdw = df_to(df_add(va, df_neg(vb)));
+        }
+        base = jq ? 4 : 6;  // qq or qp slot
+      }
     }
-    int base = jq ? 4 : 6;  // qq or qp slot
-    add_energy(d.ws, w, E_Q0 + 8 * s + base, e_el);
-    add_energy(d.ws, w, E_Q0 + 8 * s + base + 1, e_vdw);
   }
+  wave_energy(d.ws, w, E_Q0 + 8 * s + base, e_el, valid && en);
+  wave_energy(d.ws, w, E_Q0 + 8 * s + base + 1, e_vdw, valid && en);
 }
 
 // ------------------------------------------------------------- LRF --
-// Build: one block per target charge group accumulates the Taylor
-// expansion of every source atom of every group beyond the explicit cut-off
-// and within RcLRF (nbpplist_lrf 5448-5700, nbpwlist_lrf 6864-7068,
-// nbwwlist_lrf 8370-8556); FP64, fixed summation order per block.
-__global__ void __launch_bounds__(256) k_lrf_build(Dev d, int ngrp_total) {
-  const int g = blockIdx.x;
-  if (g >= ngrp_total) return;
-  const int w = d.g_win[g];
+// LRF expansions (the phi sums of the list builders, e.g. nbpplist_lrf,
+// md.f90:5589-5700): for every target charge group g, the Taylor terms about
+// its centre of all charges of groups h whose switching atoms lie beyond the
+// cut-off and within the LRF radius.  One wave takes 16 target groups of one
+// window; lane (t, part) = (lane & 15, lane >> 4) sums the sources h = part,
+// part + 4, ... for target t, so the source data a wave reads at one time
+// comes from four groups only; the four partial sums are added at the end.
+constexpr int kLrfTargets = 16;
+__global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= d.nlrfw) return;
+  const int4 lw = d.lrf_work[wave];   // window, first target group, count
+  const int w = lw.x;
   const WinState& st = d.ws[w];
   const WinDev& W = d.win[w];
   if (!st.active || !st.list_now || !W.use_lrf) return;
-  const int swg = d.g_sw[g];
-  if (d.aflag[swg] & 1) return;
-  const bool gw = (g - W.g_off) >= W.ngrp_solute;
-  const double cx = d.g_cent[3 * g], cy = d.g_cent[3 * g + 1], cz = d.g_cent[3 * g + 2];
+  const int t = lane & (kLrfTargets - 1), part = lane >> 4;
+  const int g = lw.y + t;
+  const bool g_ok = t < lw.z && !(d.aflag[d.g_sw[g < d.ngrp_all ? g : 0]] & 1);
   double acc[20];
   for (int k = 0; k < 20; ++k) acc[k] = 0.0;
-  for (int h = W.g_off + threadIdx.x; h < W.g_off + W.ngrp; h += blockDim.x) {
-    if (h == g) continue;
-    const int swh = d.g_sw[h];
-    if (d.aflag[swh] & 1) continue;
-    const bool hw = (h - W.g_off) >= W.ngrp_solute;
-    const double rc2 = (gw && hw) ? W.rcww2 : ((gw || hw) ? W.rcpw2 : W.rcpp2);
-    const double r2g = r2_exact(d.x, swg, swh);
-    if (r2g <= rc2 || r2g > W.rclrf2) continue;
-    const int n = d.g_n[h], f0 = d.g_first[h];
-    for (int k = 0; k < n; ++k) {
-      const int at = d.g_atoms[f0 + k];
-      if (d.aflag[at] & 8) continue;  // q-atom
-      const double q = (double)d.x32[at].w;
-      const double dx = d.x[3 * at] - cx, dy = d.x[3 * at + 1] - cy, dz = d.x[3 * at + 2] - cz;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      const double f0v = q / (r2 * sqrt(r2));
-      const double f1 = 3.0 * f0v / r2;
-      const double f2 = -f1 / r2;
-      acc[0] += f0v * r2;
-      acc[1] -= f0v * dx; acc[2] -= f0v * dy; acc[3] -= f0v * dz;
-      acc[4] += f1 * dx * dx - f0v; acc[5] += f1 * dx * dy; acc[6] += f1 * dx * dz;
-      acc[7] += f1 * dy * dy - f0v; acc[8] += f1 * dy * dz; acc[9] += f1 * dz * dz - f0v;
-      acc[10] += f2 * (5.0 * dx * dx * dx - r2 * 3.0 * dx);
-      acc[11] += f2 * (5.0 * dx * dx * dy - r2 * dy);
-      acc[12] += f2 * (5.0 * dx * dx * dz - r2 * dz);
-      acc[13] += f2 * (5.0 * dx * dy * dy - r2 * dx);
-      acc[14] += f2 * (5.0 * dx * dy * dz);
-      acc[15] += f2 * (5.0 * dx * dz * dz - r2 * dx);
-      acc[16] += f2 * (5.0 * dy * dy * dy - r2 * 3.0 * dy);
-      acc[17] += f2 * (5.0 * dy * dy * dz - r2 * dz);
-      acc[18] += f2 * (5.0 * dy * dz * dz - r2 * dy);
-      acc[19] += f2 * (5.0 * dz * dz * dz - r2 * 3.0 * dz);
+  if (g_ok) {
+    const int swg = d.g_sw[g];
+    const bool gw = (g - W.g_off) >= W.ngrp_solute;
+    const double cx = d.g_cent[3 * g], cy = d.g_cent[3 * g + 1], cz = d.g_cent[3 * g + 2];
+    for (int h = W.g_off + part; h < W.g_off + W.ngrp; h += 4) {
+      if (h == g) continue;
+      const int swh = d.g_sw[h];
+      if (d.aflag[swh] & 1) continue;
+      const bool hw = (h - W.g_off) >= W.ngrp_solute;
+      const double rc2 = (gw && hw) ? W.rcww2 : ((gw || hw) ? W.rcpw2 : W.rcpp2);
+      const double r2g = r2_exact(d.x, swg, swh);
+      if (r2g <= rc2 || r2g > W.rclrf2) continue;
+      const int n = d.g_n[h], f0 = d.g_first[h];
+      for (int k = 0; k < n; ++k) {
+        const int at = d.g_atoms[f0 + k];
+        if (d.aflag[at] & 8) continue;  // q-atom
+        const double q = (double)d.x32[at].w;
+        const double dx = d.x[3 * at] - cx, dy = d.x[3 * at + 1] - cy, dz = d.x[3 * at + 2] - cz;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double f0v = q / (r2 * sqrt(r2));
+        const double f1 = 3.0 * f0v / r2;
+        const double f2 = -f1 / r2;
+        acc[0] += f0v * r2;
+        acc[1] -= f0v * dx; acc[2] -= f0v * dy; acc[3] -= f0v * dz;
+        acc[4] += f1 * dx * dx - f0v; acc[5] += f1 * dx * dy; acc[6] += f1 * dx * dz;
+        acc[7] += f1 * dy * dy - f0v; acc[8] += f1 * dy * dz; acc[9] += f1 * dz * dz - f0v;
+        acc[10] += f2 * (5.0 * dx * dx * dx - r2 * 3.0 * dx);
+        acc[11] += f2 * (5.0 * dx * dx * dy - r2 * dy);
+        acc[12] += f2 * (5.0 * dx * dx * dz - r2 * dz);
+        acc[13] += f2 * (5.0 * dx * dy * dy - r2 * dx);
+        acc[14] += f2 * (5.0 * dx * dy * dz);
+        acc[15] += f2 * (5.0 * dx * dz * dz - r2 * dx);
+        acc[16] += f2 * (5.0 * dy * dy * dy - r2 * 3.0 * dy);
+        acc[17] += f2 * (5.0 * dy * dy * dz - r2 * dz);
+        acc[18] += f2 * (5.0 * dy * dz * dz - r2 * dy);
+        acc[19] += f2 * (5.0 * dz * dz * dz - r2 * 3.0 * dz);
+      }
     }
   }
-  __shared__ double red[4][20];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // (t, part) -> sum over part: lanes t, t+16, t+32, t+48
   for (int k = 0; k < 20; ++k) {
-    double v = wave_sum(acc[k]);
-    if (lane == 0) red[wv][k] = v;
+    double v = acc[k];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    acc[k] = v;
   }
-  __syncthreads();
-  if (threadIdx.x < 20) {
-    double v = 0.0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) v += red[k][threadIdx.x];
-    d.g_lrf[20 * g + threadIdx.x] = v;
-  }
+  if (part == 0 && g_ok)
+    for (int k = 0; k < 20; ++k) d.g_lrf[20 * g + k] = acc[k];
 }
 
 // Evaluation per atom (lrf_taylor, md.f90:3646-3712), FP64.
-__global__ void k_lrf_taylor(Dev d) {
+__global__ void __launch_bounds__(256) k_lrf_taylor(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.ntot) return;
-  const int w = d.wid[i];
-  const WinState& st = d.ws[w];
-  const WinDev& W = d.win[w];
-  if (!st.active || !W.use_lrf) return;
-  if (d.aflag[i] & (1 | 8)) return;  // excluded or q-atom
-  const int g = d.grp[i];
-  const double* p = &d.g_lrf[20 * g];
-  const double r0 = d.g_cent[3 * g] - d.x[3 * i], r1 = d.g_cent[3 * g + 1] - d.x[3 * i + 1],
-               r2 = d.g_cent[3 * g + 2] - d.x[3 * i + 2];
-  const double xx = p[4], xy = p[5], xz = p[6], yy = p[7], yz = p[8], zz = p[9];
-  const double t_xxx = p[10], t_xxy = p[11], t_xxz = p[12], t_xyy = p[13], t_xyz = p[14], t_xzz = p[15],
-               t_yyy = p[16], t_yyz = p[17], t_yzz = p[18], t_zzz = p[19];
-  double vij = p[0] + p[1] * r0 + p[2] * r1 + p[3] * r2 + 0.5 * (xx * r0 + xy * r1 + xz * r2) * r0 +
-               0.5 * (xy * r0 + yy * r1 + yz * r2) * r1 + 0.5 * (xz * r0 + yz * r1 + zz * r2) * r2;
-  double f0 = p[1] + xx * r0 + xy * r1 + xz * r2 + 0.5 * (t_xxx * r0 + t_xxy * r1 + t_xxz * r2) * r0 +
-              0.5 * (t_xxy * r0 + t_xyy * r1 + t_xyz * r2) * r1 + 0.5 * (t_xxz * r0 + t_xyz * r1 + t_xzz * r2) * r2;
-  double f1 = p[2] + xy * r0 + yy * r1 + yz * r2 + 0.5 * (t_xxy * r0 + t_xyy * r1 + t_xyz * r2) * r0 +
-              0.5 * (t_xyy * r0 + t_yyy * r1 + t_yyz * r2) * r1 + 0.5 * (t_xyz * r0 + t_yyz * r1 + t_yzz * r2) * r2;
-  double f2 = p[3] + xz * r0 + yz * r1 + zz * r2 + 0.5 * (t_xxz * r0 + t_xyz * r1 + t_xzz * r2) * r0 +
-              0.5 * (t_xyz * r0 + t_yyz * r1 + t_yzz * r2) * r1 + 0.5 * (t_xzz * r0 + t_yzz * r1 + t_zzz * r2) * r2;
-  const double q = (double)d.x32[i].w;
-  add_grad(d.f, i, -q * f0, -q * f1, -q * f2);
-  if (st.energy_now) add_energy(d.ws, w, E_LRF, 0.5 * q * vij);
+  bool on = false;
+  int w = 0;
+  double e = 0.0;
+  if (i < d.ntot) {
+    w = d.wid[i];
+    const WinState& st = d.ws[w];
+    const WinDev& W = d.win[w];
+    on = st.active && W.use_lrf && !(d.aflag[i] & (1 | 8));  // not excluded, not a q-atom
+    if (on) {
+      const int g = d.grp[i];
+      const double* p = &d.g_lrf[20 * g];
+      const double r0 = d.g_cent[3 * g] - d.x[3 * i], r1 = d.g_cent[3 * g + 1] - d.x[3 * i + 1],
+                   r2 = d.g_cent[3 * g + 2] - d.x[3 * i + 2];
+      const double xx = p[4], xy = p[5], xz = p[6], yy = p[7], yz = p[8], zz = p[9];
+      const double t_xxx = p[10], t_xxy = p[11], t_xxz = p[12], t_xyy = p[13], t_xyz = p[14], t_xzz = p[15],
+                   t_yyy = p[16], t_yyz = p[17], t_yzz = p[18], t_zzz = p[19];
+      const double vij = p[0] + p[1] * r0 + p[2] * r1 + p[3] * r2 + 0.5 * (xx * r0 + xy * r1 + xz * r2) * r0 +
+                         0.5 * (xy * r0 + yy * r1 + yz * r2) * r1 + 0.5 * (xz * r0 + yz * r1 + zz * r2) * r2;
+      const double f0 = p[1] + xx * r0 + xy * r1 + xz * r2 + 0.5 * (t_xxx * r0 + t_xxy * r1 + t_xxz * r2) * r0 +
+                        0.5 * (t_xxy * r0 + t_xyy * r1 + t_xyz * r2) * r1 +
+                        0.5 * (t_xxz * r0 + t_xyz * r1 + t_xzz * r2) * r2;
+      const double f1 = p[2] + xy * r0 + yy * r1 + yz * r2 + 0.5 * (t_xxy * r0 + t_xyy * r1 + t_xyz * r2) * r0 +
+                        0.5 * (t_xyy * r0 + t_yyy * r1 + t_yyz * r2) * r1 +
+                        0.5 * (t_xyz * r0 + t_yyz * r1 + t_yzz * r2) * r2;
+      const double f2 = p[3] + xz * r0 + yz * r1 + zz * r2 + 0.5 * (t_xxz * r0 + t_xyz * r1 + t_xzz * r2) * r0 +
+                        0.5 * (t_xyz * r0 + t_yyz * r1 + t_yzz * r2) * r1 +
+                        0.5 * (t_xzz * r0 + t_yzz * r1 + t_zzz * r2) * r2;
+      const double q = (double)d.x32[i].w;
+      add_grad(d.f, i, -q * f0, -q * f1, -q * f2);
+      e = 0.5 * q * vij;
+      on = st.energy_now;
+    }
+  }
+  wave_energy(d.ws, w, E_LRF, e, on);
 }
 
 // ---------------------------------------------------------- bonded --
@@ -957,17 +1031,12 @@ __device__ __forceinline__ void torsion_geom(const double* x, int i, int j, int 
 // slot of their own (no atomics); k_bonded_gather then sums every atom's
 // slots in a fixed order.  Integer sums make the result independent of the
 // order, so it equals the atomic form bit for bit.
-__device__ __forceinline__ void put_slot(long long* sl, int k, double gx, double gy, double gz) {
-  sl[3 * k + 0] = fix(gx, kFScale);
-  sl[3 * k + 1] = fix(gy, kFScale);
-  sl[3 * k + 2] = fix(gz, kFScale);
-}
 
-__device__ __forceinline__ void put4(long long* sl, int k, double dv, const double dp[12]) {
-  put_slot(sl, k + 0, dv * dp[0], dv * dp[1], dv * dp[2]);
-  put_slot(sl, k + 1, dv * dp[3], dv * dp[4], dv * dp[5]);
-  put_slot(sl, k + 2, dv * dp[6], dv * dp[7], dv * dp[8]);
-  put_slot(sl, k + 3, dv * dp[9], dv * dp[10], dv * dp[11]);
+__device__ __forceinline__ void put4(const Dev& d, int k, double dv, const double dp[12]) {
+  put_slot(d, k + 0, dv * dp[0], dv * dp[1], dv * dp[2]);
+  put_slot(d, k + 1, dv * dp[3], dv * dp[4], dv * dp[5]);
+  put_slot(d, k + 2, dv * dp[6], dv * dp[7], dv * dp[8]);
+  put_slot(d, k + 3, dv * dp[9], dv * dp[10], dv * dp[11]);
 }
 
 // angle (md.f90:814-901): energy and the gradient factors di, dk of the end
@@ -995,22 +1064,6 @@ __device__ __forceinline__ void angle_geom(const double* x, int i, int j, int k,
   }
 }
 
-// energy of a term into slot `slot` of window w: lanes of a wave that agree
-// on (w, slot) are summed first and added with one atomic
-__device__ __forceinline__ void wave_energy(WinState* ws, int w, int slot, double e, bool valid) {
-  const unsigned long long vm = __ballot(valid);
-  if (vm == 0ull) return;
-  const int first = __ffsll((long long)vm) - 1;
-  const int key = valid ? w * E_NSLOT + slot : -1;
-  const int key0 = __shfl(key, first, 64);
-  const bool uniform = __ballot(valid && key != key0) == 0ull;
-  if (uniform) {
-    const double sum = wave_sum(valid ? e : 0.0);
-    if ((int)(threadIdx.x & 63) == first) add_energy(ws, w, slot, sum);
-  } else if (valid) {
-    add_energy(ws, w, slot, e);
-  }
-}
 
 __global__ void __launch_bounds__(256) k_bonded(Dev d) {
   const int n1 = d.nbond, n2 = n1 + d.nang, n3 = n2 + d.ntor, n4 = n3 + d.nimp;
@@ -1020,7 +1073,6 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
   const int b5 = b4 + 4 * d.nimp, b6 = b5 + 2 * d.nqb, b7 = b6 + 3 * d.nqa, b8 = b7 + 4 * d.nqt;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const double* x = d.x;
-  long long* sl = d.bslot;
   bool valid = false;
   int w = 0, eslot = 0;
   double e = 0.0;
@@ -1034,8 +1086,8 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
       const double bl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
       const double db = bl - p.y;
       const double dv = p.x * db / bl;
-      put_slot(sl, b1 + 2 * t, -r[0] * dv, -r[1] * dv, -r[2] * dv);
-      put_slot(sl, b1 + 2 * t + 1, r[0] * dv, r[1] * dv, r[2] * dv);
+      put_slot(d, b1 + 2 * t, -r[0] * dv, -r[1] * dv, -r[2] * dv);
+      put_slot(d, b1 + 2 * t + 1, r[0] * dv, r[1] * dv, r[2] * dv);
       valid = true;
       e = 0.5 * p.x * db * db;
       eslot = d.b_solv[t] ? E_W_BOND : E_P_BOND;
@@ -1060,9 +1112,9 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
         const double du = 2 * p.z * ru / dik;
         for (int c = 0; c < 3; ++c) { gk[c] += du * r[c]; gi[c] -= du * r[c]; }
       }
-      put_slot(sl, b2 + 3 * a, gi[0], gi[1], gi[2]);
-      put_slot(sl, b2 + 3 * a + 1, gj[0], gj[1], gj[2]);
-      put_slot(sl, b2 + 3 * a + 2, gk[0], gk[1], gk[2]);
+      put_slot(d, b2 + 3 * a, gi[0], gi[1], gi[2]);
+      put_slot(d, b2 + 3 * a + 1, gj[0], gj[1], gj[2]);
+      put_slot(d, b2 + 3 * a + 2, gk[0], gk[1], gk[2]);
       valid = true;
       eslot = d.a_solv[a] ? E_W_ANG : E_P_ANG;
     }
@@ -1077,7 +1129,7 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
       const double arg = (double)p.y * phi - (double)p.z;
       e = (double)p.x * (1.0 + cos(arg)) * (double)p.w;
       const double dv = -(double)p.y * (double)p.x * sin(arg) * (double)p.w;
-      put4(sl, b3 + 4 * a, dv, dp);
+      put4(d, b3 + 4 * a, dv, dp);
       valid = true;
       eslot = d.t_solv[a] ? E_W_TOR : E_P_TOR;
     }
@@ -1101,7 +1153,7 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
         dv = (double)p.x * arg;
         e = 0.5 * dv * arg;
       }
-      put4(sl, b4 + 4 * a, dv, dp);
+      put4(d, b4 + 4 * a, dv, dp);
       valid = true;
       eslot = d.i_solv[a] ? E_W_IMP : E_P_IMP;
     }
@@ -1121,8 +1173,8 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
       const double fexp = exp(-p.y * db);
       e = p.x * (fexp * fexp - 2. * fexp) + 0.5 * p.w * db * db;
       const double dv = (2. * p.x * p.y * (fexp - fexp * fexp) + p.w * db) * lam / bl;
-      put_slot(sl, b5 + 2 * a, -dv * r[0], -dv * r[1], -dv * r[2]);
-      put_slot(sl, b5 + 2 * a + 1, dv * r[0], dv * r[1], dv * r[2]);
+      put_slot(d, b5 + 2 * a, -dv * r[0], -dv * r[1], -dv * r[2]);
+      put_slot(d, b5 + 2 * a + 1, dv * r[0], dv * r[1], dv * r[2]);
       valid = true;
       eslot = E_Q0 + 8 * s + 0;
     }
@@ -1154,9 +1206,9 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
         di[c] = f1 * (rjk[c] / (bji * bjk) - scp * rji[c] / (bji * bji));
         dk[c] = f1 * (rji[c] / (bji * bjk) - scp * rjk[c] / (bjk * bjk));
       }
-      put_slot(sl, b6 + 3 * a, dv * di[0], dv * di[1], dv * di[2]);
-      put_slot(sl, b6 + 3 * a + 1, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
-      put_slot(sl, b6 + 3 * a + 2, dv * dk[0], dv * dk[1], dv * dk[2]);
+      put_slot(d, b6 + 3 * a, dv * di[0], dv * di[1], dv * di[2]);
+      put_slot(d, b6 + 3 * a + 1, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
+      put_slot(d, b6 + 3 * a + 2, dv * dk[0], dv * dk[1], dv * dk[2]);
       valid = true;
       eslot = E_Q0 + 8 * s + 1;
     }
@@ -1186,7 +1238,7 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
         e = 0.5 * dv * arg;
         dv = dv * lam;
       }
-      put4(sl, tors ? b7 + 4 * a : b8 + 4 * a, dv, dp);
+      put4(d, tors ? b7 + 4 * a : b8 + 4 * a, dv, dp);
       valid = true;
       eslot = E_Q0 + 8 * s + (tors ? 2 : 3);
     }
@@ -1204,56 +1256,64 @@ __global__ void __launch_bounds__(256) k_bonded_gather(Dev d) {
   if (!d.ws[d.wid[atom]].active) return;
   long long gx = 0, gy = 0, gz = 0;
   for (int m = d.bg_start[k]; m < d.bg_start[k + 1]; ++m) {
-    const int s = d.bg_slot[m];
-    gx += d.bslot[3 * s];
-    gy += d.bslot[3 * s + 1];
-    gz += d.bslot[3 * s + 2];
+    gx += d.bslot[3 * m];
+    gy += d.bslot[3 * m + 1];
+    gz += d.bslot[3 * m + 2];
   }
   d.f[3 * atom] += gx;
   d.f[3 * atom + 1] += gy;
   d.f[3 * atom + 2] += gz;
 }
 
-__global__ void k_restr(Dev d) {
+// fixed/shell restraints (fix_shell, md.f90:1414-1450) and the radial
+// solvent restraint (restrain_solvent, 14910-14957), one thread per atom
+__global__ void __launch_bounds__(256) k_restr(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.ntot) return;
-  const int w = d.wid[i];
-  const WinState& st = d.ws[w];
-  if (!st.active) return;
-  const WinDev& W = d.win[w];
-  const uint8_t fl = d.aflag[i];
-  if (fl & 3) {
-    const double fk = (fl & 1) ? 200.0 : W.fk_pshell;
-    double dr[3];
-    for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * i + c] - d.xtop[3 * i + c];
-    add_grad(d.f, i, fk * dr[0], fk * dr[1], fk * dr[2]);
-    if (st.energy_now) {
-      double e = 0.5 * fk * (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-      if (fl & 1) add_energy(d.ws, w, E_FIX, e);
-      if (fl & 2) add_energy(d.ws, w, E_SHELL, e);
+  int w = 0;
+  bool en = false, is_fix = false, is_shell = false, is_rad = false;
+  double e_fs = 0.0, e_rad = 0.0;
+  if (i < d.ntot) {
+    w = d.wid[i];
+    const WinState& st = d.ws[w];
+    const WinDev& W = d.win[w];
+    if (st.active) {
+      en = st.energy_now;
+      const uint8_t fl = d.aflag[i];
+      if (fl & 3) {
+        const double fk = (fl & 1) ? 200.0 : W.fk_pshell;
+        double dr[3];
+        for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * i + c] - d.xtop[3 * i + c];
+        add_grad(d.f, i, fk * dr[0], fk * dr[1], fk * dr[2]);
+        e_fs = 0.5 * fk * (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+        is_fix = fl & 1;
+        is_shell = fl & 2;
+      }
+      const int li = i - W.ao;
+      if (li >= W.nat_solute && ((li - W.nat_solute) % 3) == 0 && !(fl & 1)) {
+        const double shift = (W.fk_wsphere != 0.) ? sqrt((double)0.001986f * st.tfree / W.fk_wsphere) : 0.0;
+        double dr[3];
+        for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * i + c] - W.xwcent[c];
+        const double b = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+        const double db = b - (W.rwat - shift);
+        double dv;
+        if (db > 0) {
+          e_rad = 0.5 * W.fk_wsphere * db * db - W.dwmz;
+          dv = W.fk_wsphere * db / b;
+        } else if (b > 0.0) {
+          const double fexp = exp(W.awmz * db);
+          e_rad = W.dwmz * (fexp * fexp - 2. * fexp);
+          dv = -2. * W.dwmz * W.awmz * (fexp - fexp * fexp) / b;
+        } else {
+          dv = 0.0;
+        }
+        add_grad(d.f, i, dv * dr[0], dv * dr[1], dv * dr[2]);
+        is_rad = true;
+      }
     }
   }
-  const int li = i - W.ao;
-  if (li >= W.nat_solute && ((li - W.nat_solute) % 3) == 0 && !(fl & 1)) {
-    const double shift = (W.fk_wsphere != 0.) ? sqrt((double)0.001986f * st.tfree / W.fk_wsphere) : 0.0;
-    double dr[3];
-    for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * i + c] - W.xwcent[c];
-    double b = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-    double db = b - (W.rwat - shift);
-    double erst, dv;
-    if (db > 0) {
-      erst = 0.5 * W.fk_wsphere * db * db - W.dwmz;
-      dv = W.fk_wsphere * db / b;
-    } else if (b > 0.0) {
-      double fexp = exp(W.awmz * db);
-      erst = W.dwmz * (fexp * fexp - 2. * fexp);
-      dv = -2. * W.dwmz * W.awmz * (fexp - fexp * fexp) / b;
-    } else {
-      dv = 0; erst = 0;
-    }
-    add_grad(d.f, i, dv * dr[0], dv * dr[1], dv * dr[2]);
-    if (st.energy_now) add_energy(d.ws, w, E_RAD, erst);
-  }
+  wave_energy(d.ws, w, E_FIX, e_fs, en && is_fix);
+  wave_energy(d.ws, w, E_SHELL, e_fs, en && is_shell);
+  wave_energy(d.ws, w, E_RAD, e_rad, en && is_rad);
 }
 
 // p_restrain (md.f90:12940-13274): one 64-lane block per restraint record.
@@ -1430,11 +1490,11 @@ __global__ void __launch_bounds__(256) k_watpol_theta(Dev d) {
 }
 
 // Pass 2, one block per (window, shell): the shell's waters are gathered in
-// LDS with their theta; a water's place in the theta order (ascending, ties
-// by water index: the first-minimum rule of the reference's selection sort)
-// is counted against the LDS copy, which gives its target angle theta0.
+// LDS with their theta and sorted (theta ascending, ties by water index: the
+// order of the reference's first-minimum selection sort); a water's place in
+// that order gives its target angle theta0.
 constexpr int kWpolMaxShell = 8;
-constexpr int kWpolLds = 3072;       // waters of one shell held in LDS
+constexpr int kWpolLds = 4096;       // waters of one shell held (and sorted) in LDS
 __global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
   const int w = blockIdx.x / kWpolMaxShell, is = blockIdx.x % kWpolMaxShell;
   if (w >= d.nwin) return;
@@ -1468,9 +1528,32 @@ __global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
   __syncthreads();
   const int n = s_n;
   const bool in_lds = n <= kWpolLds;
+  if (in_lds) {
+    // bitonic sort of (theta, water index) ascending; padding keys sort last
+    int np = 1;
+    while (np < n) np <<= 1;
+    for (int k = n + threadIdx.x; k < np; k += blockDim.x) { s_th[k] = 1e300; s_iw[k] = 0x7fffffff; }
+    __syncthreads();
+    for (int size = 2; size <= np; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int k = threadIdx.x; k < np; k += blockDim.x) {
+          const int l = k ^ stride;
+          if (l > k) {
+            const bool up = (k & size) == 0;
+            const double tk = s_th[k], tl = s_th[l];
+            const int ik = s_iw[k], il = s_iw[l];
+            const bool k_after_l = tk > tl || (tk == tl && ik > il);
+            if (k_after_l == up) { s_th[k] = tl; s_th[l] = tk; s_iw[k] = il; s_iw[l] = ik; }
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
   const double cstb = (double)sh.cstb, tc = (double)sh.theta_corr;
   double eacc = 0.0, tsum = 0.0;
-  // members: from LDS, or (shell larger than LDS) straight from the window's waters
+  // members: sorted in LDS (place = index), or (shell larger than LDS)
+  // straight from the window's waters with their place counted
   const int nloop = in_lds ? n : W.nwat;
   for (int k = threadIdx.x; k < nloop; k += blockDim.x) {
     int iw;
@@ -1485,10 +1568,7 @@ __global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
     }
     int rank = 0;
     if (in_lds) {
-      for (int m = 0; m < n; ++m) {
-        const double tj = s_th[m];
-        rank += (tj < th || (tj == th && s_iw[m] < iw)) ? 1 : 0;
-      }
+      rank = k;
     } else {
       for (int jw = 0; jw < W.nwat; ++jw) {
         if (d.wshell[wo + jw] != is) continue;
@@ -1569,8 +1649,70 @@ __global__ void k_leapfrog(Dev d) {
 
 // SHAKE (md.f90:14445-14566): one thread per connected component of a
 // molecule's constraint graph; components share no atoms, so iterating them
-// separately gives the reference's result for the whole molecule
-__global__ void k_shake(Dev d) {
+// separately gives the reference's result for the whole molecule.
+// Constraints are swept in order, Gauss-Seidel, until every one is within
+// tol*d0^2 (at most 1000 sweeps).  Units of up to kShakeRegAtoms atoms
+// hold their coordinates in registers during the sweeps; larger ones work on
+// the coordinates in memory.  Both do the same arithmetic in the same order.
+// a unit of NA atoms held in registers (constant indices after unrolling;
+// the constraint's atoms are picked with selects)
+template <int NA>
+__device__ __forceinline__ void shake_regs(const Dev& d, WinState& st, int c0, int nc, const int* al) {
+#pragma clang fp contract(off)
+  const double tol = (double)0.0001f;
+  unsigned long long ready = 0ull;
+  const unsigned long long all = nc >= 64 ? ~0ull : ((1ull << nc) - 1ull);
+  double px[NA][3], pxx[NA][3];
+  float wv[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) {
+    const int at = al[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { px[k][c] = d.x[3 * at + c]; pxx[k][c] = d.xx[3 * at + c]; }
+    wv[k] = d.winv[at];
+  }
+  int nits = 0;
+  while (true) {
+    for (int c = 0; c < nc; ++c) {
+      if (ready & (1ull << c)) continue;
+      const int li = d.sc_loc[2 * (c0 + c)], lj = d.sc_loc[2 * (c0 + c) + 1];
+      double xi[3], xj[3], xxi[3], xxj[3];
+      float wvi = 0.f, wvj = 0.f;
+#pragma unroll
+      for (int k = 0; k < NA; ++k) {
+        if (k == li) { for (int q = 0; q < 3; ++q) { xi[q] = px[k][q]; xxi[q] = pxx[k][q]; } wvi = wv[k]; }
+        if (k == lj) { for (int q = 0; q < 3; ++q) { xj[q] = px[k][q]; xxj[q] = pxx[k][q]; } wvj = wv[k]; }
+      }
+      double xij[3], xxij[3];
+      for (int q = 0; q < 3; ++q) xij[q] = xi[q] - xj[q];
+      const double xij2 = xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2];
+      const double d2 = d.sc_d2[c0 + c];
+      const double diff = d2 - xij2;
+      if (fabs(diff) < tol * d2) ready |= 1ull << c;
+      for (int q = 0; q < 3; ++q) xxij[q] = xxi[q] - xxj[q];
+      const double scp = xij[0] * xxij[0] + xij[1] * xxij[1] + xij[2] * xxij[2];
+      const float wsum = wvi + wvj;
+      const double corr = diff / (2. * scp * (double)wsum);
+      const double wi = (double)wvi, wj = (double)wvj;
+#pragma unroll
+      for (int k = 0; k < NA; ++k) {
+        if (k == li)
+          for (int q = 0; q < 3; ++q) px[k][q] = px[k][q] + xxij[q] * corr * wi;
+        if (k == lj)
+          for (int q = 0; q < 3; ++q) px[k][q] = px[k][q] - xxij[q] * corr * wj;
+      }
+    }
+    ++nits;
+    if ((ready & all) == all) break;
+    if (nits >= 1000) { st.shake_fail = 1; break; }
+  }
+#pragma unroll
+  for (int k = 0; k < NA; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d.x[3 * al[k] + c] = px[k][c];
+}
+
+__global__ void __launch_bounds__(64) k_shake(Dev d) {
 #pragma clang fp contract(off)
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= d.nsu) return;
@@ -1578,12 +1720,17 @@ __global__ void k_shake(Dev d) {
   WinState& st = d.ws[w];
   if (!st.active) return;
   const int c0 = d.su_start[u], nc = d.su_n[u];
+  const int na = d.su_natom[u];
+  const int* al = d.su_atom_list + d.su_atoms[u];
+  if (na == 3) { shake_regs<3>(d, st, c0, nc, al); return; }
+  if (na == 2) { shake_regs<2>(d, st, c0, nc, al); return; }
+  if (na == 4) { shake_regs<4>(d, st, c0, nc, al); return; }
   const double tol = (double)0.0001f;
   unsigned long long ready = 0ull;  // up to 64 constraints per component
   const unsigned long long all = nc >= 64 ? ~0ull : ((1ull << nc) - 1ull);
-  double* x = d.x;
   const double* xx = d.xx;
   int nits = 0;
+  double* x = d.x;
   while (true) {
     for (int c = 0; c < nc; ++c) {
       if (ready & (1ull << c)) continue;
@@ -1698,22 +1845,22 @@ void launch_begin(const Dev& d, hipStream_t s, int force_energy, int eval_only, 
 void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
   k_centres<<<blocks(ngrp, 128), 128, 0, s>>>(d, ngrp);
   k_bounds<<<blocks(nwc + nsc, 128), 128, 0, s>>>(d, nwc, nsc);
-  if (d.tl[T_WW].nicl) k_build_tiles<T_WW><<<blocks((long long)d.tl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_WP].nicl) k_build_tiles<T_WP><<<blocks((long long)d.tl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_PW].nicl) k_build_tiles<T_PW><<<blocks((long long)d.tl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_PP].nicl) k_build_tiles<T_PP><<<blocks((long long)d.tl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
-  if (ngrp) k_lrf_build<<<ngrp, 256, 0, s>>>(d, ngrp);
+  if (d.rl[T_WW].nicl) k_build_rows<T_WW><<<blocks((long long)d.rl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.rl[T_WP].nicl) k_build_rows<T_WP><<<blocks((long long)d.rl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.rl[T_PW].nicl) k_build_rows<T_PW><<<blocks((long long)d.rl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.rl[T_PP].nicl) k_build_rows<T_PP><<<blocks((long long)d.rl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.nlrfw) k_lrf_build<<<blocks((long long)d.nlrfw * 64, 256), 256, 0, s>>>(d);
 }
 
 void launch_ww(const Dev& d, hipStream_t s) {
-  if (d.tl[T_WW].nicl) k_ww<<<blocks((long long)d.tl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.rl[T_WW].nicl) k_ww<<<blocks((long long)d.rl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
 }
 
 void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww) {
   if (with_ww) launch_ww(d, s);
-  if (d.tl[T_WP].nicl) k_pw<true><<<blocks((long long)d.tl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_PW].nicl) k_pw<false><<<blocks((long long)d.tl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_PP].nicl) k_pp<<<blocks((long long)d.tl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.rl[T_WP].nicl) k_pw<true><<<blocks((long long)d.rl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.rl[T_PW].nicl) k_pw<false><<<blocks((long long)d.rl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
+  if (d.rl[T_PP].nicl) k_pp<<<blocks((long long)d.rl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_QW].nwork) k_qw<true><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_WQ].nwork) k_qw<false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
@@ -1724,10 +1871,9 @@ void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww) {
 void launch_other_forces(const Dev& d, hipStream_t s) {
   k_lrf_taylor<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
-  if (nb) {
-    k_bonded<<<blocks(nb, 256), 256, 0, s>>>(d);
-    k_bonded_gather<<<blocks(d.nbg, 256), 256, 0, s>>>(d);
-  }
+  if (nb) k_bonded<<<blocks(nb, 256), 256, 0, s>>>(d);
+  // the gather also collects the q-q pair slots written by k_qq
+  if (nb || d.nqq) k_bonded_gather<<<blocks(d.nbg, 256), 256, 0, s>>>(d);
   k_restr<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   if (d.nrst) k_prestr<<<d.nrst, 64, 0, s>>>(d);
   if (d.nwat_total) {

@@ -318,10 +318,11 @@ def report(R: QfepResult, nstates: int) -> str:
         L.append(f"# Max energy-gap is: {R.gapmax:9.2f}")
         L.append("# Part 2: Reaction free energy summary:")
         for row in R.bins:
-            L.append(f"   {row[0]:8.6f}{row[1]:5d}  {row[2]:9.2f}{row[3]:9.2f}{row[4]:9.2f}{row[5]:9.2f}  {row[6]:5d}")
+            L.append(f"   {row[0]:8.6f}{row[1]:5d}  {row[2]:9.2f}{row[3]:9.2f}{row[4]:9.2f}{row[5]:9.2f}  {row[6]:5d}"
+                     f"{0.0:9.3f}{0.0:9.3f}")
         L.append("# Part 3: Bin-averaged summary:")
         for row in R.binavg:
-            L.append(f"{row[0]:4d} {row[1]:9.2f}{row[2]:9.2f}{row[3]:9.2f}  {row[4]:5d}")
+            L.append(f"{row[0]:4d} {row[1]:9.2f}{row[2]:9.2f}{row[3]:9.2f}  {row[4]:5d}{0.0:8.3f}{0.0:8.3f}{0.0:8.3f}")
         for title, part, psum in (("# Part 4: Termodynamic integration:", R.dgti, R.dgtisum),
                                   ("# Part 5: Overlap sampling Lu et al:", R.dglu, R.dglusum),
                                   ("# Part 6: BAR Bennet:", R.dgbar, R.dgbarsum)):
