@@ -144,6 +144,7 @@ def main():
 
     e, el = run_engine(inps, args.warmup, args.steps, timing=False)
     infos = [e.info(w) for w in range(len(inps))]
+    tuning = e.tuning()
     dt_fs = 1.0
     ns_sim = len(inps) * args.steps * dt_fs * 1e-6
     value = ns_sim / el * 86400.0
@@ -201,6 +202,7 @@ def main():
             "ww_kernel": round(fam[4], 2) if fam else None,
             "wall_per_step_timed_pass": round(fam[5], 2) if fam else None,
         },
+        "autotune": tuning,
         "roofline": {
             "bound": "mfma", "kernel": "k_ww (water-water tiles, FP32 vector ALU; compute roof = FP32 peak)",
             "achieved": round(achieved, 4) if achieved else None, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -112,6 +112,17 @@ int qmi_window_info(qmi_engine* e, int w, int32_t* info, int cap);
 /* Write window w's restart file (write_xfin, md.f90:15452-15470). */
 int qmi_write_restart(qmi_engine* e, int w, const char* path);
 
+/* The autotuner (on by default) times candidate launch shapes of the Q-tile,
+ * pair-row, LRF-build and bonded kernels on the engine's own windows during
+ * qmi_prepare and keeps the fastest; the candidates do not change results.
+ * Call before qmi_prepare.  No reference counterpart. */
+int qmi_set_autotune(qmi_engine* e, int on);
+
+/* Choices of the autotuner: Q tiles per wave, threads per block of the row,
+ * LRF-build and bonded kernels, then the measured microseconds of each
+ * chosen configuration (8 values).  Returns the number written. */
+int qmi_tuning(qmi_engine* e, double* out, int cap);
+
 /* Turn per-step HIP-event timing of the kernel families on or off for the
  * following qmi_run calls (no reference counterpart; for measurement). */
 int qmi_set_timing(qmi_engine* e, int on);
@@ -126,6 +137,29 @@ int qmi_timings(qmi_engine* e, double* out, int cap);
 
 /* Last error message of the calling thread. */
 const char* qmi_last_error(void);
+
+/* ---- Overlap predictor of the learned lambda schedule (no reference
+ * counterpart: the reference's lambda schedules are written by hand).  An MLP
+ * n_in -> n_hidden (SiLU) -> n_hidden (SiLU) -> 1 (sigmoid), trained and run on
+ * the GPU; BF16 weights with an FP32 master copy, FP32 accumulation.
+ * Parameters are ordered W1 (n_hidden x n_in, row-major), b1, W2 (n_hidden x
+ * n_hidden), b2, W3 (1 x n_hidden), b3.  The loss is the mean squared error
+ * of the output against targets in [0, 1]. */
+typedef struct qmi_mlp qmi_mlp;
+int qmi_mlp_create(int hip_device, int n_in, int n_hidden, unsigned long long seed, qmi_mlp** out);
+long long qmi_mlp_num_params(qmi_mlp* m);
+int qmi_mlp_get_params(qmi_mlp* m, float* params, long long cap);
+int qmi_mlp_set_params(qmi_mlp* m, const float* params, long long n);
+/* outputs y[batch] for inputs x[batch x n_in] */
+int qmi_mlp_forward(qmi_mlp* m, const float* x, int batch, float* y);
+/* loss and gradient (num_params floats, may be NULL) for one batch */
+int qmi_mlp_grad(qmi_mlp* m, const float* x, const float* t, int batch, float* loss, float* grad);
+/* Adam training on n samples, shuffled by `seed`, for `epochs` passes of
+ * minibatches of `batch`; final_loss = mean loss over the last pass */
+int qmi_mlp_train(qmi_mlp* m, const float* x, const float* t, int n, int epochs, int batch, float lr,
+                  unsigned long long seed, float* final_loss);
+const char* qmi_mlp_last_error(void);
+void qmi_mlp_destroy(qmi_mlp* m);
 
 void qmi_engine_destroy(qmi_engine* e);
 

@@ -74,13 +74,15 @@ struct TileList {
   uint2* m14;      // 1-4 mask (P<-P, Q<-P, P<-Q)
   int nicl;        // number of i-clusters (all windows)
   int cap_total;
-  // static lists are cut into work items of at most kTileChunk tiles, one
-  // wave each: (i-cluster, first entry, number of entries); null for
-  // dynamic lists, which run one wave per i-cluster
+  // static lists are cut into work items (a multiple of kQGroup tiles, the
+  // size picked by the autotuner), one wave each: (i-cluster, first entry,
+  // number of entries)
   const int4* work;
   int nwork;
 };
-constexpr int kTileChunk = 16;
+constexpr int kQGroup = 4;        // Q-tile sums go to fixed point every 4 tiles of a list
+constexpr int kQChunkN = 4;
+constexpr int kQChunks[kQChunkN] = {4, 8, 16, 32};   // autotuner candidates, tiles per wave
 
 // Neighbour rows of the dynamic pair lists (WW, W<-P, P<-W, P<-P): row
 // ic*8 + a belongs to member a of i-cluster ic and holds the j entries of
@@ -118,8 +120,14 @@ struct Dev {
   double* g_cent;       // 3 per group
   double* g_lrf;        // 20 per group: phi0, phi1(3), phi2(6: xx xy xz yy yz zz), phi3(10)
   int ngrp_all;         // groups of all windows
-  int nlrfw;            // LRF build work items (window, first target group, count)
+  int nlrfw;            // LRF build blocks: (window, first target group, count)
   const int4* lrf_work;
+  // LRF sources, packed at every build: switching atom of each group
+  // (x, y, z, excluded) and the group's non-q atoms (x, y, z, charge)
+  double4* ls_sw;
+  double4* ls_xq;
+  const int* ls_first; const int* ls_n; const int* ls_atom;
+  int nls;
   // special partners of solute atoms (exclusions / 1-4), CSR
   const int* sp_start;  // per atom (ntot+1)
   const int* sp_j;
@@ -143,6 +151,9 @@ struct Dev {
   // tiles (static Q lists) and neighbour rows (dynamic lists, index T_WW..T_PP)
   TileList tl[T_NTYPES];
   RowList rl[4];
+  // launch shapes chosen by the autotuner (threads per block); none of
+  // them changes the arithmetic of a wave or thread
+  int bs_rows, bs_lrf, bs_bonded;
   // q-q pairs (static)
   int nqq;
   const int4* qq_ij;     // iq (window-local), j (global), jq (1-based q or 0), ljcod

@@ -77,6 +77,14 @@ struct qmi_engine {
   std::vector<double> timings;
   bool timing_on = false;
   hipEvent_t ev[8] = {};
+  // autotuner: work-item variants of the static Q lists and the choices made
+  int4* qwork[T_NTYPES][4] = {};
+  int nqwork[T_NTYPES][4] = {};
+  int4* lrfwork[3] = {};        // LRF build blocks for 64, 128, 256 threads
+  int nlrfwork[3] = {};
+  bool autotune = true;
+  int tuned[4] = {2, 256, 256, 256};   // Q chunk index, row / LRF / bonded block sizes
+  double tune_us[4] = {0, 0, 0, 0};    // time of the chosen configuration per group
   // the launch sequence of one MD step, captured once and replayed
   hipGraph_t step_graph = nullptr;
   hipGraphExec_t step_exec = nullptr;
@@ -637,6 +645,7 @@ static int build_device(qmi_engine* e) {
 
   Dev& d = e->d;
   d.nwin = NW;
+  d.bs_rows = d.bs_lrf = d.bs_bonded = 256;
   d.ntot = ao;
   int rc = 0;
 #define UP(dst, src) if ((rc = dev_upload(e, &dst, src)) != 0) return rc
@@ -663,15 +672,40 @@ static int build_device(qmi_engine* e) {
   if ((rc = dev_alloc(e, &d.g_lrf, 20 * (size_t)goff)) != 0) return rc;
   d.ngrp_all = goff;
   {
-    // LRF build work: 16 target groups of one window per wave
-    std::vector<int4> lw;
-    for (int wi = 0; wi < NW; ++wi)
-      for (int g0 = 0; g0 < wd[wi].ngrp; g0 += 16)
-        lw.push_back(make_int4(wi, wd[wi].g_off + g0, std::min(16, wd[wi].ngrp - g0), 0));
-    d.nlrfw = (int)lw.size();
-    int4* lwp;
-    UP(lwp, lw);
-    d.lrf_work = lwp;
+    // LRF build: one block per (window, block-size consecutive targets), a
+    // variant per candidate block size of the autotuner
+    for (int vi = 0; vi < 3; ++vi) {
+      const int bsz = 64 << vi;
+      std::vector<int4> lw;
+      for (int wi = 0; wi < NW; ++wi)
+        for (int g0 = 0; g0 < wd[wi].ngrp; g0 += bsz)
+          lw.push_back(make_int4(wi, wd[wi].g_off + g0, std::min(bsz, wd[wi].ngrp - g0), 0));
+      int4* lwp;
+      UP(lwp, lw);
+      e->lrfwork[vi] = lwp;
+      e->nlrfwork[vi] = (int)lw.size();
+    }
+    d.lrf_work = e->lrfwork[2];
+    d.nlrfw = e->nlrfwork[2];
+    // packed sources: non-q atoms of every group, in group order
+    std::vector<int> ls_first(goff), ls_n(goff), ls_atom;
+    for (int g = 0; g < goff; ++g) {
+      ls_first[g] = (int)ls_atom.size();
+      int n = 0;
+      for (int k = 0; k < g_n[g]; ++k) {
+        const int at = g_atoms[g_first[g] + k];
+        if (aflag[at] & 8) continue;
+        ls_atom.push_back(at);
+        ++n;
+      }
+      ls_n[g] = n;
+    }
+    d.nls = (int)ls_atom.size();
+    UP(ip, ls_first); d.ls_first = ip;
+    UP(ip, ls_n); d.ls_n = ip;
+    UP(ip, ls_atom); d.ls_atom = ip;
+    if ((rc = dev_alloc(e, &d.ls_sw, std::max<size_t>(1, goff))) != 0) return rc;
+    if ((rc = dev_alloc(e, &d.ls_xq, std::max<size_t>(1, ls_atom.size()))) != 0) return rc;
   }
   UP(ip, sp_start); d.sp_start = ip;
   UP(ip, sp_j); d.sp_j = ip;
@@ -864,15 +898,22 @@ static int build_device(qmi_engine* e) {
     }
     L.nicl = (int)h.l.size();
     L.cap_total = (int)jc.size();
-    std::vector<int4> work;
-    for (size_t ic = 0; ic < h.l.size(); ++ic)
-      for (int k = 0; k < count[ic]; k += kTileChunk)
-        work.push_back(make_int4((int)ic, start[ic] + k, std::min(kTileChunk, count[ic] - k), 0));
-    L.nwork = (int)work.size();
+    // work items for every candidate size of the autotuner; the default
+    // until it runs is the 16-tile form
     int r;
-    int4* wp = nullptr;
-    if ((r = dev_upload(e, &wp, work)) != 0) return r;
-    L.work = wp;
+    for (int ci = 0; ci < kQChunkN; ++ci) {
+      const int ch = kQChunks[ci];
+      std::vector<int4> work;
+      for (size_t ic = 0; ic < h.l.size(); ++ic)
+        for (int k = 0; k < count[ic]; k += ch)
+          work.push_back(make_int4((int)ic, start[ic] + k, std::min(ch, count[ic] - k), 0));
+      int4* wp = nullptr;
+      if ((r = dev_upload(e, &wp, work)) != 0) return r;
+      e->qwork[tt][ci] = wp;
+      e->nqwork[tt][ci] = (int)work.size();
+    }
+    L.work = e->qwork[tt][2];
+    L.nwork = e->nqwork[tt][2];
     if ((r = dev_upload(e, &L.start, start)) != 0) return r;
     if ((r = dev_upload(e, &L.count, count)) != 0) return r;
     if ((r = dev_upload(e, &L.jc, jc)) != 0) return r;
@@ -1037,6 +1078,82 @@ int qmi_window_add(qmi_engine* e, const char* inp_path, long long seed_override,
   return 0;
 }
 
+// Autotuner: on the engine's own windows, time every candidate launch shape
+// of a kernel family (a forces-only evaluation, a few repeats, HIP events on
+// the engine's stream) and keep the fastest.  The candidates differ only in
+// how work is grouped into blocks and waves, never in the arithmetic of a
+// pair or term, so the choice does not change results.
+static int autotune(qmi_engine* e) {
+  Dev& d = e->d;
+  hipEvent_t t0, t1;
+  HIP_OK(hipEventCreate(&t0));
+  HIP_OK(hipEventCreate(&t1));
+  auto time_us = [&](auto&& launch) -> double {
+    launch();  // warm-up
+    double best = 1e30;
+    for (int r = 0; r < 3; ++r) {
+      (void)hipEventRecord(t0, e->stream);
+      launch();
+      (void)hipEventRecord(t1, e->stream);
+      (void)hipEventSynchronize(t1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, t0, t1);
+      best = std::min(best, (double)ms * 1000.0);
+    }
+    return best;
+  };
+  // a forces-only state with fresh lists and LRF expansions
+  launch_begin(d, e->stream, 0, 2, 1);
+  launch_zero(d, e->stream);
+  launch_lists(d, e->stream, e->ngrp, e->nwc, e->nsc);
+  HIP_OK(hipStreamSynchronize(e->stream));
+  const int bs[3] = {64, 128, 256};
+  // 0: Q-tile work-item size
+  {
+    int bi = 2;
+    double bt = 1e30;
+    for (int ci = 0; ci < kQChunkN; ++ci) {
+      for (int tt : {T_QP, T_PQ, T_QW, T_WQ}) { d.tl[tt].work = e->qwork[tt][ci]; d.tl[tt].nwork = e->nqwork[tt][ci]; }
+      const double t = time_us([&] { launch_q_tiles(d, e->stream); });
+      if (t < bt) { bt = t; bi = ci; }
+    }
+    for (int tt : {T_QP, T_PQ, T_QW, T_WQ}) { d.tl[tt].work = e->qwork[tt][bi]; d.tl[tt].nwork = e->nqwork[tt][bi]; }
+    e->tuned[0] = bi;
+    e->tune_us[0] = bt;
+  }
+  // 1-3: threads per block of the row kernels, the LRF build and the bonded terms
+  int* field[3] = {&d.bs_rows, &d.bs_lrf, &d.bs_bonded};
+  for (int g = 0; g < 3; ++g) {
+    int bb = 256;
+    double bt = 1e30;
+    for (int b : bs) {
+      *field[g] = b;
+      if (g == 1) {
+        const int vi = b == 64 ? 0 : b == 128 ? 1 : 2;
+        d.lrf_work = e->lrfwork[vi];
+        d.nlrfw = e->nlrfwork[vi];
+      }
+      double t;
+      if (g == 0) t = time_us([&] { launch_nonbonded(d, e->stream, true); });
+      else if (g == 1) t = time_us([&] { launch_lrf_build(d, e->stream); });
+      else t = time_us([&] { launch_bonded(d, e->stream); });
+      if (t < bt) { bt = t; bb = b; }
+    }
+    *field[g] = bb;
+    if (g == 1) {
+      const int vi = bb == 64 ? 0 : bb == 128 ? 1 : 2;
+      d.lrf_work = e->lrfwork[vi];
+      d.nlrfw = e->nlrfwork[vi];
+    }
+    e->tuned[1 + g] = bb;
+    e->tune_us[1 + g] = bt;
+  }
+  HIP_OK(hipGetLastError());
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return 0;
+}
+
 int qmi_prepare(qmi_engine* e) {
   if (!e) return set_err(-1, "null engine");
   if (e->prepared) return 0;
@@ -1048,6 +1165,7 @@ int qmi_prepare(qmi_engine* e) {
   if (!e->stream) HIP_OK(hipStreamCreate(&e->stream));
   int rc = build_device(e);
   if (rc) return rc;
+  if (e->autotune && (rc = autotune(e)) != 0) return rc;
   for (auto& h : e->wins) {
     Window& W = *h.w;
     if (W.inp.iene_cycle > 0 && !W.inp.ene_file.empty()) {
@@ -1428,6 +1546,23 @@ int qmi_write_restart(qmi_engine* e, int w, const char* path) {
   Error err{0, ""};
   if (!write_restart(path, *h->w, &err)) return set_err(err.code, err.msg);
   return 0;
+}
+
+int qmi_set_autotune(qmi_engine* e, int on) {
+  if (!e) return set_err(-1, "null engine");
+  if (e->prepared) return set_err(-2, "the autotuner runs in qmi_prepare");
+  e->autotune = on != 0;
+  return 0;
+}
+
+int qmi_tuning(qmi_engine* e, double* out, int cap) {
+  if (!e || !out) return set_err(-1, "null argument");
+  if (!e->prepared) return set_err(-2, "engine not prepared");
+  const double v[8] = {(double)kQChunks[e->tuned[0]], (double)e->tuned[1], (double)e->tuned[2], (double)e->tuned[3],
+                       e->tune_us[0], e->tune_us[1], e->tune_us[2], e->tune_us[3]};
+  const int n = std::min(cap, 8);
+  for (int k = 0; k < n; ++k) out[k] = v[k];
+  return n;
 }
 
 int qmi_set_timing(qmi_engine* e, int on) {

@@ -12,6 +12,7 @@
 // (two-FP32) arithmetic; LRF, bonded terms, restraints, leapfrog and SHAKE
 // run in FP64.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 
 #include "qmi_device.h"
@@ -59,6 +60,26 @@ __device__ __forceinline__ void wave_energy(WinState* ws, int w, int slot, doubl
   } else if (valid) {
     add_energy(ws, w, slot, e);
   }
+}
+
+// integer forms: exact, so the order of the sums does not matter
+__device__ __forceinline__ long long wave_sum_i(long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ long long row8_sum_i(long long v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  return v;
+}
+__device__ __forceinline__ void add_grad_fix(long long* f, int atom, long long gx, long long gy, long long gz) {
+  atomicAdd((unsigned long long*)&f[3 * atom + 0], (unsigned long long)gx);
+  atomicAdd((unsigned long long*)&f[3 * atom + 1], (unsigned long long)gy);
+  atomicAdd((unsigned long long*)&f[3 * atom + 2], (unsigned long long)gz);
+}
+__device__ __forceinline__ void add_energy_fix(WinState* ws, int w, int slot, long long e) {
+  if (e != 0) atomicAdd((unsigned long long*)&ws[w].E[slot], (unsigned long long)e);
 }
 
 // sum over the 8 lanes that share lane>>3
@@ -170,9 +191,10 @@ __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
   WinState& s = d.ws[w];
   const WinDev& W = d.win[w];
   if (eval_only) {
+    // 1: energy evaluation; 2: forces only (the autotuner's timing runs)
     s.active = 1;
     s.list_now = rebuild;
-    s.energy_now = 1;
+    s.energy_now = eval_only == 1 ? 1 : 0;
   } else {
     s.active = s.istep < s.stop_at ? 1 : 0;
     s.list_now = (s.istep % W.nbcycle) == 0;
@@ -607,7 +629,14 @@ __device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int s, int q
   *vdw = df_to(df_add(va, df_neg(vb)));
 }
 
-// Q-atom side of q-water tiles: one lane = (q-atom a, water b)
+// Q-atom tiles: lane (a, b) of a work item takes member a of the i-cluster
+// and member b of each listed j-cluster.  The force pass is FP32 (hardware
+// reciprocal); on energy steps a second pass over the same tiles computes
+// the per-state energies in df32.  Sums go to fixed point every kQGroup tiles
+// of the i-cluster's list, so they do not depend on the work-item size.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// q-atom vs water (nonbond_qw_spc, md.f90:11302-11411)
 template <bool Q_SIDE>
 __global__ void __launch_bounds__(256) k_qw(Dev d) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -621,6 +650,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
   if (!st.active) return;
   const WinDev& W = d.win[w];
   const int S = W.S;
+  const bool qvdw = W.qvdw;
   const int a = lane >> 3, b = lane & 7;
   int iq_l = -1, qa = -1, ow = -1;
   if (Q_SIDE) {
@@ -629,84 +659,116 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
   } else {
     ow = d.wc_o[ic * kCl + a];
   }
-  const bool en = st.energy_now && Q_SIDE;
+  const float aO = W.aO1, bO = W.bO1, qow = W.crg_ow, qhw = W.crg_hw;
+  float lam[kMaxS];
+  for (int s = 0; s < kMaxS; ++s) lam[s] = s < S ? (float)W.lambda[s] : 0.f;
+  // per-state parameters of the lane's q-atom (fixed on the q side)
+  float h_qc[kMaxS] = {}, h_sc[kMaxS] = {};
+  float4 h_ql[kMaxS] = {};
+  if (Q_SIDE && iq_l >= 0) {
+    for (int s = 0; s < S; ++s) {
+      h_ql[s] = d.q_lj[W.qs_off + iq_l * S + s];
+      h_qc[s] = d.q_crg[W.qs_off + iq_l * S + s];
+      h_sc[s] = qvdw ? d.q_sc[W.qsc_off + ((size_t)iq_l * W.natyps + W.iac_ow) * S + s] : 0.f;
+    }
+  }
+  const float4 pq_fixed = (Q_SIDE && qa >= 0) ? d.x32[qa] : make_float4(0, 0, 0, 0);
+  const int s0 = wk.y, n = wk.z, lst = L.start[ic];
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
-  double eel[kMaxS], evdw[kMaxS];
-  for (int s = 0; s < kMaxS; ++s) { eel[s] = 0.0; evdw[s] = 0.0; }
-  const float aO = W.aO1, bO = W.bO1;
-  const int s0 = wk.y, n = wk.z;
+  long long gfix[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int t = 0; t < n; ++t) {
     const int jc = L.jc[s0 + t];
     const uint2 m = L.mask[s0 + t];
-    if (!lane_bit(m, lane)) continue;
-    int iq = iq_l, q_at = qa, o = ow;
-    if (Q_SIDE) o = d.wc_o[jc * kCl + b];
-    else { iq = d.qc_q[jc * kCl + b]; q_at = d.q_atom[W.q_off + iq]; }
-    const float4 pq = d.x32[q_at];
-    float dxv[3], dyv[3], dzv[3], r2v[3];
-    for (int k = 0; k < 3; ++k) {
-      float4 pw = d.x32[o + k];
-      dxv[k] = pw.x - pq.x; dyv[k] = pw.y - pq.y; dzv[k] = pw.z - pq.z;
-      r2v[k] = dxv[k] * dxv[k] + dyv[k] * dyv[k] + dzv[k] * dzv[k];
-    }
-    float rO = rsqrtf(r2v[0]), r2O = rO * rO;
-    float rH1 = rsqrtf(r2v[1]), r2H1 = rH1 * rH1;
-    float rH2 = rsqrtf(r2v[2]), r2H2 = rH2 * rH2;
-    float r6hc = r2v[0] * r2v[0] * r2v[0];
-    float dvO = 0.f, dvH1 = 0.f, dvH2 = 0.f;
-    for (int s = 0; s < S; ++s) {
-      const float lam = (float)W.lambda[s];
-      const float4 ql = d.q_lj[W.qs_off + iq * S + s];
-      float va, vb, ratio = 1.f;
-      if (W.qvdw) {
-        float r6sc = r6hc + d.q_sc[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s];
-        float inv = 1.f / r6sc;
-        va = ql.x * aO * inv * inv;
-        vb = ql.y * bO * inv;
-        ratio = r6hc * inv;
+    if (lane_bit(m, lane)) {
+      int iq = iq_l, o = ow;
+      float4 pq = pq_fixed;
+      if (Q_SIDE) {
+        o = d.wc_o[jc * kCl + b];
       } else {
-        float r6 = 1.f / r6hc;
-        va = ql.x * aO * r6 * r6;
-        vb = ql.y * bO * r6;
+        iq = d.qc_q[jc * kCl + b];
+        pq = d.x32[d.q_atom[W.q_off + iq]];
       }
-      float qc = d.q_crg[W.qs_off + iq * S + s];
-      float velO = (W.crg_ow * qc) * rO, velH1 = (W.crg_hw * qc) * rH1, velH2 = (W.crg_hw * qc) * rH2;
-      dvO += r2O * (-velO - (12.f * va - 6.f * vb) * ratio) * lam;
-      dvH1 -= r2H1 * velH1 * lam;
-      dvH2 -= r2H2 * velH2 * lam;
-      if (en) {
+      float dxv[3], dyv[3], dzv[3], r2v[3];
+      for (int k = 0; k < 3; ++k) {
+        const float4 pw = d.x32[o + k];
+        dxv[k] = pw.x - pq.x; dyv[k] = pw.y - pq.y; dzv[k] = pw.z - pq.z;
+        r2v[k] = dxv[k] * dxv[k] + dyv[k] * dyv[k] + dzv[k] * dzv[k];
+      }
+      const float rO = rsqrtf(r2v[0]), r2O = rO * rO;
+      const float rH1 = rsqrtf(r2v[1]), r2H1 = rH1 * rH1;
+      const float rH2 = rsqrtf(r2v[2]), r2H2 = rH2 * rH2;
+      const float r6hc = r2v[0] * r2v[0] * r2v[0];
+      float dvO = 0.f, dvH1 = 0.f, dvH2 = 0.f;
+      for (int s = 0; s < S; ++s) {
+        const float4 ql = Q_SIDE ? h_ql[s] : d.q_lj[W.qs_off + iq * S + s];
+        const float qc = Q_SIDE ? h_qc[s] : d.q_crg[W.qs_off + iq * S + s];
+        float va, vb, ratio = 1.f;
+        if (qvdw) {
+          const float sc = Q_SIDE ? h_sc[s] : d.q_sc[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s];
+          const float inv = rcp(r6hc + sc);
+          va = ql.x * aO * inv * inv;
+          vb = ql.y * bO * inv;
+          ratio = r6hc * inv;
+        } else {
+          const float r6 = r2O * r2O * r2O;
+          va = ql.x * aO * r6 * r6;
+          vb = ql.y * bO * r6;
+        }
+        const float velO = (qow * qc) * rO, velH1 = (qhw * qc) * rH1, velH2 = (qhw * qc) * rH2;
+        dvO += r2O * (-velO - (12.f * va - 6.f * vb) * ratio) * lam[s];
+        dvH1 -= r2H1 * velH1 * lam[s];
+        dvH2 -= r2H2 * velH2 * lam[s];
+      }
+      if (Q_SIDE) {
+        g[0] -= dvO * dxv[0] + dvH1 * dxv[1] + dvH2 * dxv[2];
+        g[1] -= dvO * dyv[0] + dvH1 * dyv[1] + dvH2 * dyv[2];
+        g[2] -= dvO * dzv[0] + dvH1 * dzv[1] + dvH2 * dzv[2];
+      } else {
+        g[0] += dvO * dxv[0]; g[1] += dvO * dyv[0]; g[2] += dvO * dzv[0];
+        g[3] += dvH1 * dxv[1]; g[4] += dvH1 * dyv[1]; g[5] += dvH1 * dzv[1];
+        g[6] += dvH2 * dxv[2]; g[7] += dvH2 * dyv[2]; g[8] += dvH2 * dzv[2];
+      }
+    }
+    if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1)
+      for (int k = 0; k < 9; ++k) { gfix[k] += fix((double)g[k], kFScale); g[k] = 0.f; }
+  }
+  const int nat = Q_SIDE ? 1 : 3;
+  const int own = Q_SIDE ? qa : ow;
+  long long gs[9];
+  for (int k = 0; k < 3 * nat; ++k) gs[k] = row8_sum_i(gfix[k]);
+  if (b == 0 && own >= 0)
+    for (int p = 0; p < nat; ++p) add_grad_fix(d.f, own + p, gs[3 * p], gs[3 * p + 1], gs[3 * p + 2]);
+  if (!(Q_SIDE && st.energy_now)) return;
+  // energy pass (df32), q side only
+  long long efix[kMaxS][2] = {};
+  double eel[kMaxS] = {}, evdw[kMaxS] = {};
+  for (int t = 0; t < n; ++t) {
+    const int jc = L.jc[s0 + t];
+    const uint2 m = L.mask[s0 + t];
+    if (lane_bit(m, lane)) {
+      const int o = d.wc_o[jc * kCl + b];
+      for (int s = 0; s < S; ++s) {
         double e_el, e_vdw;
-        qw_energy_df(d, W, iq, s, q_at, o, &e_el, &e_vdw);
+        qw_energy_df(d, W, iq_l, s, qa, o, &e_el, &e_vdw);
         eel[s] += e_el;
         evdw[s] += e_vdw;
       }
     }
-    if (Q_SIDE) {
-      g[0] -= dvO * dxv[0] + dvH1 * dxv[1] + dvH2 * dxv[2];
-      g[1] -= dvO * dyv[0] + dvH1 * dyv[1] + dvH2 * dyv[2];
-      g[2] -= dvO * dzv[0] + dvH1 * dzv[1] + dvH2 * dzv[2];
-    } else {
-      g[0] += dvO * dxv[0]; g[1] += dvO * dyv[0]; g[2] += dvO * dzv[0];
-      g[3] += dvH1 * dxv[1]; g[4] += dvH1 * dyv[1]; g[5] += dvH1 * dzv[1];
-      g[6] += dvH2 * dxv[2]; g[7] += dvH2 * dyv[2]; g[8] += dvH2 * dzv[2];
-    }
+    if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1)
+      for (int s = 0; s < kMaxS; ++s) {
+        efix[s][0] += fix(eel[s], kEScale); efix[s][1] += fix(evdw[s], kEScale);
+        eel[s] = 0.0; evdw[s] = 0.0;
+      }
   }
-  const int nat = Q_SIDE ? 1 : 3;
-  const int own = Q_SIDE ? qa : ow;
-  double gs[9];
-  for (int k = 0; k < 3 * nat; ++k) gs[k] = row8_sum((double)g[k]);
-  if (b == 0 && own >= 0)
-    for (int p = 0; p < nat; ++p) add_grad(d.f, own + p, gs[3 * p], gs[3 * p + 1], gs[3 * p + 2]);
-  if (en) {
-    for (int s = 0; s < S; ++s) {
-      double e1 = wave_sum(eel[s]), e2 = wave_sum(evdw[s]);
-      if (lane == 0) { add_energy(d.ws, w, E_QW0 + 2 * s, e1); add_energy(d.ws, w, E_QW0 + 2 * s + 1, e2); }
-    }
+  for (int s = 0; s < S; ++s) {
+    const long long e1 = wave_sum_i(efix[s][0]), e2 = wave_sum_i(efix[s][1]);
+    if (lane == 0) { add_energy_fix(d.ws, w, E_QW0 + 2 * s, e1); add_energy_fix(d.ws, w, E_QW0 + 2 * s + 1, e2); }
   }
 }
 
-// q-atom vs solute atom tiles
+// q-atom vs solute atom (nonbond_qp_qvdw / nonbond_qp, md.f90:11138-11204,
+// 10980-11045)
 template <bool Q_SIDE>
 __global__ void __launch_bounds__(256) k_qp(Dev d) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -720,6 +782,8 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   if (!st.active) return;
   const WinDev& W = d.win[w];
   const int S = W.S;
+  const bool qvdw = W.qvdw;
+  const float el14 = W.el14;
   const int a = lane >> 3, b = lane & 7;
   int iq_l = -1, qa = -1, pa = -1;
   if (Q_SIDE) {
@@ -728,65 +792,101 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   } else {
     pa = d.sc_a[ic * kCl + a];
   }
-  const bool en = st.energy_now && Q_SIDE;
+  float lam[kMaxS];
+  for (int s = 0; s < kMaxS; ++s) lam[s] = s < S ? (float)W.lambda[s] : 0.f;
+  float h_qc[kMaxS] = {};
+  float4 h_ql[kMaxS] = {};
+  if (Q_SIDE && iq_l >= 0) {
+    for (int s = 0; s < S; ++s) {
+      h_ql[s] = d.q_lj[W.qs_off + iq_l * S + s];
+      h_qc[s] = d.q_crg[W.qs_off + iq_l * S + s];
+    }
+  }
+  const float4 pq_fixed = (Q_SIDE && qa >= 0) ? d.x32[qa] : make_float4(0, 0, 0, 0);
+  const float4 pp_fixed = (!Q_SIDE && pa >= 0) ? d.x32[pa] : make_float4(0, 0, 0, 0);
+  const float4 lj_fixed = (!Q_SIDE && pa >= 0) ? d.ljp[pa] : make_float4(0, 0, 0, 0);
+  const int iac_fixed = (!Q_SIDE && pa >= 0) ? d.iac[pa] : 0;
+  const int s0 = wk.y, n = wk.z, lst = L.start[ic];
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
-  double eel[kMaxS], evdw[kMaxS];
-  for (int s = 0; s < kMaxS; ++s) { eel[s] = 0.0; evdw[s] = 0.0; }
-  const int s0 = wk.y, n = wk.z;
+  long long gx = 0, gy = 0, gz = 0;
   for (int t = 0; t < n; ++t) {
     const int jc = L.jc[s0 + t];
     const uint2 m = L.mask[s0 + t];
-    if (!lane_bit(m, lane)) continue;
-    const bool is14 = lane_bit(L.m14[s0 + t], lane);
-    int iq = iq_l, q_at = qa, j = pa;
-    if (Q_SIDE) j = d.sc_a[jc * kCl + b];
-    else { iq = d.qc_q[jc * kCl + b]; q_at = d.q_atom[W.q_off + iq]; }
-    const float4 pq = d.x32[q_at];
-    const float4 pj = d.x32[j];
-    const float4 lj = d.ljp[j];
-    float dx = pj.x - pq.x, dy = pj.y - pq.y, dz = pj.z - pq.z;
-    float r2 = dx * dx + dy * dy + dz * dz;
-    float rinv = rsqrtf(r2), r2inv = rinv * rinv;
-    float r6hc = r2 * r2 * r2;
-    float dv = 0.f;
-    for (int s = 0; s < S; ++s) {
-      const float lam = (float)W.lambda[s];
-      const float4 ql = d.q_lj[W.qs_off + iq * S + s];
-      float aLJ = is14 ? ql.z * lj.z : ql.x * lj.x;
-      float bLJ = is14 ? ql.w * lj.w : ql.y * lj.y;
-      float vel = (d.q_crg[W.qs_off + iq * S + s] * pj.w) * rinv;
-      if (is14) vel *= W.el14;
-      float va, vb, ratio = 1.f;
-      if (W.qvdw) {
-        float r6sc = r6hc + d.q_sc[W.qsc_off + ((size_t)iq * W.natyps + d.iac[j]) * S + s];
-        float inv = 1.f / r6sc;
-        va = aLJ * inv * inv;
-        vb = bLJ * inv;
-        ratio = r6hc * inv;
+    if (lane_bit(m, lane)) {
+      const bool is14 = lane_bit(L.m14[s0 + t], lane);
+      int iq = iq_l, j = pa;
+      float4 pq = pq_fixed, pj = pp_fixed, lj = lj_fixed;
+      int iacj = iac_fixed;
+      if (Q_SIDE) {
+        j = d.sc_a[jc * kCl + b];
+        pj = d.x32[j];
+        lj = d.ljp[j];
+        iacj = d.iac[j];
       } else {
-        float r6 = r2inv * r2inv * r2inv;
-        va = aLJ * r6 * r6;
-        vb = bLJ * r6;
+        iq = d.qc_q[jc * kCl + b];
+        pq = d.x32[d.q_atom[W.q_off + iq]];
       }
-      dv += r2inv * (-vel - (12.f * va - 6.f * vb) * ratio) * lam;
-      if (en) {
+      const float dx = pj.x - pq.x, dy = pj.y - pq.y, dz = pj.z - pq.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float rinv = rsqrtf(r2), r2inv = rinv * rinv;
+      const float r6hc = r2 * r2 * r2;
+      float dv = 0.f;
+      for (int s = 0; s < S; ++s) {
+        const float4 ql = Q_SIDE ? h_ql[s] : d.q_lj[W.qs_off + iq * S + s];
+        const float qc = Q_SIDE ? h_qc[s] : d.q_crg[W.qs_off + iq * S + s];
+        const float aLJ = is14 ? ql.z * lj.z : ql.x * lj.x;
+        const float bLJ = is14 ? ql.w * lj.w : ql.y * lj.y;
+        float vel = (qc * pj.w) * rinv;
+        if (is14) vel *= el14;
+        float va, vb, ratio = 1.f;
+        if (qvdw) {
+          const float inv = rcp(r6hc + d.q_sc[W.qsc_off + ((size_t)iq * W.natyps + iacj) * S + s]);
+          va = aLJ * inv * inv;
+          vb = bLJ * inv;
+          ratio = r6hc * inv;
+        } else {
+          const float r6 = r2inv * r2inv * r2inv;
+          va = aLJ * r6 * r6;
+          vb = bLJ * r6;
+        }
+        dv += r2inv * (-vel - (12.f * va - 6.f * vb) * ratio) * lam[s];
+      }
+      if (Q_SIDE) { g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz; }
+      else { g0 += dv * dx; g1 += dv * dy; g2 += dv * dz; }
+    }
+    if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1) {
+      gx += fix((double)g0, kFScale); gy += fix((double)g1, kFScale); gz += fix((double)g2, kFScale);
+      g0 = g1 = g2 = 0.f;
+    }
+  }
+  const int own = Q_SIDE ? qa : pa;
+  const long long sx = row8_sum_i(gx), sy = row8_sum_i(gy), sz = row8_sum_i(gz);
+  if (b == 0 && own >= 0) add_grad_fix(d.f, own, sx, sy, sz);
+  if (!(Q_SIDE && st.energy_now)) return;
+  long long efix[kMaxS][2] = {};
+  double eel[kMaxS] = {}, evdw[kMaxS] = {};
+  for (int t = 0; t < n; ++t) {
+    const int jc = L.jc[s0 + t];
+    const uint2 m = L.mask[s0 + t];
+    if (lane_bit(m, lane)) {
+      const bool is14 = lane_bit(L.m14[s0 + t], lane);
+      const int j = d.sc_a[jc * kCl + b];
+      for (int s = 0; s < S; ++s) {
         double e_el, e_vdw;
-        qp_energy_df(d, W, iq, s, q_at, j, is14, &e_el, &e_vdw);
+        qp_energy_df(d, W, iq_l, s, qa, j, is14, &e_el, &e_vdw);
         eel[s] += e_el;
         evdw[s] += e_vdw;
       }
     }
-    if (Q_SIDE) { g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz; }
-    else { g0 += dv * dx; g1 += dv * dy; g2 += dv * dz; }
+    if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1)
+      for (int s = 0; s < kMaxS; ++s) {
+        efix[s][0] += fix(eel[s], kEScale); efix[s][1] += fix(evdw[s], kEScale);
+        eel[s] = 0.0; evdw[s] = 0.0;
+      }
   }
-  const int own = Q_SIDE ? qa : pa;
-  double sx = row8_sum((double)g0), sy = row8_sum((double)g1), sz = row8_sum((double)g2);
-  if (b == 0 && own >= 0) add_grad(d.f, own, sx, sy, sz);
-  if (en) {
-    for (int s = 0; s < S; ++s) {
-      double e1 = wave_sum(eel[s]), e2 = wave_sum(evdw[s]);
-      if (lane == 0) { add_energy(d.ws, w, E_Q0 + 8 * s + 6, e1); add_energy(d.ws, w, E_Q0 + 8 * s + 7, e2); }
-    }
+  for (int s = 0; s < S; ++s) {
+    const long long e1 = wave_sum_i(efix[s][0]), e2 = wave_sum_i(efix[s][1]);
+    if (lane == 0) { add_energy_fix(d.ws, w, E_Q0 + 8 * s + 6, e1); add_energy_fix(d.ws, w, E_Q0 + 8 * s + 7, e2); }
   }
 }
 
@@ -867,43 +967,81 @@ __global__ void __launch_bounds__(128) k_qq(Dev d) {
 // LRF expansions (the phi sums of the list builders, e.g. nbpplist_lrf,
 // md.f90:5589-5700): for every target charge group g, the Taylor terms about
 // its centre of all charges of groups h whose switching atoms lie beyond the
-// cut-off and within the LRF radius.  One wave takes 16 target groups of one
-// window; lane (t, part) = (lane & 15, lane >> 4) sums the sources h = part,
-// part + 4, ... for target t, so the source data a wave reads at one time
-// comes from four groups only; the four partial sums are added at the end.
-constexpr int kLrfTargets = 16;
+// cut-off and within the LRF radius.
+// Pack: the switching atom of every group (x, y, z, excluded) and the
+// non-q atoms of every group (x, y, z, charge), contiguous per group.
+__global__ void __launch_bounds__(256) k_lrf_pack(Dev d) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < d.ngrp_all) {
+    const int w = d.g_win[k];
+    if (d.ws[w].active && d.ws[w].list_now && d.win[w].use_lrf) {
+      const int sw = d.g_sw[k];
+      d.ls_sw[k] = make_double4(d.x[3 * sw], d.x[3 * sw + 1], d.x[3 * sw + 2], (d.aflag[sw] & 1) ? 1.0 : 0.0);
+    }
+  }
+  if (k < d.nls) {
+    const int at = d.ls_atom[k];
+    const int w = d.wid[at];
+    if (d.ws[w].active && d.ws[w].list_now && d.win[w].use_lrf)
+      d.ls_xq[k] = make_double4(d.x[3 * at], d.x[3 * at + 1], d.x[3 * at + 2], (double)d.x32[at].w);
+  }
+}
+
+// Build: one thread per target group, one block per (window, blockDim
+// consecutive targets).  Source groups pass through LDS in chunks of
+// blockDim, so all threads read the same source at the same time; each
+// thread sums its own 20 terms in a fixed order (sources in index order).
+__device__ __forceinline__ double r2_pos(double ax, double ay, double az, double bx, double by, double bz) {
+#pragma clang fp contract(off)
+  const double dx = __dsub_rn(ax, bx), dy = __dsub_rn(ay, by), dz = __dsub_rn(az, bz);
+  return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
 __global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= d.nlrfw) return;
-  const int4 lw = d.lrf_work[wave];   // window, first target group, count
+  const int4 lw = d.lrf_work[blockIdx.x];   // window, first target group, count
   const int w = lw.x;
   const WinState& st = d.ws[w];
   const WinDev& W = d.win[w];
-  if (!st.active || !st.list_now || !W.use_lrf) return;
-  const int t = lane & (kLrfTargets - 1), part = lane >> 4;
+  if (!st.active || !st.list_now || !W.use_lrf) return;   // uniform over the block
+  __shared__ double4 s_sw[256];
+  __shared__ int s_first[256], s_n[256];
+  const int t = threadIdx.x;
   const int g = lw.y + t;
-  const bool g_ok = t < lw.z && !(d.aflag[d.g_sw[g < d.ngrp_all ? g : 0]] & 1);
+  bool g_ok = t < lw.z;
+  double4 sg = make_double4(0, 0, 0, 1);
+  double cx = 0, cy = 0, cz = 0;
+  bool gw = false;
+  if (g_ok) {
+    sg = d.ls_sw[g];
+    g_ok = sg.w == 0.0;
+    gw = (g - W.g_off) >= W.ngrp_solute;
+    cx = d.g_cent[3 * g]; cy = d.g_cent[3 * g + 1]; cz = d.g_cent[3 * g + 2];
+  }
   double acc[20];
   for (int k = 0; k < 20; ++k) acc[k] = 0.0;
-  if (g_ok) {
-    const int swg = d.g_sw[g];
-    const bool gw = (g - W.g_off) >= W.ngrp_solute;
-    const double cx = d.g_cent[3 * g], cy = d.g_cent[3 * g + 1], cz = d.g_cent[3 * g + 2];
-    for (int h = W.g_off + part; h < W.g_off + W.ngrp; h += 4) {
-      if (h == g) continue;
-      const int swh = d.g_sw[h];
-      if (d.aflag[swh] & 1) continue;
+  const int h_end = W.g_off + W.ngrp;
+  for (int h0 = W.g_off; h0 < h_end; h0 += blockDim.x) {
+    const int hl = h0 + t;
+    if (hl < h_end) {
+      s_sw[t] = d.ls_sw[hl];
+      s_first[t] = d.ls_first[hl];
+      s_n[t] = d.ls_n[hl];
+    }
+    __syncthreads();
+    const int nh = min((int)blockDim.x, h_end - h0);
+    for (int j = 0; j < nh; ++j) {
+      const int h = h0 + j;
+      const double4 sh = s_sw[j];
+      if (!g_ok || h == g || sh.w != 0.0) continue;
       const bool hw = (h - W.g_off) >= W.ngrp_solute;
       const double rc2 = (gw && hw) ? W.rcww2 : ((gw || hw) ? W.rcpw2 : W.rcpp2);
-      const double r2g = r2_exact(d.x, swg, swh);
+      const double r2g = r2_pos(sg.x, sg.y, sg.z, sh.x, sh.y, sh.z);
       if (r2g <= rc2 || r2g > W.rclrf2) continue;
-      const int n = d.g_n[h], f0 = d.g_first[h];
+      const int f0 = s_first[j], n = s_n[j];
       for (int k = 0; k < n; ++k) {
-        const int at = d.g_atoms[f0 + k];
-        if (d.aflag[at] & 8) continue;  // q-atom
-        const double q = (double)d.x32[at].w;
-        const double dx = d.x[3 * at] - cx, dy = d.x[3 * at + 1] - cy, dz = d.x[3 * at + 2] - cz;
+        const double4 a = d.ls_xq[f0 + k];
+        const double q = a.w;
+        const double dx = a.x - cx, dy = a.y - cy, dz = a.z - cz;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const double f0v = q / (r2 * sqrt(r2));
         const double f1 = 3.0 * f0v / r2;
@@ -924,15 +1062,9 @@ __global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
         acc[19] += f2 * (5.0 * dz * dz * dz - r2 * 3.0 * dz);
       }
     }
+    __syncthreads();
   }
-  // (t, part) -> sum over part: lanes t, t+16, t+32, t+48
-  for (int k = 0; k < 20; ++k) {
-    double v = acc[k];
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    acc[k] = v;
-  }
-  if (part == 0 && g_ok)
+  if (g_ok)
     for (int k = 0; k < 20; ++k) d.g_lrf[20 * g + k] = acc[k];
 }
 
@@ -1842,6 +1974,17 @@ void launch_begin(const Dev& d, hipStream_t s, int force_energy, int eval_only, 
   k_begin<<<blocks(d.nwin, 64), 64, 0, s>>>(d, force_energy, eval_only, rebuild);
 }
 
+void launch_lrf_build(const Dev& d, hipStream_t s) {
+  if (!d.nlrfw) return;
+  k_lrf_pack<<<blocks(std::max(d.ngrp_all, d.nls), 256), 256, 0, s>>>(d);
+  k_lrf_build<<<d.nlrfw, d.bs_lrf, 0, s>>>(d);
+}
+
+void launch_bonded(const Dev& d, hipStream_t s) {
+  int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
+  if (nb) k_bonded<<<blocks(nb, d.bs_bonded), d.bs_bonded, 0, s>>>(d);
+}
+
 void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
   k_centres<<<blocks(ngrp, 128), 128, 0, s>>>(d, ngrp);
   k_bounds<<<blocks(nwc + nsc, 128), 128, 0, s>>>(d, nwc, nsc);
@@ -1849,29 +1992,33 @@ void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
   if (d.rl[T_WP].nicl) k_build_rows<T_WP><<<blocks((long long)d.rl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.rl[T_PW].nicl) k_build_rows<T_PW><<<blocks((long long)d.rl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.rl[T_PP].nicl) k_build_rows<T_PP><<<blocks((long long)d.rl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.nlrfw) k_lrf_build<<<blocks((long long)d.nlrfw * 64, 256), 256, 0, s>>>(d);
+  launch_lrf_build(d, s);
 }
 
-void launch_ww(const Dev& d, hipStream_t s) {
-  if (d.rl[T_WW].nicl) k_ww<<<blocks((long long)d.rl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
-}
-
-void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww) {
-  if (with_ww) launch_ww(d, s);
-  if (d.rl[T_WP].nicl) k_pw<true><<<blocks((long long)d.rl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.rl[T_PW].nicl) k_pw<false><<<blocks((long long)d.rl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
-  if (d.rl[T_PP].nicl) k_pp<<<blocks((long long)d.rl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
+void launch_q_tiles(const Dev& d, hipStream_t s) {
   if (d.tl[T_QW].nwork) k_qw<true><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_WQ].nwork) k_qw<false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PQ].nwork) k_qp<false><<<blocks((long long)d.tl[T_PQ].nwork * 64, 256), 256, 0, s>>>(d);
+}
+
+void launch_ww(const Dev& d, hipStream_t s) {
+  if (d.rl[T_WW].nicl) k_ww<<<blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+}
+
+void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww) {
+  if (with_ww) launch_ww(d, s);
+  if (d.rl[T_WP].nicl) k_pw<true><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_PW].nicl) k_pw<false><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_PP].nicl) k_pp<<<blocks((long long)d.rl[T_PP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+  launch_q_tiles(d, s);
   if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
 }
 
 void launch_other_forces(const Dev& d, hipStream_t s) {
   k_lrf_taylor<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
-  if (nb) k_bonded<<<blocks(nb, 256), 256, 0, s>>>(d);
+  if (nb) k_bonded<<<blocks(nb, d.bs_bonded), d.bs_bonded, 0, s>>>(d);
   // the gather also collects the q-q pair slots written by k_qq
   if (nb || d.nqq) k_bonded_gather<<<blocks(d.nbg, 256), 256, 0, s>>>(d);
   k_restr<<<blocks(d.ntot, 256), 256, 0, s>>>(d);

@@ -72,6 +72,8 @@ def load_library(path: str = LIB_PATH):
         "qmi_write_restart": (i32, [p, i32, C.c_char_p]),
         "qmi_timings": (i32, [p, dp, i32]),
         "qmi_set_timing": (i32, [p, i32]),
+        "qmi_set_autotune": (i32, [p, i32]),
+        "qmi_tuning": (i32, [p, dp, i32]),
         "qmi_last_error": (C.c_char_p, []),
         "qmi_engine_destroy": (None, [p]),
     }
@@ -88,7 +90,9 @@ EXPORTED = [
     "qmi_evaluate",
     "qmi_get_energies", "qmi_get_q_energies", "qmi_get_coords", "qmi_get_velocities", "qmi_get_forces",
     "qmi_set_state", "qmi_get_pairlist", "qmi_window_info", "qmi_write_restart", "qmi_timings",
-    "qmi_set_timing", "qmi_last_error", "qmi_engine_destroy",
+    "qmi_set_timing", "qmi_set_autotune", "qmi_tuning", "qmi_last_error", "qmi_engine_destroy",
+    "qmi_mlp_create", "qmi_mlp_num_params", "qmi_mlp_get_params", "qmi_mlp_set_params", "qmi_mlp_forward",
+    "qmi_mlp_grad", "qmi_mlp_train", "qmi_mlp_last_error", "qmi_mlp_destroy",
 ]
 
 
@@ -192,6 +196,16 @@ class Engine:
 
     def write_restart(self, w: int, path: str):
         _check(self.lib.qmi_write_restart(self._h, w, os.fspath(path).encode()))
+
+    def set_autotune(self, on: bool):
+        _check(self.lib.qmi_set_autotune(self._h, 1 if on else 0))
+
+    def tuning(self) -> dict:
+        """launch shapes chosen by the autotuner and their measured times (us)"""
+        a = np.zeros(8)
+        _check(self.lib.qmi_tuning(self._h, a.ctypes.data_as(C.POINTER(C.c_double)), 8))
+        return dict(q_tiles_per_wave=int(a[0]), rows_block=int(a[1]), lrf_block=int(a[2]), bonded_block=int(a[3]),
+                    us=dict(q_tiles=a[4], rows=a[5], lrf_build=a[6], bonded=a[7]))
 
     def set_timing(self, on: bool):
         _check(self.lib.qmi_set_timing(self._h, 1 if on else 0))
