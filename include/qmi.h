@@ -119,8 +119,10 @@ int qmi_write_restart(qmi_engine* e, int w, const char* path);
 int qmi_set_autotune(qmi_engine* e, int on);
 
 /* Choices of the autotuner: Q tiles per wave, threads per block of the row,
- * LRF-build and bonded kernels, then the measured microseconds of each
- * chosen configuration (8 values).  Returns the number written. */
+ * LRF-build and bonded kernels, whether blocks are grouped by XCD (0/1),
+ * whether bonded gradients use atomics (0/1), the number of source parts
+ * of the LRF build, then the measured microseconds of the first six chosen
+ * configurations (13 values).  Returns the number written. */
 int qmi_tuning(qmi_engine* e, double* out, int cap);
 
 /* Turn per-step HIP-event timing of the kernel families on or off for the

@@ -80,6 +80,8 @@ struct TileList {
   const int4* work;
   int nwork;
 };
+constexpr int kGatherAtoms = 256;  // atoms per block of the bonded gather
+constexpr int kGatherSlots = 2048; // slots staged in LDS per gather block
 constexpr int kQGroup = 4;        // Q-tile sums go to fixed point every 4 tiles of a list
 constexpr int kQChunkN = 4;
 constexpr int kQChunks[kQChunkN] = {4, 8, 16, 32};   // autotuner candidates, tiles per wave
@@ -120,8 +122,10 @@ struct Dev {
   double* g_cent;       // 3 per group
   double* g_lrf;        // 20 per group: phi0, phi1(3), phi2(6: xx xy xz yy yz zz), phi3(10)
   int ngrp_all;         // groups of all windows
-  int nlrfw;            // LRF build blocks: (window, first target group, count)
+  int nlrfw;            // LRF build blocks: (window, first target group, count, source part)
   const int4* lrf_work;
+  int lrf_parts;         // source groups of a window split in this many parts (1: no split)
+  double* g_lrf_part;    // [part][group][20] partial sums when lrf_parts > 1
   // LRF sources, packed at every build: switching atom of each group
   // (x, y, z, excluded) and the group's non-q atoms (x, y, z, charge)
   double4* ls_sw;
@@ -154,6 +158,12 @@ struct Dev {
   // launch shapes chosen by the autotuner (threads per block); none of
   // them changes the arithmetic of a wave or thread
   int bs_rows, bs_lrf, bs_bonded;
+  // 1: blocks on one XCD take consecutive work (block_id in the kernels)
+  int xcd;
+  // 1: bonded terms and q-q pairs add their gradients with atomics; 0: they
+  // write per-term slots that k_bonded_gather sums per atom.  Both give the
+  // same int64 sums.
+  int bond_atomic;
   // q-q pairs (static)
   int nqq;
   const int4* qq_ij;     // iq (window-local), j (global), jq (1-based q or 0), ljcod
@@ -183,7 +193,12 @@ struct Dev {
   int qq_slot0;          // first slot of the q-q pairs (2 per pair, after the bonded terms)
   int nbg;
   const int* bg_atom; const int* bg_start;
+  // gather blocks: (first k, end k) with at most kGatherAtoms atoms and, for
+  // all but single-atom blocks, at most kGatherSlots slots
+  int ngw;
+  const int2* gather_work;
   const int* slot_map;   // term atom (in term order) -> slot; an atom's slots are contiguous
+  const int* slot_atom;  // term atom (in term order) -> global atom
   // restraints: (kind, window, i, j) global atoms; (k, flag) with flag = ih
   // (sequence, wall) or ipsi (position, distance, angle); 8 parameters each
   int nrst;

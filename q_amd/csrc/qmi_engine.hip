@@ -80,11 +80,15 @@ struct qmi_engine {
   // autotuner: work-item variants of the static Q lists and the choices made
   int4* qwork[T_NTYPES][4] = {};
   int nqwork[T_NTYPES][4] = {};
-  int4* lrfwork[3] = {};        // LRF build blocks for 64, 128, 256 threads
-  int nlrfwork[3] = {};
+  // LRF build blocks for 64, 128, 256 threads (index 4 * b) and 1, 2, 4, 8
+  // source parts (+ p)
+  int4* lrfwork[12] = {};
+  int nlrfwork[12] = {};
   bool autotune = true;
-  int tuned[4] = {2, 256, 256, 256};   // Q chunk index, row / LRF / bonded block sizes
-  double tune_us[4] = {0, 0, 0, 0};    // time of the chosen configuration per group
+  // Q chunk index, row / LRF / bonded block sizes, XCD map, atomic bonded
+  // gradients, LRF source parts
+  int tuned[7] = {2, 256, 256, 256, 0, 0, 1};
+  double tune_us[6] = {0, 0, 0, 0, 0, 0};   // time of the chosen configuration per group
   // the launch sequence of one MD step, captured once and replayed
   hipGraph_t step_graph = nullptr;
   hipGraphExec_t step_exec = nullptr;
@@ -646,6 +650,8 @@ static int build_device(qmi_engine* e) {
   Dev& d = e->d;
   d.nwin = NW;
   d.bs_rows = d.bs_lrf = d.bs_bonded = 256;
+  d.xcd = 0;
+  d.bond_atomic = 0;
   d.ntot = ao;
   int rc = 0;
 #define UP(dst, src) if ((rc = dev_upload(e, &dst, src)) != 0) return rc
@@ -672,21 +678,24 @@ static int build_device(qmi_engine* e) {
   if ((rc = dev_alloc(e, &d.g_lrf, 20 * (size_t)goff)) != 0) return rc;
   d.ngrp_all = goff;
   {
-    // LRF build: one block per (window, block-size consecutive targets), a
-    // variant per candidate block size of the autotuner
-    for (int vi = 0; vi < 3; ++vi) {
-      const int bsz = 64 << vi;
+    // LRF build: one block per (window, block-size consecutive targets,
+    // part of the sources), a variant per candidate of the autotuner
+    for (int vi = 0; vi < 12; ++vi) {
+      const int bsz = 64 << (vi / 4), parts = 1 << (vi % 4);
       std::vector<int4> lw;
       for (int wi = 0; wi < NW; ++wi)
         for (int g0 = 0; g0 < wd[wi].ngrp; g0 += bsz)
-          lw.push_back(make_int4(wi, wd[wi].g_off + g0, std::min(bsz, wd[wi].ngrp - g0), 0));
+          for (int p = 0; p < parts; ++p)
+            lw.push_back(make_int4(wi, wd[wi].g_off + g0, std::min(bsz, wd[wi].ngrp - g0), p));
       int4* lwp;
       UP(lwp, lw);
       e->lrfwork[vi] = lwp;
       e->nlrfwork[vi] = (int)lw.size();
     }
-    d.lrf_work = e->lrfwork[2];
-    d.nlrfw = e->nlrfwork[2];
+    d.lrf_work = e->lrfwork[8];
+    d.nlrfw = e->nlrfwork[8];
+    d.lrf_parts = 1;
+    if ((rc = dev_alloc(e, &d.g_lrf_part, 8 * 20 * (size_t)std::max(goff, 1))) != 0) return rc;
     // packed sources: non-q atoms of every group, in group order
     std::vector<int> ls_first(goff), ls_n(goff), ls_atom;
     for (int g = 0; g < goff; ++g) {
@@ -783,9 +792,22 @@ static int build_device(qmi_engine* e) {
       slot_map[sidx] = bg_start[k] + fill[k]++;
     }
     d.nbg = (int)bg_atom.size();
+    // gather blocks: consecutive atoms, at most kGatherAtoms of them and at
+    // most kGatherSlots slots unless a single atom has more
+    std::vector<int2> gwork;
+    for (int k0 = 0; k0 < d.nbg;) {
+      int k1 = k0 + 1;
+      while (k1 < d.nbg && k1 - k0 < kGatherAtoms && bg_start[k1 + 1] - bg_start[k0] <= kGatherSlots) ++k1;
+      gwork.push_back(make_int2(k0, k1));
+      k0 = k1;
+    }
+    d.ngw = (int)gwork.size();
+    int2* gwp;
+    UP(gwp, gwork); d.gather_work = gwp;
     UP(ip, bg_atom); d.bg_atom = ip;
     UP(ip, bg_start); d.bg_start = ip;
     UP(ip, slot_map); d.slot_map = ip;
+    UP(ip, slot_atom); d.slot_atom = ip;
     if ((rc = dev_alloc(e, &d.bslot, 3 * std::max<size_t>(1, slot_atom.size()))) != 0) return rc;
   }
   d.nrst = (int)rs_kwij.size();
@@ -1121,32 +1143,69 @@ static int autotune(qmi_engine* e) {
     e->tuned[0] = bi;
     e->tune_us[0] = bt;
   }
-  // 1-3: threads per block of the row kernels, the LRF build and the bonded terms
-  int* field[3] = {&d.bs_rows, &d.bs_lrf, &d.bs_bonded};
-  for (int g = 0; g < 3; ++g) {
+  // 1, 3: threads per block of the row kernels and of the bonded terms
+  int* field[3] = {&d.bs_rows, nullptr, &d.bs_bonded};
+  for (int g : {0, 2}) {
     int bb = 256;
     double bt = 1e30;
     for (int b : bs) {
       *field[g] = b;
-      if (g == 1) {
-        const int vi = b == 64 ? 0 : b == 128 ? 1 : 2;
-        d.lrf_work = e->lrfwork[vi];
-        d.nlrfw = e->nlrfwork[vi];
-      }
-      double t;
-      if (g == 0) t = time_us([&] { launch_nonbonded(d, e->stream, true); });
-      else if (g == 1) t = time_us([&] { launch_lrf_build(d, e->stream); });
-      else t = time_us([&] { launch_bonded(d, e->stream); });
+      const double t = g == 0 ? time_us([&] { launch_nonbonded(d, e->stream, true); })
+                              : time_us([&] { launch_bonded(d, e->stream); });
       if (t < bt) { bt = t; bb = b; }
     }
     *field[g] = bb;
-    if (g == 1) {
-      const int vi = bb == 64 ? 0 : bb == 128 ? 1 : 2;
-      d.lrf_work = e->lrfwork[vi];
-      d.nlrfw = e->nlrfwork[vi];
-    }
     e->tuned[1 + g] = bb;
     e->tune_us[1 + g] = bt;
+  }
+  // 2, 6: threads per block and source parts of the LRF build
+  {
+    int bv = 8;
+    double bt = 1e30;
+    auto set_lrf = [&](int vi) {
+      d.bs_lrf = 64 << (vi / 4);
+      d.lrf_parts = 1 << (vi % 4);
+      d.lrf_work = e->lrfwork[vi];
+      d.nlrfw = e->nlrfwork[vi];
+    };
+    for (int vi = 0; vi < 12; ++vi) {
+      set_lrf(vi);
+      const double t = time_us([&] { launch_lrf_build(d, e->stream); });
+      if (t < bt) { bt = t; bv = vi; }
+    }
+    set_lrf(bv);
+    e->tuned[2] = d.bs_lrf;
+    e->tune_us[2] = bt;
+    e->tuned[6] = d.lrf_parts;
+  }
+  // 4: XCD-grouped block order of the pair, Q-tile, LRF-build and gather kernels
+  {
+    double tx[2];
+    for (int x = 0; x < 2; ++x) {
+      d.xcd = x;
+      tx[x] = time_us([&] {
+        launch_nonbonded(d, e->stream, true);
+        launch_q_tiles(d, e->stream);
+        launch_lrf_build(d, e->stream);
+      });
+    }
+    d.xcd = tx[1] < tx[0] ? 1 : 0;
+    e->tuned[4] = d.xcd;
+    e->tune_us[4] = tx[d.xcd];
+  }
+  // 5: bonded and q-q gradients by atomics or by slots and a per-atom gather
+  {
+    double ta[2];
+    for (int a = 0; a < 2; ++a) {
+      d.bond_atomic = a;
+      ta[a] = time_us([&] {
+        launch_nonbonded(d, e->stream, false);
+        launch_other_forces(d, e->stream);
+      });
+    }
+    d.bond_atomic = ta[1] < ta[0] ? 1 : 0;
+    e->tuned[5] = d.bond_atomic;
+    e->tune_us[5] = ta[d.bond_atomic];
   }
   HIP_OK(hipGetLastError());
   (void)hipEventDestroy(t0);
@@ -1558,9 +1617,10 @@ int qmi_set_autotune(qmi_engine* e, int on) {
 int qmi_tuning(qmi_engine* e, double* out, int cap) {
   if (!e || !out) return set_err(-1, "null argument");
   if (!e->prepared) return set_err(-2, "engine not prepared");
-  const double v[8] = {(double)kQChunks[e->tuned[0]], (double)e->tuned[1], (double)e->tuned[2], (double)e->tuned[3],
-                       e->tune_us[0], e->tune_us[1], e->tune_us[2], e->tune_us[3]};
-  const int n = std::min(cap, 8);
+  const double v[13] = {(double)kQChunks[e->tuned[0]], (double)e->tuned[1], (double)e->tuned[2],
+                        (double)e->tuned[3], (double)e->tuned[4], (double)e->tuned[5], (double)e->tuned[6],
+                        e->tune_us[0], e->tune_us[1], e->tune_us[2], e->tune_us[3], e->tune_us[4], e->tune_us[5]};
+  const int n = std::min(cap, 13);
   for (int k = 0; k < n; ++k) out[k] = v[k];
   return n;
 }

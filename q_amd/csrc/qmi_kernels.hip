@@ -21,6 +21,18 @@
 namespace qmi {
 
 // ------------------------------------------------------------ helpers --
+// Workgroups go to the 8 XCDs in turn (block b to XCD b mod 8), each with its
+// own L2.  With xcd set, the blocks that land on one XCD take one contiguous
+// range of the work (the clusters or terms of a few windows), so a window's
+// coordinates are cached in one L2 instead of all eight.  The map is a
+// permutation of 0..gridDim.x-1.
+__device__ __forceinline__ int block_id(int xcd) {
+  const int b = blockIdx.x;
+  if (!xcd) return b;
+  const int nb = gridDim.x, x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 __device__ __forceinline__ long long fix(double v, double scale) { return __double2ll_rn(v * scale); }
 
 __device__ __forceinline__ void add_grad(long long* f, int atom, double gx, double gy, double gz) {
@@ -29,12 +41,22 @@ __device__ __forceinline__ void add_grad(long long* f, int atom, double gx, doub
   atomicAdd((unsigned long long*)&f[3 * atom + 2], (unsigned long long)fix(gz, kFScale));
 }
 
-// gradient of one term atom into its own slot (fixed point, no atomic)
+// gradient of term atom k (in term order): into its own slot (fixed point, no
+// atomic), or added to the atom's gradient with atomics; the int64 values are
+// the same either way
 __device__ __forceinline__ void put_slot(const Dev& d, int k, double gx, double gy, double gz) {
+  const long long fx = fix(gx, kFScale), fy = fix(gy, kFScale), fz = fix(gz, kFScale);
+  if (d.bond_atomic) {
+    const int a = d.slot_atom[k];
+    atomicAdd((unsigned long long*)&d.f[3 * a + 0], (unsigned long long)fx);
+    atomicAdd((unsigned long long*)&d.f[3 * a + 1], (unsigned long long)fy);
+    atomicAdd((unsigned long long*)&d.f[3 * a + 2], (unsigned long long)fz);
+    return;
+  }
   const int m = d.slot_map[k];
-  d.bslot[3 * m + 0] = fix(gx, kFScale);
-  d.bslot[3 * m + 1] = fix(gy, kFScale);
-  d.bslot[3 * m + 2] = fix(gz, kFScale);
+  d.bslot[3 * m + 0] = fx;
+  d.bslot[3 * m + 1] = fy;
+  d.bslot[3 * m + 2] = fz;
 }
 
 __device__ __forceinline__ void add_energy(WinState* ws, int w, int slot, double e) {
@@ -275,7 +297,7 @@ __device__ __forceinline__ int special_type(const Dev& d, int i, int j) {
 // members are appended, in j order, to row a.
 template <int TT>
 __global__ void __launch_bounds__(256) k_build_rows(Dev d) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[TT];
   if (wave >= L.nicl) return;
@@ -336,6 +358,8 @@ __global__ void __launch_bounds__(256) k_build_rows(Dev d) {
 }
 
 // the number of chunks of 8 entries the wave must walk: the longest row
+constexpr int kUnroll = 4;   // row entries per pass of the pair kernels
+
 __device__ __forceinline__ int row_chunks(int n) {
   int m = n;
   m = max(m, __shfl_xor(m, 8, 64));
@@ -351,7 +375,7 @@ __device__ __forceinline__ int row_chunks(int n) {
 // added once.
 // water-water: 9 interactions per molecule pair (nonbond_ww_spc, md.f90:11823-12047)
 __global__ void __launch_bounds__(256) k_ww(Dev d) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[T_WW];
   if (wave >= L.nicl) return;
@@ -428,7 +452,7 @@ __global__ void __launch_bounds__(256) k_ww(Dev d) {
 // which also counts the energies.
 template <bool WATER_SIDE>
 __global__ void __launch_bounds__(256) k_pw(Dev d) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[WATER_SIDE ? T_WP : T_PW];
   if (wave >= L.nicl) return;
@@ -455,40 +479,61 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   double eel = 0.0, evdw = 0.0;
   const int* ent = L.ent + (size_t)row * L.cap;
-  for (int t = 0; t < nch; ++t) {
-    const int kk = t * kCl + b;
-    if (kk >= n) continue;
-    const int eb = ent[kk];
-    float4 ps, pw[3];
-    float4 lj;
-    if (WATER_SIDE) {
-      ps = d.x32[eb]; lj = d.ljp[eb];
-      pw[0] = pi[0]; pw[1] = pi[1]; pw[2] = pi[2];
-    } else {
-      ps = pi[0]; lj = ljs;
-      pw[0] = d.x32[eb]; pw[1] = d.x32[eb + 1]; pw[2] = d.x32[eb + 2];
+  // kUnroll entries per pass, loads first (as in k_pp)
+  for (int t0 = 0; t0 < nch; t0 += kUnroll) {
+    int e[kUnroll];
+    bool v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int kk = (t0 + u) * kCl + b;
+      v[u] = kk < n;
+      const int le = ent[min(kk, L.cap - 1)];   // in the row, so the load needs no branch
+      e[u] = v[u] ? le : 0;
     }
-    for (int k = 0; k < 3; ++k) {
-      // dx = x(water atom) - x(solute atom), as nonbond_pw with i solute, j water
-      float dx = pw[k].x - ps.x, dy = pw[k].y - ps.y, dz = pw[k].z - ps.z;
-      float r2 = dx * dx + dy * dy + dz * dz;
-      float rinv = rsqrtf(r2);
-      float r2inv = rinv * rinv;
-      float vel = (ps.w * pw[k].w) * rinv;
-      float dv;
-      if (k == 0) {
-        float r6 = r2inv * r2inv * r2inv;
-        float va = (lj.x * W.aO1) * r6 * r6, vb = (lj.y * W.bO1) * r6;
-        dv = r2inv * (-vel - 12.f * va + 6.f * vb);
-        if (en) evdw += (double)(va - vb);
-      } else {
-        dv = r2inv * (-vel);
-      }
-      if (en) eel += (double)vel;
+    float4 ld[kUnroll][3];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      ld[u][0] = d.x32[e[u]];
       if (WATER_SIDE) {
-        g[3 * k + 0] += dv * dx; g[3 * k + 1] += dv * dy; g[3 * k + 2] += dv * dz;
+        ld[u][1] = d.ljp[e[u]];
       } else {
-        g[0] -= dv * dx; g[1] -= dv * dy; g[2] -= dv * dz;
+        ld[u][1] = d.x32[e[u] + 1]; ld[u][2] = d.x32[e[u] + 2];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      float4 ps, pw[3];
+      float4 lj;
+      if (WATER_SIDE) {
+        ps = ld[u][0]; lj = ld[u][1];
+        pw[0] = pi[0]; pw[1] = pi[1]; pw[2] = pi[2];
+      } else {
+        ps = pi[0]; lj = ljs;
+        pw[0] = ld[u][0]; pw[1] = ld[u][1]; pw[2] = ld[u][2];
+      }
+      for (int k = 0; k < 3; ++k) {
+        // dx = x(water atom) - x(solute atom), as nonbond_pw with i solute, j water
+        float dx = pw[k].x - ps.x, dy = pw[k].y - ps.y, dz = pw[k].z - ps.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float rinv = rsqrtf(r2);
+        float r2inv = rinv * rinv;
+        float vel = (ps.w * pw[k].w) * rinv;
+        float dv, ev = 0.f;
+        if (k == 0) {
+          float r6 = r2inv * r2inv * r2inv;
+          float va = (lj.x * W.aO1) * r6 * r6, vb = (lj.y * W.bO1) * r6;
+          dv = r2inv * (-vel - 12.f * va + 6.f * vb);
+          ev = va - vb;
+        } else {
+          dv = r2inv * (-vel);
+        }
+        if (!v[u]) continue;
+        if (en) { eel += (double)vel; if (k == 0) evdw += (double)ev; }
+        if (WATER_SIDE) {
+          g[3 * k + 0] += dv * dx; g[3 * k + 1] += dv * dy; g[3 * k + 2] += dv * dz;
+        } else {
+          g[0] -= dv * dx; g[1] -= dv * dy; g[2] -= dv * dz;
+        }
       }
     }
   }
@@ -507,7 +552,7 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
 // solute-solute (nonbond_pp, md.f90:10387-10491): 1-4 pairs use the third
 // LJ column and the el14 scale
 __global__ void __launch_bounds__(256) k_pp(Dev d) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[T_PP];
   if (wave >= L.nicl) return;
@@ -527,27 +572,45 @@ __global__ void __launch_bounds__(256) k_pp(Dev d) {
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   double eel = 0.0, evdw = 0.0;
   const int* ent = L.ent + (size_t)row * L.cap;
-  for (int t = 0; t < nch; ++t) {
-    const int kk = t * kCl + b;
-    if (kk >= n) continue;
-    const int e = ent[kk];
-    const bool is14 = (e & kRow14) != 0;
-    const int eb = e & ~kRow14;
-    const float4 pj = d.x32[eb];
-    const float4 lj = d.ljp[eb];
-    float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-    float r2 = dx * dx + dy * dy + dz * dz;
-    float rinv = rsqrtf(r2);
-    float r2inv = rinv * rinv;
-    float r6 = r2inv * r2inv * r2inv;
-    float vel = (pi.w * pj.w) * rinv;
-    float aa, bb;
-    if (is14) { vel *= W.el14; aa = li.z * lj.z; bb = li.w * lj.w; }
-    else { aa = li.x * lj.x; bb = li.y * lj.y; }
-    float va = r6 * r6 * aa, vb = r6 * bb;
-    float dv = r2inv * (-vel - 12.f * va + 6.f * vb);
-    g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz;
-    if (en && ea < eb) { eel += (double)vel; evdw += (double)(va - vb); }
+  // kUnroll entries per pass: all their loads are issued before the
+  // arithmetic; entries past the row end read atom 0 and are masked, and
+  // the sums run in entry order as in a plain loop
+  for (int t = 0; t < nch; t += kUnroll) {
+    int e[kUnroll];
+    bool v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int kk = (t + u) * kCl + b;
+      v[u] = kk < n;
+      const int ld = ent[min(kk, L.cap - 1)];   // in the row, so the load needs no branch
+      e[u] = v[u] ? ld : 0;
+    }
+    float4 pj[kUnroll], lj[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      pj[u] = d.x32[e[u] & ~kRow14];
+      lj[u] = d.ljp[e[u] & ~kRow14];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const bool is14 = (e[u] & kRow14) != 0;
+      const int eb = e[u] & ~kRow14;
+      float dx = pj[u].x - pi.x, dy = pj[u].y - pi.y, dz = pj[u].z - pi.z;
+      float r2 = dx * dx + dy * dy + dz * dz;
+      float rinv = rsqrtf(r2);
+      float r2inv = rinv * rinv;
+      float r6 = r2inv * r2inv * r2inv;
+      float vel = (pi.w * pj[u].w) * rinv;
+      float aa, bb;
+      if (is14) { vel *= W.el14; aa = li.z * lj[u].z; bb = li.w * lj[u].w; }
+      else { aa = li.x * lj[u].x; bb = li.y * lj[u].y; }
+      float va = r6 * r6 * aa, vb = r6 * bb;
+      float dv = r2inv * (-vel - 12.f * va + 6.f * vb);
+      if (v[u]) {
+        g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz;
+        if (en && ea < eb) { eel += (double)vel; evdw += (double)(va - vb); }
+      }
+    }
   }
   double s0x = row8_sum((double)g0), s0y = row8_sum((double)g1), s0z = row8_sum((double)g2);
   if (b == 0 && ea >= 0) add_grad(d.f, ea, s0x, s0y, s0z);
@@ -639,7 +702,7 @@ __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x);
 // q-atom vs water (nonbond_qw_spc, md.f90:11302-11411)
 template <bool Q_SIDE>
 __global__ void __launch_bounds__(256) k_qw(Dev d) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const TileList L = d.tl[Q_SIDE ? T_QW : T_WQ];
   if (wave >= L.nwork) return;
@@ -771,7 +834,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
 // 10980-11045)
 template <bool Q_SIDE>
 __global__ void __launch_bounds__(256) k_qp(Dev d) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const TileList L = d.tl[Q_SIDE ? T_QP : T_PQ];
   if (wave >= L.nwork) return;
@@ -997,8 +1060,50 @@ __device__ __forceinline__ double r2_pos(double ax, double ay, double az, double
   return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
 }
 
+// 1/sqrt(r2) in FP64: the hardware estimate and two Newton steps (each
+// squares the relative error, so the result is within a few ulp)
+__device__ __forceinline__ double rsqrt_d(double r2) {
+  const double h = 0.5 * r2;
+  double y = __builtin_amdgcn_rsq(r2);
+  y = fma(y, fma(-(h * y), y, 0.5), y);
+  y = fma(y, fma(-(h * y), y, 0.5), y);
+  return y;
+}
+
+// Taylor terms of one source charge q at d = x(source) - centre: potential,
+// field, field gradient and its derivative (the sums of nbpw/nbww lists with
+// LRF, md.f90:5589-5700), with the shared factors hoisted
+__device__ __forceinline__ void lrf_add(double acc[20], double q, double dx, double dy, double dz) {
+  const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+  const double ir = rsqrt_d(r2);
+  const double ir2 = ir * ir;
+  const double qr = q * ir;            // q / r
+  const double f0 = qr * ir2;          // q / r^3
+  const double f1 = 3.0 * f0 * ir2;    // 3 q / r^5
+  const double f2 = -f1 * ir2;         // -3 q / r^7
+  acc[0] += qr;
+  acc[1] -= f0 * dx; acc[2] -= f0 * dy; acc[3] -= f0 * dz;
+  const double ax = f1 * dx, ay = f1 * dy;
+  acc[4] += fma(ax, dx, -f0); acc[5] += ax * dy; acc[6] += ax * dz;
+  acc[7] += fma(ay, dy, -f0); acc[8] += ay * dz; acc[9] += fma(f1 * dz, dz, -f0);
+  // f2 (5 d_a d_b d_c - r2 (delta_ab d_c + delta_ac d_b + delta_bc d_a))
+  const double c5 = 5.0 * f2, g = f2 * r2, g3 = 3.0 * g;
+  const double px = c5 * dx, py = c5 * dy;
+  const double xx = px * dx, yy = py * dy, zz = c5 * dz * dz;
+  acc[10] += (xx - g3) * dx;
+  acc[11] += (xx - g) * dy;
+  acc[12] += (xx - g) * dz;
+  acc[13] += (yy - g) * dx;
+  acc[14] += px * dy * dz;
+  acc[15] += (zz - g) * dx;
+  acc[16] += (yy - g3) * dy;
+  acc[17] += (yy - g) * dz;
+  acc[18] += (zz - g) * dy;
+  acc[19] += (zz - g3) * dz;
+}
+
 __global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
-  const int4 lw = d.lrf_work[blockIdx.x];   // window, first target group, count
+  const int4 lw = d.lrf_work[block_id(d.xcd)];   // window, first target group, count, source part
   const int w = lw.x;
   const WinState& st = d.ws[w];
   const WinDev& W = d.win[w];
@@ -1017,10 +1122,17 @@ __global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
     gw = (g - W.g_off) >= W.ngrp_solute;
     cx = d.g_cent[3 * g]; cy = d.g_cent[3 * g + 1]; cz = d.g_cent[3 * g + 2];
   }
+  // cut-off against a water source group and against a solute source group
+  const double rc_hw = gw ? W.rcww2 : W.rcpw2, rc_hs = gw ? W.rcpw2 : W.rcpp2, rclrf2 = W.rclrf2;
+  const int h_sol = W.g_off + W.ngrp_solute;
   double acc[20];
   for (int k = 0; k < 20; ++k) acc[k] = 0.0;
-  const int h_end = W.g_off + W.ngrp;
-  for (int h0 = W.g_off; h0 < h_end; h0 += blockDim.x) {
+  // this block's part of the window's source groups
+  const int P = d.lrf_parts;
+  const int h_beg = W.g_off + (int)(((long long)lw.w * W.ngrp) / P);
+  const int h_end = W.g_off + (int)(((long long)(lw.w + 1) * W.ngrp) / P);
+  const double4* __restrict__ xq = d.ls_xq;
+  for (int h0 = h_beg; h0 < h_end; h0 += blockDim.x) {
     const int hl = h0 + t;
     if (hl < h_end) {
       s_sw[t] = d.ls_sw[hl];
@@ -1033,39 +1145,32 @@ __global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
       const int h = h0 + j;
       const double4 sh = s_sw[j];
       if (!g_ok || h == g || sh.w != 0.0) continue;
-      const bool hw = (h - W.g_off) >= W.ngrp_solute;
-      const double rc2 = (gw && hw) ? W.rcww2 : ((gw || hw) ? W.rcpw2 : W.rcpp2);
+      const double rc2 = h >= h_sol ? rc_hw : rc_hs;
       const double r2g = r2_pos(sg.x, sg.y, sg.z, sh.x, sh.y, sh.z);
-      if (r2g <= rc2 || r2g > W.rclrf2) continue;
+      if (r2g <= rc2 || r2g > rclrf2) continue;
       const int f0 = s_first[j], n = s_n[j];
       for (int k = 0; k < n; ++k) {
-        const double4 a = d.ls_xq[f0 + k];
-        const double q = a.w;
-        const double dx = a.x - cx, dy = a.y - cy, dz = a.z - cz;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const double f0v = q / (r2 * sqrt(r2));
-        const double f1 = 3.0 * f0v / r2;
-        const double f2 = -f1 / r2;
-        acc[0] += f0v * r2;
-        acc[1] -= f0v * dx; acc[2] -= f0v * dy; acc[3] -= f0v * dz;
-        acc[4] += f1 * dx * dx - f0v; acc[5] += f1 * dx * dy; acc[6] += f1 * dx * dz;
-        acc[7] += f1 * dy * dy - f0v; acc[8] += f1 * dy * dz; acc[9] += f1 * dz * dz - f0v;
-        acc[10] += f2 * (5.0 * dx * dx * dx - r2 * 3.0 * dx);
-        acc[11] += f2 * (5.0 * dx * dx * dy - r2 * dy);
-        acc[12] += f2 * (5.0 * dx * dx * dz - r2 * dz);
-        acc[13] += f2 * (5.0 * dx * dy * dy - r2 * dx);
-        acc[14] += f2 * (5.0 * dx * dy * dz);
-        acc[15] += f2 * (5.0 * dx * dz * dz - r2 * dx);
-        acc[16] += f2 * (5.0 * dy * dy * dy - r2 * 3.0 * dy);
-        acc[17] += f2 * (5.0 * dy * dy * dz - r2 * dz);
-        acc[18] += f2 * (5.0 * dy * dz * dz - r2 * dy);
-        acc[19] += f2 * (5.0 * dz * dz * dz - r2 * 3.0 * dz);
+        const double4 a = xq[f0 + k];
+        lrf_add(acc, a.w, a.x - cx, a.y - cy, a.z - cz);
       }
     }
     __syncthreads();
   }
-  if (g_ok)
-    for (int k = 0; k < 20; ++k) d.g_lrf[20 * g + k] = acc[k];
+  if (!g_ok) return;
+  double* out = P == 1 ? d.g_lrf + 20 * (size_t)g : d.g_lrf_part + 20 * ((size_t)lw.w * d.ngrp_all + g);
+  for (int k = 0; k < 20; ++k) out[k] = acc[k];
+}
+
+// sum of the source parts in part order, one thread per (group, term)
+__global__ void __launch_bounds__(256) k_lrf_reduce(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 20 * d.ngrp_all) return;
+  const int g = i / 20;
+  const int w = d.g_win[g];
+  if (!d.ws[w].active || !d.ws[w].list_now || !d.win[w].use_lrf || d.ls_sw[g].w != 0.0) return;
+  double v = 0.0;
+  for (int p = 0; p < d.lrf_parts; ++p) v += d.g_lrf_part[(size_t)p * 20 * d.ngrp_all + i];
+  d.g_lrf[i] = v;
 }
 
 // Evaluation per atom (lrf_taylor, md.f90:3646-3712), FP64.
@@ -1379,22 +1484,42 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
   wave_energy(d.ws, w, eslot, e, en);
 }
 
-// sum of each bonded atom's slots (CSR in slot order) into the gradient; one
-// thread per atom, so the add needs no atomic
+// sum of each bonded atom's slots (CSR in slot order) into the gradient.  A
+// block takes consecutive atoms whose slots form one contiguous run: the run
+// is read into LDS with coalesced loads, then one thread per atom adds its
+// own slots, so the add to the gradient needs no atomic.  A block of one
+// atom with more slots than fit reads them from global memory.
 __global__ void __launch_bounds__(256) k_bonded_gather(Dev d) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.nbg) return;
-  const int atom = d.bg_atom[k];
-  if (!d.ws[d.wid[atom]].active) return;
-  long long gx = 0, gy = 0, gz = 0;
-  for (int m = d.bg_start[k]; m < d.bg_start[k + 1]; ++m) {
-    gx += d.bslot[3 * m];
-    gy += d.bslot[3 * m + 1];
-    gz += d.bslot[3 * m + 2];
+  __shared__ long long s_slot[3 * kGatherSlots];
+  const int2 gw = d.gather_work[block_id(d.xcd)];
+  const int s0 = d.bg_start[gw.x], s1 = d.bg_start[gw.y];
+  const bool staged = s1 - s0 <= kGatherSlots;
+  if (staged) {
+    for (int m = threadIdx.x; m < 3 * (s1 - s0); m += blockDim.x) s_slot[m] = d.bslot[3 * (size_t)s0 + m];
+    __syncthreads();
   }
-  d.f[3 * atom] += gx;
-  d.f[3 * atom + 1] += gy;
-  d.f[3 * atom + 2] += gz;
+  for (int k = gw.x + (int)threadIdx.x; k < gw.y; k += blockDim.x) {
+    const int atom = d.bg_atom[k];
+    if (!d.ws[d.wid[atom]].active) continue;
+    long long gx = 0, gy = 0, gz = 0;
+    const int m0 = d.bg_start[k], m1 = d.bg_start[k + 1];
+    if (staged) {
+      for (int m = m0 - s0; m < m1 - s0; ++m) {
+        gx += s_slot[3 * m];
+        gy += s_slot[3 * m + 1];
+        gz += s_slot[3 * m + 2];
+      }
+    } else {
+      for (int m = m0; m < m1; ++m) {
+        gx += d.bslot[3 * (size_t)m];
+        gy += d.bslot[3 * (size_t)m + 1];
+        gz += d.bslot[3 * (size_t)m + 2];
+      }
+    }
+    d.f[3 * atom] += gx;
+    d.f[3 * atom + 1] += gy;
+    d.f[3 * atom + 2] += gz;
+  }
 }
 
 // fixed/shell restraints (fix_shell, md.f90:1414-1450) and the radial
@@ -1978,6 +2103,7 @@ void launch_lrf_build(const Dev& d, hipStream_t s) {
   if (!d.nlrfw) return;
   k_lrf_pack<<<blocks(std::max(d.ngrp_all, d.nls), 256), 256, 0, s>>>(d);
   k_lrf_build<<<d.nlrfw, d.bs_lrf, 0, s>>>(d);
+  if (d.lrf_parts > 1) k_lrf_reduce<<<blocks(20LL * d.ngrp_all, 256), 256, 0, s>>>(d);
 }
 
 void launch_bonded(const Dev& d, hipStream_t s) {
@@ -2020,7 +2146,7 @@ void launch_other_forces(const Dev& d, hipStream_t s) {
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
   if (nb) k_bonded<<<blocks(nb, d.bs_bonded), d.bs_bonded, 0, s>>>(d);
   // the gather also collects the q-q pair slots written by k_qq
-  if (nb || d.nqq) k_bonded_gather<<<blocks(d.nbg, 256), 256, 0, s>>>(d);
+  if ((nb || d.nqq) && !d.bond_atomic && d.ngw) k_bonded_gather<<<d.ngw, 256, 0, s>>>(d);
   k_restr<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   if (d.nrst) k_prestr<<<d.nrst, 64, 0, s>>>(d);
   if (d.nwat_total) {

@@ -202,10 +202,11 @@ class Engine:
 
     def tuning(self) -> dict:
         """launch shapes chosen by the autotuner and their measured times (us)"""
-        a = np.zeros(8)
-        _check(self.lib.qmi_tuning(self._h, a.ctypes.data_as(C.POINTER(C.c_double)), 8))
+        a = np.zeros(13)
+        _check(self.lib.qmi_tuning(self._h, a.ctypes.data_as(C.POINTER(C.c_double)), 13))
         return dict(q_tiles_per_wave=int(a[0]), rows_block=int(a[1]), lrf_block=int(a[2]), bonded_block=int(a[3]),
-                    us=dict(q_tiles=a[4], rows=a[5], lrf_build=a[6], bonded=a[7]))
+                    xcd_grouped=int(a[4]), bonded_atomic=int(a[5]), lrf_source_parts=int(a[6]),
+                    us=dict(q_tiles=a[7], rows=a[8], lrf_build=a[9], bonded=a[10], xcd=a[11], bonded_mode=a[12]))
 
     def set_timing(self, on: bool):
         _check(self.lib.qmi_set_timing(self._h, 1 if on else 0))

@@ -152,14 +152,15 @@ def propose_n(predict, lams, dEs, alpha, nmin=16, nmax=24):
 @dataclass
 class Perturbation:
     seed: int
-    leg: str            # "ligwat" or "protein"
+    leg: str                 # "ligwat" or "protein"
     n_pert: int
     alpha: float = 10.0
+    scaffold: int = -1       # protein leg: index of the environment it is drawn in
 
 
 def build_system(p: Perturbation):
-    """the synthetic system of a perturbation; the ligand generator retries
-    with derived seeds when a growth attempt fails"""
+    """the synthetic system of a ligand-in-water perturbation; the ligand
+    generator retries with derived seeds when a growth attempt fails"""
     from q_amd import synth
     for k in range(50):
         s = p.seed + 100003 * k
@@ -170,6 +171,12 @@ def build_system(p: Perturbation):
         except RuntimeError:
             continue
     raise RuntimeError(f"no system for {p}")
+
+
+def build_scaffold(seed: int):
+    """a protein-leg environment (chain, ligand and water); perturbations
+    are redrawn in it by synth.reperturb"""
+    return build_system(Perturbation(seed, "protein", 6))
 
 
 def write_ladder(p: Perturbation, system, lams, outdir, steps, energy=5, md_seed=1, tag="w"):
@@ -198,13 +205,87 @@ def run_batched(inps, steps, batch=64, log=None):
     return [os.path.splitext(p)[0] + ".en" for p in inps]
 
 
+def bar_var(wF: np.ndarray, wR: np.ndarray) -> float:
+    """Bennett's large-sample variance of the BAR free energy (kT^2 units),
+    for uncorrelated samples."""
+    nF, nR = len(wF), len(wR)
+    M = math.log(nF / nR)
+    df = bar_df(wF, wR)
+    fF = 1.0 / (1.0 + np.exp(np.clip(M + wF - df, -700, 700)))
+    fR = 1.0 / (1.0 + np.exp(np.clip(-M + wR + df, -700, 700)))
+    return float((np.mean(fF ** 2) / np.mean(fF) ** 2 - 1.0) / nF + (np.mean(fR ** 2) / np.mean(fR) ** 2 - 1.0) / nR)
+
+
+def ladder_stats(en_paths, nskip=0) -> dict:
+    """BAR free energy of a ladder (qfep's estimator), the smallest
+    neighbour overlap and Bennett's standard error (kcal/mol)."""
+    from q_amd.qfep import qfep
+    gaps = [energy_gap(p, nskip) for p in en_paths]
+    ovl, var = [], 0.0
+    for (la, da), (lb, db) in zip(gaps[:-1], gaps[1:]):
+        d = lb - la
+        wF, wR = d * da / KT, -d * db / KT
+        ovl.append(bar_overlap(wF, wR))
+        var += bar_var(wF, wR)
+    return dict(dG=float(qfep(en_paths, 2, KT, nskip, 20, 5).dG_bar), min_overlap=float(min(ovl)),
+                sigma=float(KT * math.sqrt(var)))
+
+
 def ladder_dG(en_paths, nskip=0) -> float:
     from q_amd.qfep import qfep
     return qfep(en_paths, 2, KT, nskip, 20, 5).dG_bar
 
 
+def simulated_fraction(n, prod_steps, pilot_steps, n_ref=51):
+    """simulated time of pilots + proposed ladder over that of the reference ladder"""
+    return (n * prod_steps + len(PILOT_LAMBDAS) * pilot_steps) / float(n_ref * prod_steps)
+
+
+def load_predictor(path=None):
+    """the trained overlap network shipped with the package, on the GPU"""
+    from q_amd.mlp import MLP
+    path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "overlap_mlp.npz")
+    z = np.load(path)
+    if int(z["n_in"]) != N_IN or int(z["nbins"]) != NBINS:
+        raise ValueError(f"{path}: network built for other features")
+    net = MLP(N_IN, int(z["hidden"]), seed=0)
+    net.params = z["params"].astype(np.float32)
+    return net
+
+
+def evaluate(p: Perturbation, system, predict, workdir, pilot_steps, prod_steps, nskip, nmin=16, nmax=None,
+             pilots=None, log=None) -> dict:
+    """pilots (unless given), the proposed ladder, the uniform ladder with as
+    many windows and the uniform 51-window ladder of one perturbation"""
+    if nmax is None:
+        # the largest ladder that keeps pilots + ladder within half of the 51-window time
+        nmax = max(nmin, min(24, int((0.5 * 51 * prod_steps - len(PILOT_LAMBDAS) * pilot_steps) // prod_steps)))
+    if pilots is None:
+        inps = write_ladder(p, system, PILOT_LAMBDAS, os.path.join(workdir, "pilot"), pilot_steps, tag="p")
+        pilots = [energy_gap(en, nskip) for en in run_batched(inps, pilot_steps)]
+    lams = [a for a, _ in pilots]
+    dEs = [b for _, b in pilots]
+    sched = propose_n(predict, lams, dEs, p.alpha, nmin, nmax)
+    n = len(sched)
+    ladders = dict(ref51=np.linspace(0.0, 1.0, 51), learned=sched, uniform=np.linspace(0.0, 1.0, n))
+    inps, owner = [], []
+    for name, lamv in ladders.items():
+        w = write_ladder(p, system, lamv, os.path.join(workdir, name), prod_steps, tag="e")
+        inps += w
+        owner += [name] * len(w)
+    ens = run_batched(inps, prod_steps, log=log)
+    st = {name: ladder_stats([e for e, o in zip(ens, owner) if o == name], nskip) for name in ladders}
+    return dict(perturbation=p.seed, leg=p.leg, n_pert=p.n_pert, windows=n,
+                schedule=[round(float(v), 4) for v in sched], stats=st,
+                err_learned=abs(st["learned"]["dG"] - st["ref51"]["dG"]),
+                err_uniform=abs(st["uniform"]["dG"] - st["ref51"]["dG"]),
+                simulated_fraction=simulated_fraction(n, prod_steps, pilot_steps))
+
+
 # ------------------------------------------------------------- driver --
 def build(args):
+    from concurrent.futures import ProcessPoolExecutor
+    from q_amd import synth
     from q_amd.mlp import MLP
     os.makedirs(args.out, exist_ok=True)
     t0 = time.time()
@@ -213,33 +294,47 @@ def build(args):
         print(f"[schedule {time.time() - t0:7.1f}s] {msg}", flush=True)
 
     rng = np.random.default_rng(args.seed)
-    train = [Perturbation(1000 + k, "ligwat", int(rng.integers(2, 11))) for k in range(args.n_train)]
-    train += [Perturbation(5000 + k, "protein", int(rng.integers(2, 11))) for k in range(args.n_train_protein)]
-    held = [Perturbation(9000 + k, "ligwat", int(rng.integers(2, 11))) for k in range(args.n_eval)]
-    # pilot ladders
-    from concurrent.futures import ProcessPoolExecutor
+    nsc = args.n_scaffolds
+    # every perturbation runs in both legs: its own ligand in water, and a
+    # redrawn perturbation of one of nsc protein environments
+    train, held = [], []
+    for k in range(args.n_train):
+        n_pert = int(rng.integers(2, 11))
+        train += [Perturbation(1000 + k, "ligwat", n_pert), Perturbation(1000 + k, "protein", n_pert, scaffold=k % nsc)]
+    for k in range(args.n_eval):
+        n_pert = int(rng.integers(2, 11))
+        held += [Perturbation(9000 + k, "ligwat", n_pert), Perturbation(9000 + k, "protein", n_pert, scaffold=nsc + k % 2)]
     work = os.path.join(args.out, "work")
     os.makedirs(work, exist_ok=True)
+    lig = [p for p in train + held if p.leg == "ligwat"]
     with ProcessPoolExecutor(max_workers=args.procs) as ex:
-        systems = list(ex.map(build_system, train + held))
-    log(f"built {len(systems)} systems")
-    pilot_inps, owners = [], []
-    for k, (p, s) in enumerate(zip(train + held, systems)):
-        d = os.path.join(work, f"pilot_{k:03d}")
-        inps = write_ladder(p, s, PILOT_LAMBDAS, d, args.pilot_steps, tag="p")
-        pilot_inps += inps
-        owners += [k] * len(inps)
-    log(f"wrote {len(pilot_inps)} pilot windows")
-    ens = run_batched(pilot_inps, args.pilot_steps, log=log)
+        lig_sys = list(ex.map(build_system, lig))
+        scaffolds = list(ex.map(build_scaffold, [7000 + j for j in range(nsc + 2)]))
+    lig_of = {id(p): s for p, s in zip(lig, lig_sys)}
+
+    def system_of(p):
+        if p.leg == "ligwat":
+            return lig_of[id(p)]
+        return synth.reperturb(scaffolds[p.scaffold], p.seed, p.n_pert, p.alpha)
+
+    log(f"built {len(lig_sys)} ligand-in-water systems and {len(scaffolds)} protein environments")
+    # pilot ladders, one batch per leg so that windows of one size share engines
     pil = {}
-    for k, en in zip(owners, ens):
-        lam, dE = energy_gap(en, args.nskip)
-        pil.setdefault(k, []).append((lam, dE))
+    for leg in ("ligwat", "protein"):
+        inps, owners = [], []
+        for k, p in enumerate(train + held):
+            if p.leg != leg:
+                continue
+            w = write_ladder(p, system_of(p), PILOT_LAMBDAS, os.path.join(work, f"pilot_{leg}_{k:04d}"),
+                             args.pilot_steps, tag="p")
+            inps += w
+            owners += [k] * len(w)
+        log(f"{leg}: {len(inps)} pilot windows written")
+        for k, en in zip(owners, run_batched(inps, args.pilot_steps, log=log)):
+            pil.setdefault(k, []).append(energy_gap(en, args.nskip))
     X, Y = [], []
     for k, p in enumerate(train):
-        lams = [a for a, _ in pil[k]]
-        dEs = [b for _, b in pil[k]]
-        x, y = pilot_samples(lams, dEs, p.alpha)
+        x, y = pilot_samples([a for a, _ in pil[k]], [b for _, b in pil[k]], p.alpha)
         X.append(x)
         Y.append(y)
     X = np.concatenate(X)
@@ -247,42 +342,32 @@ def build(args):
     log(f"training set: {len(Y)} rows, overlap mean {Y.mean():.3f}")
     net = MLP(N_IN, 128, seed=args.seed)
     l0, _ = net.grad(X, Y)
-    loss = net.train(X, Y, epochs=args.epochs, batch=256, lr=1e-3, seed=args.seed)
+    net.train(X, Y, epochs=args.epochs, batch=256, lr=1e-3, seed=args.seed)
     l1, _ = net.grad(X, Y)
-    log(f"trained: loss {l0:.4f} -> {l1:.4f}")
+    log(f"trained: loss {l0:.5f} -> {l1:.5f}")
     np.savez(os.path.join(args.out, "overlap_mlp.npz"), params=net.params, n_in=N_IN, hidden=128,
              u_lo=U_LO, u_hi=U_HI, nbins=NBINS, kt=KT)
-    # evaluation on held-out perturbations
-    report = dict(n_train=len(train), rows=int(len(Y)), loss_before=l0, loss_after=l1, evals=[])
-    for k, p in enumerate(held):
-        kk = len(train) + k
-        lams = [a for a, _ in pil[kk]]
-        dEs = [b for _, b in pil[kk]]
-        sched = propose_n(net.forward, lams, dEs, p.alpha)
-        n = len(sched)
-        uni = np.linspace(0.0, 1.0, n)
-        ref = np.linspace(0.0, 1.0, 51)
-        s = systems[kk]
-        base = os.path.join(work, f"eval_{k:03d}")
-        runs = {}
-        for name, lamv in (("ref51", ref), ("learned", sched), ("uniform", uni)):
-            inps = write_ladder(p, s, lamv, os.path.join(base, name), args.prod_steps, tag="e")
-            ens = run_batched(inps, args.prod_steps)
-            runs[name] = ladder_dG(ens, args.nskip)
-        sim_frac = (n * args.prod_steps + len(PILOT_LAMBDAS) * args.pilot_steps) / (51.0 * args.prod_steps)
-        row = dict(perturbation=p.seed, n_pert=p.n_pert, windows=n, schedule=[round(float(v), 4) for v in sched],
-                   dG_ref51=runs["ref51"], dG_learned=runs["learned"], dG_uniform_same_n=runs["uniform"],
-                   err_learned=abs(runs["learned"] - runs["ref51"]), err_uniform=abs(runs["uniform"] - runs["ref51"]),
-                   simulated_fraction=sim_frac)
+    np.savez_compressed(os.path.join(args.out, "pilot_data.npz"), X=X, Y=Y)
+    report = dict(n_perturbations=args.n_train, rows=int(len(Y)), loss_before=l0, loss_after=l1,
+                  pilot_steps=args.pilot_steps, prod_steps=args.prod_steps, nskip=args.nskip, evals=[])
+    for j, p in enumerate(held):
+        kk = len(train) + j
+        row = evaluate(p, system_of(p), net.forward, os.path.join(work, f"eval_{j:03d}"), args.pilot_steps,
+                       args.prod_steps, args.nskip, pilots=pil[kk])
         report["evals"].append(row)
-        log(json.dumps(row))
+        log(json.dumps({k: v for k, v in row.items() if k != "schedule"}))
     ev = report["evals"]
-    report["mae_learned"] = float(np.mean([r["err_learned"] for r in ev]))
-    report["mae_uniform_same_n"] = float(np.mean([r["err_uniform"] for r in ev]))
-    report["max_err_learned"] = float(np.max([r["err_learned"] for r in ev]))
+    for key in ("err_learned", "err_uniform"):
+        report["mae_" + key[4:]] = float(np.mean([r[key] for r in ev]))
+        report["max_" + key] = float(np.max([r[key] for r in ev]))
     report["mean_simulated_fraction"] = float(np.mean([r["simulated_fraction"] for r in ev]))
+    for name in ("learned", "uniform"):
+        report["mean_sigma_" + name] = float(np.mean([r["stats"][name]["sigma"] for r in ev]))
+        report["mean_min_overlap_" + name] = float(np.mean([r["stats"][name]["min_overlap"] for r in ev]))
     json.dump(report, open(os.path.join(args.out, "schedule_report.json"), "w"), indent=1)
-    log(f"MAE learned {report['mae_learned']:.3f}  uniform(same n) {report['mae_uniform_same_n']:.3f} kcal/mol")
+    log(f"MAE vs 51 windows: learned {report['mae_learned']:.3f}, uniform with as many windows "
+        f"{report['mae_uniform']:.3f} kcal/mol; sigma {report['mean_sigma_learned']:.3f} vs "
+        f"{report['mean_sigma_uniform']:.3f}")
 
 
 def main(argv=None):
@@ -291,7 +376,7 @@ def main(argv=None):
     b = sub.add_parser("build", help="pilot data, GPU training of the overlap MLP, held-out evaluation")
     b.add_argument("--out", required=True)
     b.add_argument("--n-train", type=int, default=200)
-    b.add_argument("--n-train-protein", type=int, default=0)
+    b.add_argument("--n-scaffolds", type=int, default=16)
     b.add_argument("--n-eval", type=int, default=8)
     b.add_argument("--pilot-steps", type=int, default=500)
     b.add_argument("--prod-steps", type=int, default=2000)

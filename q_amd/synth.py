@@ -444,6 +444,30 @@ def _ligand_fep(rng, sysm: System, first: int, n_lig: int, types: list, n_pert: 
             sysm.qimpropers.append((i, j, k, l, 1, 1))
 
 
+def reperturb(base: System, seed: int, n_pert: int, alpha: float = 10.0) -> System:
+    """A copy of ``base`` whose two FEP states are redrawn: the same ligand
+    (the first ``len(base.qatoms)`` topology atoms) and environment, a new
+    seeded choice of ``n_pert`` perturbed atoms with new charges and types.
+    Used to make many ligand perturbations in one costly environment."""
+    import copy
+    if not base.qatoms or base.qatoms[0] != 1:
+        raise ValueError("reperturb needs a system whose ligand atoms come first")
+    n_lig = len(base.qatoms)
+    s = copy.deepcopy(base)
+    for name in ("qatoms", "qcharges", "qtypes", "qalpha", "qbonds", "qangles", "qtorsions", "qimpropers",
+                 "qbond_lib", "qangle_lib"):
+        setattr(s, name, [])
+    types = [int(t) for t in s.iac[:n_lig]]
+    heavy = [t in HEAVY_TYPES for t in types]
+    nb: list = [[] for _ in range(n_lig)]
+    for (i, j, _c) in s.bonds:
+        if i <= n_lig and j <= n_lig:
+            nb[i - 1].append(j - 1)
+            nb[j - 1].append(i - 1)
+    _ligand_fep(np.random.default_rng(seed), s, 1, n_lig, types, n_pert, alpha, nb, heavy)
+    return s
+
+
 def water_droplet(radius: float = 15.0, seed: int = 0) -> System:
     rng = np.random.default_rng(seed)
     s = System(name=f"water{radius:g}", radius=radius)

@@ -57,3 +57,44 @@ def test_greedy_ladder_follows_the_predictor():
     assert s[0] == 0.0 and s[-1] == 1.0 and np.all(np.diff(s) > 0)
     steps = np.diff(s)
     assert steps[:3].mean() > 2 * steps[-3:].mean()
+
+
+def test_bar_variance_matches_replica_spread():
+    # Bennett's variance against the spread of BAR over independent replicas
+    rng = np.random.default_rng(3)
+    mu, s2, n = 2.0, 3.0, 300
+    est, var = [], []
+    for _ in range(400):
+        wF = rng.normal(mu, np.sqrt(s2), n)
+        wR = rng.normal(-(mu - s2), np.sqrt(s2), n)
+        est.append(S.bar_df(wF, wR))
+        var.append(S.bar_var(wF, wR))
+    ratio = np.var(est) / np.mean(var)
+    assert 0.8 < ratio < 1.25, ratio
+    # identical states: no uncertainty
+    w = np.zeros(100)
+    assert S.bar_var(w, w) < 1e-12
+
+
+def test_simulated_fraction_bound():
+    # pilots included: the largest ladder within half the 51-window time
+    assert S.simulated_fraction(22, 2000, 500) <= 0.5 < S.simulated_fraction(23, 2000, 500)
+
+
+def test_reperturb_redraws_only_the_fep_states():
+    from q_amd import synth
+    base = S.build_system(S.Perturbation(1001, "ligwat", 6))
+    a = synth.reperturb(base, 11, 3)
+    b = synth.reperturb(base, 11, 3)
+    c = synth.reperturb(base, 12, 9)
+    assert a.qcharges == b.qcharges and a.qtypes == b.qtypes and a.qbonds == b.qbonds
+    assert np.array_equal(np.array(a.x), np.array(base.x)) and a.crg == base.crg and a.iac == base.iac and a.bonds == base.bonds
+    assert a.qatoms == base.qatoms
+    for s, n in ((a, 3), (c, 9)):
+        changed = [k for k, (t, q) in enumerate(zip(s.qtypes, s.qcharges)) if t[0] != t[1] or q[0] != q[1]]
+        assert 1 <= len(changed) <= n
+        assert sum(1 for al in s.qalpha if al[0] > 0) == n
+        # both states keep the ligand's net charge
+        assert abs(sum(q for q, _ in s.qcharges) - sum(q for _, q in s.qcharges)) < 1e-9
+    # state 1 is the topology
+    assert [q for q, _ in a.qcharges] == base.crg[:len(base.qatoms)]
