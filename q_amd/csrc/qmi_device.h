@@ -58,6 +58,7 @@ struct WinState {
   double ekin_parts[4];  // solute all, solute free, solvent all, solvent free
   int shake_fail;
   int list_overflow;  // a neighbour row exceeded its capacity
+  int list_missed;    // list_now was set in a step launched without the list kernels
   long long E[E_NSLOT];
 };
 

@@ -89,10 +89,13 @@ struct qmi_engine {
   // gradients, LRF source parts
   int tuned[7] = {2, 256, 256, 256, 0, 0, 1};
   double tune_us[6] = {0, 0, 0, 0, 0, 0};   // time of the chosen configuration per group
-  // the launch sequence of one MD step, captured once and replayed
-  hipGraph_t step_graph = nullptr;
-  hipGraphExec_t step_exec = nullptr;
+  // the launch sequence of one MD step, captured once and replayed: [1]
+  // with the pair-list and LRF kernels, [0] without them for steps where no
+  // window updates its lists
+  hipGraph_t step_graph[2] = {nullptr, nullptr};
+  hipGraphExec_t step_exec[2] = {nullptr, nullptr};
   bool graph_failed = false;
+  std::vector<int> nbcycle;   // per window, as on the device
   ~qmi_engine();
 };
 
@@ -118,8 +121,10 @@ static int dev_alloc(qmi_engine* e, T** dst, size_t n) {
 }
 
 qmi_engine::~qmi_engine() {
-  if (step_exec) (void)hipGraphExecDestroy(step_exec);
-  if (step_graph) (void)hipGraphDestroy(step_graph);
+  for (int k = 0; k < 2; ++k) {
+    if (step_exec[k]) (void)hipGraphExecDestroy(step_exec[k]);
+    if (step_graph[k]) (void)hipGraphDestroy(step_graph[k]);
+  }
   for (auto& x : ev) if (x) (void)hipEventDestroy(x);
   for (void* p : bufs) (void)hipFree(p);
   if (stream) (void)hipStreamDestroy(stream);
@@ -210,6 +215,7 @@ static int build_device(qmi_engine* e) {
     D.q_off = qoff; D.t_off = toff;
     D.qs_off = (int)q_crg.size(); D.qsc_off = (int)q_sc.size(); D.g_off = goff; D.ngrp = (int)t.cgp.size(); D.ngrp_solute = t.ncgp_solute;
     D.nsteps = p.nsteps; D.nbcycle = p.nbcycle > 0 ? p.nbcycle : 1;
+    e->nbcycle.push_back(D.nbcycle);
     D.iene = p.iene_cycle > 0 ? p.iene_cycle : 0; D.iout = p.iout_cycle > 0 ? p.iout_cycle : 0;
     D.use_lrf = p.use_lrf; D.wpol = p.wpol_restr && W.nwat > 0; D.separate_scaling = p.separate_scaling;
     D.iac_ow = W.nwat > 0 ? t.iac[NS] - 1 : 0;
@@ -1326,40 +1332,60 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   // one step = the same kernels with the same arguments every time (the
   // kernels read per-window step state from device memory), so outside the
   // timing pass it runs as one captured graph
-  auto launch_step = [&]() {
-    launch_begin(e->d, e->stream, 0, 0, 0);
+  auto launch_step = [&](bool lists) {
+    launch_begin(e->d, e->stream, 0, 0, lists ? 1 : 0);
     launch_zero(e->d, e->stream);
-    launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
+    if (lists) launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
     launch_ww(e->d, e->stream);
     launch_nonbonded(e->d, e->stream, false);
     launch_other_forces(e->d, e->stream);
     launch_integrate(e->d, e->stream, e->shake_any);
   };
-  if (!e->timing_on && !e->step_exec && !e->graph_failed) {
-    hipGraph_t g = nullptr;
-    bool ok = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
-    if (ok) {
-      launch_step();
-      ok = hipStreamEndCapture(e->stream, &g) == hipSuccess && g != nullptr;
+  if (!e->timing_on && !e->step_exec[0] && !e->graph_failed) {
+    bool ok = true;
+    for (int k = 0; k < 2 && ok; ++k) {
+      hipGraph_t g = nullptr;
+      ok = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+      if (ok) {
+        launch_step(k == 1);
+        ok = hipStreamEndCapture(e->stream, &g) == hipSuccess && g != nullptr;
+      }
+      if (ok) ok = hipGraphInstantiate(&e->step_exec[k], g, nullptr, nullptr, 0) == hipSuccess;
+      if (ok) {
+        e->step_graph[k] = g;
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
     }
-    if (ok) ok = hipGraphInstantiate(&e->step_exec, g, nullptr, nullptr, 0) == hipSuccess;
-    if (ok) {
-      e->step_graph = g;
-    } else {
+    if (!ok) {
       // run with direct launches instead
-      if (g) (void)hipGraphDestroy(g);
-      e->step_exec = nullptr;
+      for (int k = 0; k < 2; ++k) {
+        if (e->step_exec[k]) (void)hipGraphExecDestroy(e->step_exec[k]);
+        if (e->step_graph[k]) (void)hipGraphDestroy(e->step_graph[k]);
+        e->step_exec[k] = nullptr;
+        e->step_graph[k] = nullptr;
+      }
       e->graph_failed = true;
       (void)hipGetLastError();
     }
   }
+  // whether any window updates its pair lists at step `k` of this run (the
+  // condition k_begin evaluates per window; k_begin flags a window whose
+  // update would be skipped)
+  auto lists_at = [&](long long k) {
+    for (int wi = 0; wi < NW; ++wi) {
+      const long long is = (long long)ws[wi].istep + k;
+      if (is < ws[wi].stop_at && is % e->nbcycle[wi] == 0) return true;
+    }
+    return false;
+  };
   while (done < nsteps) {
     int chunk = (int)std::min<long long>(R, nsteps - done);
-    if (!e->timing_on && e->step_exec) {
-      for (int k = 0; k < chunk; ++k) HIP_OK(hipGraphLaunch(e->step_exec, e->stream));
+    if (!e->timing_on && e->step_exec[0]) {
+      for (int k = 0; k < chunk; ++k) HIP_OK(hipGraphLaunch(e->step_exec[lists_at(done + k) ? 1 : 0], e->stream));
     }
-    for (int k = 0; k < chunk && (e->timing_on || !e->step_exec); ++k) {
-      launch_begin(e->d, e->stream, 0, 0, 0);
+    for (int k = 0; k < chunk && (e->timing_on || !e->step_exec[0]); ++k) {
+      launch_begin(e->d, e->stream, 0, 0, 1);
       launch_zero(e->d, e->stream);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[0], e->stream));
       launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
@@ -1399,6 +1425,8 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     if (ws[wi].shake_fail) return set_err(-60, "shake failure in window " + std::to_string(wi));
     if (ws[wi].list_overflow)
       return set_err(-61, "neighbour list capacity exceeded in window " + std::to_string(wi));
+    if (ws[wi].list_missed)
+      return set_err(-62, "internal error: a pair-list update was skipped in window " + std::to_string(wi));
     e->wins[wi].istep = ws[wi].istep;
   }
   int rc = sync_state_to_host(e);

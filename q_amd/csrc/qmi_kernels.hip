@@ -218,8 +218,10 @@ __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
     s.list_now = rebuild;
     s.energy_now = eval_only == 1 ? 1 : 0;
   } else {
+    // MD step; rebuild = 0: this step was launched without the list kernels
     s.active = s.istep < s.stop_at ? 1 : 0;
     s.list_now = (s.istep % W.nbcycle) == 0;
+    if (s.active && s.list_now && !rebuild) s.list_missed = 1;
     bool e = (W.iene > 0 && s.istep % W.iene == 0 && s.istep > 0) || (W.iout > 0 && s.istep % W.iout == 0) ||
              force_energy;
     s.energy_now = e ? 1 : 0;
@@ -472,7 +474,7 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
     for (int k = 0; k < 3; ++k) pi[k] = ea >= 0 ? d.x32[ea + k] : make_float4(0, 0, 0, 0);
   } else {
     pi[0] = ea >= 0 ? d.x32[ea] : make_float4(0, 0, 0, 0);
-    ljs = ea >= 0 ? d.ljp[ea] : ljs;
+    if (ea >= 0) ljs = d.ljp[ea];
   }
   const bool en = st.energy_now && !WATER_SIDE;
   float g[9];
@@ -511,6 +513,7 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
         ps = pi[0]; lj = ljs;
         pw[0] = ld[u][0]; pw[1] = ld[u][1]; pw[2] = ld[u][2];
       }
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
         // dx = x(water atom) - x(solute atom), as nonbond_pw with i solute, j water
         float dx = pw[k].x - ps.x, dy = pw[k].y - ps.y, dz = pw[k].z - ps.z;
@@ -736,25 +739,45 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
     }
   }
   const float4 pq_fixed = (Q_SIDE && qa >= 0) ? d.x32[qa] : make_float4(0, 0, 0, 0);
+  float4 pw_fixed[3];
+  for (int k = 0; k < 3; ++k) pw_fixed[k] = (!Q_SIDE && ow >= 0) ? d.x32[ow + k] : make_float4(0, 0, 0, 0);
   const int s0 = wk.y, n = wk.z, lst = L.start[ic];
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   long long gfix[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int t = 0; t < n; ++t) {
-    const int jc = L.jc[s0 + t];
-    const uint2 m = L.mask[s0 + t];
-    if (lane_bit(m, lane)) {
-      int iq = iq_l, o = ow;
-      float4 pq = pq_fixed;
+  // kQGroup tiles per pass (work items start on a kQGroup boundary of the
+  // list): the tile, index and coordinate loads of the pass are issued
+  // before its arithmetic; the sums run in tile order and go to fixed point
+  // after each pass
+  for (int t0 = 0; t0 < n; t0 += kQGroup) {
+    bool on[kQGroup];
+    int jx[kQGroup];
+#pragma unroll
+    for (int u = 0; u < kQGroup; ++u) {
+      const int tt = s0 + min(t0 + u, n - 1);
+      const int jc = L.jc[tt];
+      on[u] = t0 + u < n && lane_bit(L.mask[tt], lane);
+      jx[u] = Q_SIDE ? d.wc_o[jc * kCl + b] : d.qc_q[jc * kCl + b];
+    }
+    float4 pwv[kQGroup][3];
+    float4 pqv[kQGroup];
+#pragma unroll
+    for (int u = 0; u < kQGroup; ++u) {
+      const int jj = on[u] ? jx[u] : 0;
       if (Q_SIDE) {
-        o = d.wc_o[jc * kCl + b];
+        for (int k = 0; k < 3; ++k) pwv[u][k] = d.x32[jj + k];
       } else {
-        iq = d.qc_q[jc * kCl + b];
-        pq = d.x32[d.q_atom[W.q_off + iq]];
+        pqv[u] = d.x32[d.q_atom[W.q_off + jj]];
       }
+    }
+#pragma unroll
+    for (int u = 0; u < kQGroup; ++u) {
+      if (!on[u]) continue;
+      const int iq = Q_SIDE ? iq_l : jx[u];
+      const float4 pq = Q_SIDE ? pq_fixed : pqv[u];
       float dxv[3], dyv[3], dzv[3], r2v[3];
       for (int k = 0; k < 3; ++k) {
-        const float4 pw = d.x32[o + k];
+        const float4 pw = Q_SIDE ? pwv[u][k] : pw_fixed[k];
         dxv[k] = pw.x - pq.x; dyv[k] = pw.y - pq.y; dzv[k] = pw.z - pq.z;
         r2v[k] = dxv[k] * dxv[k] + dyv[k] * dyv[k] + dzv[k] * dzv[k];
       }
@@ -793,8 +816,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
         g[6] += dvH2 * dxv[2]; g[7] += dvH2 * dyv[2]; g[8] += dvH2 * dzv[2];
       }
     }
-    if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1)
-      for (int k = 0; k < 9; ++k) { gfix[k] += fix((double)g[k], kFScale); g[k] = 0.f; }
+    for (int k = 0; k < 9; ++k) { gfix[k] += fix((double)g[k], kFScale); g[k] = 0.f; }
   }
   const int nat = Q_SIDE ? 1 : 3;
   const int own = Q_SIDE ? qa : ow;
@@ -872,23 +894,40 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   const int s0 = wk.y, n = wk.z, lst = L.start[ic];
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   long long gx = 0, gy = 0, gz = 0;
-  for (int t = 0; t < n; ++t) {
-    const int jc = L.jc[s0 + t];
-    const uint2 m = L.mask[s0 + t];
-    if (lane_bit(m, lane)) {
-      const bool is14 = lane_bit(L.m14[s0 + t], lane);
-      int iq = iq_l, j = pa;
-      float4 pq = pq_fixed, pj = pp_fixed, lj = lj_fixed;
-      int iacj = iac_fixed;
+  // kQGroup tiles per pass, loads first (as in k_qw)
+  for (int t0 = 0; t0 < n; t0 += kQGroup) {
+    bool on[kQGroup], o14[kQGroup];
+    int jx[kQGroup];
+#pragma unroll
+    for (int u = 0; u < kQGroup; ++u) {
+      const int tt = s0 + min(t0 + u, n - 1);
+      const int jc = L.jc[tt];
+      on[u] = t0 + u < n && lane_bit(L.mask[tt], lane);
+      o14[u] = lane_bit(L.m14[tt], lane);
+      jx[u] = Q_SIDE ? d.sc_a[jc * kCl + b] : d.qc_q[jc * kCl + b];
+    }
+    float4 pv[kQGroup], lv[kQGroup];
+    int iacv[kQGroup];
+#pragma unroll
+    for (int u = 0; u < kQGroup; ++u) {
+      const int jj = on[u] ? jx[u] : 0;
       if (Q_SIDE) {
-        j = d.sc_a[jc * kCl + b];
-        pj = d.x32[j];
-        lj = d.ljp[j];
-        iacj = d.iac[j];
+        pv[u] = d.x32[jj];
+        lv[u] = d.ljp[jj];
+        iacv[u] = d.iac[jj];
       } else {
-        iq = d.qc_q[jc * kCl + b];
-        pq = d.x32[d.q_atom[W.q_off + iq]];
+        pv[u] = d.x32[d.q_atom[W.q_off + jj]];
       }
+    }
+#pragma unroll
+    for (int u = 0; u < kQGroup; ++u) {
+      if (!on[u]) continue;
+      const bool is14 = o14[u];
+      const int iq = Q_SIDE ? iq_l : jx[u];
+      const float4 pq = Q_SIDE ? pq_fixed : pv[u];
+      const float4 pj = Q_SIDE ? pv[u] : pp_fixed;
+      const float4 lj = Q_SIDE ? lv[u] : lj_fixed;
+      const int iacj = Q_SIDE ? iacv[u] : iac_fixed;
       const float dx = pj.x - pq.x, dy = pj.y - pq.y, dz = pj.z - pq.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
       const float rinv = rsqrtf(r2), r2inv = rinv * rinv;
@@ -917,10 +956,8 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
       if (Q_SIDE) { g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz; }
       else { g0 += dv * dx; g1 += dv * dy; g2 += dv * dz; }
     }
-    if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1) {
-      gx += fix((double)g0, kFScale); gy += fix((double)g1, kFScale); gz += fix((double)g2, kFScale);
-      g0 = g1 = g2 = 0.f;
-    }
+    gx += fix((double)g0, kFScale); gy += fix((double)g1, kFScale); gz += fix((double)g2, kFScale);
+    g0 = g1 = g2 = 0.f;
   }
   const int own = Q_SIDE ? qa : pa;
   const long long sx = row8_sum_i(gx), sy = row8_sum_i(gy), sz = row8_sum_i(gz);

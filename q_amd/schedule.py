@@ -29,7 +29,7 @@ import numpy as np
 
 KT = 0.001986 * 300.0          # kcal/mol at 300 K
 NBINS = 32
-U_LO, U_HI = -10.0, 22.0       # histogram range of u = delta * dE / kT
+U_LO, U_HI = -8.0, 8.0         # histogram range of the centred reduced work
 N_IN = NBINS + 3               # histogram, lambda, delta, alpha / 10
 PILOT_LAMBDAS = np.linspace(0.0, 1.0, 11)
 
@@ -50,7 +50,11 @@ def histogram(u: np.ndarray) -> np.ndarray:
 
 
 def features(dE: np.ndarray, lam: float, delta: float, alpha: float) -> np.ndarray:
-    u = delta * dE / KT
+    """Inputs of the overlap network for a step lambda -> lambda + delta from
+    a window with energy gaps dE: the histogram of the reduced work
+    u = delta * dE / kT about its mean (a constant shift of u changes the
+    free energy, not the overlap), then lambda, delta and alpha / 10."""
+    u = delta * (dE - np.mean(dE)) / KT
     return np.concatenate([histogram(u), [lam, delta, alpha / 10.0]]).astype(np.float32)
 
 
@@ -313,9 +317,9 @@ def build(args):
     lig_of = {id(p): s for p, s in zip(lig, lig_sys)}
 
     def system_of(p):
-        if p.leg == "ligwat":
-            return lig_of[id(p)]
-        return synth.reperturb(scaffolds[p.scaffold], p.seed, p.n_pert, p.alpha)
+        # the perturbation itself is drawn by synth.reperturb in both legs
+        base = lig_of[id(p)] if p.leg == "ligwat" else scaffolds[p.scaffold]
+        return synth.reperturb(base, p.seed, p.n_pert, p.alpha)
 
     log(f"built {len(lig_sys)} ligand-in-water systems and {len(scaffolds)} protein environments")
     # pilot ladders, one batch per leg so that windows of one size share engines

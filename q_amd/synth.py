@@ -389,9 +389,11 @@ def _neutral_charges(rng, n: int, sd: float = 0.3) -> list:
 
 
 def _ligand_fep(rng, sysm: System, first: int, n_lig: int, types: list, n_pert: int, alpha: float,
-                nb: list, heavy: list):
+                nb: list, heavy: list, spread: bool = False):
     """All ligand atoms become q-atoms; ``n_pert`` of them change charge and
-    LJ type between state 1 and state 2 and get soft-core alpha."""
+    LJ type between state 1 and state 2 and get soft-core alpha.  State 2
+    is kept neutral by moving the residual charge onto the last perturbed
+    heavy atom, or, with ``spread``, in equal shares onto all of them."""
     pert = sorted(int(v) for v in rng.choice(n_lig, size=n_pert, replace=False))
     # at least one perturbed heavy atom carries the neutrality correction
     if not any(heavy[k] for k in pert):
@@ -404,6 +406,12 @@ def _ligand_fep(rng, sysm: System, first: int, n_lig: int, types: list, n_pert: 
         q2[k] = round(q1[k] + float(rng.normal(0.0, 0.3)), 4) if heavy[k] else 0.0
     # keep state 2 neutral: put the residual on a perturbed heavy atom
     resid = round(sum(q2) - sum(q1), 4)
+    if spread:
+        hs = [k for k in pert if heavy[k]]
+        share = round(resid / len(hs), 4)
+        for k in hs[:-1]:
+            q2[k] = round(q2[k] - share, 4)
+        resid = round(sum(q2) - sum(q1), 4)
     kh = [k for k in pert if heavy[k]][-1]
     q2[kh] = round(q2[kh] - resid, 4)
     names = {t: LJ_TYPES[t - 1][0] for t in range(1, len(LJ_TYPES) + 1)}
@@ -444,11 +452,13 @@ def _ligand_fep(rng, sysm: System, first: int, n_lig: int, types: list, n_pert: 
             sysm.qimpropers.append((i, j, k, l, 1, 1))
 
 
-def reperturb(base: System, seed: int, n_pert: int, alpha: float = 10.0) -> System:
+def reperturb(base: System, seed: int, n_pert: int, alpha: float = 10.0, spread: bool = True) -> System:
     """A copy of ``base`` whose two FEP states are redrawn: the same ligand
     (the first ``len(base.qatoms)`` topology atoms) and environment, a new
-    seeded choice of ``n_pert`` perturbed atoms with new charges and types.
-    Used to make many ligand perturbations in one costly environment."""
+    seeded choice of ``n_pert`` perturbed atoms with new charges and types
+    (neutrality residual spread over the perturbed heavy atoms unless
+    ``spread`` is false).  Used to make many ligand perturbations in one
+    costly environment."""
     import copy
     if not base.qatoms or base.qatoms[0] != 1:
         raise ValueError("reperturb needs a system whose ligand atoms come first")
@@ -464,7 +474,7 @@ def reperturb(base: System, seed: int, n_pert: int, alpha: float = 10.0) -> Syst
         if i <= n_lig and j <= n_lig:
             nb[i - 1].append(j - 1)
             nb[j - 1].append(i - 1)
-    _ligand_fep(np.random.default_rng(seed), s, 1, n_lig, types, n_pert, alpha, nb, heavy)
+    _ligand_fep(np.random.default_rng(seed), s, 1, n_lig, types, n_pert, alpha, nb, heavy, spread=spread)
     return s
 
 
