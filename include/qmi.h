@@ -106,7 +106,11 @@ int qmi_set_state(qmi_engine* e, int w, const double* xyz, const double* v, long
 int qmi_get_pairlist(qmi_engine* e, int w, int kind, int32_t* i, int32_t* j, int16_t* ljcod,
                      long long cap, long long* n);
 
-/* Size of window w: natom, nat_solute, nwat, nstates, nqat (5 ints). */
+/* Size of window w: natom, nat_solute, nwat, nstates, nqat (5 ints), and
+ * with cap >= 6 its status: 0 running, 1 stopped by a SHAKE failure, 2
+ * stopped because a neighbour row overflowed.  A failed window stops at the
+ * failing step while the other windows of the engine run on; qmi_run then
+ * returns an error naming every failed window. */
 int qmi_window_info(qmi_engine* e, int w, int32_t* info, int cap);
 
 /* Write window w's restart file (write_xfin, md.f90:15452-15470). */

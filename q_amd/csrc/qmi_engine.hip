@@ -57,6 +57,7 @@ struct HostWin {
   std::vector<double> EQ;                     // S x 15
   double temp = 0, tfree = 0, ekin = 0;
   double rad = 0, wpol = 0;                   // solvent radial / polarisation restraint energies
+  int status = 0;                             // 0 running, 1 SHAKE failure, 2 list capacity exceeded
   int istep = 0;                              // host mirror of the device step
   bool done = false;
 };
@@ -1421,14 +1422,23 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   for (int f = 0; f < 5; ++f) e->timings[f] = nsteps > 0 ? fam[f] / (double)nsteps : 0.0;
   e->timings[5] = nsteps > 0 ? us / (double)nsteps : 0.0;
   HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
+  // a failed window stops at its failing step (as qdyn ends that run); the
+  // others run to the end, and the error names every failed window
+  std::string failed_shake, failed_list;
   for (int wi = 0; wi < NW; ++wi) {
-    if (ws[wi].shake_fail) return set_err(-60, "shake failure in window " + std::to_string(wi));
-    if (ws[wi].list_overflow)
-      return set_err(-61, "neighbour list capacity exceeded in window " + std::to_string(wi));
     if (ws[wi].list_missed)
       return set_err(-62, "internal error: a pair-list update was skipped in window " + std::to_string(wi));
+    if (ws[wi].shake_fail) {
+      e->wins[wi].status = 1;
+      failed_shake += (failed_shake.empty() ? "" : " ") + std::to_string(wi);
+    } else if (ws[wi].list_overflow) {
+      e->wins[wi].status = 2;
+      failed_list += (failed_list.empty() ? "" : " ") + std::to_string(wi);
+    }
     e->wins[wi].istep = ws[wi].istep;
   }
+  if (!failed_shake.empty()) return set_err(-60, "shake failure in window(s) " + failed_shake);
+  if (!failed_list.empty()) return set_err(-61, "neighbour list capacity exceeded in window(s) " + failed_list);
   int rc = sync_state_to_host(e);
   if (rc) return rc;
   if (finalize) {
@@ -1554,7 +1564,9 @@ int qmi_window_info(qmi_engine* e, int w, int32_t* info, int cap) {
   if (cap < 5) return set_err(-3, "buffer too small");
   info[0] = h->w->natom; info[1] = h->w->top.nat_solute; info[2] = h->w->nwat;
   info[3] = h->w->inp.nstates; info[4] = h->w->fep.nqat;
-  return 5;
+  if (cap < 6) return 5;
+  info[5] = h->status;
+  return 6;
 }
 
 int qmi_get_pairlist(qmi_engine* e, int w, int kind, int32_t* pi, int32_t* pj, int16_t* pl, long long cap,

@@ -219,7 +219,7 @@ __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
     s.energy_now = eval_only == 1 ? 1 : 0;
   } else {
     // MD step; rebuild = 0: this step was launched without the list kernels
-    s.active = s.istep < s.stop_at ? 1 : 0;
+    s.active = s.istep < s.stop_at && !s.shake_fail && !s.list_overflow ? 1 : 0;
     s.list_now = (s.istep % W.nbcycle) == 0;
     if (s.active && s.list_now && !rebuild) s.list_missed = 1;
     bool e = (W.iene > 0 && s.istep % W.iene == 0 && s.istep > 0) || (W.iout > 0 && s.istep % W.iout == 0) ||

@@ -143,9 +143,9 @@ class Engine:
         _check(self.lib.qmi_evaluate(self._h, 1 if rebuild_lists else 0))
 
     def info(self, w: int) -> dict:
-        a = (C.c_int32 * 5)()
-        _check(self.lib.qmi_window_info(self._h, w, a, 5))
-        return dict(natom=a[0], nat_solute=a[1], nwat=a[2], nstates=a[3], nqat=a[4])
+        a = (C.c_int32 * 6)()
+        _check(self.lib.qmi_window_info(self._h, w, a, 6))
+        return dict(natom=a[0], nat_solute=a[1], nwat=a[2], nstates=a[3], nqat=a[4], status=a[5])
 
     def energies(self, w: int) -> dict:
         a = np.zeros(N_ENERGIES)

@@ -29,32 +29,67 @@ def test_overlap_limits():
 
 
 def test_features_layout():
-    dE = np.linspace(-5, 5, 200)
-    f = S.features(dE, 0.3, 0.1, 10.0)
+    dE = np.linspace(-5, 5, 200) + 40.0
+    f = S.features(dE, 0.3, 10.0)
     assert f.shape == (S.N_IN,) and f.dtype == np.float32
     assert abs(f[:S.NBINS].sum() - 1.0) < 1e-6
-    assert np.allclose(f[S.NBINS:], [0.3, 0.1, 1.0])
+    assert np.allclose(f[S.NBINS:], [0.3, 1.0])
+    # a constant shift of the energy gap does not change the features
+    assert np.array_equal(f, S.features(dE - 40.0, 0.3, 10.0))
 
 
-def test_greedy_ladder_follows_the_predictor():
-    # predicted overlap exp(-k |delta| sd(dE)), with the energy gap spread
-    # ten times wider at the lambda = 1 end: steps must shrink there
+def test_gaussian_overlap_matches_sampled_bar():
+    # the closed form against bar_overlap on Gaussian works of known spread
+    rng = np.random.default_rng(4)
+    for sw in (0.3, 1.0, 2.5):
+        df = 1.7
+        wF = rng.normal(df + sw ** 2 / 2, sw, 200000)
+        wR = rng.normal(-df + sw ** 2 / 2, sw, 200000)
+        assert abs(S.gaussian_overlap(sw)[0] - S.bar_overlap(wF, wR)) < 0.01
+    assert abs(S.gaussian_overlap(0.0)[0] - 1.0) < 1e-12
+    o = S.gaussian_overlap(np.linspace(0, 10, 50))
+    assert np.all(np.diff(o) < 0)
+    sw = np.array([0.05, 0.7, 3.0, 8.0])
+    assert np.allclose(S.work_spread(S.gaussian_overlap(sw)), sw, rtol=1e-6)
+
+
+def test_equal_length_ladder():
+    lams = S.PILOT_LAMBDAS
+    # equal overlaps: the ladder is uniform
+    s, implied = S.ladder_from_overlaps(lams, np.full(10, 0.8), 16, 22)
+    assert np.allclose(s, np.linspace(0, 1, len(s)))
+    # ten times the work spread on the last interval: windows crowd there,
+    # and every step carries the same spread
+    ovl = S.gaussian_overlap(np.array([0.5] * 9 + [5.0]))
+    s, implied = S.ladder_from_overlaps(lams, ovl, 16, 22)
+    assert 16 <= len(s) <= 22 and s[0] == 0.0 and s[-1] == 1.0 and np.all(np.diff(s) > 0)
+    rate = np.array([5.0] * 9 + [50.0])
+    spread = [np.sum(rate * np.clip(np.minimum(b, lams[1:]) - np.maximum(a, lams[:-1]), 0, None))
+              for a, b in zip(s[:-1], s[1:])]
+    assert np.allclose(spread, spread[0], rtol=1e-9)
+    assert np.mean(s > 0.9) >= 0.45     # the last interval holds 5 of the 9.5 units of length
+    assert abs(implied - S.gaussian_overlap(spread[0])[0]) < 1e-9
+    # the window count follows the target overlap within the bounds
+    assert len(S.ladder_from_overlaps(lams, np.full(10, 0.99), 16, 22)[0]) == 16
+    assert len(S.ladder_from_overlaps(lams, np.full(10, 0.05), 16, 22)[0]) == 22
+    assert len(S.ladder_from_overlaps(lams, np.full(10, 0.05), n=19)[0]) == 19
+
+
+def test_learned_ladder_follows_the_predictor():
+    # a predictor whose overlap falls with the spread of the centred work
+    # histogram: the ladder crowds where the pilot energy gaps are wide
     lams = list(S.PILOT_LAMBDAS)
     rng = np.random.default_rng(2)
     dEs = [rng.normal(0, 1 + 9 * lam ** 4, 400) for lam in lams]
-    spread = {round(l, 6): d.std() for l, d in zip(lams, dEs)}
+    centres = np.linspace(S.U_LO, S.U_HI, S.NBINS + 1)[:-1] + (S.U_HI - S.U_LO) / S.NBINS / 2
 
     def predict(F):
-        out = []
-        for f in F:
-            lam, delta = float(f[S.NBINS]), float(f[S.NBINS + 1])
-            k = int(np.argmin(np.abs(np.array(lams) - lam)))
-            out.append(np.exp(-3.0 * abs(delta) * spread[round(lams[k], 6)]))
-        return np.array(out)
+        h = F[:, :S.NBINS]
+        sd = np.sqrt(np.maximum((h * centres ** 2).sum(1) - (h * centres).sum(1) ** 2, 0))
+        return S.gaussian_overlap(sd)
 
-    s = S.propose_n(predict, lams, dEs, 10.0, 16, 24)
-    assert 16 <= len(s) <= 24
-    assert s[0] == 0.0 and s[-1] == 1.0 and np.all(np.diff(s) > 0)
+    s = S.propose_n(predict, lams, dEs, 10.0, 16, 22)
+    assert 16 <= len(s) <= 22
     steps = np.diff(s)
     assert steps[:3].mean() > 2 * steps[-3:].mean()
 
@@ -78,7 +113,7 @@ def test_bar_variance_matches_replica_spread():
 
 def test_simulated_fraction_bound():
     # pilots included: the largest ladder within half the 51-window time
-    assert S.simulated_fraction(22, 2000, 500) <= 0.5 < S.simulated_fraction(23, 2000, 500)
+    assert S.simulated_fraction(22, 4000, 1000) <= 0.5 < S.simulated_fraction(23, 4000, 1000)
 
 
 def test_reperturb_redraws_only_the_fep_states():
