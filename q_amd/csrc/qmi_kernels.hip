@@ -696,11 +696,62 @@ __device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int s, int q
 }
 
 // Q-atom tiles: lane (a, b) of a work item takes member a of the i-cluster
-// and member b of each listed j-cluster.  The force pass is FP32 (hardware
-// reciprocal); on energy steps a second pass over the same tiles computes
-// the per-state energies in df32.  Sums go to fixed point every kQGroup tiles
-// of the i-cluster's list, so they do not depend on the work-item size.
+// and member b of each listed j-cluster.  Forces are computed once per pair,
+// on the non-q side (rows of water or solute clusters, entries q-clusters),
+// in FP32 with the hardware reciprocal.  The row member's gradient sums go
+// to fixed point every kQGroup tiles of its list, so they do not depend on
+// the work-item size; the q-atoms' share of each tile is summed over the
+// tile's rows and stored in the tile's own slots (q_reaction), which
+// k_q_gather adds up per q-atom in entry order.  On energy steps the q side
+// makes its own pass for the per-state energies in df32.
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// q-atom share of kQGroup tiles starting at entry e0 of a non-q-side list:
+// each lane's reaction r (gradient of its row member, summed over the
+// member's sites) is summed over the 8 rows in FP32 in a fixed order, and
+// the q-atom b of the tile takes -r in its slot (no atomics)
+__device__ __forceinline__ void q_reaction(const Dev& d, int slot0, int lane, int e0, int t0, int n, const float* rx,
+                                           const float* ry, const float* rz) {
+#pragma unroll
+  for (int u = 0; u < kQGroup; ++u) {
+    float x = rx[u], y = ry[u], z = rz[u];
+    for (int o = 8; o < 64; o <<= 1) {
+      x += __shfl_xor(x, o, 64);
+      y += __shfl_xor(y, o, 64);
+      z += __shfl_xor(z, o, 64);
+    }
+    if (lane < kCl && t0 + u < n) {
+      long long* q = d.qslot + 3 * ((size_t)(slot0 + e0 + u) * kCl + lane);
+      q[0] = fix(-(double)x, kFScale);
+      q[1] = fix(-(double)y, kFScale);
+      q[2] = fix(-(double)z, kFScale);
+    }
+  }
+}
+
+// q-atom gradients from the tile slots: one wave per q-cluster, lane (b, k)
+// sums the entries k, k+8, ... of member b; integer sums, so the result does
+// not depend on the order
+__global__ void __launch_bounds__(256) k_q_gather(Dev d) {
+  const int qc = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (qc >= d.nqc_all) return;
+  const int w = d.qc_win[qc];
+  if (!d.ws[w].active) return;
+  const int b = lane & 7, k = lane >> 3;
+  long long gx = 0, gy = 0, gz = 0;
+  for (int m = d.qg_start[qc] + k; m < d.qg_start[qc + 1]; m += kCl) {
+    const long long* q = d.qslot + 3 * ((size_t)d.qg_ent[m] * kCl + b);
+    gx += q[0]; gy += q[1]; gz += q[2];
+  }
+  for (int o = 8; o < 64; o <<= 1) {
+    gx += __shfl_xor(gx, o, 64);
+    gy += __shfl_xor(gy, o, 64);
+    gz += __shfl_xor(gz, o, 64);
+  }
+  const int iq = d.qc_q[qc * kCl + b];
+  if (k == 0 && iq >= 0) add_grad_fix(d.f, d.q_atom[d.win[w].q_off + iq], gx, gy, gz);
+}
 
 // q-atom vs water (nonbond_qw_spc, md.f90:11302-11411)
 template <bool Q_SIDE>
@@ -745,11 +796,13 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   long long gfix[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // the q side only computes energies (on energy steps)
+  if (Q_SIDE && !st.energy_now) return;
   // kQGroup tiles per pass (work items start on a kQGroup boundary of the
   // list): the tile, index and coordinate loads of the pass are issued
   // before its arithmetic; the sums run in tile order and go to fixed point
   // after each pass
-  for (int t0 = 0; t0 < n; t0 += kQGroup) {
+  for (int t0 = 0; !Q_SIDE && t0 < n; t0 += kQGroup) {
     bool on[kQGroup];
     int jx[kQGroup];
 #pragma unroll
@@ -761,6 +814,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
     }
     float4 pwv[kQGroup][3];
     float4 pqv[kQGroup];
+    float rx[kQGroup] = {}, ry[kQGroup] = {}, rz[kQGroup] = {};   // water side: reaction per tile
 #pragma unroll
     for (int u = 0; u < kQGroup; ++u) {
       const int jj = on[u] ? jx[u] : 0;
@@ -814,17 +868,21 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
         g[0] += dvO * dxv[0]; g[1] += dvO * dyv[0]; g[2] += dvO * dzv[0];
         g[3] += dvH1 * dxv[1]; g[4] += dvH1 * dyv[1]; g[5] += dvH1 * dzv[1];
         g[6] += dvH2 * dxv[2]; g[7] += dvH2 * dyv[2]; g[8] += dvH2 * dzv[2];
+        rx[u] = dvO * dxv[0] + dvH1 * dxv[1] + dvH2 * dxv[2];
+        ry[u] = dvO * dyv[0] + dvH1 * dyv[1] + dvH2 * dyv[2];
+        rz[u] = dvO * dzv[0] + dvH1 * dzv[1] + dvH2 * dzv[2];
       }
     }
     for (int k = 0; k < 9; ++k) { gfix[k] += fix((double)g[k], kFScale); g[k] = 0.f; }
+    if (!Q_SIDE) q_reaction(d, d.qslot_wq, lane, s0 + t0, t0, n, rx, ry, rz);
   }
-  const int nat = Q_SIDE ? 1 : 3;
-  const int own = Q_SIDE ? qa : ow;
-  long long gs[9];
-  for (int k = 0; k < 3 * nat; ++k) gs[k] = row8_sum_i(gfix[k]);
-  if (b == 0 && own >= 0)
-    for (int p = 0; p < nat; ++p) add_grad_fix(d.f, own + p, gs[3 * p], gs[3 * p + 1], gs[3 * p + 2]);
-  if (!(Q_SIDE && st.energy_now)) return;
+  if (!Q_SIDE) {
+    long long gs[9];
+    for (int k = 0; k < 9; ++k) gs[k] = row8_sum_i(gfix[k]);
+    if (b == 0 && ow >= 0)
+      for (int p = 0; p < 3; ++p) add_grad_fix(d.f, ow + p, gs[3 * p], gs[3 * p + 1], gs[3 * p + 2]);
+    return;
+  }
   // energy pass (df32), q side only
   long long efix[kMaxS][2] = {};
   double eel[kMaxS] = {}, evdw[kMaxS] = {};
@@ -894,8 +952,9 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   const int s0 = wk.y, n = wk.z, lst = L.start[ic];
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   long long gx = 0, gy = 0, gz = 0;
+  if (Q_SIDE && !st.energy_now) return;   // the q side only computes energies
   // kQGroup tiles per pass, loads first (as in k_qw)
-  for (int t0 = 0; t0 < n; t0 += kQGroup) {
+  for (int t0 = 0; !Q_SIDE && t0 < n; t0 += kQGroup) {
     bool on[kQGroup], o14[kQGroup];
     int jx[kQGroup];
 #pragma unroll
@@ -908,6 +967,7 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
     }
     float4 pv[kQGroup], lv[kQGroup];
     int iacv[kQGroup];
+    float rx[kQGroup] = {}, ry[kQGroup] = {}, rz[kQGroup] = {};   // solute side: reaction per tile
 #pragma unroll
     for (int u = 0; u < kQGroup; ++u) {
       const int jj = on[u] ? jx[u] : 0;
@@ -954,15 +1014,20 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
         dv += r2inv * (-vel - (12.f * va - 6.f * vb) * ratio) * lam[s];
       }
       if (Q_SIDE) { g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz; }
-      else { g0 += dv * dx; g1 += dv * dy; g2 += dv * dz; }
+      else {
+        g0 += dv * dx; g1 += dv * dy; g2 += dv * dz;
+        rx[u] = dv * dx; ry[u] = dv * dy; rz[u] = dv * dz;
+      }
     }
     gx += fix((double)g0, kFScale); gy += fix((double)g1, kFScale); gz += fix((double)g2, kFScale);
     g0 = g1 = g2 = 0.f;
+    if (!Q_SIDE) q_reaction(d, 0, lane, s0 + t0, t0, n, rx, ry, rz);
   }
-  const int own = Q_SIDE ? qa : pa;
-  const long long sx = row8_sum_i(gx), sy = row8_sum_i(gy), sz = row8_sum_i(gz);
-  if (b == 0 && own >= 0) add_grad_fix(d.f, own, sx, sy, sz);
-  if (!(Q_SIDE && st.energy_now)) return;
+  if (!Q_SIDE) {
+    const long long sx = row8_sum_i(gx), sy = row8_sum_i(gy), sz = row8_sum_i(gz);
+    if (b == 0 && pa >= 0) add_grad_fix(d.f, pa, sx, sy, sz);
+    return;
+  }
   long long efix[kMaxS][2] = {};
   double eel[kMaxS] = {}, evdw[kMaxS] = {};
   for (int t = 0; t < n; ++t) {
@@ -2163,6 +2228,7 @@ void launch_q_tiles(const Dev& d, hipStream_t s) {
   if (d.tl[T_WQ].nwork) k_qw<false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PQ].nwork) k_qp<false><<<blocks((long long)d.tl[T_PQ].nwork * 64, 256), 256, 0, s>>>(d);
+  if (d.nqc_all) k_q_gather<<<blocks((long long)d.nqc_all * 64, 256), 256, 0, s>>>(d);
 }
 
 void launch_ww(const Dev& d, hipStream_t s) {
