@@ -81,6 +81,7 @@ struct TileList {
   const int4* work;
   int nwork;
 };
+constexpr int kTempParts = 8;      // blocks per window of the kinetic-energy sum
 constexpr int kGatherAtoms = 256;  // atoms per block of the bonded gather
 constexpr int kGatherSlots = 2048; // slots staged in LDS per gather block
 constexpr int kQGroup = 4;        // Q-tile sums go to fixed point every 4 tiles of a list
@@ -137,6 +138,7 @@ struct Dev {
   const int* sp_start;  // per atom (ntot+1)
   const int* sp_j;
   const uint8_t* sp_t;  // 1 = excluded, 2 = 1-4
+  const int2* sp_rng;   // per atom: smallest and largest partner in sp_j
   // clusters
   const int* wc_o;      // 8 per water cluster: O atom or -1
   const int* sc_a;      // 8 per solute cluster: atom or -1
@@ -227,6 +229,7 @@ struct Dev {
   // energy snapshots for output: [ring slot][window][E_NSLOT] + temperature
   long long* e_ring;
   double* t_ring;        // [slot][window][4]: temp, tfree, ekin, istep
+  double* t_part;        // [window][kTempParts][4]: kinetic-energy partial sums
   int ring_n;
 };
 

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <climits>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -644,6 +645,10 @@ static int build_device(qmi_engine* e) {
     toff += t.natyps;
   }
   sp_start.push_back((int)sp_j.size());
+  std::vector<int2> sp_rng(ao, make_int2(INT_MAX, -1));
+  for (int i = 0; i < ao; ++i)
+    for (int k = sp_start[i]; k < sp_start[i + 1]; ++k)
+      sp_rng[i] = make_int2(std::min(sp_rng[i].x, sp_j[k]), std::max(sp_rng[i].y, sp_j[k]));
   e->ntot = ao;
   e->ngrp = goff;
   e->nwc = (int)wc_win.size();
@@ -726,6 +731,10 @@ static int build_device(qmi_engine* e) {
   UP(ip, sp_start); d.sp_start = ip;
   UP(ip, sp_j); d.sp_j = ip;
   UP(up, sp_t); d.sp_t = up;
+  {
+    int2* rp;
+    UP(rp, sp_rng); d.sp_rng = rp;
+  }
   UP(ip, wc_o); d.wc_o = ip;
   UP(ip, sc_a); d.sc_a = ip;
   UP(ip, qc_q); d.qc_q = ip;
@@ -835,6 +844,7 @@ static int build_device(qmi_engine* e) {
   d.ring_n = 64;
   if ((rc = dev_alloc(e, &d.e_ring, (size_t)d.ring_n * NW * E_NSLOT)) != 0) return rc;
   if ((rc = dev_alloc(e, &d.t_ring, (size_t)d.ring_n * NW * 4)) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.t_part, (size_t)NW * kTempParts * 4)) != 0) return rc;
 
   // --- neighbour rows of the dynamic lists: capacity per row from the largest
   // candidate count at the start (switching atoms within the cut-off plus
@@ -1158,7 +1168,6 @@ static int autotune(qmi_engine* e) {
   };
   // a forces-only state with fresh lists and LRF expansions
   launch_begin(d, e->stream, 0, 2, 1);
-  launch_zero(d, e->stream);
   launch_lists(d, e->stream, e->ngrp, e->nwc, e->nsc);
   HIP_OK(hipStreamSynchronize(e->stream));
   const int bs[3] = {64, 128, 256};
@@ -1360,7 +1369,6 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   // timing pass it runs as one captured graph
   auto launch_step = [&](bool lists) {
     launch_begin(e->d, e->stream, 0, 0, lists ? 1 : 0);
-    launch_zero(e->d, e->stream);
     if (lists) launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
     launch_ww(e->d, e->stream);
     launch_nonbonded(e->d, e->stream, false);
@@ -1412,7 +1420,6 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     }
     for (int k = 0; k < chunk && (e->timing_on || !e->step_exec[0]); ++k) {
       launch_begin(e->d, e->stream, 0, 0, 1);
-      launch_zero(e->d, e->stream);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[0], e->stream));
       launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[1], e->stream));
@@ -1487,7 +1494,6 @@ int qmi_evaluate(qmi_engine* e, int rebuild_lists) {
   if (!e || !e->prepared) return set_err(-2, "engine not prepared");
   HIP_OK(hipSetDevice(e->device));
   launch_begin(e->d, e->stream, 1, 1, rebuild_lists ? 1 : 0);
-  launch_zero(e->d, e->stream);
   launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
   launch_nonbonded(e->d, e->stream);
   launch_other_forces(e->d, e->stream);

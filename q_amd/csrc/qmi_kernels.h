@@ -6,7 +6,6 @@
 
 namespace qmi {
 void launch_begin(const Dev& d, hipStream_t s, int force_energy, int eval_only, int rebuild);
-void launch_zero(const Dev& d, hipStream_t s);
 void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc);
 // water-water tiles alone, and all nonbonded tile kernels (with or without them)
 void launch_ww(const Dev& d, hipStream_t s);

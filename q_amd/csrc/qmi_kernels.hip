@@ -207,27 +207,38 @@ __device__ __forceinline__ df df_dx(double xj, double xi) {
  return df_from(xj - xi); }
 
 // ------------------------------------------------------ step control --
+// Step start, one thread per window (flags, energy accumulators) and per
+// atom (gradient accumulators of running windows start at zero).  The two
+// parts touch different fields of the window state.
+__device__ __forceinline__ bool window_runs(const WinState& s) {
+  return s.istep < s.stop_at && !s.shake_fail && !s.list_overflow;
+}
+
 __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= d.nwin) return;
-  WinState& s = d.ws[w];
-  const WinDev& W = d.win[w];
-  if (eval_only) {
-    // 1: energy evaluation; 2: forces only (the autotuner's timing runs)
-    s.active = 1;
-    s.list_now = rebuild;
-    s.energy_now = eval_only == 1 ? 1 : 0;
-  } else {
-    // MD step; rebuild = 0: this step was launched without the list kernels
-    s.active = s.istep < s.stop_at && !s.shake_fail && !s.list_overflow ? 1 : 0;
-    s.list_now = (s.istep % W.nbcycle) == 0;
-    if (s.active && s.list_now && !rebuild) s.list_missed = 1;
-    bool e = (W.iene > 0 && s.istep % W.iene == 0 && s.istep > 0) || (W.iout > 0 && s.istep % W.iout == 0) ||
-             force_energy;
-    s.energy_now = e ? 1 : 0;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.nwin) {
+    WinState& s = d.ws[i];
+    const WinDev& W = d.win[i];
+    if (eval_only) {
+      // 1: energy evaluation; 2: forces only (the autotuner's timing runs)
+      s.active = 1;
+      s.list_now = rebuild;
+      s.energy_now = eval_only == 1 ? 1 : 0;
+    } else {
+      // MD step; rebuild = 0: this step was launched without the list kernels
+      s.active = window_runs(s) ? 1 : 0;
+      s.list_now = (s.istep % W.nbcycle) == 0;
+      if (s.active && s.list_now && !rebuild) s.list_missed = 1;
+      bool e = (W.iene > 0 && s.istep % W.iene == 0 && s.istep > 0) || (W.iout > 0 && s.istep % W.iout == 0) ||
+               force_energy;
+      s.energy_now = e ? 1 : 0;
+    }
+    if (s.energy_now)
+      for (int k = 0; k < E_NSLOT; ++k) s.E[k] = 0;
   }
-  if (s.energy_now)
-    for (int k = 0; k < E_NSLOT; ++k) s.E[k] = 0;
+  if (i < d.ntot && (eval_only || window_runs(d.ws[d.wid[i]]))) {
+    d.f[3 * i] = 0; d.f[3 * i + 1] = 0; d.f[3 * i + 2] = 0;
+  }
 }
 
 // charge-group centres (cgp_centers, md.f90:1000-1019) and cluster bounding
@@ -284,6 +295,8 @@ __global__ void k_bounds(Dev d, int nwc_total, int nsc_total) {
 __device__ __forceinline__ int special_type(const Dev& d, int i, int j) {
   // 1 = excluded pair, 2 = 1-4 pair, 0 = normal (listex/list14 and the long
   // lists of topo.f90, as nbpplist checks them, md.f90:5256-5294)
+  const int2 r = d.sp_rng[i];
+  if (j < r.x || j > r.y) return 0;   // outside the partners' index range
   int s0 = d.sp_start[i], s1 = d.sp_start[i + 1];
   for (int k = s0; k < s1; ++k)
     if (d.sp_j[k] == j) return d.sp_t[k];
@@ -2136,14 +2149,18 @@ __global__ void k_post(Dev d, int shake_any) {
 
 // temperature and Berendsen scaling factors (md.f90:3813-3914): one block per
 // window, fixed-order reduction; advances the window's step counter
-__global__ void __launch_bounds__(256) k_temperature(Dev d, int advance) {
-  const int w = blockIdx.x;
+// Kinetic energy per window in two launches: kTempParts blocks per window
+// sum strided atoms (solute, solute free, solvent, solvent free) and store
+// the block's partials; k_temperature adds the partials in part order, so
+// the sum does not depend on scheduling.
+__global__ void __launch_bounds__(256) k_ekin_part(Dev d, int advance) {
+  const int w = blockIdx.x / kTempParts, part = blockIdx.x % kTempParts;
   if (w >= d.nwin) return;
-  WinState& st = d.ws[w];
+  const WinState& st = d.ws[w];
   const WinDev& W = d.win[w];
   if (!st.active && advance) return;
   double acc[4] = {0, 0, 0, 0};  // solute, solute free, solvent, solvent free
-  for (int li = threadIdx.x; li < W.natom; li += blockDim.x) {
+  for (int li = part * blockDim.x + threadIdx.x; li < W.natom; li += kTempParts * blockDim.x) {
     const int i = W.ao + li;
     const double* v = &d.v[3 * i];
     const double ek = 0.5 * d.mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -2159,10 +2176,25 @@ __global__ void __launch_bounds__(256) k_temperature(Dev d, int advance) {
     if (lane == 0) red[wv][k] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[4] = {0, 0, 0, 0};
-    for (int k = 0; k < 4; ++k)
-      for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t[k] += red[q][k];
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += red[q][threadIdx.x];
+    d.t_part[((size_t)w * kTempParts + part) * 4 + threadIdx.x] = t;
+  }
+}
+
+// temperatures and Berendsen factors from the partial sums (one thread per
+// window); with `advance`, the ring records of the step and the step count
+__global__ void __launch_bounds__(64) k_temperature(Dev d, int advance) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= d.nwin) return;
+  WinState& st = d.ws[w];
+  const WinDev& W = d.win[w];
+  if (!st.active && advance) return;
+  double t[4] = {0, 0, 0, 0};
+  for (int p = 0; p < kTempParts; ++p)
+    for (int k = 0; k < 4; ++k) t[k] += d.t_part[((size_t)w * kTempParts + p) * 4 + k];
+  {
     const double boltz = (double)0.001986f;
     const double tfree_e = t[1] + t[3];
     const double temp_e = t[0] + t[2];
@@ -2198,7 +2230,7 @@ __global__ void __launch_bounds__(256) k_temperature(Dev d, int advance) {
 static inline int blocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 void launch_begin(const Dev& d, hipStream_t s, int force_energy, int eval_only, int rebuild) {
-  k_begin<<<blocks(d.nwin, 64), 64, 0, s>>>(d, force_energy, eval_only, rebuild);
+  k_begin<<<blocks(std::max(d.nwin, d.ntot), 256), 256, 0, s>>>(d, force_energy, eval_only, rebuild);
 }
 
 void launch_lrf_build(const Dev& d, hipStream_t s) {
@@ -2262,21 +2294,17 @@ void launch_integrate(const Dev& d, hipStream_t s, int shake_any) {
   k_leapfrog<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   if (d.nsu) k_shake<<<blocks(d.nsu, 64), 64, 0, s>>>(d);
   k_post<<<blocks(d.ntot, 256), 256, 0, s>>>(d, shake_any);
-  k_temperature<<<d.nwin, 256, 0, s>>>(d, 1);
+  k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d, 1);
+  k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 1);
 }
 
-void launch_temperature(const Dev& d, hipStream_t s) { k_temperature<<<d.nwin, 256, 0, s>>>(d, 0); }
+void launch_temperature(const Dev& d, hipStream_t s) {
+  k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d, 0);
+  k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 0);
+}
 
 void launch_refresh32(const Dev& d, hipStream_t s) { k_post<<<blocks(d.ntot, 256), 256, 0, s>>>(d, -1); }
 
-// gradient accumulators of active windows start each evaluation at zero
-__global__ void k_zero(Dev d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.ntot) return;
-  if (!d.ws[d.wid[i]].active) return;
-  d.f[3 * i] = 0; d.f[3 * i + 1] = 0; d.f[3 * i + 2] = 0;
-}
 
-void launch_zero(const Dev& d, hipStream_t s) { k_zero<<<blocks(d.ntot, 256), 256, 0, s>>>(d); }
 
 }  // namespace qmi
