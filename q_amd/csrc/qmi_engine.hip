@@ -45,6 +45,9 @@ static int set_err(int code, const std::string& m) {
 
 namespace {
 
+// steps per drain of the energy / temperature ring
+constexpr int kChunk = 64;
+
 struct HostWin {
   std::unique_ptr<Window> w;
   std::string inp_path, log_path;
@@ -98,6 +101,12 @@ struct qmi_engine {
   hipGraphExec_t step_exec[2] = {nullptr, nullptr};
   bool graph_failed = false;
   std::vector<int> nbcycle;   // per window, as on the device
+  // ring drain: a chunk's records are copied on copy_stream into pinned
+  // host memory while the next chunk runs on stream
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t chunk_ev[2] = {nullptr, nullptr};
+  long long* h_er = nullptr;
+  double* h_tr = nullptr;
   ~qmi_engine();
 };
 
@@ -129,6 +138,10 @@ qmi_engine::~qmi_engine() {
   }
   for (auto& x : ev) if (x) (void)hipEventDestroy(x);
   for (void* p : bufs) (void)hipFree(p);
+  for (auto& x : chunk_ev) if (x) (void)hipEventDestroy(x);
+  if (h_er) (void)hipHostFree(h_er);
+  if (h_tr) (void)hipHostFree(h_tr);
+  if (copy_stream) (void)hipStreamDestroy(copy_stream);
   if (stream) (void)hipStreamDestroy(stream);
   for (auto& h : wins) {
     if (h.en) fclose(h.en);
@@ -841,9 +854,12 @@ static int build_device(qmi_engine* e) {
   UP(ip, w_off); d.w_off = ip;
   UP(ip, wat_win); d.wat_win = ip;
   d.nwat_total = e->nwat_total;
-  d.ring_n = 64;
+  // two chunks of kChunk steps: one drains while the next runs
+  d.ring_n = 2 * kChunk;
   if ((rc = dev_alloc(e, &d.e_ring, (size_t)d.ring_n * NW * E_NSLOT)) != 0) return rc;
   if ((rc = dev_alloc(e, &d.t_ring, (size_t)d.ring_n * NW * 4)) != 0) return rc;
+  HIP_OK(hipHostMalloc((void**)&e->h_er, (size_t)d.ring_n * NW * E_NSLOT * sizeof(long long), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&e->h_tr, (size_t)d.ring_n * NW * 4 * sizeof(double), hipHostMallocDefault));
   if ((rc = dev_alloc(e, &d.t_part, (size_t)NW * kTempParts * 4)) != 0) return rc;
 
   // --- neighbour rows of the dynamic lists: capacity per row from the largest
@@ -1263,6 +1279,9 @@ int qmi_prepare(qmi_engine* e) {
   if (e->device >= ndev) return set_err(-1, "invalid HIP device");
   HIP_OK(hipSetDevice(e->device));
   if (!e->stream) HIP_OK(hipStreamCreate(&e->stream));
+  if (!e->copy_stream) HIP_OK(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
+  for (auto& x : e->chunk_ev)
+    if (!x) HIP_OK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
   int rc = build_device(e);
   if (rc) return rc;
   if (e->autotune && (rc = autotune(e)) != 0) return rc;
@@ -1284,15 +1303,29 @@ int qmi_prepare(qmi_engine* e) {
   return 0;
 }
 
-static int drain(qmi_engine* e, int first_step, int nsteps_done) {
-  // copy the energy / temperature ring and write .en records and summaries
+// Records of `nsteps_done` steps from `first_step`, once the event `done`
+// (recorded after their launches) has passed: the ring slots are copied on
+// the copy stream, then .en records, summaries and temperature lines are
+// written.  The main stream may run the next chunk meanwhile (it writes the
+// other half of the ring).
+static int drain(qmi_engine* e, int first_step, int nsteps_done, hipEvent_t done) {
   const int NW = (int)e->wins.size();
   const int R = e->d.ring_n;
-  std::vector<long long> er((size_t)R * NW * E_NSLOT);
-  std::vector<double> tr((size_t)R * NW * 4);
-  HIP_OK(hipMemcpyAsync(er.data(), e->d.e_ring, er.size() * 8, hipMemcpyDeviceToHost, e->stream));
-  HIP_OK(hipMemcpyAsync(tr.data(), e->d.t_ring, tr.size() * 8, hipMemcpyDeviceToHost, e->stream));
-  HIP_OK(hipStreamSynchronize(e->stream));
+  HIP_OK(hipStreamWaitEvent(e->copy_stream, done, 0));
+  const int s0 = first_step % R;
+  const int n1 = std::min(nsteps_done, R - s0), n2 = nsteps_done - n1;
+  const size_t er_row = (size_t)NW * E_NSLOT, tr_row = (size_t)NW * 4;
+  HIP_OK(hipMemcpyAsync(e->h_er + s0 * er_row, e->d.e_ring + s0 * er_row, n1 * er_row * 8, hipMemcpyDeviceToHost,
+                        e->copy_stream));
+  HIP_OK(hipMemcpyAsync(e->h_tr + s0 * tr_row, e->d.t_ring + s0 * tr_row, n1 * tr_row * 8, hipMemcpyDeviceToHost,
+                        e->copy_stream));
+  if (n2 > 0) {
+    HIP_OK(hipMemcpyAsync(e->h_er, e->d.e_ring, n2 * er_row * 8, hipMemcpyDeviceToHost, e->copy_stream));
+    HIP_OK(hipMemcpyAsync(e->h_tr, e->d.t_ring, n2 * tr_row * 8, hipMemcpyDeviceToHost, e->copy_stream));
+  }
+  HIP_OK(hipStreamSynchronize(e->copy_stream));
+  const long long* er = e->h_er;
+  const double* tr = e->h_tr;
   for (int wi = 0; wi < NW; ++wi) {
     HostWin& h = e->wins[wi];
     Window& W = *h.w;
@@ -1363,7 +1396,6 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   double fam[5] = {0, 0, 0, 0, 0};
   auto t0 = std::chrono::steady_clock::now();
   long long done = 0;
-  const int R = e->d.ring_n;
   // one step = the same kernels with the same arguments every time (the
   // kernels read per-window step state from device memory), so outside the
   // timing pass it runs as one captured graph
@@ -1413,8 +1445,10 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     }
     return false;
   };
+  // chunk c is drained while chunk c + 1 runs
+  int pend_first = 0, pend_n = 0, cidx = 0;
   while (done < nsteps) {
-    int chunk = (int)std::min<long long>(R, nsteps - done);
+    int chunk = (int)std::min<long long>(kChunk, nsteps - done);
     if (!e->timing_on && e->step_exec[0]) {
       for (int k = 0; k < chunk; ++k) HIP_OK(hipGraphLaunch(e->step_exec[lists_at(done + k) ? 1 : 0], e->stream));
     }
@@ -1443,9 +1477,19 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
       }
     }
     HIP_OK(hipGetLastError());
-    int rc = drain(e, first + (int)done, chunk);
-    if (rc) return rc;
+    HIP_OK(hipEventRecord(e->chunk_ev[cidx & 1], e->stream));
+    if (pend_n > 0) {
+      int rc = drain(e, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
+      if (rc) return rc;
+    }
+    pend_first = first + (int)done;
+    pend_n = chunk;
+    ++cidx;
     done += chunk;
+  }
+  if (pend_n > 0) {
+    int rc = drain(e, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
+    if (rc) return rc;
   }
   HIP_OK(hipStreamSynchronize(e->stream));
   auto t1 = std::chrono::steady_clock::now();

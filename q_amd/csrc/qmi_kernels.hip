@@ -389,7 +389,10 @@ __device__ __forceinline__ int row_chunks(int n) {
 // evaluated from both sides); the 8 lanes of a row are then reduced and
 // added once.
 // water-water: 9 interactions per molecule pair (nonbond_ww_spc, md.f90:11823-12047)
-__global__ void __launch_bounds__(256) k_ww(Dev d) {
+// The energy sums run only on energy steps: EN selects the form of the
+// loop for the wave's window (uniform over the wave).
+template <bool EN>
+__device__ __forceinline__ void ww_impl(const Dev& d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[T_WW];
@@ -412,7 +415,7 @@ __global__ void __launch_bounds__(256) k_ww(Dev d) {
   const float qo = W.crg_ow, qh = W.crg_hw;
   const float qq[3][3] = {{qo * qo, qo * qh, qo * qh}, {qh * qo, qh * qh, qh * qh}, {qh * qo, qh * qh, qh * qh}};
   const float A = W.aO1 * W.aO1, B = W.bO1 * W.bO1;
-  const bool en = st.energy_now;
+  constexpr bool en = EN;
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   double eel = 0.0, evdw = 0.0;
@@ -461,12 +464,19 @@ __global__ void __launch_bounds__(256) k_ww(Dev d) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_ww(Dev d) {
+  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  if (wave >= d.rl[T_WW].nicl) return;
+  if (d.ws[d.wc_win[wave]].energy_now) ww_impl<true>(d);
+  else ww_impl<false>(d);
+}
+
 // solute atom - water: LJ on O, Coulomb on all three sites (nonbond_pw,
 // md.f90:10631-10682).  WATER_SIDE: rows are waters and entries solute atoms
 // (W<-P); otherwise rows are solute atoms and entries water O atoms (P<-W),
 // which also counts the energies.
-template <bool WATER_SIDE>
-__global__ void __launch_bounds__(256) k_pw(Dev d) {
+template <bool WATER_SIDE, bool EN>
+__device__ __forceinline__ void pw_impl(const Dev& d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[WATER_SIDE ? T_WP : T_PW];
@@ -489,7 +499,7 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
     pi[0] = ea >= 0 ? d.x32[ea] : make_float4(0, 0, 0, 0);
     if (ea >= 0) ljs = d.ljp[ea];
   }
-  const bool en = st.energy_now && !WATER_SIDE;
+  constexpr bool en = EN && !WATER_SIDE;
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   double eel = 0.0, evdw = 0.0;
@@ -565,9 +575,19 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
   }
 }
 
+template <bool WATER_SIDE>
+__global__ void __launch_bounds__(256) k_pw(Dev d) {
+  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  if (wave >= d.rl[WATER_SIDE ? T_WP : T_PW].nicl) return;
+  const int w = WATER_SIDE ? d.wc_win[wave] : d.sc_win[wave];
+  if (!WATER_SIDE && d.ws[w].energy_now) pw_impl<WATER_SIDE, true>(d);
+  else pw_impl<WATER_SIDE, false>(d);
+}
+
 // solute-solute (nonbond_pp, md.f90:10387-10491): 1-4 pairs use the third
 // LJ column and the el14 scale
-__global__ void __launch_bounds__(256) k_pp(Dev d) {
+template <bool EN>
+__device__ __forceinline__ void pp_impl(const Dev& d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[T_PP];
@@ -584,7 +604,7 @@ __global__ void __launch_bounds__(256) k_pp(Dev d) {
   const int nch = row_chunks(n);
   const float4 pi = ea >= 0 ? d.x32[ea] : make_float4(0, 0, 0, 0);
   const float4 li = ea >= 0 ? d.ljp[ea] : make_float4(0, 0, 0, 0);
-  const bool en = st.energy_now;
+  constexpr bool en = EN;
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   double eel = 0.0, evdw = 0.0;
   const int* ent = L.ent + (size_t)row * L.cap;
@@ -635,6 +655,13 @@ __global__ void __launch_bounds__(256) k_pp(Dev d) {
     evdw = wave_sum(evdw);
     if (lane == 0) { add_energy(d.ws, w, E_PP_EL, eel); add_energy(d.ws, w, E_PP_VDW, evdw); }
   }
+}
+
+__global__ void __launch_bounds__(256) k_pp(Dev d) {
+  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  if (wave >= d.rl[T_PP].nicl) return;
+  if (d.ws[d.sc_win[wave]].energy_now) pp_impl<true>(d);
+  else pp_impl<false>(d);
 }
 
 // ---- Q-atom tiles: q-atom vs water (nonbond_qw_spc, md.f90:11302-11411) and
