@@ -153,6 +153,10 @@ struct Dev {
   const int* wc_win; const int* sc_win; const int* qc_win;
   double* wc_bs;        // bounding sphere per water cluster (x, y, z, r)
   double* sc_bs;        // bounding sphere of the switch atoms of the members
+  // per cluster member, written at list steps: its switching atom's x, y, z
+  // and w = 0 (listed), 1 (switching atom excluded) or -1 (no member)
+  double4* wc_cm;
+  double4* sc_cm;
   // q tables (per window, offset q_off)
   const int* q_atom;     // global atom
   const float* q_crg;    // [q*S + s]

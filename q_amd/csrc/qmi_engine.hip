@@ -756,6 +756,8 @@ static int build_device(qmi_engine* e) {
   UP(ip, qc_win); d.qc_win = ip;
   if ((rc = dev_alloc(e, &d.wc_bs, 4 * (size_t)e->nwc)) != 0) return rc;
   if ((rc = dev_alloc(e, &d.sc_bs, 4 * (size_t)e->nsc)) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.wc_cm, (size_t)kCl * e->nwc)) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.sc_cm, (size_t)kCl * e->nsc)) != 0) return rc;
   UP(ip, q_atom); d.q_atom = ip;
   float* fp_;
   UP(fp_, q_crg); d.q_crg = fp_;

@@ -265,21 +265,25 @@ __global__ void k_bounds(Dev d, int nwc_total, int nsc_total) {
   int cl = water ? c : c - nwc_total;
   int w = water ? d.wc_win[cl] : d.sc_win[cl];
   if (!d.ws[w].active || !d.ws[w].list_now) return;
-  double p[8][3];
+  // member records (switching-atom position and flag), then the sphere
+  double4* cm = (water ? d.wc_cm : d.sc_cm) + (size_t)cl * kCl;
+  double cx = 0, cy = 0, cz = 0;
   int n = 0;
   for (int k = 0; k < kCl; ++k) {
     int a = water ? d.wc_o[cl * kCl + k] : d.sc_a[cl * kCl + k];
-    if (a < 0) continue;
+    if (a < 0) { cm[k] = make_double4(0, 0, 0, -1.0); continue; }
     int sw = water ? a : d.g_sw[d.grp[a]];
-    p[n][0] = d.x[3 * sw]; p[n][1] = d.x[3 * sw + 1]; p[n][2] = d.x[3 * sw + 2];
+    const double px = d.x[3 * sw], py = d.x[3 * sw + 1], pz = d.x[3 * sw + 2];
+    cm[k] = make_double4(px, py, pz, (d.aflag[sw] & 1) ? 1.0 : 0.0);
+    cx += px; cy += py; cz += pz;
     ++n;
   }
-  double cx = 0, cy = 0, cz = 0;
-  for (int k = 0; k < n; ++k) { cx += p[k][0]; cy += p[k][1]; cz += p[k][2]; }
   if (n > 0) { cx /= n; cy /= n; cz /= n; }
   double r = 0;
-  for (int k = 0; k < n; ++k) {
-    double dx = p[k][0] - cx, dy = p[k][1] - cy, dz = p[k][2] - cz;
+  for (int k = 0; k < kCl; ++k) {
+    const double4 m = cm[k];
+    if (m.w < 0.0) continue;
+    double dx = m.x - cx, dy = m.y - cy, dz = m.z - cz;
     r = fmax(r, sqrt(dx * dx + dy * dy + dz * dz));
   }
   double* bs = water ? &d.wc_bs[4 * cl] : &d.sc_bs[4 * cl];
@@ -331,8 +335,10 @@ __global__ void __launch_bounds__(256) k_build_rows(Dev d) {
   const int a = lane >> 3, b = lane & 7;
   const int row = ic * kCl + a;
   const int ea = iw ? d.wc_o[ic * kCl + a] : d.sc_a[ic * kCl + a];
-  const int swa = ea < 0 ? -1 : (iw ? ea : d.g_sw[d.grp[ea]]);
-  const bool a_ok = ea >= 0 && !(d.aflag[swa] & 1);
+  const double4 mi = (iw ? d.wc_cm : d.sc_cm)[ic * kCl + a];
+  const bool a_ok = mi.w == 0.0;
+  const double4* cmj = jw ? d.wc_cm : d.sc_cm;
+  const int* memj = jw ? d.wc_o : d.sc_a;
   int cnt = 0;  // entries of row a (the same in the 8 lanes of the row)
   for (int base = 0; base < nj && bi[3] >= 0.0; base += 64) {
     bool cand = false;
@@ -343,18 +349,25 @@ __global__ void __launch_bounds__(256) k_build_rows(Dev d) {
       cand = bj[3] >= 0.0 && dc - bi[3] - bj[3] <= rc + 1e-3;
     }
     unsigned long long cb = __ballot(cand);
+    // candidates in j order; the next one's member record is loaded while
+    // the current one is tested
+    int k = cb ? __ffsll((long long)cb) - 1 : 0;
+    double4 mb = make_double4(0, 0, 0, -1.0);
+    int eb = -1;
+    if (cb) { mb = cmj[(size_t)(j0 + base + k) * kCl + b]; eb = memj[(j0 + base + k) * kCl + b]; }
     while (cb) {
-      const int k = __ffsll((long long)cb) - 1;
       cb &= cb - 1ull;
-      const int jc = j0 + base + k;
-      const int eb = jw ? d.wc_o[jc * kCl + b] : d.sc_a[jc * kCl + b];
+      const int kn = cb ? __ffsll((long long)cb) - 1 : k;
+      const double4 mn = cmj[(size_t)(j0 + base + kn) * kCl + b];
+      const int en = memj[(j0 + base + kn) * kCl + b];
       bool keep = false;
       int sp = 0;
-      if (a_ok && eb >= 0 && !(TT == T_WW && ea == eb) && !(TT == T_PP && ea == eb)) {
-        const int swb = jw ? eb : d.g_sw[d.grp[eb]];
-        if (!(d.aflag[swb] & 1)) {
-          if (TT == T_PP) sp = special_type(d, ea, eb);
-          if (sp != 1) keep = r2_exact(d.x, swa, swb) <= rc2;
+      if (a_ok && mb.w == 0.0 && !(TT == T_WW && ea == eb) && !(TT == T_PP && ea == eb)) {
+        if (TT == T_PP) sp = special_type(d, ea, eb);
+        if (sp != 1) {
+#pragma clang fp contract(off)
+          const double dx = __dsub_rn(mi.x, mb.x), dy = __dsub_rn(mi.y, mb.y), dz = __dsub_rn(mi.z, mb.z);
+          keep = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)) <= rc2;
         }
       }
       const unsigned long long kb = __ballot(keep);
@@ -364,6 +377,9 @@ __global__ void __launch_bounds__(256) k_build_rows(Dev d) {
         if (idx < L.cap) L.ent[(size_t)row * L.cap + idx] = (TT == T_PP && sp == 2) ? (eb | kRow14) : eb;
       }
       cnt += __popc(rowbits);
+      k = kn;
+      mb = mn;
+      eb = en;
     }
   }
   if (b == 0) {
