@@ -684,12 +684,11 @@ __global__ void __launch_bounds__(256) k_pp(Dev d) {
 // q-atom vs solute atom (nonbond_qp_qvdw / nonbond_qp, md.f90:11138-11204,
 // 10980-11045), all states; gradients are lambda-weighted over states.
 
-// df32 energies of one q-atom / water pair in state s (energy steps only)
-__device__ void qw_energy_df(const Dev& d, const WinDev& W, int iq, int s, int qa, int ow, double* el,
-                             double* vdw) {
+// df32 energies of one q-atom / water pair in every state (energy steps
+// only); the geometry is shared by the states
+__device__ void qw_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int ow, double* el, double* vdw) {
   const double* x = d.x;
   const int S = W.S;
-  float qc = d.q_crg[W.qs_off + iq * S + s];
   df r2[3];
   for (int k = 0; k < 3; ++k) {
     df dx = df_dx(x[3 * (ow + k)], x[3 * qa]);
@@ -697,58 +696,59 @@ __device__ void qw_energy_df(const Dev& d, const WinDev& W, int iq, int s, int q
     df dz = df_dx(x[3 * (ow + k) + 2], x[3 * qa + 2]);
     r2[k] = df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz));
   }
-  df one = {1.f, 0.f};
-  double e = 0.0;
-  for (int k = 0; k < 3; ++k) {
-    float qq = (k == 0 ? W.crg_ow : W.crg_hw) * qc;   // real(4) product as in the reference
-    df rinv = df_div(one, df_sqrt(r2[k]));
-    e += df_to(df_mulf(rinv, qq));
-  }
-  *el = e;
-  // LJ on the oxygen
-  df r6 = df_mul(df_mul(r2[0], r2[0]), r2[0]);
-  double aLJ, bLJ, sc = 0.0;
+  const df one = {1.f, 0.f};
+  df rinv[3];
+  for (int k = 0; k < 3; ++k) rinv[k] = df_div(one, df_sqrt(r2[k]));
+  const df r6 = df_mul(df_mul(r2[0], r2[0]), r2[0]);
   const double* tl = &d.t_lj[(W.t_off + W.iac_ow) * 4];
-  if (W.qvdw) {
+  for (int s = 0; s < S; ++s) {
+    const float qc = d.q_crg[W.qs_off + iq * S + s];
+    double e = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const float qq = (k == 0 ? W.crg_ow : W.crg_hw) * qc;   // real(4) product as in the reference
+      e += df_to(df_mulf(rinv[k], qq));
+    }
+    el[s] = e;
+    // LJ on the oxygen
     const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
-    aLJ = ql[0] * tl[0];
-    bLJ = ql[1] * tl[1];
-    sc = d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s];
-  } else {
-    const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
-    aLJ = ql[0] * tl[0];
-    bLJ = ql[1] * tl[1];
+    const double aLJ = ql[0] * tl[0], bLJ = ql[1] * tl[1];
+    const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s] : 0.0;
+    const df r6sc = df_add(r6, df_from(sc));
+    const df va = df_div(df_from(aLJ), df_mul(r6sc, r6sc));
+    const df vb = df_div(df_from(bLJ), r6sc);
+    vdw[s] = df_to(df_add(va, df_neg(vb)));
   }
-  df r6sc = df_add(r6, df_from(sc));
-  df va = df_div(df_from(aLJ), df_mul(r6sc, r6sc));
-  df vb = df_div(df_from(bLJ), r6sc);
-  *vdw = df_to(df_add(va, df_neg(vb)));
 }
 
-// df32 energies of one q-atom / solute-atom pair in state s
-__device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int s, int qa, int j, bool is14, double* el,
+// df32 energies of one q-atom / solute-atom pair in every state
+__device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int j, bool is14, double* el,
                              double* vdw) {
   const double* x = d.x;
   const int S = W.S;
-  df dx = df_dx(x[3 * j], x[3 * qa]);
-  df dy = df_dx(x[3 * j + 1], x[3 * qa + 1]);
-  df dz = df_dx(x[3 * j + 2], x[3 * qa + 2]);
-  df r2 = df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz));
-  df one = {1.f, 0.f};
-  float qq = d.q_crg[W.qs_off + iq * S + s] * d.x32[j].w;
-  df vel = df_mulf(df_div(one, df_sqrt(r2)), qq);
-  if (is14) vel = df_mul(vel, df_from((double)W.el14));
-  *el = df_to(vel);
-  const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
-  const double* tl = &d.t_lj[(W.t_off + d.iac[j]) * 4];
-  int k = is14 ? 2 : 0;
-  double aLJ = ql[k] * tl[k], bLJ = ql[k + 1] * tl[k + 1];
-  double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + d.iac[j]) * S + s] : 0.0;
-  df r6 = df_mul(df_mul(r2, r2), r2);
-  df r6sc = df_add(r6, df_from(sc));
-  df va = df_div(df_from(aLJ), df_mul(r6sc, r6sc));
-  df vb = df_div(df_from(bLJ), r6sc);
-  *vdw = df_to(df_add(va, df_neg(vb)));
+  const df dx = df_dx(x[3 * j], x[3 * qa]);
+  const df dy = df_dx(x[3 * j + 1], x[3 * qa + 1]);
+  const df dz = df_dx(x[3 * j + 2], x[3 * qa + 2]);
+  const df r2 = df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz));
+  const df one = {1.f, 0.f};
+  const df rinv = df_div(one, df_sqrt(r2));
+  const df r6 = df_mul(df_mul(r2, r2), r2);
+  const float qj = d.x32[j].w;
+  const int iacj = d.iac[j];
+  const double* tl = &d.t_lj[(W.t_off + iacj) * 4];
+  const int k = is14 ? 2 : 0;
+  for (int s = 0; s < S; ++s) {
+    const float qq = d.q_crg[W.qs_off + iq * S + s] * qj;
+    df vel = df_mulf(rinv, qq);
+    if (is14) vel = df_mul(vel, df_from((double)W.el14));
+    el[s] = df_to(vel);
+    const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
+    const double aLJ = ql[k] * tl[k], bLJ = ql[k + 1] * tl[k + 1];
+    const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + iacj) * S + s] : 0.0;
+    const df r6sc = df_add(r6, df_from(sc));
+    const df va = df_div(df_from(aLJ), df_mul(r6sc, r6sc));
+    const df vb = df_div(df_from(bLJ), r6sc);
+    vdw[s] = df_to(df_add(va, df_neg(vb)));
+  }
 }
 
 // Q-atom tiles: lane (a, b) of a work item takes member a of the i-cluster
@@ -947,11 +947,11 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
     const uint2 m = L.mask[s0 + t];
     if (lane_bit(m, lane)) {
       const int o = d.wc_o[jc * kCl + b];
+      double e_el[kMaxS], e_vdw[kMaxS];
+      qw_energy_df(d, W, iq_l, qa, o, e_el, e_vdw);
       for (int s = 0; s < S; ++s) {
-        double e_el, e_vdw;
-        qw_energy_df(d, W, iq_l, s, qa, o, &e_el, &e_vdw);
-        eel[s] += e_el;
-        evdw[s] += e_vdw;
+        eel[s] += e_el[s];
+        evdw[s] += e_vdw[s];
       }
     }
     if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1)
@@ -1092,11 +1092,11 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
     if (lane_bit(m, lane)) {
       const bool is14 = lane_bit(L.m14[s0 + t], lane);
       const int j = d.sc_a[jc * kCl + b];
+      double e_el[kMaxS], e_vdw[kMaxS];
+      qp_energy_df(d, W, iq_l, qa, j, is14, e_el, e_vdw);
       for (int s = 0; s < S; ++s) {
-        double e_el, e_vdw;
-        qp_energy_df(d, W, iq_l, s, qa, j, is14, &e_el, &e_vdw);
-        eel[s] += e_el;
-        evdw[s] += e_vdw;
+        eel[s] += e_el[s];
+        evdw[s] += e_vdw[s];
       }
     }
     if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1)
