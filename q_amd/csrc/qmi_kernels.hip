@@ -135,8 +135,8 @@ __device__ __forceinline__ double r2_exact(const double* x, int i, int j) {
 
 // ---------------------------------------------------- double-float (df32)
 // Error-free transformations (Dekker/Knuth two-sum, fma two-product) and the
-// double-float add/mul/div/sqrt built on them (Bailey's double-double scheme
-// carried out in FP32): about 44 significant bits.
+// double-float add/mul built on them (Bailey's double-double scheme carried
+// out in FP32), with Newton reciprocals: about 44 significant bits.
 struct df { float h, l; };
 __device__ __forceinline__ df df_q2s(float a, float b) {
 #pragma clang fp contract(off)
@@ -182,24 +182,23 @@ __device__ __forceinline__ df df_mulf(df a, float b) {
   p.l += a.l * b;
   return df_q2s(p.h, p.l);
 }
-__device__ __forceinline__ df df_div(df a, df b) {
+// 1/sqrt(a) and 1/a for a > 0: the hardware FP32 estimate y (relative error
+// about 2^-22) and one Newton step carried in df32, y + y (1 - a y^2) / 2 and
+// y + y (1 - a y); 1 - a y^2 and 1 - a y are near 0, so their leading part
+// is an exact FP32 difference, and the result keeps about 44 bits.
+__device__ __forceinline__ df df_rsqrt(df a) {
 #pragma clang fp contract(off)
-
-  float q1 = a.h / b.h;
-  df r = df_add(a, df_neg(df_mulf(b, q1)));
-  float q2 = r.h / b.h;
-  r = df_add(r, df_neg(df_mulf(b, q2)));
-  float q3 = r.h / b.h;
-  df q = df_q2s(q1, q2);
-  return df_add(q, df{q3, 0.f});
+  const float y = __builtin_amdgcn_rsqf(a.h);
+  const df t = df_mul(a, df_2p(y, y));
+  const float e = (1.0f - t.h) - t.l;
+  return df_q2s(y, y * (0.5f * e));
 }
-__device__ __forceinline__ df df_sqrt(df a) {
+__device__ __forceinline__ df df_rcp(df a) {
 #pragma clang fp contract(off)
-
-  if (a.h <= 0.f) return {0.f, 0.f};
-  float x = sqrtf(a.h);
-  df r = df_add(a, df_neg(df_2p(x, x)));
-  return df_q2s(x, r.h * (0.5f / x));
+  const float y = __builtin_amdgcn_rcpf(a.h);
+  const df t = df_mulf(a, y);
+  const float e = (1.0f - t.h) - t.l;
+  return df_q2s(y, y * e);
 }
 // difference of two FP64 coordinates as a df
 __device__ __forceinline__ df df_dx(double xj, double xi) {
@@ -696,9 +695,8 @@ __device__ void qw_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int 
     df dz = df_dx(x[3 * (ow + k) + 2], x[3 * qa + 2]);
     r2[k] = df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz));
   }
-  const df one = {1.f, 0.f};
   df rinv[3];
-  for (int k = 0; k < 3; ++k) rinv[k] = df_div(one, df_sqrt(r2[k]));
+  for (int k = 0; k < 3; ++k) rinv[k] = df_rsqrt(r2[k]);
   const df r6 = df_mul(df_mul(r2[0], r2[0]), r2[0]);
   const double* tl = &d.t_lj[(W.t_off + W.iac_ow) * 4];
   for (int s = 0; s < S; ++s) {
@@ -713,9 +711,9 @@ __device__ void qw_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int 
     const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
     const double aLJ = ql[0] * tl[0], bLJ = ql[1] * tl[1];
     const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s] : 0.0;
-    const df r6sc = df_add(r6, df_from(sc));
-    const df va = df_div(df_from(aLJ), df_mul(r6sc, r6sc));
-    const df vb = df_div(df_from(bLJ), r6sc);
+    const df inv = df_rcp(df_add(r6, df_from(sc)));
+    const df va = df_mul(df_from(aLJ), df_mul(inv, inv));
+    const df vb = df_mul(df_from(bLJ), inv);
     vdw[s] = df_to(df_add(va, df_neg(vb)));
   }
 }
@@ -729,8 +727,7 @@ __device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int 
   const df dy = df_dx(x[3 * j + 1], x[3 * qa + 1]);
   const df dz = df_dx(x[3 * j + 2], x[3 * qa + 2]);
   const df r2 = df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz));
-  const df one = {1.f, 0.f};
-  const df rinv = df_div(one, df_sqrt(r2));
+  const df rinv = df_rsqrt(r2);
   const df r6 = df_mul(df_mul(r2, r2), r2);
   const float qj = d.x32[j].w;
   const int iacj = d.iac[j];
@@ -744,9 +741,9 @@ __device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int 
     const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
     const double aLJ = ql[k] * tl[k], bLJ = ql[k + 1] * tl[k + 1];
     const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + iacj) * S + s] : 0.0;
-    const df r6sc = df_add(r6, df_from(sc));
-    const df va = df_div(df_from(aLJ), df_mul(r6sc, r6sc));
-    const df vb = df_div(df_from(bLJ), r6sc);
+    const df inv = df_rcp(df_add(r6, df_from(sc)));
+    const df va = df_mul(df_from(aLJ), df_mul(inv, inv));
+    const df vb = df_mul(df_from(bLJ), inv);
     vdw[s] = df_to(df_add(va, df_neg(vb)));
   }
 }
@@ -1162,8 +1159,7 @@ __global__ void __launch_bounds__(128) k_qq(Dev d) {
         df ddx = df_dx(d.x[3 * j], d.x[3 * i]), ddy = df_dx(d.x[3 * j + 1], d.x[3 * i + 1]),
            ddz = df_dx(d.x[3 * j + 2], d.x[3 * i + 2]);
         df rr2 = df_add(df_add(df_mul(ddx, ddx), df_mul(ddy, ddy)), df_mul(ddz, ddz));
-        df one = {1.f, 0.f};
-        df rinv = df_div(one, df_sqrt(rr2));
+        df rinv = df_rsqrt(rr2);
         df qq = df_2p((float)qi, (float)qj);
         df v = df_mul(df_mul(qq, rinv), df_from((double)es));
         if (iLJ == 3) v = df_mul(v, df_from((double)W.el14));
@@ -1171,9 +1167,9 @@ __global__ void __launch_bounds__(128) k_qq(Dev d) {
         if (expo) {
           e_vdw = aLJ * exp(-bLJ * sqrt(df_to(rr2)));
         } else {
-          df r6s = df_add(df_mul(df_mul(rr2, rr2), rr2), df_from(sc));
-          df va = df_div(df_from(aLJ), df_mul(r6s, r6s));
-          df vb = df_div(df_from(bLJ), r6s);
+          df inv = df_rcp(df_add(df_mul(df_mul(rr2, rr2), rr2), df_from(sc)));
+          df va = df_mul(df_from(aLJ), df_mul(inv, inv));
+          df vb = df_mul(df_from(bLJ), inv);
           e_vdw = df_to(df_add(va, df_neg(vb)));
         }
         base = jq ? 4 : 6;  // qq or qp slot
