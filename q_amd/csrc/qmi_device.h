@@ -143,13 +143,13 @@ struct Dev {
   const int* wc_o;      // 8 per water cluster: O atom or -1
   const int* sc_a;      // 8 per solute cluster: atom or -1
   const int* qc_q;      // 8 per q cluster: window-local q index or -1
-  // q-atom share of the Q tiles, written per tile by the non-q side: 8
-  // members x 3 int64 per entry of the P<-Q list, then of the W<-Q list
-  // (from qslot_wq); k_q_gather sums the entries of each q-cluster
+  // q-atom share of the Q tiles, written per tile by the non-q side into
+  // slot ent_slot[entry] (8 members x 3 int64); a q-cluster's slots are
+  // contiguous, qg_start[c] .. qg_start[c+1]-1, so k_q_gather reads runs
   long long* qslot;
-  int qslot_wq, nqc_all;
-  const int* qg_start;  // per q-cluster: first of its entries in qg_ent
-  const int* qg_ent;    // entry slots (P<-Q entries, then qslot_wq + W<-Q entries)
+  int qslot_wq, nqc_all;  // qslot_wq: offset of the W<-Q entries in ent_slot
+  const int* qg_start;
+  const int* ent_slot;    // P<-Q entries, then W<-Q entries
   const int* wc_win; const int* sc_win; const int* qc_win;
   double* wc_bs;        // bounding sphere per water cluster (x, y, z, r)
   double* sc_bs;        // bounding sphere of the switch atoms of the members

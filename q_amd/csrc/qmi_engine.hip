@@ -985,17 +985,18 @@ static int build_device(qmi_engine* e) {
   if ((rc = make_static(T_QW, qt[2])) != 0) return rc;
   if ((rc = make_static(T_WQ, qt[3])) != 0) return rc;
   // q-atom gather: per q-cluster, the P<-Q and W<-Q entries that list it, in
-  // entry order (the sum order of k_q_gather)
+  // entry order, get consecutive slots
   {
     const int nqc = d.tl[T_QW].nicl;
     const int npq = (int)jc_all[T_PQ].size(), nwq = (int)jc_all[T_WQ].size();
     std::vector<std::vector<int>> by(nqc);
     for (int k = 0; k < npq; ++k) by[jc_all[T_PQ][k]].push_back(k);
     for (int k = 0; k < nwq; ++k) by[jc_all[T_WQ][k]].push_back(npq + k);
-    std::vector<int> qg_start(nqc + 1, 0), qg_ent;
+    std::vector<int> qg_start(nqc + 1, 0), ent_slot(npq + nwq, 0);
+    int pos = 0;
     for (int c = 0; c < nqc; ++c) {
-      qg_ent.insert(qg_ent.end(), by[c].begin(), by[c].end());
-      qg_start[c + 1] = (int)qg_ent.size();
+      for (int k : by[c]) ent_slot[k] = pos++;
+      qg_start[c + 1] = pos;
     }
     d.nqc_all = nqc;
     d.qslot_wq = npq;
@@ -1003,8 +1004,8 @@ static int build_device(qmi_engine* e) {
     int* gp = nullptr;
     if ((r = dev_upload(e, &gp, qg_start)) != 0) return r;
     d.qg_start = gp;
-    if ((r = dev_upload(e, &gp, qg_ent)) != 0) return r;
-    d.qg_ent = gp;
+    if ((r = dev_upload(e, &gp, ent_slot)) != 0) return r;
+    d.ent_slot = gp;
     if ((r = dev_alloc(e, &d.qslot, (size_t)std::max(1, npq + nwq) * kCl * 3)) != 0) return r;
   }
   return 0;

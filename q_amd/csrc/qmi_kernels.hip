@@ -774,7 +774,7 @@ __device__ __forceinline__ void q_reaction(const Dev& d, int slot0, int lane, in
       z += __shfl_xor(z, o, 64);
     }
     if (lane < kCl && t0 + u < n) {
-      long long* q = d.qslot + 3 * ((size_t)(slot0 + e0 + u) * kCl + lane);
+      long long* q = d.qslot + 3 * ((size_t)d.ent_slot[slot0 + e0 + u] * kCl + lane);
       q[0] = fix(-(double)x, kFScale);
       q[1] = fix(-(double)y, kFScale);
       q[2] = fix(-(double)z, kFScale);
@@ -783,8 +783,8 @@ __device__ __forceinline__ void q_reaction(const Dev& d, int slot0, int lane, in
 }
 
 // q-atom gradients from the tile slots: one wave per q-cluster, lane (b, k)
-// sums the entries k, k+8, ... of member b; integer sums, so the result does
-// not depend on the order
+// sums slots k, k+8, ... of the cluster's run for member b (coalesced);
+// integer sums, so the result does not depend on the order
 __global__ void __launch_bounds__(256) k_q_gather(Dev d) {
   const int qc = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -794,7 +794,7 @@ __global__ void __launch_bounds__(256) k_q_gather(Dev d) {
   const int b = lane & 7, k = lane >> 3;
   long long gx = 0, gy = 0, gz = 0;
   for (int m = d.qg_start[qc] + k; m < d.qg_start[qc + 1]; m += kCl) {
-    const long long* q = d.qslot + 3 * ((size_t)d.qg_ent[m] * kCl + b);
+    const long long* q = d.qslot + 3 * ((size_t)m * kCl + b);
     gx += q[0]; gy += q[1]; gz += q[2];
   }
   for (int o = 8; o < 64; o <<= 1) {
@@ -1328,8 +1328,7 @@ __global__ void __launch_bounds__(256) k_lrf_reduce(Dev d) {
 }
 
 // Evaluation per atom (lrf_taylor, md.f90:3646-3712), FP64.
-__global__ void __launch_bounds__(256) k_lrf_taylor(Dev d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void lrf_taylor_atom(const Dev& d, int i) {
   bool on = false;
   int w = 0;
   double e = 0.0;
@@ -1677,9 +1676,8 @@ __global__ void __launch_bounds__(256) k_bonded_gather(Dev d) {
 }
 
 // fixed/shell restraints (fix_shell, md.f90:1414-1450) and the radial
-// solvent restraint (restrain_solvent, 14910-14957), one thread per atom
-__global__ void __launch_bounds__(256) k_restr(Dev d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// solvent restraint (restrain_solvent, 14910-14957), per atom
+__device__ __forceinline__ void restr_atom(const Dev& d, int i) {
   int w = 0;
   bool en = false, is_fix = false, is_shell = false, is_rad = false;
   double e_fs = 0.0, e_rad = 0.0;
@@ -1725,6 +1723,15 @@ __global__ void __launch_bounds__(256) k_restr(Dev d) {
   wave_energy(d.ws, w, E_FIX, e_fs, en && is_fix);
   wave_energy(d.ws, w, E_SHELL, e_fs, en && is_shell);
   wave_energy(d.ws, w, E_RAD, e_rad, en && is_rad);
+}
+
+// the per-atom terms of a step in one pass: LRF evaluation, then the
+// fixed, shell and solvent restraints (one thread per atom; every lane
+// reaches the wave energy sums of both)
+__global__ void __launch_bounds__(256) k_atom_terms(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  lrf_taylor_atom(d, i);
+  restr_atom(d, i);
 }
 
 // p_restrain (md.f90:12940-13274): one 64-lane block per restraint record.
@@ -2316,12 +2323,11 @@ void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww) {
 }
 
 void launch_other_forces(const Dev& d, hipStream_t s) {
-  k_lrf_taylor<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+  k_atom_terms<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
   if (nb) k_bonded<<<blocks(nb, d.bs_bonded), d.bs_bonded, 0, s>>>(d);
   // the gather also collects the q-q pair slots written by k_qq
   if ((nb || d.nqq) && !d.bond_atomic && d.ngw) k_bonded_gather<<<d.ngw, 256, 0, s>>>(d);
-  k_restr<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   if (d.nrst) k_prestr<<<d.nrst, 64, 0, s>>>(d);
   if (d.nwat_total) {
     k_watpol_theta<<<blocks(d.nwat_total, 256), 256, 0, s>>>(d);
