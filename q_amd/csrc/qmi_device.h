@@ -143,6 +143,13 @@ struct Dev {
   const int* wc_o;      // 8 per water cluster: O atom or -1
   const int* sc_a;      // 8 per solute cluster: atom or -1
   const int* qc_q;      // 8 per q cluster: window-local q index or -1
+  // per q-cluster member (8 per cluster): its global atom (or -1), and per
+  // state (kMaxS per member) the water-side factors of k_qw: A and B of the
+  // pair with the water oxygen, the charge products with O and with H, and
+  // the soft-core term against the oxygen type
+  const int* qc_atom;
+  const float4* qcw_par;
+  const float* qcw_sc;
   // q-atom share of the Q tiles, written per tile by the non-q side into
   // slot ent_slot[entry] (8 members x 3 int64); a q-cluster's slots are
   // contiguous, qg_start[c] .. qg_start[c+1]-1, so k_q_gather reads runs

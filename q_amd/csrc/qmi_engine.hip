@@ -759,6 +759,33 @@ static int build_device(qmi_engine* e) {
   if ((rc = dev_alloc(e, &d.wc_cm, (size_t)kCl * e->nwc)) != 0) return rc;
   if ((rc = dev_alloc(e, &d.sc_cm, (size_t)kCl * e->nsc)) != 0) return rc;
   UP(ip, q_atom); d.q_atom = ip;
+  {
+    // water-side factors per q-cluster member, as k_qw forms them (FP32
+    // products in the same order)
+    std::vector<int> qc_atom(qc_q.size(), -1);
+    std::vector<float4> qcw_par(qc_q.size() * kMaxS, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<float> qcw_sc(qc_q.size() * kMaxS, 0.f);
+    for (int wi = 0; wi < NW; ++wi) {
+      const WinDev& D = wd[wi];
+      for (int m = D.qc_off * kCl; m < (D.qc_off + D.nqc) * kCl; ++m) {
+        const int iq = qc_q[m];
+        if (iq < 0) continue;
+        qc_atom[m] = q_atom[D.q_off + iq];
+        for (int st = 0; st < D.S; ++st) {
+          const float4 ql = q_lj[D.qs_off + iq * D.S + st];
+          const float qcv = q_crg[D.qs_off + iq * D.S + st];
+          qcw_par[(size_t)m * kMaxS + st] = make_float4(ql.x * D.aO1, ql.y * D.bO1, D.crg_ow * qcv, D.crg_hw * qcv);
+          if (D.qvdw && D.nwat > 0)
+            qcw_sc[(size_t)m * kMaxS + st] = q_sc[D.qsc_off + ((size_t)iq * D.natyps + D.iac_ow) * D.S + st];
+        }
+      }
+    }
+    UP(ip, qc_atom); d.qc_atom = ip;
+    float4* pp;
+    UP(pp, qcw_par); d.qcw_par = pp;
+    float* fp;
+    UP(fp, qcw_sc); d.qcw_sc = fp;
+  }
   float* fp_;
   UP(fp_, q_crg); d.q_crg = fp_;
   float4* f4;

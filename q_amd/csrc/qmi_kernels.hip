@@ -863,24 +863,24 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
       const int tt = s0 + min(t0 + u, n - 1);
       const int jc = L.jc[tt];
       on[u] = t0 + u < n && lane_bit(L.mask[tt], lane);
-      jx[u] = Q_SIDE ? d.wc_o[jc * kCl + b] : d.qc_q[jc * kCl + b];
+      // water O atom (q side), or the q-cluster member index (water side)
+      jx[u] = Q_SIDE ? d.wc_o[jc * kCl + b] : jc * kCl + b;
     }
     float4 pwv[kQGroup][3];
     float4 pqv[kQGroup];
     float rx[kQGroup] = {}, ry[kQGroup] = {}, rz[kQGroup] = {};   // water side: reaction per tile
 #pragma unroll
     for (int u = 0; u < kQGroup; ++u) {
-      const int jj = on[u] ? jx[u] : 0;
       if (Q_SIDE) {
+        const int jj = on[u] ? jx[u] : 0;
         for (int k = 0; k < 3; ++k) pwv[u][k] = d.x32[jj + k];
       } else {
-        pqv[u] = d.x32[d.q_atom[W.q_off + jj]];
+        pqv[u] = d.x32[on[u] ? d.qc_atom[jx[u]] : 0];
       }
     }
 #pragma unroll
     for (int u = 0; u < kQGroup; ++u) {
       if (!on[u]) continue;
-      const int iq = Q_SIDE ? iq_l : jx[u];
       const float4 pq = Q_SIDE ? pq_fixed : pqv[u];
       float dxv[3], dyv[3], dzv[3], r2v[3];
       for (int k = 0; k < 3; ++k) {
@@ -894,21 +894,29 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
       const float r6hc = r2v[0] * r2v[0] * r2v[0];
       float dvO = 0.f, dvH1 = 0.f, dvH2 = 0.f;
       for (int s = 0; s < S; ++s) {
-        const float4 ql = Q_SIDE ? h_ql[s] : d.q_lj[W.qs_off + iq * S + s];
-        const float qc = Q_SIDE ? h_qc[s] : d.q_crg[W.qs_off + iq * S + s];
+        // A, B with the oxygen and the charge products (packed per member
+        // on the water side, formed from the lane's q-atom on the q side)
+        float4 P;
+        float sc;
+        if (Q_SIDE) {
+          P = make_float4(h_ql[s].x * aO, h_ql[s].y * bO, qow * h_qc[s], qhw * h_qc[s]);
+          sc = h_sc[s];
+        } else {
+          P = d.qcw_par[(size_t)jx[u] * kMaxS + s];
+          sc = d.qcw_sc[(size_t)jx[u] * kMaxS + s];
+        }
         float va, vb, ratio = 1.f;
         if (qvdw) {
-          const float sc = Q_SIDE ? h_sc[s] : d.q_sc[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s];
           const float inv = rcp(r6hc + sc);
-          va = ql.x * aO * inv * inv;
-          vb = ql.y * bO * inv;
+          va = P.x * inv * inv;
+          vb = P.y * inv;
           ratio = r6hc * inv;
         } else {
           const float r6 = r2O * r2O * r2O;
-          va = ql.x * aO * r6 * r6;
-          vb = ql.y * bO * r6;
+          va = P.x * r6 * r6;
+          vb = P.y * r6;
         }
-        const float velO = (qow * qc) * rO, velH1 = (qhw * qc) * rH1, velH2 = (qhw * qc) * rH2;
+        const float velO = P.z * rO, velH1 = P.w * rH1, velH2 = P.w * rH2;
         dvO += r2O * (-velO - (12.f * va - 6.f * vb) * ratio) * lam[s];
         dvH1 -= r2H1 * velH1 * lam[s];
         dvH2 -= r2H2 * velH2 * lam[s];
