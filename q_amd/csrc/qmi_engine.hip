@@ -53,6 +53,8 @@ struct HostWin {
   std::string inp_path, log_path;
   FILE* en = nullptr;
   FILE* log = nullptr;
+  FILE* trj = nullptr;                        // DCD trajectory (trj.f90)
+  std::vector<int> trj_atoms;                 // window-local atoms of the mask
   // static lists kept for qmi_get_pairlist
   std::vector<int> qw_waters;                 // O atoms (window-local) of nbqw
   std::vector<std::array<int, 3>> qp_pairs;   // iq (0-based), j (local), ljcod
@@ -146,6 +148,7 @@ qmi_engine::~qmi_engine() {
   for (auto& h : wins) {
     if (h.en) fclose(h.en);
     if (h.log && h.log != stdout) fclose(h.log);
+    if (h.trj) fclose(h.trj);
   }
 }
 
@@ -1300,6 +1303,8 @@ static int autotune(qmi_engine* e) {
   return 0;
 }
 
+static int open_trajectory(HostWin& h);
+
 int qmi_prepare(qmi_engine* e) {
   if (!e) return set_err(-1, "null engine");
   if (e->prepared) return 0;
@@ -1324,12 +1329,68 @@ int qmi_prepare(qmi_engine* e) {
     if (h.log_path == "-") h.log = stdout;
     else if (!h.log_path.empty()) h.log = fopen(h.log_path.c_str(), "w");
     if (h.log) fputs(W.log.c_str(), h.log);
+    if (W.inp.itrj_cycle > 0 && !h.trj && (rc = open_trajectory(h)) != 0) return rc;
   }
   // initial temperature and scaling factors (md_run, md.f90:3969-3976)
   launch_temperature(e->d, e->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(e->stream));
   e->prepared = true;
+  return 0;
+}
+
+// Fortran unformatted sequential record: 4-byte length, payload, length
+static bool write_record(FILE* f, const void* p, size_t n) {
+  const int32_t len = (int32_t)n;
+  return fwrite(&len, 4, 1, f) == 1 && (n == 0 || fwrite(p, 1, n, f) == n) && fwrite(&len, 4, 1, f) == 1;
+}
+
+static std::string fortran_chars(const std::string& t, size_t n) {
+  std::string r = t.substr(0, n);
+  r.resize(n, ' ');
+  return r;
+}
+
+// DCD header (trj_initialize / trj_create, trj.f90:75-100, 180-220): the
+// CORD record, the title record (title, topology, mask rows) and the atom
+// count
+static int open_trajectory(HostWin& h) {
+  Window& W = *h.w;
+  const InputParams& p = W.inp;
+  Error err{0, ""};
+  if (!trj_mask_atoms(W, p.trj_mask, &h.trj_atoms, &err)) return set_err(err.code, err.msg);
+  h.trj = fopen((p.dir + "/" + p.trj_file).c_str(), "wb");
+  if (!h.trj) return set_err(-43, "failure to open trajectory file");
+  int32_t r1[21] = {0};
+  std::memcpy(&r1[0], "CORD", 4);
+  r1[1] = p.nsteps / p.itrj_cycle;   // frames
+  r1[2] = p.itrj_cycle;              // steps before the first frame
+  r1[3] = p.itrj_cycle;              // interval
+  r1[4] = p.nsteps;
+  r1[8] = W.ndegfree;
+  std::string title;
+  const int32_t nrows = 2 + (int32_t)p.trj_mask.size();
+  title.append((const char*)&nrows, 4);
+  title += fortran_chars("Q DCD trajectory version 6.0.1", 80);
+  title += fortran_chars(p.top_file, 80);
+  for (auto& row : p.trj_mask) title += fortran_chars(row, 80);
+  const int32_t nat = (int32_t)h.trj_atoms.size();
+  if (!write_record(h.trj, r1, sizeof(r1)) || !write_record(h.trj, title.data(), title.size()) ||
+      !write_record(h.trj, &nat, 4))
+    return set_err(-43, "failed to write trajectory header");
+  if (h.log) fprintf(h.log, "Coordinates for%6d atoms will be written to the trajectory.\n", nat);
+  return 0;
+}
+
+// one frame (trj_write, trj.f90:228-246): x, y, z records of the masked
+// atoms in real(4)
+static int write_frame(HostWin& h, const double* x) {
+  const size_t n = h.trj_atoms.size();
+  std::vector<float> c(n);
+  for (int k = 0; k < 3; ++k) {
+    for (size_t i = 0; i < n; ++i) c[i] = (float)x[3 * h.trj_atoms[i] + k];
+    if (!write_record(h.trj, c.data(), n * 4)) return set_err(-43, "failure to write to trajectory file");
+  }
   return 0;
 }
 
@@ -1475,10 +1536,38 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     }
     return false;
   };
-  // chunk c is drained while chunk c + 1 runs
+  // trajectory frames (md.f90:4151-4154 in the loop, 4199-4203 after it):
+  // the positions after window step is, for is > 0 a multiple of the
+  // interval, and after the last step when the interval divides nsteps
+  std::vector<long long> aoff(NW + 1, 0);
+  for (int wi = 0; wi < NW; ++wi) aoff[wi + 1] = aoff[wi] + e->wins[wi].w->natom;
+  auto frames_after = [&](int wi, long long k) -> int {
+    const HostWin& h = e->wins[wi];
+    if (!h.trj) return 0;
+    const InputParams& p = h.w->inp;
+    const long long is = (long long)ws[wi].istep + k;
+    if (is >= ws[wi].stop_at) return 0;
+    int nf = 0;
+    if (is > 0 && is % p.itrj_cycle == 0) ++nf;
+    if (is == p.nsteps - 1 && p.nsteps % p.itrj_cycle == 0) ++nf;
+    return nf;
+  };
+  auto any_frame = [&](long long k) {
+    for (int wi = 0; wi < NW; ++wi)
+      if (frames_after(wi, k)) return true;
+    return false;
+  };
+  bool any_trj = false;
+  for (auto& h : e->wins) any_trj = any_trj || h.trj != nullptr;
+  std::vector<double> xh;
+  // chunk c is drained while chunk c + 1 runs; a chunk ends after a step
+  // with trajectory frames, which are written from the positions then
   int pend_first = 0, pend_n = 0, cidx = 0;
   while (done < nsteps) {
     int chunk = (int)std::min<long long>(kChunk, nsteps - done);
+    if (any_trj)
+      for (int k = 0; k < chunk; ++k)
+        if (any_frame(done + k)) { chunk = k + 1; break; }
     if (!e->timing_on && e->step_exec[0]) {
       for (int k = 0; k < chunk; ++k) HIP_OK(hipGraphLaunch(e->step_exec[lists_at(done + k) ? 1 : 0], e->stream));
     }
@@ -1507,6 +1596,16 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
       }
     }
     HIP_OK(hipGetLastError());
+    if (any_trj && any_frame(done + chunk - 1)) {
+      xh.resize((size_t)3 * e->ntot);
+      HIP_OK(hipStreamSynchronize(e->stream));
+      HIP_OK(hipMemcpy(xh.data(), e->d.x, xh.size() * 8, hipMemcpyDeviceToHost));
+      for (int wi = 0; wi < NW; ++wi)
+        for (int f = frames_after(wi, done + chunk - 1); f > 0; --f) {
+          int rc = write_frame(e->wins[wi], &xh[3 * aoff[wi]]);
+          if (rc) return rc;
+        }
+    }
     HIP_OK(hipEventRecord(e->chunk_ev[cidx & 1], e->stream));
     if (pend_n > 0) {
       int rc = drain(e, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);

@@ -282,6 +282,15 @@ bool load_input(const std::string& inp_path, InputParams* p, Error* err) {
   if (!prm_string(fi, "final", &p->xfin_file)) return fail(err, -11, "final coordinate file not specified");
   bool has_trj = prm_string(fi, "trajectory", &p->trj_file);
   if (!has_trj && p->itrj_cycle > 0) return fail(err, -11, "Trajectory file name required to write trajectory");
+  if (has_trj && p->itrj_cycle < 0) return fail(err, -11, "Trajectory file given but no output interval");
+  // trajectory atom mask (md.f90:2972-2986): 'all' when the section is absent
+  p->trj_mask.clear();
+  if (p->itrj_cycle > 0) {
+    if (const PrmSection* ta = f.find("TRAJECTORY_ATOMS"))
+      for (auto& ln : ta->lines) p->trj_mask.push_back(ln);
+    if (p->trj_mask.empty()) p->trj_mask.push_back("all");
+    if (p->trj_mask.size() > 20) return fail(err, -11, "Too many trajectory atom mask rows (max 20)");
+  }
   bool has_en = prm_string(fi, "energy", &p->ene_file);
   if (!has_en && p->iene_cycle > 0) return fail(err, -11, "Energy file name required to write energies");
   if (has_en && p->iene_cycle < 0) return fail(err, -11, "Energy file given but no energy interval");

@@ -4,6 +4,8 @@
 //   get_fep (1472-1652), prep_sim (13787-13892), init_shake (2326-2503),
 //   make_nbqqlist (1022-1040, 7312-7530), shrink_topology (14569-14665),
 //   maxwell / initial_shaking / stop_cm_translation (3787, 2506, 14670).
+#include <cctype>
+#include <sstream>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -104,8 +106,7 @@ bool prepare_window(Window* w, long long seed_override, Error* err) {
     w->ljcod[(pr.second - 1) * t.natyps + (pr.first - 1)] = 2;
   }
 
-  if (p.itrj_cycle > 0)
-    w->log += "NOTE: trajectory output (" + p.trj_file + ") is not written by this engine.\n";
+
 
   // --- prep_coord(): residue:atom restraint atoms (md.f90:13609-13623,
   // get_atom_from_resnum_atnum 16059-16095; the last residue ends at the
@@ -484,6 +485,79 @@ bool prepare_window(Window* w, long long seed_override, Error* err) {
     for (int i = 0; i < N; ++i)
       for (int k = 0; k < 3; ++k) w->v[3 * i + k] -= vcm[k];
   }
+  return true;
+}
+
+}  // namespace qmi
+
+namespace qmi {
+
+// Trajectory atom mask (mask_add / update, mask.f90:118-353): each row adds
+// the atoms of a range (atom numbers, or residue numbers after RESIDUE;
+// default all atoms) that match the requested properties; a property is
+// required (SOLUTE), required false (NOT SOLUTE) or free.  EXCLUDED is the
+// excluded-atom flag, RESTRAINED the shell or excluded flag, HEAVY a type
+// mass of 4 or more.  Words after a range are ignored, as in the reference.
+bool trj_mask_atoms(const Window& w, const std::vector<std::string>& rows, std::vector<int>* atoms, Error* err) {
+  const Topology& t = w.top;
+  const int n = t.nat_pro, nres = (int)t.res_start.size();
+  std::vector<uint8_t> in(n, 0);
+  for (const std::string& row : rows) {
+    int req[4] = {0, 0, 0, 0};  // solute, excluded, restrained, heavy: 0 free, 1 true, -1 false
+    bool notflag = false, resnum = false, empty = true;
+    int first = 1, last = n;
+    std::istringstream ss(row);
+    std::string word;
+    while (ss >> word) {
+      for (auto& c : word) c = (char)std::toupper((unsigned char)c);
+      auto prop = [&](int k) { req[k] = notflag ? -1 : 1; empty = false; notflag = false; };
+      if (word == "ALL") { notflag = false; empty = false; }
+      else if (word == "NOT" || word == "-") notflag = true;
+      else if (word == "RESIDUE" || word == "RES" || word == "RESI") { resnum = true; notflag = false; }
+      else if (word == "SOLUTE") prop(0);
+      else if (word == "EXCLUDED" || word == "EXCL") prop(1);
+      else if (word == "RESTRAINED") prop(2);
+      else if (word == "HEAVY") prop(3);
+      else if (word == "SYBYL") {
+        err->code = -11; err->msg = "trajectory mask: SYBYL selections are not supported"; return false;
+      } else {
+        double v;
+        if (!parse_num(word, &v) || v != (double)(int)v) {
+          err->code = -11; err->msg = "trajectory mask: unrecognized property " + word; return false;
+        }
+        first = (int)v;
+        if (first < 1 || (resnum ? first > nres : first > n)) {
+          err->code = -11; err->msg = "trajectory mask: invalid first atom or residue " + word; return false;
+        }
+        last = first;
+        if (ss >> word) {
+          if (!parse_num(word, &v) || v != (double)(int)v || (int)v < first || (resnum ? (int)v > nres : (int)v > n)) {
+            err->code = -11; err->msg = "trajectory mask: invalid last atom or residue " + word; return false;
+          }
+          last = (int)v;
+        }
+        empty = false;
+        break;
+      }
+    }
+    if (empty) continue;
+    int a0 = first - 1, a1 = last - 1;   // 0-based, inclusive
+    if (resnum) {
+      a0 = t.res_start[first - 1];
+      a1 = last == nres ? n - 1 : t.res_start[last] - 1;
+    }
+    for (int i = a0; i <= a1; ++i) {
+      const bool prop[4] = {i < t.nat_solute, t.excl[i] != 0, (w.shell[i] != 0) || (t.excl[i] != 0),
+                            t.iaclib[t.iac[i]].mass >= 4.0};
+      bool ok = true;
+      for (int k = 0; k < 4; ++k)
+        if ((req[k] == 1 && !prop[k]) || (req[k] == -1 && prop[k])) ok = false;
+      if (ok) in[i] = 1;
+    }
+  }
+  atoms->clear();
+  for (int i = 0; i < n; ++i)
+    if (in[i]) atoms->push_back(i);
   return true;
 }
 

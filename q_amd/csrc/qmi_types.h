@@ -142,6 +142,7 @@ struct InputParams {
   bool wpol_restr = true, wpol_born = true;
   int nbcycle = 10, iout_cycle = 10, itemp_cycle = 10, iene_cycle = 0, itrj_cycle = 0;
   std::string top_file, restart_file, xfin_file, trj_file, ene_file, fep_file;
+  std::vector<std::string> trj_mask;  // [trajectory_atoms] rows, as written
   bool restart = false;
   int nstates = 0;
   double lambda[kMaxStates] = {0};
@@ -200,6 +201,8 @@ bool load_fep(const std::string& path, const Topology& t, const InputParams& p, 
 bool read_restart(const std::string& path, Window* w, Error* err);
 bool write_restart(const std::string& path, const Window& w, Error* err);
 bool prepare_window(Window* w, long long seed_override, Error* err);
+// atoms (0-based) selected by the trajectory mask rows (mask_add, mask.f90)
+bool trj_mask_atoms(const Window& w, const std::vector<std::string>& rows, std::vector<int>* atoms, Error* err);
 
 // Fortran randm (md.f90:14408-14442): LCG mod 1e8, single-precision conversion.
 struct Randm {

@@ -931,7 +931,8 @@ def write_inp(path: str, top: str, fep: str | None, final: str, *, steps: int, s
               shake_solvent: bool = True, shake_hydrogens: bool = False, nb_cycle: int = 25,
               output: int = 5, energy: int = 10, energy_file: str | None = None,
               restart: str | None = None, bath_coupling: float = 10.0,
-              separate_scaling: bool = True, polarization: bool = True) -> None:
+              separate_scaling: bool = True, polarization: bool = True, trajectory: int = 0,
+              trajectory_file: str | None = None, trajectory_atoms: list | None = None) -> None:
     onoff = lambda v: "on" if v else "off"  # noqa: E731
     out = ["[MD]", f"steps {steps}", f"stepsize {stepsize}", f"temperature {temperature}",
            f"bath_coupling {bath_coupling}"]
@@ -949,15 +950,21 @@ def write_inp(path: str, top: str, fep: str | None, final: str, *, steps: int, s
             "[intervals]", f"output {output}", f"non_bond {nb_cycle}"]
     if energy_file:
         out.append(f"energy {energy}")
+    if trajectory_file:
+        out.append(f"trajectory {trajectory}")
     out += ["", "[files]", f"topology {top}", f"final {final}"]
     if energy_file:
         out.append(f"energy {energy_file}")
+    if trajectory_file:
+        out.append(f"trajectory {trajectory_file}")
     if restart:
         out.append(f"restart {restart}")
     if fep:
         out.append(f"fep {fep}")
     if lambdas:
         out += ["", "[lambdas]", " ".join(f"{v:.6f}" for v in lambdas)]
+    if trajectory_file and trajectory_atoms:
+        out += ["", "[trajectory_atoms]"] + list(trajectory_atoms)
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
 
