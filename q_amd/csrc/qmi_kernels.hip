@@ -2053,14 +2053,8 @@ __global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
 
 // ------------------------------------------------------- integrator --
 // leapfrog (md.f90:4081-4111): v = (v - d*winv*dt)*Tscale; xx = x; x += v*dt
-__global__ void k_leapfrog(Dev d) {
+__device__ __forceinline__ void leapfrog_atom(const Dev& d, const WinState& st, const WinDev& W, int i) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.ntot) return;
-  const int w = d.wid[i];
-  const WinState& st = d.ws[w];
-  if (!st.active) return;
-  const WinDev& W = d.win[w];
   const double ts = (i - W.ao) < W.nat_solute ? st.tscale_solute : st.tscale_solvent;
   const double wi = (double)d.winv[i];
   for (int c = 0; c < 3; ++c) {
@@ -2071,6 +2065,15 @@ __global__ void k_leapfrog(Dev d) {
     d.xx[k] = d.x[k];
     d.x[k] = d.x[k] + v * W.dt;
   }
+}
+
+__global__ void k_leapfrog(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.ntot) return;
+  const int w = d.wid[i];
+  const WinState& st = d.ws[w];
+  if (!st.active) return;
+  leapfrog_atom(d, st, d.win[w], i);
 }
 
 // SHAKE (md.f90:14445-14566): one thread per connected component of a
@@ -2207,7 +2210,10 @@ __global__ void k_post(Dev d, int shake_any) {
 // sum strided atoms (solute, solute free, solvent, solvent free) and store
 // the block's partials; k_temperature adds the partials in part order, so
 // the sum does not depend on scheduling.
-__global__ void __launch_bounds__(256) k_ekin_part(Dev d, int advance) {
+// With post >= 0 (at the end of a step) each atom is first finished as
+// k_post does it: post = 1, v = (x - xx)/dt after SHAKE; then the FP32
+// coordinates.
+__global__ void __launch_bounds__(256) k_ekin_part(Dev d, int advance, int post) {
   const int w = blockIdx.x / kTempParts, part = blockIdx.x % kTempParts;
   if (w >= d.nwin) return;
   const WinState& st = d.ws[w];
@@ -2216,6 +2222,12 @@ __global__ void __launch_bounds__(256) k_ekin_part(Dev d, int advance) {
   double acc[4] = {0, 0, 0, 0};  // solute, solute free, solvent, solvent free
   for (int li = part * blockDim.x + threadIdx.x; li < W.natom; li += kTempParts * blockDim.x) {
     const int i = W.ao + li;
+    if (post >= 0) {
+      if (post)
+        for (int c = 0; c < 3; ++c) d.v[3 * i + c] = (d.x[3 * i + c] - d.xx[3 * i + c]) / W.dt;
+      const float4 q = d.x32[i];
+      d.x32[i] = make_float4((float)d.x[3 * i], (float)d.x[3 * i + 1], (float)d.x[3 * i + 2], q.w);
+    }
     const double* v = &d.v[3 * i];
     const double ek = 0.5 * d.mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     const bool solute = li < W.nat_solute;
@@ -2346,13 +2358,12 @@ void launch_other_forces(const Dev& d, hipStream_t s) {
 void launch_integrate(const Dev& d, hipStream_t s, int shake_any) {
   k_leapfrog<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   if (d.nsu) k_shake<<<blocks(d.nsu, 64), 64, 0, s>>>(d);
-  k_post<<<blocks(d.ntot, 256), 256, 0, s>>>(d, shake_any);
-  k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d, 1);
+  k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d, 1, shake_any ? 1 : 0);
   k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 1);
 }
 
 void launch_temperature(const Dev& d, hipStream_t s) {
-  k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d, 0);
+  k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d, 0, -1);
   k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 0);
 }
 
