@@ -782,18 +782,21 @@ __device__ __forceinline__ void q_reaction(const Dev& d, int slot0, int lane, in
   }
 }
 
-// q-atom gradients from the tile slots: one wave per q-cluster, lane (b, k)
-// sums slots k, k+8, ... of the cluster's run for member b (coalesced);
-// integer sums, so the result does not depend on the order
+// q-atom gradients from the tile slots: kQGatherParts waves per q-cluster;
+// lane (b, k) of part p sums slots p*8+k, p*8+k + 8*kQGatherParts, ... of
+// the cluster's run for member b (coalesced) and the parts add with
+// atomics; integer sums, so the result does not depend on the order
+constexpr int kQGatherParts = 8;
 __global__ void __launch_bounds__(256) k_q_gather(Dev d) {
-  const int qc = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int qc = wv / kQGatherParts, part = wv % kQGatherParts;
   const int lane = threadIdx.x & 63;
   if (qc >= d.nqc_all) return;
   const int w = d.qc_win[qc];
   if (!d.ws[w].active) return;
   const int b = lane & 7, k = lane >> 3;
   long long gx = 0, gy = 0, gz = 0;
-  for (int m = d.qg_start[qc] + k; m < d.qg_start[qc + 1]; m += kCl) {
+  for (int m = d.qg_start[qc] + part * kCl + k; m < d.qg_start[qc + 1]; m += kCl * kQGatherParts) {
     const long long* q = d.qslot + 3 * ((size_t)m * kCl + b);
     gx += q[0]; gy += q[1]; gz += q[2];
   }
@@ -2326,7 +2329,7 @@ void launch_q_tiles(const Dev& d, hipStream_t s) {
   if (d.tl[T_WQ].nwork) k_qw<false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PQ].nwork) k_qp<false><<<blocks((long long)d.tl[T_PQ].nwork * 64, 256), 256, 0, s>>>(d);
-  if (d.nqc_all) k_q_gather<<<blocks((long long)d.nqc_all * 64, 256), 256, 0, s>>>(d);
+  if (d.nqc_all) k_q_gather<<<blocks((long long)d.nqc_all * kQGatherParts * 64, 256), 256, 0, s>>>(d);
 }
 
 void launch_ww(const Dev& d, hipStream_t s) {
