@@ -58,7 +58,7 @@ struct WinState {
   double ekin_parts[4];  // solute all, solute free, solvent all, solvent free
   int shake_fail;
   int list_overflow;  // a neighbour row exceeded its capacity
-  int list_missed;    // list_now was set in a step launched without the list kernels
+  int list_missed;    // 1 (2): a step needing pair lists (energies) was launched without those kernels
   long long E[E_NSLOT];
 };
 

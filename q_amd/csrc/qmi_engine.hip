@@ -96,11 +96,11 @@ struct qmi_engine {
   // gradients, LRF source parts
   int tuned[7] = {2, 256, 256, 256, 0, 0, 1};
   double tune_us[6] = {0, 0, 0, 0, 0, 0};   // time of the chosen configuration per group
-  // the launch sequence of one MD step, captured once and replayed: [1]
-  // with the pair-list and LRF kernels, [0] without them for steps where no
-  // window updates its lists
-  hipGraph_t step_graph[2] = {nullptr, nullptr};
-  hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+  // the launch sequence of one MD step, captured once and replayed: index
+  // bit 0 with the pair-list and LRF kernels (some window updates its lists),
+  // bit 1 with the q-side energy passes (some window computes energies)
+  hipGraph_t step_graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraphExec_t step_exec[4] = {nullptr, nullptr, nullptr, nullptr};
   bool graph_failed = false;
   std::vector<int> nbcycle;   // per window, as on the device
   // ring drain: a chunk's records are copied on copy_stream into pinned
@@ -134,7 +134,7 @@ static int dev_alloc(qmi_engine* e, T** dst, size_t n) {
 }
 
 qmi_engine::~qmi_engine() {
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 4; ++k) {
     if (step_exec[k]) (void)hipGraphExecDestroy(step_exec[k]);
     if (step_graph[k]) (void)hipGraphDestroy(step_graph[k]);
   }
@@ -1000,8 +1000,12 @@ static int build_device(qmi_engine* e) {
       e->qwork[tt][ci] = wp;
       e->nqwork[tt][ci] = (int)work.size();
     }
-    L.work = e->qwork[tt][2];
-    L.nwork = e->nqwork[tt][2];
+    // the q-side lists only carry the per-state energy pass (few rows, long
+    // lists): the smallest work items give it the most waves; the non-q
+    // side starts at 16 tiles until the autotuner picks
+    const int ci0 = (tt == T_QP || tt == T_QW) ? 0 : 2;
+    L.work = e->qwork[tt][ci0];
+    L.nwork = e->nqwork[tt][ci0];
     if ((r = dev_upload(e, &L.start, start)) != 0) return r;
     if ((r = dev_upload(e, &L.count, count)) != 0) return r;
     if ((r = dev_upload(e, &L.jc, jc)) != 0) return r;
@@ -1225,11 +1229,11 @@ static int autotune(qmi_engine* e) {
     int bi = 2;
     double bt = 1e30;
     for (int ci = 0; ci < kQChunkN; ++ci) {
-      for (int tt : {T_QP, T_PQ, T_QW, T_WQ}) { d.tl[tt].work = e->qwork[tt][ci]; d.tl[tt].nwork = e->nqwork[tt][ci]; }
+      for (int tt : {T_PQ, T_WQ}) { d.tl[tt].work = e->qwork[tt][ci]; d.tl[tt].nwork = e->nqwork[tt][ci]; }
       const double t = time_us([&] { launch_q_tiles(d, e->stream); });
       if (t < bt) { bt = t; bi = ci; }
     }
-    for (int tt : {T_QP, T_PQ, T_QW, T_WQ}) { d.tl[tt].work = e->qwork[tt][bi]; d.tl[tt].nwork = e->nqwork[tt][bi]; }
+    for (int tt : {T_PQ, T_WQ}) { d.tl[tt].work = e->qwork[tt][bi]; d.tl[tt].nwork = e->nqwork[tt][bi]; }
     e->tuned[0] = bi;
     e->tune_us[0] = bt;
   }
@@ -1490,21 +1494,22 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   // one step = the same kernels with the same arguments every time (the
   // kernels read per-window step state from device memory), so outside the
   // timing pass it runs as one captured graph
-  auto launch_step = [&](bool lists) {
-    launch_begin(e->d, e->stream, 0, 0, lists ? 1 : 0);
+  auto launch_step = [&](int variant) {
+    const bool lists = variant & 1, qen = variant & 2;
+    launch_begin(e->d, e->stream, 0, 0, variant);
     if (lists) launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
     launch_ww(e->d, e->stream);
-    launch_nonbonded(e->d, e->stream, false);
+    launch_nonbonded(e->d, e->stream, false, qen);
     launch_other_forces(e->d, e->stream);
     launch_integrate(e->d, e->stream, e->shake_any);
   };
   if (!e->timing_on && !e->step_exec[0] && !e->graph_failed) {
     bool ok = true;
-    for (int k = 0; k < 2 && ok; ++k) {
+    for (int k = 0; k < 4 && ok; ++k) {
       hipGraph_t g = nullptr;
       ok = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
       if (ok) {
-        launch_step(k == 1);
+        launch_step(k);
         ok = hipStreamEndCapture(e->stream, &g) == hipSuccess && g != nullptr;
       }
       if (ok) ok = hipGraphInstantiate(&e->step_exec[k], g, nullptr, nullptr, 0) == hipSuccess;
@@ -1516,7 +1521,7 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     }
     if (!ok) {
       // run with direct launches instead
-      for (int k = 0; k < 2; ++k) {
+      for (int k = 0; k < 4; ++k) {
         if (e->step_exec[k]) (void)hipGraphExecDestroy(e->step_exec[k]);
         if (e->step_graph[k]) (void)hipGraphDestroy(e->step_graph[k]);
         e->step_exec[k] = nullptr;
@@ -1533,6 +1538,17 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     for (int wi = 0; wi < NW; ++wi) {
       const long long is = (long long)ws[wi].istep + k;
       if (is < ws[wi].stop_at && is % e->nbcycle[wi] == 0) return true;
+    }
+    return false;
+  };
+  // whether any window computes energies at step k (k_begin's condition)
+  auto energy_at = [&](long long k) {
+    for (int wi = 0; wi < NW; ++wi) {
+      const long long is = (long long)ws[wi].istep + k;
+      const InputParams& p = e->wins[wi].w->inp;
+      if (is >= ws[wi].stop_at) continue;
+      if ((p.iene_cycle > 0 && is % p.iene_cycle == 0 && is > 0) || (p.iout_cycle > 0 && is % p.iout_cycle == 0))
+        return true;
     }
     return false;
   };
@@ -1569,10 +1585,11 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
       for (int k = 0; k < chunk; ++k)
         if (any_frame(done + k)) { chunk = k + 1; break; }
     if (!e->timing_on && e->step_exec[0]) {
-      for (int k = 0; k < chunk; ++k) HIP_OK(hipGraphLaunch(e->step_exec[lists_at(done + k) ? 1 : 0], e->stream));
+      for (int k = 0; k < chunk; ++k)
+        HIP_OK(hipGraphLaunch(e->step_exec[(lists_at(done + k) ? 1 : 0) | (energy_at(done + k) ? 2 : 0)], e->stream));
     }
     for (int k = 0; k < chunk && (e->timing_on || !e->step_exec[0]); ++k) {
-      launch_begin(e->d, e->stream, 0, 0, 1);
+      launch_begin(e->d, e->stream, 0, 0, 3);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[0], e->stream));
       launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[1], e->stream));
@@ -1632,7 +1649,8 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   std::string failed_shake, failed_list;
   for (int wi = 0; wi < NW; ++wi) {
     if (ws[wi].list_missed)
-      return set_err(-62, "internal error: a pair-list update was skipped in window " + std::to_string(wi));
+      return set_err(-62, std::string("internal error: a step's ") + (ws[wi].list_missed == 1 ? "pair-list update" :
+                     "energy evaluation") + " was skipped in window " + std::to_string(wi));
     if (ws[wi].shake_fail) {
       e->wins[wi].status = 1;
       failed_shake += (failed_shake.empty() ? "" : " ") + std::to_string(wi);

@@ -10,10 +10,10 @@ void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc);
 // water-water tiles alone, and all nonbonded tile kernels (with or without them)
 void launch_ww(const Dev& d, hipStream_t s);
 // single families, for the autotuner's timing runs
-void launch_q_tiles(const Dev& d, hipStream_t s);
+void launch_q_tiles(const Dev& d, hipStream_t s, bool q_energy = true);
 void launch_lrf_build(const Dev& d, hipStream_t s);
 void launch_bonded(const Dev& d, hipStream_t s);
-void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww = true);
+void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww = true, bool q_energy = true);
 void launch_other_forces(const Dev& d, hipStream_t s);
 void launch_integrate(const Dev& d, hipStream_t s, int shake_any);
 void launch_temperature(const Dev& d, hipStream_t s);

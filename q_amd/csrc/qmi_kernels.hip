@@ -224,13 +224,15 @@ __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
       s.list_now = rebuild;
       s.energy_now = eval_only == 1 ? 1 : 0;
     } else {
-      // MD step; rebuild = 0: this step was launched without the list kernels
+      // MD step; rebuild bit 0: the list kernels were launched, bit 1: the
+      // q-side energy passes were
       s.active = window_runs(s) ? 1 : 0;
       s.list_now = (s.istep % W.nbcycle) == 0;
-      if (s.active && s.list_now && !rebuild) s.list_missed = 1;
+      if (s.active && s.list_now && !(rebuild & 1)) s.list_missed = 1;
       bool e = (W.iene > 0 && s.istep % W.iene == 0 && s.istep > 0) || (W.iout > 0 && s.istep % W.iout == 0) ||
                force_energy;
       s.energy_now = e ? 1 : 0;
+      if (s.active && e && !(rebuild & 2)) s.list_missed = 2;
     }
     if (s.energy_now)
       for (int k = 0; k < E_NSLOT; ++k) s.E[k] = 0;
@@ -2324,10 +2326,12 @@ void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
   launch_lrf_build(d, s);
 }
 
-void launch_q_tiles(const Dev& d, hipStream_t s) {
-  if (d.tl[T_QW].nwork) k_qw<true><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
+// q_energy false: the q-side energy passes are left out (a step where no
+// window computes energies)
+void launch_q_tiles(const Dev& d, hipStream_t s, bool q_energy) {
+  if (q_energy && d.tl[T_QW].nwork) k_qw<true><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_WQ].nwork) k_qw<false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
+  if (q_energy && d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PQ].nwork) k_qp<false><<<blocks((long long)d.tl[T_PQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.nqc_all) k_q_gather<<<blocks((long long)d.nqc_all * kQGatherParts * 64, 256), 256, 0, s>>>(d);
 }
@@ -2336,12 +2340,12 @@ void launch_ww(const Dev& d, hipStream_t s) {
   if (d.rl[T_WW].nicl) k_ww<<<blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
 }
 
-void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww) {
+void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) {
   if (with_ww) launch_ww(d, s);
   if (d.rl[T_WP].nicl) k_pw<true><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
   if (d.rl[T_PW].nicl) k_pw<false><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
   if (d.rl[T_PP].nicl) k_pp<<<blocks((long long)d.rl[T_PP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
-  launch_q_tiles(d, s);
+  launch_q_tiles(d, s, q_energy);
   if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
 }
 
