@@ -3,7 +3,8 @@
 Only tests, the smoke check and the bench's CPU leg may use this module, and
 only as a checker: the product path never imports it.  It re-derives, with
 its own readers, what the reference computes at one step of md_run for a
-spherical-boundary, geometric-rule, SPC-water window:
+spherical-boundary, SPC-water window with the geometric or the arithmetic
+LJ combination rule:
 
   reading     initialize (src/q6/md.f90:2528-3262), topo_read (topo.f90:529-1082),
               qatom_load_atoms / qatom_load_fep (qatom.f90:253-548, 874-1745)
@@ -14,7 +15,9 @@ spherical-boundary, geometric-rule, SPC-water window:
               nbwwlist_lrf (8370-8556), nbqplist (7875-7964), nbqwlist (8066-8118)
   forces      nonbond_pp (10387-10491), nonbond_pw (10631-10682), nonbond_ww_spc
               (11823-12047), nonbond_qp_qvdw / nonbond_qp (11138-11204, 10980-11045),
-              nonbond_qw_spc (11302-11411), nonbond_qq (10763-10868), lrf_taylor
+              nonbond_qw_spc (11302-11411), nonbond_qq (10763-10868); arithmetic
+              rule: nonbon2_pp/pw/qp/qw/qq/ww (8972-9089, 9242-9297, 9603-9689,
+              9798-9936, 9385-9491, 10099-10227); lrf_taylor
               (3646-3712), bond/angle/torsion/improper2 (954-997, 814-901,
               14793-14907, 1789-1897), qbond/qangle/qtorsion/qimproper
               (14060-14114, 13896-13988, 14262-14403, 14117-14258), fix_shell
@@ -233,6 +236,11 @@ def read_top(path):
         A[:, j] = L.values(nty)
         L.line()
         B[:, j] = L.values(nty)
+    if t["vdw_rule"] not in (1, 2):
+        raise ValueError("unknown vdW combination rule")
+    if t["vdw_rule"] == 2:
+        # epsilons enter as their square roots (topology, md.f90:14747-14752)
+        B = np.sqrt(np.abs(B))
     t["avdw"], t["bvdw"] = A, B
     nlj2 = int(L.values(1)[0])
     if nlj2:
@@ -282,7 +290,21 @@ def read_top(path):
     return t
 
 
-def read_fep(path, nq_states, nat_solute, natyps):
+def lj_ab(ai, bi, aj, bj, rule):
+    """A, B of a pair's V = A/r^12 - B/r^6 from the two atoms' LJ columns:
+    products under the geometric rule (nonbond_pp, md.f90:10432-10439);
+    under the arithmetic rule R* = a_i + a_j and eps = b_i b_j (b being the
+    square root of epsilon), A = eps R*^12, B = 2 eps R*^6 (nonbon2_pp,
+    md.f90:8993-9000, 9056-9058)."""
+    if rule == 1:
+        return ai * aj, bi * bj
+    s = ai + aj
+    s6 = (s * s) ** 3
+    e = bi * bj
+    return e * s6 * s6, 2.0 * e * s6
+
+
+def read_fep(path, nq_states, nat_solute, natyps, vdw_rule=1):
     s = read_prm(path)
     S = nq_states
     f = dict(S=S)
@@ -309,6 +331,11 @@ def read_fep(path, nq_states, nat_solute, natyps):
     fep = s.get("FEP", [])
     if _onoff(fep, "softcore_use_max_potential", False):
         raise ValueError("softcore_use_max_potential is not restated")
+    if vdw_rule == 2 and f["qvdw"]:
+        # square roots of the q-type epsilons, normal and 1-4 columns only
+        # (get_fep, md.f90:1518-1523)
+        f["qbvdw"][:, 0] = np.sqrt(f["qbvdw"][:, 0])
+        f["qbvdw"][:, 2] = np.sqrt(f["qbvdw"][:, 2])
     for sec in ("SOFT_PAIRS", "EL_SCALE", "EXCLUDED_PAIRS", "OFF_DIAGONALS", "COUPLINGS"):
         if s.get(sec):
             raise ValueError(f"[{sec.lower()}] is not restated")
@@ -414,7 +441,7 @@ class Window:
         self.tor_cod = t["tor"][:, 4].copy()
         self.imp_cod = t["imp"][:, 4].copy()
         if S:
-            f = read_fep(os.path.join(p["dir"], p["fep"]), S, ns, len(t["mass"]))
+            f = read_fep(os.path.join(p["dir"], p["fep"]), S, ns, len(t["mass"]), t["vdw_rule"])
             self.f = f
             for q, a in enumerate(f["iqseq"]):
                 self.iqatom[a] = q + 1
@@ -647,6 +674,7 @@ class Window:
         E = {}
         d = np.zeros((self.n, 3))
         crg, A, B, iac = self.crg.astype(np.float64), t["avdw"], t["bvdw"], t["iac"] - 1
+        rule = t["vdw_rule"]
         el14 = t["el14"]
         L = self.lists
         # solute-solute (nonbond_pp)
@@ -661,8 +689,9 @@ class Window:
             qq = (self.crg[i] * self.crg[j]).astype(np.float64)
             vel = qq * r * np.where(is14, el14, 1.0)
             c = np.where(is14, 2, 0)
-            va = r6 * r6 * A[iac[i], c] * A[iac[j], c]
-            vb = r6 * B[iac[i], c] * B[iac[j], c]
+            aa, bb = lj_ab(A[iac[i], c], B[iac[i], c], A[iac[j], c], B[iac[j], c], rule)
+            va = r6 * r6 * aa
+            vb = r6 * bb
             dv = r2 * (-vel - 12. * va + 6. * vb)
             np.add.at(d, i, -dv[:, None] * dx)
             np.add.at(d, j, dv[:, None] * dx)
@@ -678,8 +707,9 @@ class Window:
             r = np.sqrt(r2)
             r6 = r2 ** 3
             vel = (self.crg[i] * self.crg[j]).astype(np.float64) * r
-            va = A[iac[i], 0] * A[iac[j], 0] * r6 * r6
-            vb = B[iac[i], 0] * B[iac[j], 0] * r6
+            aa, bb = lj_ab(A[iac[i], 0], B[iac[i], 0], A[iac[j], 0], B[iac[j], 0], rule)
+            va = aa * r6 * r6
+            vb = bb * r6
             dv = r2 * (-vel - 12. * va + 6. * vb)
             np.add.at(d, i, -dv[:, None] * dx)
             np.add.at(d, j, dv[:, None] * dx)
@@ -692,8 +722,7 @@ class Window:
             pr = np.array(sorted(L["ww"]), int)
             qo, qh = self.crg_ow, self.crg_hw
             ow_t = iac[self.ns]
-            AOO = A[ow_t, 0] * A[ow_t, 0]
-            BOO = B[ow_t, 0] * B[ow_t, 0]
+            AOO, BOO = lj_ab(A[ow_t, 0], B[ow_t, 0], A[ow_t, 0], B[ow_t, 0], rule)
             for a in range(3):
                 for b in range(3):
                     i, j = pr[:, 0] + a, pr[:, 1] + b
@@ -746,6 +775,7 @@ class Window:
         f, t, x, S = self.f, self.t, self.x, self.S
         lam = self.p["lambda"]
         A, B, iac = t["avdw"], t["bvdw"], t["iac"] - 1
+        rule = t["vdw_rule"]
         nty = len(t["mass"])
         el14 = t["el14"]
         iq_at = np.array(f["iqseq"])
@@ -765,12 +795,10 @@ class Window:
             for s in range(S):
                 if qv:
                     qt = f["qiac"][q, s]
-                    aLJ = f["qavdw"][qt, col] * A[iac[j], col]
-                    bLJ = f["qbvdw"][qt, col] * B[iac[j], col]
+                    aLJ, bLJ = lj_ab(f["qavdw"][qt, col], f["qbvdw"][qt, col], A[iac[j], col], B[iac[j], col], rule)
                     r6sc = r6 + self.sc[q, iac[j], s]
                 else:
-                    aLJ = A[iac[i], col] * A[iac[j], col]
-                    bLJ = B[iac[i], col] * B[iac[j], col]
+                    aLJ, bLJ = lj_ab(A[iac[i], col], B[iac[i], col], A[iac[j], col], B[iac[j], col], rule)
                     r6sc = r6
                 vel = (self.qcrg[q, s] * self.crg[j]).astype(np.float64) * r * np.where(c == 3, el14, 1.0)
                 va = aLJ / r6sc ** 2
@@ -800,15 +828,15 @@ class Window:
                 for s in range(S):
                     if qv:
                         qt = f["qiac"][q, s]
-                        aLJ = f["qavdw"][qt, 0] * A[ow_t, 0]
-                        bLJ = f["qbvdw"][qt, 0] * B[ow_t, 0]
+                        aLJ, bLJ = lj_ab(f["qavdw"][qt, 0], f["qbvdw"][qt, 0], A[ow_t, 0], B[ow_t, 0], rule)
                         r6sc = r6hc + self.sc[q, ow_t, s]
                         va = aLJ / r6sc ** 2
                         vb = bLJ / r6sc
                     else:
                         r6sc = r6hc
-                        va = A[iac[i], 0] * A[ow_t, 0] / r6hc ** 2
-                        vb = B[iac[i], 0] * B[ow_t, 0] / r6hc
+                        aLJ, bLJ = lj_ab(A[iac[i], 0], B[iac[i], 0], A[ow_t, 0], B[ow_t, 0], rule)
+                        va = aLJ / r6hc ** 2
+                        vb = bLJ / r6hc
                     qq = float(F32(self.crg_ow * self.qcrg[q, s]))
                     qh = float(F32(self.crg_hw * self.qcrg[q, s]))
                     vO, v1, v2 = qq * rO, qh * rH1, qh * rH2
@@ -844,19 +872,16 @@ class Window:
                 qj = float(self.qcrg[jq - 1, s]) if jq else float(self.crg[j])
                 if not qv:
                     # topology types for both atoms (nonbond_qq, md.f90:10786-10796)
-                    aLJ = A[iac[i], col] * A[iac[j], col]
-                    bLJ = B[iac[i], col] * B[iac[j], col]
+                    aLJ, bLJ = lj_ab(A[iac[i], col], B[iac[i], col], A[iac[j], col], B[iac[j], col], rule)
                 else:
                     qt = f["qiac"][a, s]
-                    aLJ = f["qavdw"][qt, col]
-                    bLJ = f["qbvdw"][qt, col]
                     if jq:
                         qtj = f["qiac"][jq - 1, s]
-                        aLJ *= f["qavdw"][qtj, col]
-                        bLJ *= f["qbvdw"][qtj, col]
+                        aLJ, bLJ = lj_ab(f["qavdw"][qt, col], f["qbvdw"][qt, col], f["qavdw"][qtj, col],
+                                         f["qbvdw"][qtj, col], rule)
                     else:
-                        aLJ *= A[iac[j], col]
-                        bLJ *= B[iac[j], col]
+                        aLJ, bLJ = lj_ab(f["qavdw"][qt, col], f["qbvdw"][qt, col], A[iac[j], col],
+                                         B[iac[j], col], rule)
                 dx = x[j] - x[i]
                 r2 = (dx ** 2).sum()
                 r6hc = r2 ** 3

@@ -30,6 +30,23 @@ enum ESlot {
 // restraint records of p_restrain (md.f90:12940-13274)
 enum RstKind { R_SEQ_ATOMS = 0, R_SEQ_GEOM, R_SEQ_MASS, R_POS, R_DIST, R_ANGLE, R_WALL };
 
+// A and B of a pair's V = A/r^12 - B/r^6 from the two atoms' LJ columns a, b:
+// products under the geometric rule (nonbond_pp, md.f90:10432-10439); under
+// the arithmetic rule R* = a_i + a_j and eps = b_i b_j (b being the square
+// root of epsilon), A = eps R*^12, B = 2 eps R*^6 (nonbon2_pp,
+// md.f90:8993-9000, 9056-9058)
+template <typename T>
+__host__ __device__ __forceinline__ void lj_ab(T ai, T bi, T aj, T bj, bool arith, T* A, T* B) {
+  if (!arith) {
+    *A = ai * aj;
+    *B = bi * bj;
+    return;
+  }
+  const T s = ai + aj, s2 = s * s, s6 = s2 * s2 * s2, e = bi * bj;
+  *A = e * s6 * s6;
+  *B = T(2) * e * s6;
+}
+
 struct WinDev {
   int ao, natom, nat_solute, nwat;
   int nq, S, qvdw, natyps;
@@ -41,7 +58,9 @@ struct WinDev {
   int nsteps, nbcycle, iene, iout;
   int use_lrf, wpol, separate_scaling, iac_ow;   // iac_ow: 0-based type of water O
   int ndegf, ndegfree, ndegf_solute, ndegfree_solute, ndegf_solvent, ndegfree_solvent;
-  float el14, crg_ow, crg_hw, aO1, bO1;
+  float el14, crg_ow, crg_hw, aO1, bO1;   // aO1, bO1: LJ columns of the water oxygen type
+  int arith;                        // arithmetic LJ combination rule (topology vdW rule 2)
+  float ww_a, ww_b;                 // A, B of the water O-O pair
   double dt, tau_t, temp0;
   double rcpp2, rcpw2, rcww2, rclrf2;
   double fk_pshell, fk_wsphere, fkwpol, dwmz, awmz, rwat;

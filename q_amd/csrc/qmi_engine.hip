@@ -240,12 +240,25 @@ static int build_device(qmi_engine* e) {
     D.ndegf = W.ndegf; D.ndegfree = W.ndegfree; D.ndegf_solute = W.ndegf_solute;
     D.ndegfree_solute = W.ndegfree_solute; D.ndegf_solvent = W.ndegf_solvent; D.ndegfree_solvent = W.ndegfree_solvent;
     D.el14 = (float)t.el14_scale; D.crg_ow = W.crg_ow; D.crg_hw = W.crg_hw;
+    D.arith = t.ivdw_rule == 2 ? 1 : 0;
     if (W.nwat > 0) {
-      D.aO1 = (float)t.iaclib[t.iac[NS]].avdw[0];
-      D.bO1 = (float)t.iaclib[t.iac[NS]].bvdw[0];
+      const AtomType& o = t.iaclib[t.iac[NS]];
+      D.aO1 = (float)o.avdw[0];
+      D.bO1 = (float)o.bvdw[0];
+      if (D.arith) {
+        double A, B;
+        lj_ab(o.avdw[0], o.bvdw[0], o.avdw[0], o.bvdw[0], true, &A, &B);
+        D.ww_a = (float)A;
+        D.ww_b = (float)B;
+      } else {
+        D.ww_a = D.aO1 * D.aO1;
+        D.ww_b = D.bO1 * D.bO1;
+      }
       const AtomType& h = t.iaclib[t.iac[NS + 1]];
-      if (h.avdw[0] != 0.0 || h.bvdw[0] != 0.0 || h.avdw[2] != 0.0)
-        return set_err(-10, "SPC water hydrogens must have zero LJ parameters");
+      // no LJ on the water hydrogens: zero A and B under either rule
+      const bool h_lj = D.arith ? (h.bvdw[0] != 0.0 || h.bvdw[2] != 0.0)
+                                : (h.avdw[0] != 0.0 || h.bvdw[0] != 0.0 || h.avdw[2] != 0.0);
+      if (h_lj) return set_err(-10, "SPC water hydrogens must have zero LJ parameters");
     }
     D.dt = p.dt; D.tau_t = p.tau_t; D.temp0 = p.temp0;
     D.rcpp2 = p.rcpp * p.rcpp; D.rcpw2 = p.rcpw * p.rcpw; D.rcww2 = p.rcww * p.rcww; D.rclrf2 = p.rclrf * p.rclrf;
@@ -387,21 +400,25 @@ static int build_device(qmi_engine* e) {
           if (!fp.qvdw_flag) {
             const AtomType& ti = t.iaclib[t.iac[i_at]];
             const AtomType& tj = t.iaclib[t.iac[pq.j]];
-            aLJ = ti.avdw[c] * tj.avdw[c];
-            bLJ = ti.bvdw[c] * tj.bvdw[c];
+            lj_ab(ti.avdw[c], ti.bvdw[c], tj.avdw[c], tj.bvdw[c], D.arith != 0, &aLJ, &bLJ);
           } else {
             const int qi = fp.qiac[pq.iq * S + s];
             aLJ = fp.qavdw[qi * 3 + c];
             bLJ = fp.qbvdw[qi * 3 + c];
             if (pq.jq) {
               const int qj = fp.qiac[(pq.jq - 1) * S + s];
-              aLJ *= fp.qavdw[qj * 3 + c];
-              bLJ *= fp.qbvdw[qj * 3 + c];
+              if (pq.ljcod == 2) {
+                // the exponential repulsion takes the products under either
+                // rule (nonbon2_qq, md.f90:9424-9429)
+                aLJ *= fp.qavdw[qj * 3 + c];
+                bLJ *= fp.qbvdw[qj * 3 + c];
+              } else {
+                lj_ab(aLJ, bLJ, fp.qavdw[qj * 3 + c], fp.qbvdw[qj * 3 + c], D.arith != 0, &aLJ, &bLJ);
+              }
             } else {
               if (pq.ljcod == 2) { aLJ = fp.qavdw[qi * 3]; bLJ = fp.qbvdw[qi * 3]; }
               const AtomType& tj = t.iaclib[t.iac[pq.j]];
-              aLJ *= tj.avdw[c];
-              bLJ *= tj.bvdw[c];
+              lj_ab(aLJ, bLJ, tj.avdw[c], tj.bvdw[c], D.arith != 0, &aLJ, &bLJ);
             }
           }
           qq_ab.push_back(make_double2(aLJ, bLJ));
@@ -777,7 +794,17 @@ static int build_device(qmi_engine* e) {
         for (int st = 0; st < D.S; ++st) {
           const float4 ql = q_lj[D.qs_off + iq * D.S + st];
           const float qcv = q_crg[D.qs_off + iq * D.S + st];
-          qcw_par[(size_t)m * kMaxS + st] = make_float4(ql.x * D.aO1, ql.y * D.bO1, D.crg_ow * qcv, D.crg_hw * qcv);
+          float A = ql.x * D.aO1, B = ql.y * D.bO1;
+          if (D.arith) {
+            // from the FP64 parameters, as the reference forms them
+            const double* qd = &q_ljd[(size_t)(D.qs_off + iq * D.S + st) * 4];
+            const double* od = &t_lj[(size_t)(D.t_off + D.iac_ow) * 4];
+            double Ad, Bd;
+            lj_ab(qd[0], qd[1], od[0], od[1], true, &Ad, &Bd);
+            A = (float)Ad;
+            B = (float)Bd;
+          }
+          qcw_par[(size_t)m * kMaxS + st] = make_float4(A, B, D.crg_ow * qcv, D.crg_hw * qcv);
           if (D.qvdw && D.nwat > 0)
             qcw_sc[(size_t)m * kMaxS + st] = q_sc[D.qsc_off + ((size_t)iq * D.natyps + D.iac_ow) * D.S + st];
         }

@@ -710,8 +710,20 @@ bool load_topology(const std::string& path, Topology* t, Error* err) {
   }
   if (t->solvent_type != 0 && t->nat_pro > t->nat_solute)
     return fail(err, -10, "only SPC-type 3-site solvent (solvent type 0) is supported");
-  if (t->ivdw_rule != 1) return fail(err, -10, "only the geometric vdW combination rule is supported");
+  if (t->ivdw_rule != 1 && t->ivdw_rule != 2) return fail(err, -10, "unknown vdW combination rule");
+  if (t->ivdw_rule == 2) {
+    // arithmetic rule: the epsilons enter as their square roots, so that a
+    // pair takes eps = b_i b_j (topology, md.f90:14747-14752)
+    for (int k = 1; k <= t->natyps; ++k)
+      for (int j = 0; j < kNljtyp; ++j) t->iaclib[k].bvdw[j] = std::sqrt(std::fabs(t->iaclib[k].bvdw[j]));
+  }
   return true;
+}
+
+// x^6 as x^2 x^2 x^2 (the integer power of the soft-core formulas)
+static double ipow6(double x) {
+  const double x2 = x * x;
+  return x2 * x2 * x2;
 }
 
 // ----------------------------------------------------------- fep file --
@@ -965,6 +977,9 @@ bool load_fep(const std::string& path, const Topology& t, const InputParams& p, 
       for (int s = 0; s < S; ++s) fp->alpha_max[iq * S + s] = v[1 + s];
     }
     auto idx = [&](int iq, int col, int s) { return ((size_t)iq * ncol + col) * S + s; };
+    // under the arithmetic rule the topology epsilons are square roots here
+    // and the q-type epsilons not yet (qatom.f90:1611-1620, 1655-1664)
+    const bool arith = t.ivdw_rule == 2;
     for (int i = 0; i < nq; ++i)
       for (int s = 0; s < S; ++s) {
         double am = fp->alpha_max[i * S + s];
@@ -972,7 +987,11 @@ bool load_fep(const std::string& path, const Topology& t, const InputParams& p, 
           if (fp->softcore_use_max_potential) {
             double aq = fp->qavdw[fp->qiac[i * S + s] * 3], bq = fp->qbvdw[fp->qiac[i * S + s] * 3];
             double aj = t.iaclib[j].avdw[0], bj = t.iaclib[j].bvdw[0];
-            if (am != 0) fp->sc_lookup[idx(i, j - 1, s)] = (-bq * bj + std::sqrt(bq * bq * bj * bj + 4 * am * aq * aj)) / (2 * am);
+            if (am != 0 && !arith)
+              fp->sc_lookup[idx(i, j - 1, s)] = (-bq * bj + std::sqrt(bq * bq * bj * bj + 4 * am * aq * aj)) / (2 * am);
+            else if (am != 0)
+              fp->sc_lookup[idx(i, j - 1, s)] = (-2 * std::sqrt(bq) * bj + 2 * std::sqrt(bq * bj * bj + am * std::sqrt(bq) * bj)) *
+                                                ipow6(aq + aj) / (2 * am);
           } else {
             fp->sc_lookup[idx(i, j - 1, s)] = am;
           }
@@ -986,13 +1005,25 @@ bool load_fep(const std::string& path, const Topology& t, const InputParams& p, 
               if (am == 0 || amj == 0) a = std::max(am, amj);
               double aq = fp->qavdw[fp->qiac[i * S + s] * 3], bq = fp->qbvdw[fp->qiac[i * S + s] * 3];
               double aj = fp->qavdw[fp->qiac[j * S + s] * 3], bj = fp->qbvdw[fp->qiac[j * S + s] * 3];
-              fp->sc_lookup[idx(i, t.natyps + j, s)] = (-bq * bj + std::sqrt(bq * bq * bj * bj + 4 * a * aq * aj)) / (2 * a);
+              if (!arith)
+                fp->sc_lookup[idx(i, t.natyps + j, s)] = (-bq * bj + std::sqrt(bq * bq * bj * bj + 4 * a * aq * aj)) / (2 * a);
+              else
+                fp->sc_lookup[idx(i, t.natyps + j, s)] = (-2 * std::sqrt(bq * bj) + 2 * std::sqrt(bq * bj + a * std::sqrt(bq * bj))) *
+                                                         ipow6(aq + aj) / (2 * a);
             } else {
               fp->sc_lookup[idx(i, t.natyps + j, s)] = std::max(am, amj);
             }
           }
         }
       }
+  }
+  if (t.ivdw_rule == 2 && fp->qvdw_flag) {
+    // square roots of the q-type epsilons, normal and 1-4 columns only
+    // (get_fep, md.f90:1518-1523)
+    for (size_t k = 0; k + 2 < fp->qbvdw.size(); k += 3) {
+      fp->qbvdw[k] = std::sqrt(fp->qbvdw[k]);
+      fp->qbvdw[k + 2] = std::sqrt(fp->qbvdw[k + 2]);
+    }
   }
   return true;
 }

@@ -431,7 +431,7 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
   }
   const float qo = W.crg_ow, qh = W.crg_hw;
   const float qq[3][3] = {{qo * qo, qo * qh, qo * qh}, {qh * qo, qh * qh, qh * qh}, {qh * qo, qh * qh, qh * qh}};
-  const float A = W.aO1 * W.aO1, B = W.bO1 * W.bO1;
+  const float A = W.ww_a, B = W.ww_b;
   constexpr bool en = EN;
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
@@ -517,6 +517,7 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
     if (ea >= 0) ljs = d.ljp[ea];
   }
   constexpr bool en = EN && !WATER_SIDE;
+  const bool arith = W.arith;
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   double eel = 0.0, evdw = 0.0;
@@ -564,7 +565,9 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
         float dv, ev = 0.f;
         if (k == 0) {
           float r6 = r2inv * r2inv * r2inv;
-          float va = (lj.x * W.aO1) * r6 * r6, vb = (lj.y * W.bO1) * r6;
+          float A, B;
+          lj_ab(lj.x, lj.y, W.aO1, W.bO1, arith, &A, &B);
+          float va = A * r6 * r6, vb = B * r6;
           dv = r2inv * (-vel - 12.f * va + 6.f * vb);
           ev = va - vb;
         } else {
@@ -621,6 +624,7 @@ __device__ __forceinline__ void pp_impl(const Dev& d) {
   const int nch = row_chunks(n);
   const float4 pi = ea >= 0 ? d.x32[ea] : make_float4(0, 0, 0, 0);
   const float4 li = ea >= 0 ? d.ljp[ea] : make_float4(0, 0, 0, 0);
+  const bool arith = W.arith;
   constexpr bool en = EN;
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   double eel = 0.0, evdw = 0.0;
@@ -655,8 +659,8 @@ __device__ __forceinline__ void pp_impl(const Dev& d) {
       float r6 = r2inv * r2inv * r2inv;
       float vel = (pi.w * pj[u].w) * rinv;
       float aa, bb;
-      if (is14) { vel *= W.el14; aa = li.z * lj[u].z; bb = li.w * lj[u].w; }
-      else { aa = li.x * lj[u].x; bb = li.y * lj[u].y; }
+      if (is14) { vel *= W.el14; lj_ab(li.z, li.w, lj[u].z, lj[u].w, arith, &aa, &bb); }
+      else lj_ab(li.x, li.y, lj[u].x, lj[u].y, arith, &aa, &bb);
       float va = r6 * r6 * aa, vb = r6 * bb;
       float dv = r2inv * (-vel - 12.f * va + 6.f * vb);
       if (v[u]) {
@@ -711,7 +715,8 @@ __device__ void qw_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int 
     el[s] = e;
     // LJ on the oxygen
     const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
-    const double aLJ = ql[0] * tl[0], bLJ = ql[1] * tl[1];
+    double aLJ, bLJ;
+    lj_ab(ql[0], ql[1], tl[0], tl[1], W.arith != 0, &aLJ, &bLJ);
     const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s] : 0.0;
     const df inv = df_rcp(df_add(r6, df_from(sc)));
     const df va = df_mul(df_from(aLJ), df_mul(inv, inv));
@@ -741,7 +746,8 @@ __device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int 
     if (is14) vel = df_mul(vel, df_from((double)W.el14));
     el[s] = df_to(vel);
     const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
-    const double aLJ = ql[k] * tl[k], bLJ = ql[k + 1] * tl[k + 1];
+    double aLJ, bLJ;
+    lj_ab(ql[k], ql[k + 1], tl[k], tl[k + 1], W.arith != 0, &aLJ, &bLJ);
     const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + iacj) * S + s] : 0.0;
     const df inv = df_rcp(df_add(r6, df_from(sc)));
     const df va = df_mul(df_from(aLJ), df_mul(inv, inv));
@@ -904,7 +910,9 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
         float4 P;
         float sc;
         if (Q_SIDE) {
-          P = make_float4(h_ql[s].x * aO, h_ql[s].y * bO, qow * h_qc[s], qhw * h_qc[s]);
+          float A, B;
+          lj_ab(h_ql[s].x, h_ql[s].y, aO, bO, W.arith != 0, &A, &B);
+          P = make_float4(A, B, qow * h_qc[s], qhw * h_qc[s]);
           sc = h_sc[s];
         } else {
           P = d.qcw_par[(size_t)jx[u] * kMaxS + s];
@@ -991,7 +999,7 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   if (!st.active) return;
   const WinDev& W = d.win[w];
   const int S = W.S;
-  const bool qvdw = W.qvdw;
+  const bool qvdw = W.qvdw, arith = W.arith;
   const float el14 = W.el14;
   const int a = lane >> 3, b = lane & 7;
   int iq_l = -1, qa = -1, pa = -1;
@@ -1062,8 +1070,9 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
       for (int s = 0; s < S; ++s) {
         const float4 ql = Q_SIDE ? h_ql[s] : d.q_lj[W.qs_off + iq * S + s];
         const float qc = Q_SIDE ? h_qc[s] : d.q_crg[W.qs_off + iq * S + s];
-        const float aLJ = is14 ? ql.z * lj.z : ql.x * lj.x;
-        const float bLJ = is14 ? ql.w * lj.w : ql.y * lj.y;
+        float aLJ, bLJ;
+        if (is14) lj_ab(ql.z, ql.w, lj.z, lj.w, arith, &aLJ, &bLJ);
+        else lj_ab(ql.x, ql.y, lj.x, lj.y, arith, &aLJ, &bLJ);
         float vel = (qc * pj.w) * rinv;
         if (is14) vel *= el14;
         float va, vb, ratio = 1.f;

@@ -50,6 +50,23 @@ def lj_sqrt(sigma: float, eps: float) -> tuple[float, float]:
     return math.sqrt(a), math.sqrt(b)
 
 
+def lj_columns(sigma: float, eps: float, rule: int) -> tuple:
+    """The (a, b) pairs of the three LJ columns (normal, polar, 1-4) of one
+    atom type as a Q topology or FEP file stores them.  Geometric rule (1):
+    sqrt(A), sqrt(B), with the 1-4 column at half the A and B.  Arithmetic
+    rule (2): R*_i = r_min/2 = 2^(1/6) sigma / 2 and epsilon_i, the pair
+    taking R* = R*_i + R*_j and eps = sqrt(eps_i eps_j) (topo.f90:1236-1270,
+    md.f90:14747-14752); the 1-4 column halves epsilon."""
+    if rule == 1:
+        a, b = lj_sqrt(sigma, eps)
+        r2 = math.sqrt(2.0)
+        return (a, b), (a, b), (a / r2, b / r2)
+    if sigma == 0.0 or eps == 0.0:
+        return (0.0, 0.0), (0.0, 0.0), (0.0, 0.0)
+    r = 2.0 ** (1.0 / 6.0) * sigma / 2.0
+    return (r, eps), (r, eps), (r, eps / 2.0)
+
+
 # bond codes: (fk, r0); angle codes: (fk, theta0 deg); torsion codes (fk, mult, delta deg, paths)
 BOND_LIB = [(620.0, 1.46), (680.0, 1.09), (1106.0, SPC_ROH), (0.0, 2 * SPC_ROH * math.sin(math.radians(SPC_HOH / 2)))]
 B_HEAVY, B_XH, B_OW_H, B_HW_HW = 1, 2, 3, 4
@@ -79,6 +96,8 @@ class System:
     nangles_solute: int = 0
     ncgp_solute: int = 0
     radius: float = 20.0
+    vdw_rule: int = 1                              # 1 geometric, 2 arithmetic LJ combination
+    sc_max_potential: bool = False                 # [softcore] values are maximum potentials, not alphas
     # FEP description (q-atoms are topology atoms, 1-based)
     qatoms: list = field(default_factory=list)
     qcharges: list = field(default_factory=list)   # per q-atom (state1, state2)
@@ -486,9 +505,10 @@ def water_droplet(radius: float = 15.0, seed: int = 0) -> System:
 
 
 def ligand_in_water(radius: float = 20.0, seed: int = 0, n_heavy: int = 30, n_h: int = 20,
-                    n_pert: int = 6, alpha: float = 10.0) -> System:
+                    n_pert: int = 6, alpha: float = 10.0, vdw_rule: int = 1,
+                    sc_max_potential: bool = False) -> System:
     rng = np.random.default_rng(seed)
-    s = System(name="ligwat", radius=radius)
+    s = System(name="ligwat", radius=radius, vdw_rule=vdw_rule, sc_max_potential=sc_max_potential)
     grid = _Grid()
     xyz, types, nb, heavy = _grow_ligand(rng, n_heavy, n_h, np.zeros(3), grid, 1)
     # re-centre the ligand on the sphere centre
@@ -538,11 +558,11 @@ def _unit_template():
 
 
 def protein_leg(radius: float = 25.0, seed: int = 2, n_units: int = 160, n_lig: int = 40,
-                n_pert: int = 6, alpha: float = 10.0, core: float = 21.5) -> System:
+                n_pert: int = 6, alpha: float = 10.0, core: float = 21.5, vdw_rule: int = 1) -> System:
     """Synthetic chain of ``n_units`` 16-atom units wrapped around a central
     two-state ligand, solvated by SPC water up to ``radius``."""
     rng = np.random.default_rng(seed)
-    s = System(name="protlig", radius=radius)
+    s = System(name="protlig", radius=radius, vdw_rule=vdw_rule)
     grid = _Grid()
     # ligand first in the topology (it is the FEP part), chain afterwards
     n_heavy = (n_lig * 3) // 5
@@ -818,16 +838,17 @@ def write_top(s: System, path: str) -> None:
         w("".join(f"{m:6d}" for m in members))
     nty = len(LJ_TYPES)
     w(f"{nty:8d} = No. of atom types")
-    w("       1 = vdW combination rule (1 = Geom. / 2 = Arit.)")
+    w(f"{s.vdw_rule:8d} = vdW combination rule (1 = Geom. / 2 = Arit.)")
     w(" 0.50000 332.0716 = Electrostatic 1-4 scaling factor and  Coulomb constant")
     w("Masses:")
     w("".join(f"{t[3]:9.4f}" for t in LJ_TYPES))
-    ab = [lj_sqrt(t[1], t[2]) for t in LJ_TYPES]
-    for lab, scale in (("normal", 1.0), ("polar", 1.0), ("1-4", 1.0 / math.sqrt(2.0))):
-        w(f"sqrt (Aii) {lab}:")
-        w("".join(f"{a * scale:12.4f}" for a, _b in ab))
-        w(f"sqrt (Bii) {lab}:")
-        w("".join(f"{b * scale:12.5f}" for _a, b in ab))
+    cols = [lj_columns(t[1], t[2], s.vdw_rule) for t in LJ_TYPES]
+    la, lb = ("sqrt (Aii)", "sqrt (Bii)") if s.vdw_rule == 1 else ("R*ii", "epsilon ii")
+    for k, lab in enumerate(("normal", "polar", "1-4")):
+        w(f"{la} {lab}:")
+        w("".join(f"{c[k][0]:12.4f}" for c in cols))
+        w(f"{lb} {lab}:")
+        w("".join(f"{c[k][1]:12.5f}" for c in cols))
     w("       0 = No. of type-2 vdW interactions. pairs:")
     excl, one4 = _neighbour_sets(s)
     l14, long14 = _bitmap_lines(one4, nsol)
@@ -876,7 +897,8 @@ def write_top(s: System, path: str) -> None:
 def write_fep(s: System, path: str) -> None:
     if not s.qatoms:
         return
-    out = ["!synthetic two-state perturbation", "[FEP]", "states 2", "softcore_use_max_potential off", "",
+    out = ["!synthetic two-state perturbation", "[FEP]", "states 2",
+           f"softcore_use_max_potential {'on' if s.sc_max_potential else 'off'}", "",
            "[atoms]"]
     for k, a in enumerate(s.qatoms, 1):
         out.append(f"{k:5d}{a:6d}")
@@ -885,9 +907,8 @@ def write_fep(s: System, path: str) -> None:
         out.append(f"{k:5d}{q1:10.4f}{q2:10.4f}")
     out += ["", "[atom_types]"]
     for (nm, sig, eps, mass) in LJ_TYPES:
-        a, b = lj_sqrt(sig, eps)
-        a14, b14 = a / math.sqrt(2.0), b / math.sqrt(2.0)
-        out.append(f"{nm:<6s}{a:12.4f}{b:10.5f}{a:12.4f}{b:10.5f}{a14:12.4f}{b14:10.5f}{mass:9.4f}")
+        (a, b), (ap, bp), (a14, b14) = lj_columns(sig, eps, s.vdw_rule)
+        out.append(f"{nm:<6s}{a:12.4f}{b:10.5f}{ap:12.4f}{bp:10.5f}{a14:12.4f}{b14:10.5f}{mass:9.4f}")
     out += ["", "[change_atoms]"]
     for k, (t1, t2) in enumerate(s.qtypes, 1):
         out.append(f"{k:5d}  {t1:<6s}{t2:<6s}")
@@ -973,7 +994,14 @@ CONFIGS = {
     # name: (builder, builder kwargs, inp kwargs)
     "water": (water_droplet, {"radius": 15.0}, {"lrf": False, "rc_solute": 10.0}),
     "ligwat": (ligand_in_water, {"radius": 20.0}, {"lrf": True}),
+
     "protein": (protein_leg, {"radius": 25.0}, {"lrf": True, "shake_hydrogens": True}),
+    # soft-core given as maximum potentials (qatom.f90:1600-1664), under
+    # either LJ combination rule
+    "ligmaxpot": (ligand_in_water, {"radius": 20.0, "sc_max_potential": True}, {"lrf": True}),
+    "ligarithmax": (ligand_in_water, {"radius": 20.0, "vdw_rule": 2, "sc_max_potential": True}, {"lrf": True}),
+    # the protein leg with the arithmetic LJ combination rule
+    "protarith": (protein_leg, {"radius": 25.0, "vdw_rule": 2}, {"lrf": True, "shake_hydrogens": True}),
 }
 
 

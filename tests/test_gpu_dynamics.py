@@ -34,7 +34,7 @@ def read_en(path, nstates):
     return out
 
 
-@pytest.mark.parametrize("cfg,seed", [("ligwat", 0), ("protein", 2), ("kat", 0)])
+@pytest.mark.parametrize("cfg,seed", [("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2)])
 def test_100_steps(golden_inp, cfg, seed):
     from q_amd.engine import Engine
     g = dict(np.load(os.path.join(GOLD, f"{cfg}_s{seed}.npz")))
@@ -67,13 +67,15 @@ def test_100_steps(golden_inp, cfg, seed):
 
 def test_batched_equals_single(systems):
     from q_amd.engine import Engine
-    inps = [systems("ligwat", seed=0, steps=40, lam=0.25), systems("protein", seed=2, steps=40, lam=0.75)]
+    # the two LJ combination rules in one batch
+    inps = [systems("ligwat", seed=0, steps=40, lam=0.25), systems("protein", seed=2, steps=40, lam=0.75),
+            systems("protarith", seed=2, steps=40, lam=0.5)]
     eb = Engine(0)
     for p in inps:
         eb.add_window(p)
     eb.prepare()
     eb.run(30, finalize=False)
-    xb = [eb.coords(w) for w in range(2)]
+    xb = [eb.coords(w) for w in range(len(inps))]
     for w, p in enumerate(inps):
         es = Engine(0)
         es.add_window(p)

@@ -17,9 +17,12 @@ from tests.golden.make_golden import E_NAMES, canonical_lists, sha
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-# three seeded synthetic windows and the reference's own test system (KAT
-# ligand in water, one state, sequence restraint)
-CASES = {"water": 0, "ligwat": 0, "protein": 2, "kat": 0}
+# three seeded synthetic windows, the reference's own test system (KAT
+# ligand in water, one state, sequence restraint) and the protein leg under
+# the arithmetic LJ combination rule (nonbon2_*, md.f90:8972-10227), and
+# the ligand in water with soft-core maximum potentials under either rule
+# (sc_lookup, qatom.f90:1600-1664)
+CASES = {"water": 0, "ligwat": 0, "protein": 2, "kat": 0, "protarith": 2, "ligmaxpot": 0, "ligarithmax": 0}
 
 
 def _rel(a, b, floor=1.0):
@@ -66,7 +69,7 @@ def test_step0_energies(evaluated, cfg):
         assert _rel(E[k], ref[k]) < 1e-6, k
 
 
-@pytest.mark.parametrize("cfg", ["ligwat", "protein", "kat"])
+@pytest.mark.parametrize("cfg", ["ligwat", "protein", "kat", "protarith", "ligmaxpot", "ligarithmax"])
 def test_step0_q_energies(evaluated, cfg):
     e, w, dump = evaluated[cfg]
     EQ = e.q_energies(w)
