@@ -261,6 +261,13 @@ struct Dev {
   double* t_ring;        // [slot][window][4]: temp, tfree, ekin, istep
   double* t_part;        // [window][kTempParts][4]: kinetic-energy partial sums
   int ring_n;
+  // EVB off-diagonal functions of all windows: window, q-atoms k, l (global
+  // atoms), A and mu; every MD step records Hij and r_kl at its start
+  int noffd_all;
+  const int* offd_win;
+  const int2* offd_kl;
+  const double2* offd_par;
+  double* o_ring;        // [slot][noffd_all][2]: Hij, r_kl
 };
 
 }  // namespace qmi

@@ -64,6 +64,8 @@ struct HostWin {
   double temp = 0, tfree = 0, ekin = 0;
   double rad = 0, wpol = 0;                   // solvent radial / polarisation restraint energies
   int status = 0;                             // 0 running, 1 SHAKE failure, 2 list capacity exceeded
+  int offd_off = 0;                           // first of the window's EVB off-diagonal functions
+  std::vector<double> offd;                   // Hij, r_kl per function at the last drained step
   int istep = 0;                              // host mirror of the device step
   bool done = false;
 };
@@ -108,6 +110,7 @@ struct qmi_engine {
   hipStream_t copy_stream = nullptr;
   hipEvent_t chunk_ev[2] = {nullptr, nullptr};
   long long* h_er = nullptr;
+  double* h_or = nullptr;                     // EVB off-diagonal ring, pinned
   double* h_tr = nullptr;
   ~qmi_engine();
 };
@@ -142,6 +145,7 @@ qmi_engine::~qmi_engine() {
   for (void* p : bufs) (void)hipFree(p);
   for (auto& x : chunk_ev) if (x) (void)hipEventDestroy(x);
   if (h_er) (void)hipHostFree(h_er);
+  if (h_or) (void)hipHostFree(h_or);
   if (h_tr) (void)hipHostFree(h_tr);
   if (copy_stream) (void)hipStreamDestroy(copy_stream);
   if (stream) (void)hipStreamDestroy(stream);
@@ -920,6 +924,34 @@ static int build_device(qmi_engine* e) {
   HIP_OK(hipHostMalloc((void**)&e->h_er, (size_t)d.ring_n * NW * E_NSLOT * sizeof(long long), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&e->h_tr, (size_t)d.ring_n * NW * 4 * sizeof(double), hipHostMallocDefault));
   if ((rc = dev_alloc(e, &d.t_part, (size_t)NW * kTempParts * 4)) != 0) return rc;
+  // EVB off-diagonal functions and their ring
+  {
+    std::vector<int> ow;
+    std::vector<int2> okl;
+    std::vector<double2> opar;
+    for (int wi = 0; wi < NW; ++wi) {
+      HostWin& h = e->wins[wi];
+      const Fep& fp = h.w->fep;
+      h.offd_off = (int)ow.size();
+      h.offd.assign(2 * fp.offd.size(), 0.0);
+      for (const auto& o : fp.offd) {
+        ow.push_back(wi);
+        okl.push_back(make_int2(wd[wi].ao + fp.iqseq[o.k - 1], wd[wi].ao + fp.iqseq[o.l - 1]));
+        opar.push_back(make_double2(o.A, o.mu));
+      }
+    }
+    d.noffd_all = (int)ow.size();
+    if (d.noffd_all) {
+      int* owp;
+      UP(owp, ow); d.offd_win = owp;
+      int2* klp;
+      UP(klp, okl); d.offd_kl = klp;
+      double2* pp2;
+      UP(pp2, opar); d.offd_par = pp2;
+      if ((rc = dev_alloc(e, &d.o_ring, (size_t)d.ring_n * d.noffd_all * 2)) != 0) return rc;
+      HIP_OK(hipHostMalloc((void**)&e->h_or, (size_t)d.ring_n * d.noffd_all * 2 * sizeof(double), hipHostMallocDefault));
+    }
+  }
 
   // --- neighbour rows of the dynamic lists: capacity per row from the largest
   // candidate count at the start (switching atoms within the cut-off plus
@@ -1132,9 +1164,17 @@ static void write_en(HostWin& h) {
     fwrite(&h.EQ[(size_t)s * QMI_N_Q_ENERGIES], 8, QMI_N_Q_ENERGIES, h.en);
     fwrite(&len, 4, 1, h.en);
   }
-  int32_t z = 0;
-  fwrite(&z, 4, 1, h.en);
-  fwrite(&z, 4, 1, h.en);
+  // the off-diagonal record (put_ene, nrgy.f90:94-119): per function the
+  // states i, j (int32), Hij and r_kl (f64)
+  const auto& od = h.w->fep.offd;
+  int32_t len = (int32_t)(24 * od.size());
+  fwrite(&len, 4, 1, h.en);
+  for (size_t k = 0; k < od.size(); ++k) {
+    int32_t ij[2] = {od[k].i, od[k].j};
+    fwrite(ij, 4, 2, h.en);
+    fwrite(&h.offd[2 * k], 8, 2, h.en);
+  }
+  fwrite(&len, 4, 1, h.en);
   fflush(h.en);
 }
 
@@ -1180,6 +1220,16 @@ static void write_summary(HostWin& h, int istep, bool final_) {
               q(s)[3], q(s)[4], q(s)[5]);
     fprintf(f, "\ntype   st lambda%10s%10s\n", "total", "restraint");
     for (int s = 0; s < S; ++s) fprintf(f, "%-7s%2d%7.4f%10.2f%10.2f\n", "Q-SUM", s + 1, q(s)[0], q(s)[1], q(s)[14]);
+    // EVB off-diagonals (write_out, md.f90:15393-15397): the matrix element
+    // printed is the last one stored for the pair of states
+    const auto& od = W.fep.offd;
+    for (size_t k = 0; k < od.size(); ++k) {
+      double hij = 0;
+      for (size_t m = 0; m < od.size(); ++m)
+        if (od[m].i == od[k].i && od[m].j == od[k].j) hij = h.offd[2 * m];
+      fprintf(f, "H(%2d,%2d) =%8.2f dist. between Q-atoms%4d%4d =%8.2f\n", od[k].i, od[k].j, hij, od[k].k, od[k].l,
+              h.offd[2 * k + 1]);
+    }
   }
   fprintf(f, "===============================================================================\n");
   fflush(f);
@@ -1445,6 +1495,13 @@ static int drain(qmi_engine* e, int first_step, int nsteps_done, hipEvent_t done
     HIP_OK(hipMemcpyAsync(e->h_er, e->d.e_ring, n2 * er_row * 8, hipMemcpyDeviceToHost, e->copy_stream));
     HIP_OK(hipMemcpyAsync(e->h_tr, e->d.t_ring, n2 * tr_row * 8, hipMemcpyDeviceToHost, e->copy_stream));
   }
+  const int NO = e->d.noffd_all;
+  const size_t or_row = (size_t)NO * 2;
+  if (NO) {
+    HIP_OK(hipMemcpyAsync(e->h_or + s0 * or_row, e->d.o_ring + s0 * or_row, n1 * or_row * 8, hipMemcpyDeviceToHost,
+                          e->copy_stream));
+    if (n2 > 0) HIP_OK(hipMemcpyAsync(e->h_or, e->d.o_ring, n2 * or_row * 8, hipMemcpyDeviceToHost, e->copy_stream));
+  }
   HIP_OK(hipStreamSynchronize(e->copy_stream));
   const long long* er = e->h_er;
   const double* tr = e->h_tr;
@@ -1458,6 +1515,8 @@ static int drain(qmi_engine* e, int first_step, int nsteps_done, hipEvent_t done
       const double* t = &tr[((size_t)slot * NW + wi) * 4];
       if ((int)t[3] != istep) continue;
       h.temp = t[0]; h.tfree = t[1]; h.ekin = t[2];
+      if (!h.offd.empty())
+        std::copy_n(&e->h_or[(size_t)slot * or_row + 2 * (size_t)h.offd_off], h.offd.size(), h.offd.begin());
       const bool en_step = W.inp.iene_cycle > 0 && istep % W.inp.iene_cycle == 0 && istep > 0;
       const bool out_step = W.inp.iout_cycle > 0 && istep % W.inp.iout_cycle == 0;
       if (en_step || out_step) {

@@ -958,8 +958,22 @@ bool load_fep(const std::string& path, const Topology& t, const InputParams& p, 
       for (int s = 0; s < S; ++s) fp->qshake_dist.push_back(v[2 + s]);
     }
   }
-  if (const PrmSection* od = f.find("OFF_DIAGONALS")) fp->noffd = (int)od->lines.size();
-  if (fp->noffd > 0) return fail(err, -30, "EVB off-diagonal couplings are not supported");
+  // EVB off-diagonal functions (qatom.f90:1514-1557)
+  if (const PrmSection* od = f.find("OFF_DIAGONALS")) {
+    if (od->lines.size() > 10) return fail(err, -31, "Too many off-diagonal functions (max_states is 10)");
+    for (auto& l : od->lines) {
+      auto v = lead_nums(l);
+      if (v.size() < 6) return fail(err, -31, "bad [off_diagonals] line");
+      Fep::Offd o{(int)v[0], (int)v[1], (int)v[2], (int)v[3], v[4], v[5]};
+      if (o.i < 1 || o.i > S || o.j < 1 || o.j > S || o.i == o.j)
+        return fail(err, -31, "Invalid combination of states: " + std::to_string(o.i) + " " + std::to_string(o.j));
+      if (o.k < 1 || o.k > nq || o.l < 1 || o.l > nq)
+        return fail(err, -31, "invalid q-atom numbers in [off_diagonals]: " + std::to_string(o.k) + " " +
+                                  std::to_string(o.l));
+      fp->offd.push_back(o);
+    }
+  }
+  fp->noffd = (int)fp->offd.size();
   for (const char* sec : {"MONITOR_GROUPS", "MONITOR_GROUP_PAIRS"})
     if (f.find(sec)) return fail(err, -30, "monitor groups are not supported");
 

@@ -240,6 +240,20 @@ __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
   if (i < d.ntot && (eval_only || window_runs(d.ws[d.wid[i]]))) {
     d.f[3 * i] = 0; d.f[3 * i + 1] = 0; d.f[3 * i + 2] = 0;
   }
+  // EVB off-diagonal elements at the step's starting coordinates (offdiag,
+  // md.f90:12911-12937), into the step's ring slot; the ring is read for
+  // steps the window ran, so a stopped window's slot goes unread
+  if (!eval_only && i < d.noffd_all) {
+    const WinState& s = d.ws[d.offd_win[i]];
+    const int2 kl = d.offd_kl[i];
+    const double dx = d.x[3 * kl.y] - d.x[3 * kl.x], dy = d.x[3 * kl.y + 1] - d.x[3 * kl.x + 1],
+                 dz = d.x[3 * kl.y + 2] - d.x[3 * kl.x + 2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double2 p = d.offd_par[i];
+    double* o = &d.o_ring[((size_t)(s.istep % d.ring_n) * d.noffd_all + i) * 2];
+    o[0] = p.x * exp(-p.y * r);
+    o[1] = r;
+  }
 }
 
 // charge-group centres (cgp_centers, md.f90:1000-1019) and cluster bounding
@@ -2310,7 +2324,8 @@ __global__ void __launch_bounds__(64) k_temperature(Dev d, int advance) {
 static inline int blocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 void launch_begin(const Dev& d, hipStream_t s, int force_energy, int eval_only, int rebuild) {
-  k_begin<<<blocks(std::max(d.nwin, d.ntot), 256), 256, 0, s>>>(d, force_energy, eval_only, rebuild);
+  k_begin<<<blocks(std::max(std::max(d.nwin, d.ntot), d.noffd_all), 256), 256, 0, s>>>(d, force_energy, eval_only,
+                                                                                         rebuild);
 }
 
 void launch_lrf_build(const Dev& d, hipStream_t s) {

@@ -114,6 +114,8 @@ struct Fep {
   std::vector<int> iqshake, jqshake;
   std::vector<double> qshake_dist;   // [b*S + s]
   int noffd = 0;
+  struct Offd { int i, j, k, l; double A, mu; };   // states i, j (1-based), q-atoms k, l (1-based)
+  std::vector<Offd> offd;            // [off_diagonals]: Hij = A exp(-mu r_kl)
   std::vector<double> alpha_max;     // [iq*S + s]
   std::vector<double> sc_lookup;     // [(iq*(natyps+nqat) + t)*S + s], t 0-based column
 };

@@ -98,6 +98,7 @@ class System:
     radius: float = 20.0
     vdw_rule: int = 1                              # 1 geometric, 2 arithmetic LJ combination
     sc_max_potential: bool = False                 # [softcore] values are maximum potentials, not alphas
+    offdiag: list = field(default_factory=list)    # EVB (state i, state j, q-atom k, q-atom l, A, mu)
     # FEP description (q-atoms are topology atoms, 1-based)
     qatoms: list = field(default_factory=list)
     qcharges: list = field(default_factory=list)   # per q-atom (state1, state2)
@@ -506,9 +507,10 @@ def water_droplet(radius: float = 15.0, seed: int = 0) -> System:
 
 def ligand_in_water(radius: float = 20.0, seed: int = 0, n_heavy: int = 30, n_h: int = 20,
                     n_pert: int = 6, alpha: float = 10.0, vdw_rule: int = 1,
-                    sc_max_potential: bool = False) -> System:
+                    sc_max_potential: bool = False, offdiag: tuple = ()) -> System:
     rng = np.random.default_rng(seed)
-    s = System(name="ligwat", radius=radius, vdw_rule=vdw_rule, sc_max_potential=sc_max_potential)
+    s = System(name="ligwat", radius=radius, vdw_rule=vdw_rule, sc_max_potential=sc_max_potential,
+               offdiag=list(offdiag))
     grid = _Grid()
     xyz, types, nb, heavy = _grow_ligand(rng, n_heavy, n_h, np.zeros(3), grid, 1)
     # re-centre the ligand on the sphere centre
@@ -936,6 +938,10 @@ def write_fep(s: System, path: str) -> None:
         out += ["", "[change_torsions]"]
         for (i, j, k, l, c1, c2) in s.qtorsions:
             out.append(f"{i:6d}{j:6d}{k:6d}{l:6d}{c1:4d}{c2:4d}")
+    if s.offdiag:
+        out += ["", "[off_diagonals]"]
+        for (i, j, k, l, a, mu) in s.offdiag:
+            out.append(f"{i:3d}{j:3d}{k:5d}{l:5d}{a:10.3f}{mu:8.3f}")
     if s.qimpropers:
         out += ["", "[improper_types]"]
         out.append(f"{1:4d}{10.5:10.3f}{0.0:10.3f}")
@@ -1000,6 +1006,9 @@ CONFIGS = {
     # either LJ combination rule
     "ligmaxpot": (ligand_in_water, {"radius": 20.0, "sc_max_potential": True}, {"lrf": True}),
     "ligarithmax": (ligand_in_water, {"radius": 20.0, "vdw_rule": 2, "sc_max_potential": True}, {"lrf": True}),
+    # EVB off-diagonal functions Hij = A exp(-mu r_kl) (md.f90:12911-12937)
+    "ligevb": (ligand_in_water, {"radius": 20.0, "offdiag": ((1, 2, 1, 5, 25.0, 0.5), (2, 1, 3, 7, 10.0, 0.2))},
+               {"lrf": True}),
     # the protein leg with the arithmetic LJ combination rule
     "protarith": (protein_leg, {"radius": 25.0, "vdw_rule": 2}, {"lrf": True, "shake_hydrogens": True}),
 }

@@ -65,6 +65,34 @@ def test_100_steps(golden_inp, cfg, seed):
         assert r.max() < 1e-4
 
 
+def test_evb_offdiagonal_records(golden_inp):
+    """EVB off-diagonal functions (md.f90:12911-12937): the .en off-diagonal
+    records (i, j, Hij, r_kl) of a 100-step run against the reference's, and
+    the log's H(i,j) lines."""
+    from q_amd.engine import Engine
+    g = dict(np.load(os.path.join(GOLD, "ligevb_s0.npz")))
+    inp = golden_inp("ligevb", 0)
+    d = os.path.dirname(inp)
+    tag = os.path.splitext(os.path.basename(inp))[0]
+    e = Engine(0)
+    w = e.add_window(inp)
+    e.prepare()
+    print("[gpu] engine run ligevb", flush=True)
+    e.run(100, finalize=True)
+    from tests.golden.make_golden import read_en as read_en_offd
+    en, od = read_en_offd(os.path.join(d, tag + ".en"), 2, offd=True)
+    ref = g["en_offd"]
+    assert od.shape == ref.shape
+    assert np.array_equal(od[:, :, :2], ref[:, :, :2])
+    rel = np.abs(od[:, :, 2:] - ref[:, :, 2:]) / np.abs(ref[:, :, 2:])
+    print("offd max rel", rel.max())
+    assert rel.max() < 1e-4
+    r = np.abs(en - g["en"]) / np.maximum(np.abs(g["en"]), 1.0)
+    assert r.max() < 1e-4
+    log = open(os.path.join(d, tag + ".log")).read()
+    assert log.count("H( 1, 2) =") >= 1 and log.count("dist. between Q-atoms   3   7 =") >= 1
+
+
 def test_batched_equals_single(systems):
     from q_amd.engine import Engine
     # the two LJ combination rules in one batch

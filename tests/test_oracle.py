@@ -163,3 +163,21 @@ def test_gradient_is_derivative_of_energy(golden_inp, cfg, seed, npick):
             em, _ = _energy(w, xm, 300.0)
             fd = (ep - em) / (2 * h)
             assert abs(fd - d[a, c]) < 2e-4 * max(1.0, abs(d[a, c])), (a, c, fd, d[a, c])
+
+
+def test_evb_offdiagonal_records_golden():
+    """The reference's EVB off-diagonal records (offdiag, md.f90:12911-12937;
+    put_ene, nrgy.f90:94-119): one (i, j, Hij, r_kl) per [off_diagonals]
+    line in input order with Hij = A exp(-mu r_kl), and the functions leave
+    the dynamics alone (state records equal to the same window without
+    them)."""
+    from q_amd import synth
+    g = load_gold("ligevb", 0)
+    od = g["en_offd"]
+    funcs = synth.CONFIGS["ligevb"][1]["offdiag"]
+    assert od.shape == (len(g["en"]), len(funcs), 4)
+    for m, (i, j, _k, _l, a, mu) in enumerate(funcs):
+        assert np.all(od[:, m, 0] == i) and np.all(od[:, m, 1] == j)
+        assert np.allclose(od[:, m, 2], a * np.exp(-mu * od[:, m, 3]), rtol=1e-13, atol=0)
+        assert np.all((od[:, m, 3] > 0.5) & (od[:, m, 3] < 20.0))
+    assert np.array_equal(g["en"], load_gold("ligwat", 0)["en"])
