@@ -13,9 +13,12 @@ nrgy.f90:51-58) for each state and one off-diagonal record (nrgy.f90:94-119)
   Part 6  Bennett acceptance ratio, iterated to 1e-3    (523-583)
 
 Sums run in the reference's order in double precision, so the printed values
-agree with qfep's.  EVB off-diagonal couplings (noffd > 0 or off-diagonal
-functions) are not supported: the engine writes energy files without
-off-diagonal elements.
+agree with qfep's.  EVB: off-diagonal functions given to qfep (i, j, A, mu,
+eta, r0: Hij = A exp(-mu (r - r0)) exp(-eta (r - r0)^2), r the first r_kl of
+the energy file's off-diagonal record) set the ground-state energy and the
+c1**2, c2**2 and <r_xy> columns (qfep.f90:155-170, 275-322, 606-722).
+Off-diagonal elements taken from the files (a non-zero second number on the
+states line) are refused: the reference reads none of them in that case.
 
 Command line, reading qfep's own stdin answers:
     python -m q_amd.qfep < qfep.inp
@@ -37,10 +40,12 @@ class QfepError(RuntimeError):
     pass
 
 
-def read_en(path: str, nstates: int) -> np.ndarray:
-    """All records of an energy file as an array (npts, nstates, 15)."""
+def read_en(path: str, nstates: int, noffd: int = 0):
+    """All records of an energy file as an array (npts, nstates, 15); with
+    ``noffd`` > 0 also the r_kl of the first ``noffd`` entries of each
+    off-diagonal record, (npts, noffd) (get_ene, nrgy.f90:123-160)."""
     raw = open(path, "rb").read()
-    out, k = [], 0
+    out, rkl, k = [], [], 0
     n_raw = len(raw)
     while k < n_raw:
         block = []
@@ -55,9 +60,14 @@ def read_en(path: str, nstates: int) -> np.ndarray:
         if k + 4 > n_raw:
             raise QfepError(f"{path}: missing off-diagonal record")
         n = int(np.frombuffer(raw, np.int32, 1, k)[0])
+        if noffd:
+            if n < 24 * noffd:
+                raise QfepError(f"{path}: maybe there are less than {noffd} off-diagonal elements?")
+            rkl.append([float(np.frombuffer(raw, np.float64, 1, k + 4 + 24 * m + 16)[0]) for m in range(noffd)])
         k += 8 + n
         out.append(np.array(block))
-    return np.array(out).reshape(-1, nstates, 15)
+    E = np.array(out).reshape(-1, nstates, 15)
+    return (E, np.array(rkl).reshape(-1, noffd)) if noffd else E
 
 
 @dataclass
@@ -68,6 +78,9 @@ class FepFile:
     vg: np.ndarray           # (npts,): ground state energy
     gap: np.ndarray          # (npts,)
     avg: np.ndarray          # (nstates, 15) averages over the points kept
+    c1: np.ndarray = None    # (npts,): ground-state weights (EVB, two states)
+    c2: np.ndarray = None
+    r: np.ndarray = None     # (npts,): first r_kl of the file (EVB)
 
 
 @dataclass
@@ -108,10 +121,29 @@ def _veff(lam, v):
     return acc
 
 
+def _hij(funcs, nstates, r):
+    """Off-diagonal matrix elements from the functions at distances r
+    (qfep.f90:284-300): (i, j) takes A(i,j), or A(j,i) where A(i,j) is 0."""
+    A = np.zeros((nstates, nstates))
+    mu, eta, r0 = np.zeros_like(A), np.zeros_like(A), np.zeros_like(A)
+    for (i, j, a, m, et, x0) in funcs:
+        A[i - 1, j - 1], mu[i - 1, j - 1], eta[i - 1, j - 1], r0[i - 1, j - 1] = a, m, et, x0
+    H = np.zeros((len(r), nstates, nstates))
+    for i in range(nstates):
+        for j in range(nstates):
+            if i == j:
+                continue
+            a, b = (j, i) if A[i, j] == 0.0 else (i, j)
+            H[:, i, j] = A[a, b] * np.exp(-mu[a, b] * (r - r0[a, b])) * np.exp(-eta[a, b] * (r - r0[a, b]) ** 2)
+    return H
+
+
 def qfep(paths, nstates: int, kT: float, nskip: int = 0, nbins: int = 50, nmin: int = 10,
-         alpha=None, coeff=None, gas: int = 0) -> QfepResult:
-    """Run the qfep analysis on energy files `paths` (ordered along lambda)."""
+         alpha=None, coeff=None, gas: int = 0, offdiag=None) -> QfepResult:
+    """Run the qfep analysis on energy files `paths` (ordered along lambda).
+    ``offdiag``: EVB off-diagonal functions (i, j, A, mu, eta, r0)."""
     nfiles = len(paths)
+    funcs = list(offdiag or [])
     alfa = np.zeros(nstates)
     if alpha is not None:
         alfa[1:] = np.asarray(alpha, float)[: nstates - 1]
@@ -119,7 +151,10 @@ def qfep(paths, nstates: int, kT: float, nskip: int = 0, nbins: int = 50, nmin: 
     files = []
     gapmin, gapmax = 999.0, -999.0
     for p in paths:
-        rec = read_en(p, nstates)
+        if funcs:
+            rec, rkl = read_en(p, nstates, len(funcs))
+        else:
+            rec, rkl = read_en(p, nstates), None
         if len(rec) == 0:
             raise QfepError(f"{p}: no energies")
         lam = rec[0, :, 0].copy()
@@ -132,9 +167,23 @@ def qfep(paths, nstates: int, kT: float, nskip: int = 0, nbins: int = 50, nmin: 
             raise QfepError(f"number of data sets in {p} is less than number of points to skip")
         avg = E[nskip:].sum(axis=0) * (1.0 / (npts - nskip))
         v = E[:, :, 1] + alfa[None, :]
+        # ground state of the secular determinant (qfep.f90:284-322): the
+        # diagonal is v, the off-diagonal the functions of the first r_kl
+        H = _hij(funcs, nstates, rkl[:, 0] if funcs else np.zeros(len(v)))
+        c1 = np.zeros(len(v))
+        c2 = np.zeros(len(v))
+        r1 = rkl[:, 0].copy() if funcs else None
         if nstates == 2:
-            # no off-diagonal coupling: Hij = 0 (qfep.f90:294-322)
-            vg = 0.5 * (v[:, 0] + v[:, 1]) - 0.5 * np.sqrt((v[:, 0] - v[:, 1]) ** 2)
+            h12 = H[:, 0, 1]
+            vg = 0.5 * (v[:, 0] + v[:, 1]) - 0.5 * np.sqrt((v[:, 0] - v[:, 1]) ** 2 + 4. * h12 ** 2)
+            if funcs:
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    c1 = 1. / (1. + ((vg - v[:, 0]) / h12) ** 2)
+                c2 = 1 - c1
+        elif funcs:
+            for s in range(nstates):
+                H[:, s, s] = v[:, s]
+            vg = np.linalg.eigvalsh(H, UPLO="L").min(axis=1)
         else:
             vg = v.min(axis=1)
         gap = np.zeros(npts)
@@ -143,7 +192,7 @@ def qfep(paths, nstates: int, kT: float, nskip: int = 0, nbins: int = 50, nmin: 
         if npts > nskip:
             gapmin = min(gapmin, float(gap[nskip:].min()))
             gapmax = max(gapmax, float(gap[nskip:].max()))
-        files.append(FepFile(p, lam, v, vg, gap, avg))
+        files.append(FepFile(p, lam, v, vg, gap, avg, c1, c2, r1))
     R = QfepResult(files=files, gapmin=gapmin, gapmax=gapmax)
     if nfiles < 2:
         return R
@@ -248,6 +297,7 @@ def _reaction_free_energy(R: QfepResult, nstates, rt, nskip, nbins, nmin):
     gaprange = R.gapmax - R.gapmin
     xint = gaprange / float(nbins)
     binsum = np.zeros((nbins + 2, 4))
+    evb = R.files[0].r is not None
     ptsum = np.zeros(nbins + 2, dtype=np.int64)
     for fi, F in enumerate(R.files):
         avdvv = np.zeros((nstates, nbins + 2))
@@ -255,6 +305,9 @@ def _reaction_free_energy(R: QfepResult, nstates, rt, nskip, nbins, nmin):
         nbinpts = np.zeros(nbins + 2, dtype=np.int64)
         sumv = np.zeros((nstates, nbins + 2))
         sumg = np.zeros(nbins + 2)
+        avc1 = np.zeros(nbins + 2)
+        avc2 = np.zeros(nbins + 2)
+        avr = np.zeros(nbins + 2)
         kept = range(nskip, len(F.v))
         ibins = []
         for ip in kept:
@@ -265,9 +318,16 @@ def _reaction_free_energy(R: QfepResult, nstates, rt, nskip, nbins, nmin):
                 veff += F.lam[s] * F.v[ip, s]
             avdvv[:, ib] += F.v[ip, :] - veff
             avdvg[ib] += F.vg[ip] - veff
+            avc1[ib] += F.c1[ip]
+            avc2[ib] += F.c2[ip]
+            if evb:
+                avr[ib] += F.r[ip]
             nbinpts[ib] += 1
         for ib in range(1, nbins + 1):
             if nbinpts[ib] != 0:
+                avc1[ib] /= float(nbinpts[ib])
+                avc2[ib] /= float(nbinpts[ib])
+                avr[ib] /= float(nbinpts[ib])
                 avdvv[:, ib] /= nbinpts[ib]
                 avdvg[ib] /= nbinpts[ib]
         for ip, ib in zip(kept, ibins):
@@ -280,6 +340,9 @@ def _reaction_free_energy(R: QfepResult, nstates, rt, nskip, nbins, nmin):
             sumg[ib] += math.exp(-dvg / rt)
         for ib in range(1, nbins + 1):
             if nbinpts[ib] >= nmin:
+                binsum[ib, 1] += avc1[ib] * nbinpts[ib]
+                binsum[ib, 2] += avc2[ib] * nbinpts[ib]
+                binsum[ib, 3] += avr[ib] * nbinpts[ib]
                 sumv[:, ib] /= float(nbinpts[ib])
                 sumg[ib] /= float(nbinpts[ib])
                 ptsum[ib] += nbinpts[ib]
@@ -289,13 +352,14 @@ def _reaction_free_energy(R: QfepResult, nstates, rt, nskip, nbins, nmin):
                 dGg = R.dG[fi + 1] + sumg2
                 binsum[ib, 0] += dGg * int(nbinpts[ib])
                 R.bins.append((F.lam[0], ib, R.gapmin + ib * xint - xint / 2., dGv[0], dGv[1] if nstates > 1 else 0.0,
-                               dGg, int(nbinpts[ib])))
+                               dGg, int(nbinpts[ib]), avc1[ib], avc2[ib]))
     valid = [ib for ib in range(1, nbins + 1) if ptsum[ib] >= nmin]
     for ib in valid:
-        binsum[ib, 0] /= float(ptsum[ib])
+        binsum[ib, :] /= float(ptsum[ib])
     mn = binsum[1:nbins + 1, 0].min()
     for ib in valid:
-        R.binavg.append((ib, R.gapmin + ib * xint - xint / 2., binsum[ib, 0], binsum[ib, 0] - mn, int(ptsum[ib])))
+        R.binavg.append((ib, R.gapmin + ib * xint - xint / 2., binsum[ib, 0], binsum[ib, 0] - mn, int(ptsum[ib]),
+                         binsum[ib, 1], binsum[ib, 2], binsum[ib, 3]))
 
 
 def report(R: QfepResult, nstates: int) -> str:
@@ -319,10 +383,11 @@ def report(R: QfepResult, nstates: int) -> str:
         L.append("# Part 2: Reaction free energy summary:")
         for row in R.bins:
             L.append(f"   {row[0]:8.6f}{row[1]:5d}  {row[2]:9.2f}{row[3]:9.2f}{row[4]:9.2f}{row[5]:9.2f}  {row[6]:5d}"
-                     f"{0.0:9.3f}{0.0:9.3f}")
+                     f"{row[7]:9.3f}{row[8]:9.3f}")
         L.append("# Part 3: Bin-averaged summary:")
         for row in R.binavg:
-            L.append(f"{row[0]:4d} {row[1]:9.2f}{row[2]:9.2f}{row[3]:9.2f}  {row[4]:5d}{0.0:8.3f}{0.0:8.3f}{0.0:8.3f}")
+            L.append(f"{row[0]:4d} {row[1]:9.2f}{row[2]:9.2f}{row[3]:9.2f}  {row[4]:5d}{row[5]:8.3f}{row[6]:8.3f}"
+                     f"{row[7]:8.3f}")
         for title, part, psum in (("# Part 4: Termodynamic integration:", R.dgti, R.dgtisum),
                                   ("# Part 5: Overlap sampling Lu et al:", R.dglu, R.dglusum),
                                   ("# Part 6: BAR Bennet:", R.dgbar, R.dgbarsum)):
@@ -346,7 +411,8 @@ def main(argv=None):
     t = next(toks)
     nstates, noffd = int(t[0]), int(t[1])
     if noffd != 0:
-        raise QfepError("off-diagonal elements are not supported")
+        raise QfepError("off-diagonal elements from the energy files are not read by qfep (qfep.f90:251, 279): "
+                        "give 0 and the off-diagonal functions instead")
     t = next(toks)
     kT, nskip = float(t[0]), int(t[1])
     gas = int(t[2]) if len(t) > 2 else 0
@@ -354,11 +420,13 @@ def main(argv=None):
     nmin = int(next(toks)[0])
     alpha = [float(next(toks)[0]) for _ in range(2, nstates + 1)]
     nnoffd = int(next(toks)[0])
-    if nnoffd != 0:
-        raise QfepError("off-diagonal coupling functions are not supported")
+    offdiag = []
+    for _ in range(nnoffd):
+        t = next(toks)
+        offdiag.append((int(t[0]), int(t[1])) + tuple(float(v) for v in t[2:6]))
     coeff = [float(v) for v in next(toks)[:nstates]]
     paths = [next(toks)[0] for _ in range(nfiles)]
-    R = qfep(paths, nstates, kT, nskip, nbins, nmin, alpha, coeff, gas)
+    R = qfep(paths, nstates, kT, nskip, nbins, nmin, alpha, coeff, gas, offdiag)
     sys.stdout.write(report(R, nstates))
 
 

@@ -1,8 +1,11 @@
 """The qfep judge (q_amd/qfep.py) against the reference qfep
-(src/q6/qfep.f90) on a five-window lambda ladder: tests/golden/qfep holds the
-energy files written by the reference qdyn and the tables the reference
-qfep printed for them (tests/golden/make_golden.py, make_qfep).  Every
-printed number of Parts 0-6 must agree to the printed digits."""
+(src/q6/qfep.f90) on two five-window lambda ladders: tests/golden/qfep holds
+the energy files written by the reference qdyn and the tables the reference
+qfep printed for them (tests/golden/make_golden.py, make_qfep);
+tests/golden/qfep_evb the same for a window with EVB off-diagonal functions,
+analysed with a qfep off-diagonal function of r_kl (ground-state mixing,
+c1**2, c2**2, <r_xy>).  Every printed number of Parts 0-6 must agree to the
+printed digits."""
 import io
 import os
 import sys
@@ -12,7 +15,7 @@ import pytest
 
 from q_amd import qfep as QF
 
-GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qfep")
+GOLDS = {k: os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", k) for k in ("qfep", "qfep_evb")}
 
 
 def _rows(txt, title):
@@ -33,11 +36,12 @@ def _rows(txt, title):
     return out
 
 
-@pytest.fixture(scope="module")
-def both(monkeypatch_module=None):
-    ref = open(os.path.join(GOLD, "qfep.out")).read()
+@pytest.fixture(scope="module", params=list(GOLDS))
+def both(request):
+    gold = GOLDS[request.param]
+    ref = open(os.path.join(gold, "qfep.out")).read()
     cwd = os.getcwd()
-    os.chdir(GOLD)
+    os.chdir(gold)
     try:
         stdin, stdout = sys.stdin, sys.stdout
         sys.stdin = open("qfep.inp")
