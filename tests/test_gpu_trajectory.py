@@ -29,10 +29,12 @@ def records(path):
     return out
 
 
-def test_dcd_matches_reference(tmp_path):
+@pytest.fixture(scope="module")
+def written(tmp_path_factory):
+    """The engine's trajectory for the golden input (directory, path)."""
     import make_golden as M
     from q_amd.engine import Engine
-    d = str(tmp_path)
+    d = str(tmp_path_factory.mktemp("trj"))
     inp = M.trj_window(d)
     sha = M.file_sha([inp, os.path.join(d, "ligwat_s0.top"), os.path.join(d, "ligwat_s0.fep")])
     assert sha == open(os.path.join(HERE, "golden", "trj", "input_sha.txt")).read().strip()
@@ -42,7 +44,13 @@ def test_dcd_matches_reference(tmp_path):
     e.prepare()
     e.run(M.TRJ_STEPS, finalize=True)
     e.close()
-    got = records(os.path.join(d, "trj.dcd"))
+    return d, os.path.join(d, "trj.dcd")
+
+
+def test_dcd_matches_reference(written):
+    import make_golden as M
+    d, path = written
+    got = records(path)
     ref = records(os.path.join(HERE, "golden", "trj", "trj.dcd"))
     assert len(got) == len(ref) == 3 + 3 * (M.TRJ_STEPS // M.TRJ_EVERY)
     for k in range(3):
@@ -54,3 +62,33 @@ def test_dcd_matches_reference(tmp_path):
         worst = max(worst, float(np.abs(a - b).max()))
     print("frames max |dx|", worst)
     assert worst < 1e-3
+
+
+QCALC = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "qcalc")
+
+
+def _qcalc_rows(d, dcd):
+    """Result rows (frame, values) of the reference qcalc on a trajectory:
+    distances, an angle and a torsion (with their energies) over atoms of
+    the mask."""
+    import subprocess
+    cmds = "ligwat_s0.top\ndist\n1 5\ndist\n3 54\nangle\n1 2 3\ntorsion\n1 2 3 4\ngo\n" + dcd + "\n.\n"
+    r = subprocess.run([QCALC], cwd=d, input=cmds, capture_output=True, text=True, timeout=60)
+    rows = [ln.split() for ln in r.stdout.splitlines() if ln.startswith(dcd)]
+    return [(int(t[1]), [float(v) for v in t[2:]]) for t in rows]
+
+
+@pytest.mark.skipif(not os.path.exists(QCALC), reason="the reference qcalc is not built in oracle/_ref")
+def test_qcalc_reads_engine_trajectory(written):
+    """qcalc compatibility: the reference qcalc (src/q6/qcalc.f90, built in
+    oracle/_ref) reads the engine's DCD and prints the same geometry as for
+    the reference qdyn's file."""
+    import shutil
+    d, path = written
+    shutil.copy(os.path.join(HERE, "golden", "trj", "trj.dcd"), os.path.join(d, "ref.dcd"))
+    mine, ref = _qcalc_rows(d, "trj.dcd"), _qcalc_rows(d, "ref.dcd")
+    print("qcalc rows", mine)
+    assert len(mine) == len(ref) == 4 and all(len(v) == 6 for _, v in mine)
+    for (fa, va), (fb, vb) in zip(mine, ref):
+        assert fa == fb
+        assert np.allclose(va, vb, atol=0.011), (va, vb)
