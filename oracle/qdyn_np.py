@@ -275,7 +275,9 @@ def read_top(path):
     L.k += (nty + 7) // 8
     L.line()
     L.k += (nty + 12) // 13
-    L.values(1)
+    t["solvent_type"] = int(L.values(1)[0])
+    if t["solvent_type"] not in (0, 1):
+        raise ValueError("only SPC (0) and 3-atom (1) solvent are restated")
     v = _nums(L.line().split())
     t["rexcl_o"], t["rwat"] = v[0], v[1]
     t["xpcent"] = L.values(3)
@@ -515,6 +517,12 @@ class Window:
         self.crg = (t["crg"].astype(np.float64) * sq).astype(F32)
         self.crg_ow = F32(float(self.crg_ow) * sq)
         self.crg_hw = F32(float(self.crg_hw) * sq)
+        # the three solvent charges: O, H, H of the SPC routines, or the
+        # topology charges of the first solvent molecule for a 3-atom
+        # solvent (nonbond_3atomsolvent, md.f90:12335-12340)
+        self.wat3 = t["solvent_type"] == 1 and self.nwat > 0
+        self.crg_w = ([self.crg[ns], self.crg[ns + 1], self.crg[ns + 2]] if self.wat3
+                      else [self.crg_ow, self.crg_hw, self.crg_hw])
         if S:
             self.qcrg = (self.qcrg.astype(np.float64) * sq).astype(F32)
         self.cgp_of = np.zeros(n, int)
@@ -716,22 +724,23 @@ class Window:
             E["pw.el"], E["pw.vdw"] = vel.sum(), (va - vb).sum()
         else:
             E["pw.el"] = E["pw.vdw"] = 0.0
-        # water-water (nonbond_ww_spc)
+        # water-water (nonbond_ww_spc; 3-atom solvent: nonbond_3atomsolvent,
+        # md.f90:12317-12591, LJ on all nine atom pairs)
         E["ww.el"] = E["ww.vdw"] = 0.0
         if L["ww"]:
             pr = np.array(sorted(L["ww"]), int)
-            qo, qh = self.crg_ow, self.crg_hw
-            ow_t = iac[self.ns]
-            AOO, BOO = lj_ab(A[ow_t, 0], B[ow_t, 0], A[ow_t, 0], B[ow_t, 0], rule)
+            qw = self.crg_w
+            wt = [iac[self.ns + k] for k in range(3)]
             for a in range(3):
                 for b in range(3):
                     i, j = pr[:, 0] + a, pr[:, 1] + b
                     dx = x[j] - x[i]
                     rinv = np.sqrt(1.0 / (dx ** 2).sum(1))
                     r2 = rinv * rinv
-                    qq = float(F32((qo if a == 0 else qh) * (qo if b == 0 else qh)))
+                    qq = float(F32(qw[a] * qw[b]))
                     vel = qq * rinv
-                    if a == 0 and b == 0:
+                    if (a == 0 and b == 0) or self.wat3:
+                        AOO, BOO = lj_ab(A[wt[a], 0], B[wt[a], 0], A[wt[b], 0], B[wt[b], 0], rule)
                         va = AOO * r2 ** 6
                         vb = BOO * r2 ** 3
                         dv = r2 * (-vel - 12. * va + 6. * vb)
@@ -808,47 +817,39 @@ class Window:
                 EQ[s, 11] += (va - vb).sum()
             np.add.at(d, i, -dv[:, None] * dx)
             np.add.at(d, j, dv[:, None] * dx)
-        # q-water (nonbond_qw_spc)
+        # q-water (nonbond_qw_spc; 3-atom solvent: nonbond_qw_3atom,
+        # md.f90:11545-11670, LJ and soft-core on every solvent atom)
         qw = np.array(self.lists["qw"], int)
         if len(qw):
-            ow_t = iac[self.ns]
+            wt = [iac[self.ns + k] for k in range(3)]
+            nlj = 3 if self.wat3 else 1
             for q, i in enumerate(iq_at):
-                dO = x[qw] - x[i]
-                d1 = x[qw + 1] - x[i]
-                d2 = x[qw + 2] - x[i]
-                r2Oh = (dO ** 2).sum(1)
-                rH1 = np.sqrt(1.0 / (d1 ** 2).sum(1))
-                rH2 = np.sqrt(1.0 / (d2 ** 2).sum(1))
-                r6hc = r2Oh ** 3
-                r2O = 1.0 / r2Oh
-                rO = np.sqrt(r2O)
-                dvO = np.zeros(len(qw))
-                dvH1 = np.zeros(len(qw))
-                dvH2 = np.zeros(len(qw))
+                dk = [x[qw + k] - x[i] for k in range(3)]
+                r2h = [(v ** 2).sum(1) for v in dk]
+                rk = [np.sqrt(1.0 / r) for r in r2h]
+                r6hc = [r ** 3 for r in r2h]
+                dv = [np.zeros(len(qw)) for _ in range(3)]
                 for s in range(S):
-                    if qv:
-                        qt = f["qiac"][q, s]
-                        aLJ, bLJ = lj_ab(f["qavdw"][qt, 0], f["qbvdw"][qt, 0], A[ow_t, 0], B[ow_t, 0], rule)
-                        r6sc = r6hc + self.sc[q, ow_t, s]
-                        va = aLJ / r6sc ** 2
-                        vb = bLJ / r6sc
-                    else:
-                        r6sc = r6hc
-                        aLJ, bLJ = lj_ab(A[iac[i], 0], B[iac[i], 0], A[ow_t, 0], B[ow_t, 0], rule)
-                        va = aLJ / r6hc ** 2
-                        vb = bLJ / r6hc
-                    qq = float(F32(self.crg_ow * self.qcrg[q, s]))
-                    qh = float(F32(self.crg_hw * self.qcrg[q, s]))
-                    vO, v1, v2 = qq * rO, qh * rH1, qh * rH2
-                    dvO += r2O * (-vO - (12. * va - 6. * vb) * (r6hc / r6sc)) * lam[s]
-                    dvH1 -= rH1 * rH1 * v1 * lam[s]
-                    dvH2 -= rH2 * rH2 * v2 * lam[s]
-                    EQ[s, 12] += (vO + v1 + v2).sum()
-                    EQ[s, 13] += (va - vb).sum()
-                d[i] -= (dvO[:, None] * dO + dvH1[:, None] * d1 + dvH2[:, None] * d2).sum(0)
-                np.add.at(d, qw, dvO[:, None] * dO)
-                np.add.at(d, qw + 1, dvH1[:, None] * d1)
-                np.add.at(d, qw + 2, dvH2[:, None] * d2)
+                    for k in range(3):
+                        qq = float(F32(self.crg_w[k] * self.qcrg[q, s]))
+                        vel = qq * rk[k]
+                        va = vb = 0.0
+                        r6sc = r6hc[k]
+                        if k < nlj:
+                            if qv:
+                                qt = f["qiac"][q, s]
+                                aLJ, bLJ = lj_ab(f["qavdw"][qt, 0], f["qbvdw"][qt, 0], A[wt[k], 0], B[wt[k], 0], rule)
+                                r6sc = r6hc[k] + self.sc[q, wt[k], s]
+                            else:
+                                aLJ, bLJ = lj_ab(A[iac[i], 0], B[iac[i], 0], A[wt[k], 0], B[wt[k], 0], rule)
+                            va = aLJ / r6sc ** 2
+                            vb = bLJ / r6sc
+                        dv[k] += (1.0 / r2h[k]) * (-vel - (12. * va - 6. * vb) * (r6hc[k] / r6sc)) * lam[s]
+                        EQ[s, 12] += vel.sum()
+                        EQ[s, 13] += np.sum(va - vb)
+                d[i] -= sum(dv[k][:, None] * dk[k] for k in range(3)).sum(0)
+                for k in range(3):
+                    np.add.at(d, qw + k, dv[k][:, None] * dk[k])
         # q-q and q-connected pairs (nbqqlist + nonbond_qq)
         qc = self.qconn
         nq = f["nq"]

@@ -61,6 +61,13 @@ struct WinDev {
   float el14, crg_ow, crg_hw, aO1, bO1;   // aO1, bO1: LJ columns of the water oxygen type
   int arith;                        // arithmetic LJ combination rule (topology vdW rule 2)
   float ww_a, ww_b;                 // A, B of the water O-O pair
+  // 3-atom solvent (topology solvent type 1): LJ on all three atoms and the
+  // charges of the first solvent molecule; crg_w is (crg_ow, crg_hw, crg_hw)
+  // for SPC
+  int wat3;
+  int iac_w[3];                     // 0-based types of the solvent atoms
+  float crg_w[3], aw[3], bw[3];     // charges and LJ columns of the solvent atoms
+  float ww3_a[9], ww3_b[9];         // A, B of solvent atom pairs (p, q) at [3p + q]
   double dt, tau_t, temp0;
   double rcpp2, rcpw2, rcww2, rclrf2;
   double fk_pshell, fk_wsphere, fkwpol, dwmz, awmz, rwat;
@@ -169,6 +176,10 @@ struct Dev {
   const int* qc_atom;
   const float4* qcw_par;
   const float* qcw_sc;
+  // 3-atom solvent: per member, state and hydrogen k = 1, 2 of the water:
+  // A, B, charge product and soft-core term ([(m * kMaxS + s) * 2 + k - 1])
+  const float4* qcw_h;
+  int any_wat3;          // some window has a 3-atom solvent
   // q-atom share of the Q tiles, written per tile by the non-q side into
   // slot ent_slot[entry] (8 members x 3 int64); a q-cluster's slots are
   // contiguous, qg_start[c] .. qg_start[c+1]-1, so k_q_gather reads runs

@@ -245,6 +245,26 @@ static int build_device(qmi_engine* e) {
     D.ndegfree_solute = W.ndegfree_solute; D.ndegf_solvent = W.ndegf_solvent; D.ndegfree_solvent = W.ndegfree_solvent;
     D.el14 = (float)t.el14_scale; D.crg_ow = W.crg_ow; D.crg_hw = W.crg_hw;
     D.arith = t.ivdw_rule == 2 ? 1 : 0;
+    D.wat3 = (W.nwat > 0 && t.solvent_type == 1) ? 1 : 0;
+    for (int k = 0; k < 3; ++k) {
+      D.iac_w[k] = W.nwat > 0 ? t.iac[NS + k] - 1 : 0;
+      D.crg_w[k] = W.nwat > 0 ? (D.wat3 ? t.crg[NS + k] : (k == 0 ? W.crg_ow : W.crg_hw)) : 0.f;
+      D.aw[k] = W.nwat > 0 ? (float)t.iaclib[t.iac[NS + k]].avdw[0] : 0.f;
+      D.bw[k] = W.nwat > 0 ? (float)t.iaclib[t.iac[NS + k]].bvdw[0] : 0.f;
+    }
+    if (D.wat3) {
+      // A, B of the nine solvent atom pairs (nonbond_3atomsolvent,
+      // md.f90:12331-12362), formed in FP64
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) {
+          const AtomType& tp = t.iaclib[t.iac[NS + p]];
+          const AtomType& tq = t.iaclib[t.iac[NS + q]];
+          double A, B;
+          lj_ab(tp.avdw[0], tp.bvdw[0], tq.avdw[0], tq.bvdw[0], D.arith != 0, &A, &B);
+          D.ww3_a[3 * p + q] = (float)A;
+          D.ww3_b[3 * p + q] = (float)B;
+        }
+    }
     if (W.nwat > 0) {
       const AtomType& o = t.iaclib[t.iac[NS]];
       D.aO1 = (float)o.avdw[0];
@@ -262,7 +282,7 @@ static int build_device(qmi_engine* e) {
       // no LJ on the water hydrogens: zero A and B under either rule
       const bool h_lj = D.arith ? (h.bvdw[0] != 0.0 || h.bvdw[2] != 0.0)
                                 : (h.avdw[0] != 0.0 || h.bvdw[0] != 0.0 || h.avdw[2] != 0.0);
-      if (h_lj) return set_err(-10, "SPC water hydrogens must have zero LJ parameters");
+      if (h_lj && !D.wat3) return set_err(-10, "SPC water hydrogens must have zero LJ parameters");
     }
     D.dt = p.dt; D.tau_t = p.tau_t; D.temp0 = p.temp0;
     D.rcpp2 = p.rcpp * p.rcpp; D.rcpw2 = p.rcpw * p.rcpw; D.rcww2 = p.rcww * p.rcww; D.rclrf2 = p.rclrf * p.rclrf;
@@ -819,6 +839,34 @@ static int build_device(qmi_engine* e) {
     UP(pp, qcw_par); d.qcw_par = pp;
     float* fp;
     UP(fp, qcw_sc); d.qcw_sc = fp;
+    // 3-atom solvent: the hydrogens' factors, formed like the oxygen's
+    bool any3 = false;
+    for (int wi = 0; wi < NW; ++wi) any3 = any3 || wd[wi].wat3;
+    if (any3) {
+      std::vector<float4> qcw_h(qc_q.size() * kMaxS * 2, make_float4(0.f, 0.f, 0.f, 0.f));
+      for (int wi = 0; wi < NW; ++wi) {
+        const WinDev& D = wd[wi];
+        if (!D.wat3) continue;
+        for (int m = D.qc_off * kCl; m < (D.qc_off + D.nqc) * kCl; ++m) {
+          const int iq = qc_q[m];
+          if (iq < 0) continue;
+          for (int st = 0; st < D.S; ++st) {
+            const double* qd = &q_ljd[(size_t)(D.qs_off + iq * D.S + st) * 4];
+            const float qcv = q_crg[D.qs_off + iq * D.S + st];
+            for (int k = 1; k < 3; ++k) {
+              const double* hd = &t_lj[(size_t)(D.t_off + D.iac_w[k]) * 4];
+              double A, B;
+              lj_ab(qd[0], qd[1], hd[0], hd[1], D.arith != 0, &A, &B);
+              const float sc = D.qvdw ? q_sc[D.qsc_off + ((size_t)iq * D.natyps + D.iac_w[k]) * D.S + st] : 0.f;
+              qcw_h[((size_t)m * kMaxS + st) * 2 + k - 1] = make_float4((float)A, (float)B, D.crg_w[k] * qcv, sc);
+            }
+          }
+        }
+      }
+      float4* hp;
+      UP(hp, qcw_h); d.qcw_h = hp;
+    }
+    d.any_wat3 = any3 ? 1 : 0;
   }
   float* fp_;
   UP(fp_, q_crg); d.q_crg = fp_;

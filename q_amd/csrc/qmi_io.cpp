@@ -708,8 +708,8 @@ bool load_topology(const std::string& path, Topology* t, Error* err) {
       t->excl[got++] = (ch == 'T' || ch == 't') ? 1 : 0;
     }
   }
-  if (t->solvent_type != 0 && t->nat_pro > t->nat_solute)
-    return fail(err, -10, "only SPC-type 3-site solvent (solvent type 0) is supported");
+  if (t->solvent_type != 0 && t->solvent_type != 1 && t->nat_pro > t->nat_solute)
+    return fail(err, -10, "Topology contains mixed or non-3-atomic solvent. This feature is not implemented yet.");
   if (t->ivdw_rule != 1 && t->ivdw_rule != 2) return fail(err, -10, "unknown vdW combination rule");
   if (t->ivdw_rule == 2) {
     // arithmetic rule: the epsilons enter as their square roots, so that a

@@ -419,10 +419,11 @@ __device__ __forceinline__ int row_chunks(int n) {
 // row a and accumulates the gradient of its i-member only (each pair is
 // evaluated from both sides); the 8 lanes of a row are then reduced and
 // added once.
-// water-water: 9 interactions per molecule pair (nonbond_ww_spc, md.f90:11823-12047)
+// water-water: 9 interactions per molecule pair (nonbond_ww_spc, md.f90:11823-12047;
+// W3, a 3-atom solvent: LJ on all nine, nonbond_3atomsolvent, 12317-12591).
 // The energy sums run only on energy steps: EN selects the form of the
 // loop for the wave's window (uniform over the wave).
-template <bool EN>
+template <bool EN, bool W3>
 __device__ __forceinline__ void ww_impl(const Dev& d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -443,8 +444,8 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
     float4 q = oa >= 0 ? d.x32[oa + k] : make_float4(0, 0, 0, 0);
     pi[k] = make_float3(q.x, q.y, q.z);
   }
-  const float qo = W.crg_ow, qh = W.crg_hw;
-  const float qq[3][3] = {{qo * qo, qo * qh, qo * qh}, {qh * qo, qh * qh, qh * qh}, {qh * qo, qh * qh, qh * qh}};
+  const float q0 = W.crg_w[0], q1 = W.crg_w[1], q2 = W.crg_w[2];
+  const float qq[3][3] = {{q0 * q0, q0 * q1, q0 * q2}, {q1 * q0, q1 * q1, q1 * q2}, {q2 * q0, q2 * q1, q2 * q2}};
   const float A = W.ww_a, B = W.ww_b;
   constexpr bool en = EN;
   float g[9];
@@ -469,9 +470,10 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
         float r2inv = rinv * rinv;
         float vel = qq[p][q] * rinv;
         float dv;
-        if (p == 0 && q == 0) {
+        if (W3 || (p == 0 && q == 0)) {
           float r6 = r2inv * r2inv * r2inv;
-          float va = A * r6 * r6, vb = B * r6;
+          const float a_ = W3 ? W.ww3_a[3 * p + q] : A, b_ = W3 ? W.ww3_b[3 * p + q] : B;
+          float va = a_ * r6 * r6, vb = b_ * r6;
           dv = r2inv * (-vel - 12.f * va + 6.f * vb);
           if (cnt) evdw += (double)(va - vb);
         } else {
@@ -495,18 +497,23 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
   }
 }
 
+// W3: the instance for windows with a 3-atom solvent (launched only when
+// there are any); each instance skips the other windows' rows
+template <bool W3>
 __global__ void __launch_bounds__(256) k_ww(Dev d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   if (wave >= d.rl[T_WW].nicl) return;
-  if (d.ws[d.wc_win[wave]].energy_now) ww_impl<true>(d);
-  else ww_impl<false>(d);
+  const int w = d.wc_win[wave];
+  if ((d.win[w].wat3 != 0) != W3) return;
+  if (d.ws[w].energy_now) ww_impl<true, W3>(d);
+  else ww_impl<false, W3>(d);
 }
 
-// solute atom - water: LJ on O, Coulomb on all three sites (nonbond_pw,
-// md.f90:10631-10682).  WATER_SIDE: rows are waters and entries solute atoms
+// solute atom - water: LJ on O (on all three sites for a 3-atom solvent),
+// Coulomb on all three sites (nonbond_pw, md.f90:10631-10682).  WATER_SIDE: rows are waters and entries solute atoms
 // (W<-P); otherwise rows are solute atoms and entries water O atoms (P<-W),
 // which also counts the energies.
-template <bool WATER_SIDE, bool EN>
+template <bool WATER_SIDE, bool EN, bool W3>
 __device__ __forceinline__ void pw_impl(const Dev& d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -532,6 +539,7 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
   }
   constexpr bool en = EN && !WATER_SIDE;
   const bool arith = W.arith;
+  constexpr bool wat3 = W3;
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   double eel = 0.0, evdw = 0.0;
@@ -577,10 +585,10 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
         float r2inv = rinv * rinv;
         float vel = (ps.w * pw[k].w) * rinv;
         float dv, ev = 0.f;
-        if (k == 0) {
+        if (k == 0 || wat3) {
           float r6 = r2inv * r2inv * r2inv;
           float A, B;
-          lj_ab(lj.x, lj.y, W.aO1, W.bO1, arith, &A, &B);
+          lj_ab(lj.x, lj.y, k == 0 ? W.aO1 : W.aw[k], k == 0 ? W.bO1 : W.bw[k], arith, &A, &B);
           float va = A * r6 * r6, vb = B * r6;
           dv = r2inv * (-vel - 12.f * va + 6.f * vb);
           ev = va - vb;
@@ -588,7 +596,7 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
           dv = r2inv * (-vel);
         }
         if (!v[u]) continue;
-        if (en) { eel += (double)vel; if (k == 0) evdw += (double)ev; }
+        if (en) { eel += (double)vel; if (k == 0 || wat3) evdw += (double)ev; }
         if (WATER_SIDE) {
           g[3 * k + 0] += dv * dx; g[3 * k + 1] += dv * dy; g[3 * k + 2] += dv * dz;
         } else {
@@ -609,13 +617,14 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
   }
 }
 
-template <bool WATER_SIDE>
+template <bool WATER_SIDE, bool W3>
 __global__ void __launch_bounds__(256) k_pw(Dev d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   if (wave >= d.rl[WATER_SIDE ? T_WP : T_PW].nicl) return;
   const int w = WATER_SIDE ? d.wc_win[wave] : d.sc_win[wave];
-  if (!WATER_SIDE && d.ws[w].energy_now) pw_impl<WATER_SIDE, true>(d);
-  else pw_impl<WATER_SIDE, false>(d);
+  if ((d.win[w].wat3 != 0) != W3) return;
+  if (!WATER_SIDE && d.ws[w].energy_now) pw_impl<WATER_SIDE, true, W3>(d);
+  else pw_impl<WATER_SIDE, false, W3>(d);
 }
 
 // solute-solute (nonbond_pp, md.f90:10387-10491): 1-4 pairs use the third
@@ -717,25 +726,32 @@ __device__ void qw_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int 
   }
   df rinv[3];
   for (int k = 0; k < 3; ++k) rinv[k] = df_rsqrt(r2[k]);
-  const df r6 = df_mul(df_mul(r2[0], r2[0]), r2[0]);
-  const double* tl = &d.t_lj[(W.t_off + W.iac_ow) * 4];
+  // LJ on the oxygen, or on all three atoms of a 3-atom solvent
+  // (nonbond_qw_3atom, md.f90:11545-11670)
+  const int nlj = W.wat3 ? 3 : 1;
+  df r6[3];
+  for (int k = 0; k < nlj; ++k) r6[k] = df_mul(df_mul(r2[k], r2[k]), r2[k]);
   for (int s = 0; s < S; ++s) {
     const float qc = d.q_crg[W.qs_off + iq * S + s];
     double e = 0.0;
     for (int k = 0; k < 3; ++k) {
-      const float qq = (k == 0 ? W.crg_ow : W.crg_hw) * qc;   // real(4) product as in the reference
+      const float qq = W.crg_w[k] * qc;   // real(4) product as in the reference
       e += df_to(df_mulf(rinv[k], qq));
     }
     el[s] = e;
-    // LJ on the oxygen
     const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
-    double aLJ, bLJ;
-    lj_ab(ql[0], ql[1], tl[0], tl[1], W.arith != 0, &aLJ, &bLJ);
-    const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + W.iac_ow) * S + s] : 0.0;
-    const df inv = df_rcp(df_add(r6, df_from(sc)));
-    const df va = df_mul(df_from(aLJ), df_mul(inv, inv));
-    const df vb = df_mul(df_from(bLJ), inv);
-    vdw[s] = df_to(df_add(va, df_neg(vb)));
+    double ev = 0.0;
+    for (int k = 0; k < nlj; ++k) {
+      const double* tl = &d.t_lj[(W.t_off + W.iac_w[k]) * 4];
+      double aLJ, bLJ;
+      lj_ab(ql[0], ql[1], tl[0], tl[1], W.arith != 0, &aLJ, &bLJ);
+      const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + W.iac_w[k]) * S + s] : 0.0;
+      const df inv = df_rcp(df_add(r6[k], df_from(sc)));
+      const df va = df_mul(df_from(aLJ), df_mul(inv, inv));
+      const df vb = df_mul(df_from(bLJ), inv);
+      ev += df_to(df_add(va, df_neg(vb)));
+    }
+    vdw[s] = ev;
   }
 }
 
@@ -832,7 +848,7 @@ __global__ void __launch_bounds__(256) k_q_gather(Dev d) {
 }
 
 // q-atom vs water (nonbond_qw_spc, md.f90:11302-11411)
-template <bool Q_SIDE>
+template <bool Q_SIDE, bool W3>
 __global__ void __launch_bounds__(256) k_qw(Dev d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -855,6 +871,10 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
     ow = d.wc_o[ic * kCl + a];
   }
   const float aO = W.aO1, bO = W.bO1, qow = W.crg_ow, qhw = W.crg_hw;
+  // the water side has an instance for 3-atom solvent windows (W3); the q
+  // side's energy pass handles both
+  constexpr bool wat3 = W3;
+  if (!Q_SIDE && (W.wat3 != 0) != W3) return;
   float lam[kMaxS];
   for (int s = 0; s < kMaxS; ++s) lam[s] = s < S ? (float)W.lambda[s] : 0.f;
   // per-state parameters of the lane's q-atom (fixed on the q side)
@@ -945,8 +965,32 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
         }
         const float velO = P.z * rO, velH1 = P.w * rH1, velH2 = P.w * rH2;
         dvO += r2O * (-velO - (12.f * va - 6.f * vb) * ratio) * lam[s];
-        dvH1 -= r2H1 * velH1 * lam[s];
-        dvH2 -= r2H2 * velH2 * lam[s];
+        if (!Q_SIDE && wat3) {
+          // 3-atom solvent: LJ and soft-core on the hydrogens too, and their
+          // own charges (nonbond_qw_3atom, md.f90:11620-11640)
+          for (int k = 1; k < 3; ++k) {
+            const float4 H = d.qcw_h[((size_t)jx[u] * kMaxS + s) * 2 + k - 1];
+            const float r2k = r2v[k], r2ik = k == 1 ? r2H1 : r2H2, rk = k == 1 ? rH1 : rH2;
+            const float r6k = r2k * r2k * r2k;
+            float vak, vbk, ratk = 1.f;
+            if (qvdw) {
+              const float inv = rcp(r6k + H.w);
+              vak = H.x * inv * inv;
+              vbk = H.y * inv;
+              ratk = r6k * inv;
+            } else {
+              const float r6i = r2ik * r2ik * r2ik;
+              vak = H.x * r6i * r6i;
+              vbk = H.y * r6i;
+            }
+            const float dvk = r2ik * (-(H.z * rk) - (12.f * vak - 6.f * vbk) * ratk) * lam[s];
+            if (k == 1) dvH1 += dvk;
+            else dvH2 += dvk;
+          }
+        } else {
+          dvH1 -= r2H1 * velH1 * lam[s];
+          dvH2 -= r2H2 * velH2 * lam[s];
+        }
       }
       if (Q_SIDE) {
         g[0] -= dvO * dxv[0] + dvH1 * dxv[1] + dvH2 * dxv[2];
@@ -2353,21 +2397,29 @@ void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
 // q_energy false: the q-side energy passes are left out (a step where no
 // window computes energies)
 void launch_q_tiles(const Dev& d, hipStream_t s, bool q_energy) {
-  if (q_energy && d.tl[T_QW].nwork) k_qw<true><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
-  if (d.tl[T_WQ].nwork) k_qw<false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
+  if (q_energy && d.tl[T_QW].nwork) k_qw<true, false><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_WQ].nwork) k_qw<false, false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_WQ].nwork && d.any_wat3)
+    k_qw<false, true><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (q_energy && d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PQ].nwork) k_qp<false><<<blocks((long long)d.tl[T_PQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.nqc_all) k_q_gather<<<blocks((long long)d.nqc_all * kQGatherParts * 64, 256), 256, 0, s>>>(d);
 }
 
 void launch_ww(const Dev& d, hipStream_t s) {
-  if (d.rl[T_WW].nicl) k_ww<<<blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_WW].nicl) k_ww<false><<<blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_WW].nicl && d.any_wat3)
+    k_ww<true><<<blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
 }
 
 void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) {
   if (with_ww) launch_ww(d, s);
-  if (d.rl[T_WP].nicl) k_pw<true><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
-  if (d.rl[T_PW].nicl) k_pw<false><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_WP].nicl) k_pw<true, false><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_PW].nicl) k_pw<false, false><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_WP].nicl && d.any_wat3)
+    k_pw<true, true><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_PW].nicl && d.any_wat3)
+    k_pw<false, true><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
   if (d.rl[T_PP].nicl) k_pp<<<blocks((long long)d.rl[T_PP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
   launch_q_tiles(d, s, q_energy);
   if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);

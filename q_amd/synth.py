@@ -37,6 +37,10 @@ LJ_TYPES = [
 T_CT, T_CA, T_NT, T_OS, T_SH, T_HC, T_HN, T_OW, T_HW, T_DU = range(1, 11)
 HEAVY_TYPES = [T_CT, T_CA, T_NT, T_OS, T_SH]
 SPC_QO = -0.82
+# a TIP3P-like 3-site water with Lennard-Jones on the hydrogens (topology
+# solvent type 1, "3-atom"), SPC geometry; its two types follow LJ_TYPES
+TIP3_TYPES = [("OT", 3.1507, 0.1521, 15.9994), ("HT", 0.4000, 0.0460, 1.008)]
+TIP3_QO = -0.834
 SPC_ROH = 1.0
 SPC_HOH = 109.47
 
@@ -99,6 +103,7 @@ class System:
     vdw_rule: int = 1                              # 1 geometric, 2 arithmetic LJ combination
     sc_max_potential: bool = False                 # [softcore] values are maximum potentials, not alphas
     offdiag: list = field(default_factory=list)    # EVB (state i, state j, q-atom k, q-atom l, A, mu)
+    water: str = "spc"                             # "spc" (solvent type 0) or "tip3" (3-atom, type 1)
     # FEP description (q-atoms are topology atoms, 1-based)
     qatoms: list = field(default_factory=list)
     qcharges: list = field(default_factory=list)   # per q-atom (state1, state2)
@@ -349,8 +354,13 @@ def _topology_from_graph(nb: list, offset: int, heavy: list):
 
 def _add_water_lattice(rng, sysm: System, radius: float, grid: _Grid | None, spacing: float = 3.1,
                        heavy_grid: _Grid | None = None):
-    """SPC waters on a cubic lattice inside ``radius``, random orientations,
-    skipping lattice points that clash with solute atoms."""
+    """Waters (SPC, or the TIP3P-like model when ``sysm.water`` is "tip3")
+    on a cubic lattice inside ``radius``, random orientations, skipping
+    lattice points that clash with solute atoms."""
+    if sysm.water == "tip3":
+        t_o, t_h, q_o = len(LJ_TYPES) + 1, len(LJ_TYPES) + 2, TIP3_QO
+    else:
+        t_o, t_h, q_o = T_OW, T_HW, SPC_QO
     n = int(math.ceil(radius / spacing)) + 1
     pts = []
     for i in range(-n, n + 1):
@@ -362,7 +372,7 @@ def _add_water_lattice(rng, sysm: System, radius: float, grid: _Grid | None, spa
     half = math.radians(SPC_HOH / 2)
     h1 = np.array([SPC_ROH * math.sin(half), SPC_ROH * math.cos(half), 0.0])
     h2 = np.array([-SPC_ROH * math.sin(half), SPC_ROH * math.cos(half), 0.0])
-    qh = -SPC_QO / 2
+    qh = -q_o / 2
     for p in pts:
         if heavy_grid is not None:
             if heavy_grid.min_dist(p) < 2.7 or grid.min_dist(p) < 2.0:
@@ -372,8 +382,8 @@ def _add_water_lattice(rng, sysm: System, radius: float, grid: _Grid | None, spa
         rot = _rand_rot(rng)
         base = sysm.natom + 1
         sysm.x += [p, p + rot @ h1, p + rot @ h2]
-        sysm.iac += [T_OW, T_HW, T_HW]
-        sysm.crg += [SPC_QO, qh, qh]
+        sysm.iac += [t_o, t_h, t_h]
+        sysm.crg += [q_o, qh, qh]
         sysm.bonds += [(base, base + 1, B_OW_H), (base, base + 2, B_OW_H), (base + 1, base + 2, B_HW_HW)]
         sysm.angles += [(base + 1, base, base + 2, A_WAT)]
         sysm.cgps.append((base, [base, base + 1, base + 2]))
@@ -507,10 +517,10 @@ def water_droplet(radius: float = 15.0, seed: int = 0) -> System:
 
 def ligand_in_water(radius: float = 20.0, seed: int = 0, n_heavy: int = 30, n_h: int = 20,
                     n_pert: int = 6, alpha: float = 10.0, vdw_rule: int = 1,
-                    sc_max_potential: bool = False, offdiag: tuple = ()) -> System:
+                    sc_max_potential: bool = False, offdiag: tuple = (), water: str = "spc") -> System:
     rng = np.random.default_rng(seed)
     s = System(name="ligwat", radius=radius, vdw_rule=vdw_rule, sc_max_potential=sc_max_potential,
-               offdiag=list(offdiag))
+               offdiag=list(offdiag), water=water)
     grid = _Grid()
     xyz, types, nb, heavy = _grow_ligand(rng, n_heavy, n_h, np.zeros(3), grid, 1)
     # re-centre the ligand on the sphere centre
@@ -838,13 +848,14 @@ def write_top(s: System, path: str) -> None:
     for (sw, members) in s.cgps:
         w(f"{len(members):6d}{sw:6d}")
         w("".join(f"{m:6d}" for m in members))
-    nty = len(LJ_TYPES)
+    types = LJ_TYPES + (TIP3_TYPES if s.water == "tip3" else [])
+    nty = len(types)
     w(f"{nty:8d} = No. of atom types")
     w(f"{s.vdw_rule:8d} = vdW combination rule (1 = Geom. / 2 = Arit.)")
     w(" 0.50000 332.0716 = Electrostatic 1-4 scaling factor and  Coulomb constant")
     w("Masses:")
-    w("".join(f"{t[3]:9.4f}" for t in LJ_TYPES))
-    cols = [lj_columns(t[1], t[2], s.vdw_rule) for t in LJ_TYPES]
+    w("".join(f"{t[3]:9.4f}" for t in types))
+    cols = [lj_columns(t[1], t[2], s.vdw_rule) for t in types]
     la, lb = ("sqrt (Aii)", "sqrt (Bii)") if s.vdw_rule == 1 else ("R*ii", "epsilon ii")
     for k, lab in enumerate(("normal", "polar", "1-4")):
         w(f"{la} {lab}:")
@@ -878,13 +889,13 @@ def write_top(s: System, path: str) -> None:
     for k in range(0, len(s.molecules), 13):
         w("".join(f"{v:6d}" for v in s.molecules[k:k + 13]))
     w(f"{nty:8d} = No. of atom types:")
-    tn = [t[0] for t in LJ_TYPES]
+    tn = [t[0] for t in types]
     for k in range(0, nty, 8):
         w("".join(f"{v:<8s} " for v in tn[k:k + 8]))
     w(f"{nty:8d} = No. of SYBYL atom types:")
     for k in range(0, nty, 13):
         w("".join(f"{'C.3':<5s} " for _v in tn[k:k + 13]))
-    w("       0 = solvent type (0=SPC,1=3-atom,2=general)")
+    w(f"{1 if s.water == 'tip3' else 0:8d} = solvent type (0=SPC,1=3-atom,2=general)")
     w(f"{s.radius:8.3f}{s.radius:8.3f} = Exclusion, solvent radii")
     w("   0.000   0.000   0.000 = Solute centre")
     w("   0.000   0.000   0.000 = Solvent centre")
@@ -1006,6 +1017,9 @@ CONFIGS = {
     # either LJ combination rule
     "ligmaxpot": (ligand_in_water, {"radius": 20.0, "sc_max_potential": True}, {"lrf": True}),
     "ligarithmax": (ligand_in_water, {"radius": 20.0, "vdw_rule": 2, "sc_max_potential": True}, {"lrf": True}),
+    # 3-atom solvent with LJ on the hydrogens (nonbond_3atomsolvent,
+    # nonbond_qw_3atom, md.f90:12317-12591, 11545-11670)
+    "ligtip3": (ligand_in_water, {"radius": 20.0, "water": "tip3"}, {"lrf": True}),
     # EVB off-diagonal functions Hij = A exp(-mu r_kl) (md.f90:12911-12937)
     "ligevb": (ligand_in_water, {"radius": 20.0, "offdiag": ((1, 2, 1, 5, 25.0, 0.5), (2, 1, 3, 7, 10.0, 0.2))},
                {"lrf": True}),

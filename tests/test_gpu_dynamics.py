@@ -34,7 +34,7 @@ def read_en(path, nstates):
     return out
 
 
-@pytest.mark.parametrize("cfg,seed", [("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2)])
+@pytest.mark.parametrize("cfg,seed", [("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2), ("ligtip3", 0)])
 def test_100_steps(golden_inp, cfg, seed):
     from q_amd.engine import Engine
     g = dict(np.load(os.path.join(GOLD, f"{cfg}_s{seed}.npz")))
@@ -95,9 +95,9 @@ def test_evb_offdiagonal_records(golden_inp):
 
 def test_batched_equals_single(systems):
     from q_amd.engine import Engine
-    # the two LJ combination rules in one batch
+    # the two LJ combination rules and both solvent models in one batch
     inps = [systems("ligwat", seed=0, steps=40, lam=0.25), systems("protein", seed=2, steps=40, lam=0.75),
-            systems("protarith", seed=2, steps=40, lam=0.5)]
+            systems("protarith", seed=2, steps=40, lam=0.5), systems("ligtip3", seed=0, steps=40, lam=0.5)]
     eb = Engine(0)
     for p in inps:
         eb.add_window(p)

@@ -21,8 +21,10 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # ligand in water, one state, sequence restraint) and the protein leg under
 # the arithmetic LJ combination rule (nonbon2_*, md.f90:8972-10227), and
 # the ligand in water with soft-core maximum potentials under either rule
-# (sc_lookup, qatom.f90:1600-1664)
-CASES = {"water": 0, "ligwat": 0, "protein": 2, "kat": 0, "protarith": 2, "ligmaxpot": 0, "ligarithmax": 0}
+# (sc_lookup, qatom.f90:1600-1664), and the ligand in a 3-atom water with LJ
+# on the hydrogens (nonbond_3atomsolvent, nonbond_qw_3atom)
+CASES = {"water": 0, "ligwat": 0, "protein": 2, "kat": 0, "protarith": 2, "ligmaxpot": 0, "ligarithmax": 0,
+         "ligtip3": 0}
 
 
 def _rel(a, b, floor=1.0):
@@ -58,10 +60,16 @@ def test_step0_energies(evaluated, cfg):
               "restraint.total", "restraint.fix", "restraint.shell", "restraint.solvent_radial",
               "restraint.water_pol", "potential"]:
         print(f"  {k:18s} gpu {E[k]: .10f} ref {ref[k]: .10f} rel {_rel(E[k], ref[k]):.2e}")
-    # bulk nonbonded sums: FP32 pair arithmetic, error budget 2e-6 relative
-    # plus 2e-2 kcal/mol absolute (about 6e-8 per term over ~1e5 terms)
+    # bulk nonbonded sums: FP32 pair arithmetic (FP32 coordinates, hardware
+    # rsqrt).  Budget: 2e-6 relative plus 1e-7 kcal/mol per pair term, at
+    # least 2e-2 (the terms are up to ~100 kcal/mol with ~3e-7 relative
+    # error each and add up like a random walk: an FP32 emulation of the
+    # water-water Coulomb sum on the CPU is off by 0.007 (SPC) and 0.032
+    # (the 3-atom water) kcal/mol over 5e5 terms)
+    g = dump["gold"]
+    nterm = {"ww": 9 * int(g["n_ww"]), "pw": int(g["n_pw"]), "pp": int(g["n_pp"])}
     for k in ["ww.el", "ww.vdw", "pw.el", "pw.vdw", "pp.el", "pp.vdw"]:
-        assert abs(E[k] - ref[k]) < 2e-6 * abs(ref[k]) + 2e-2, k
+        assert abs(E[k] - ref[k]) < 2e-6 * abs(ref[k]) + max(2e-2, 1e-7 * nterm[k[:2]]), k
     # FP64 terms
     for k in ["LRF", "p.bond", "p.angle", "p.torsion", "p.improper", "q.bond", "q.angle", "q.torsion", "q.improper",
               "restraint.total", "restraint.solvent_radial", "restraint.water_pol", "restraint.shell",
@@ -69,7 +77,7 @@ def test_step0_energies(evaluated, cfg):
         assert _rel(E[k], ref[k]) < 1e-6, k
 
 
-@pytest.mark.parametrize("cfg", ["ligwat", "protein", "kat", "protarith", "ligmaxpot", "ligarithmax"])
+@pytest.mark.parametrize("cfg", ["ligwat", "protein", "kat", "protarith", "ligmaxpot", "ligarithmax", "ligtip3"])
 def test_step0_q_energies(evaluated, cfg):
     e, w, dump = evaluated[cfg]
     EQ = e.q_energies(w)

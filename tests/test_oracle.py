@@ -1,8 +1,8 @@
 """The CPU oracle (oracle/qdyn_np.py) pinned three ways, on the CPU:
 
 1. against the golden vectors of tests/golden/ (the reference Q6 routines,
-   step-0 energies, gradient and pair lists of four seeded windows, one of
-   them under the arithmetic LJ combination rule);
+   step-0 energies, gradient and pair lists of five seeded windows, one of
+   them under the arithmetic LJ combination rule, one with a 3-atom solvent);
 2. against closed forms: the LRF Taylor expansion against direct Coulomb
    sums, a single water pair against Coulomb + Lennard-Jones by hand;
 3. its gradient against central finite differences of its own energy
@@ -18,9 +18,10 @@ from oracle import qdyn_np as O
 from tests.golden.make_golden import E_NAMES, canonical_lists, file_sha, input_files, sha
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-# four seeded synthetic windows (the last under the arithmetic LJ rule) and
-# the reference's own test system
-CASES = [("water", 0), ("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2)]
+# five seeded synthetic windows (the protein leg under the arithmetic LJ
+# rule, the ligand in a 3-atom water with LJ on the hydrogens) and the
+# reference's own test system
+CASES = [("water", 0), ("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2), ("ligtip3", 0)]
 
 
 def load_gold(cfg, seed):
@@ -138,7 +139,8 @@ def _energy(w, x, tfree):
     return E["potential"], d
 
 
-@pytest.mark.parametrize("cfg,seed,npick", [("ligwat", 0, 4), ("protein", 2, 2), ("kat", 0, 3), ("protarith", 2, 2)])
+@pytest.mark.parametrize("cfg,seed,npick", [("ligwat", 0, 4), ("protein", 2, 2), ("kat", 0, 3), ("protarith", 2, 2),
+                                                ("ligtip3", 0, 3)])
 def test_gradient_is_derivative_of_energy(golden_inp, cfg, seed, npick):
     """Central differences of the potential (lists frozen, LRF off: the LRF
     force is a Taylor field, not the derivative of its energy) against the
