@@ -133,78 +133,6 @@ __device__ __forceinline__ double r2_exact(const double* x, int i, int j) {
   return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
 }
 
-// ---------------------------------------------------- double-float (df32)
-// Error-free transformations (Dekker/Knuth two-sum, fma two-product) and the
-// double-float add/mul built on them (Bailey's double-double scheme carried
-// out in FP32), with Newton reciprocals: about 44 significant bits.
-struct df { float h, l; };
-__device__ __forceinline__ df df_q2s(float a, float b) {
-#pragma clang fp contract(off)
- float s = a + b; return {s, b - (s - a)}; }
-__device__ __forceinline__ df df_2s(float a, float b) {
-#pragma clang fp contract(off)
-
-  float s = a + b; float bb = s - a; return {s, (a - (s - bb)) + (b - bb)};
-}
-__device__ __forceinline__ df df_2p(float a, float b) {
-#pragma clang fp contract(off)
- float p = a * b; return {p, fmaf(a, b, -p)}; }
-__device__ __forceinline__ df df_from(double x) {
-#pragma clang fp contract(off)
- float h = (float)x; return {h, (float)(x - (double)h)}; }
-__device__ __forceinline__ double df_to(df a) {
-#pragma clang fp contract(off)
- return (double)a.h + (double)a.l; }
-__device__ __forceinline__ df df_add(df a, df b) {
-#pragma clang fp contract(off)
-
-  df s = df_2s(a.h, b.h);
-  df t = df_2s(a.l, b.l);
-  s.l += t.h;
-  s = df_q2s(s.h, s.l);
-  s.l += t.l;
-  return df_q2s(s.h, s.l);
-}
-__device__ __forceinline__ df df_neg(df a) {
-#pragma clang fp contract(off)
- return {-a.h, -a.l}; }
-__device__ __forceinline__ df df_mul(df a, df b) {
-#pragma clang fp contract(off)
-
-  df p = df_2p(a.h, b.h);
-  p.l += a.h * b.l + a.l * b.h;
-  return df_q2s(p.h, p.l);
-}
-__device__ __forceinline__ df df_mulf(df a, float b) {
-#pragma clang fp contract(off)
-
-  df p = df_2p(a.h, b);
-  p.l += a.l * b;
-  return df_q2s(p.h, p.l);
-}
-// 1/sqrt(a) and 1/a for a > 0: the hardware FP32 estimate y (relative error
-// about 2^-22) and one Newton step carried in df32, y + y (1 - a y^2) / 2 and
-// y + y (1 - a y); 1 - a y^2 and 1 - a y are near 0, so their leading part
-// is an exact FP32 difference, and the result keeps about 44 bits.
-__device__ __forceinline__ df df_rsqrt(df a) {
-#pragma clang fp contract(off)
-  const float y = __builtin_amdgcn_rsqf(a.h);
-  const df t = df_mul(a, df_2p(y, y));
-  const float e = (1.0f - t.h) - t.l;
-  return df_q2s(y, y * (0.5f * e));
-}
-__device__ __forceinline__ df df_rcp(df a) {
-#pragma clang fp contract(off)
-  const float y = __builtin_amdgcn_rcpf(a.h);
-  const df t = df_mulf(a, y);
-  const float e = (1.0f - t.h) - t.l;
-  return df_q2s(y, y * e);
-}
-// difference of two FP64 coordinates as a df
-__device__ __forceinline__ df df_dx(double xj, double xi) {
-#pragma clang fp contract(off)
- return df_from(xj - xi); }
-
 // ------------------------------------------------------ step control --
 // Step start, one thread per window (flags, energy accumulators) and per
 // atom (gradient accumulators of running windows start at zero).  The two
@@ -712,78 +640,20 @@ __global__ void __launch_bounds__(256) k_pp(Dev d) {
 // q-atom vs solute atom (nonbond_qp_qvdw / nonbond_qp, md.f90:11138-11204,
 // 10980-11045), all states; gradients are lambda-weighted over states.
 
-// df32 energies of one q-atom / water pair in every state (energy steps
-// only); the geometry is shared by the states
-__device__ void qw_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int ow, double* el, double* vdw) {
-  const double* x = d.x;
-  const int S = W.S;
-  df r2[3];
-  for (int k = 0; k < 3; ++k) {
-    df dx = df_dx(x[3 * (ow + k)], x[3 * qa]);
-    df dy = df_dx(x[3 * (ow + k) + 1], x[3 * qa + 1]);
-    df dz = df_dx(x[3 * (ow + k) + 2], x[3 * qa + 2]);
-    r2[k] = df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz));
-  }
-  df rinv[3];
-  for (int k = 0; k < 3; ++k) rinv[k] = df_rsqrt(r2[k]);
-  // LJ on the oxygen, or on all three atoms of a 3-atom solvent
-  // (nonbond_qw_3atom, md.f90:11545-11670)
-  const int nlj = W.wat3 ? 3 : 1;
-  df r6[3];
-  for (int k = 0; k < nlj; ++k) r6[k] = df_mul(df_mul(r2[k], r2[k]), r2[k]);
-  for (int s = 0; s < S; ++s) {
-    const float qc = d.q_crg[W.qs_off + iq * S + s];
-    double e = 0.0;
-    for (int k = 0; k < 3; ++k) {
-      const float qq = W.crg_w[k] * qc;   // real(4) product as in the reference
-      e += df_to(df_mulf(rinv[k], qq));
-    }
-    el[s] = e;
-    const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
-    double ev = 0.0;
-    for (int k = 0; k < nlj; ++k) {
-      const double* tl = &d.t_lj[(W.t_off + W.iac_w[k]) * 4];
-      double aLJ, bLJ;
-      lj_ab(ql[0], ql[1], tl[0], tl[1], W.arith != 0, &aLJ, &bLJ);
-      const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + W.iac_w[k]) * S + s] : 0.0;
-      const df inv = df_rcp(df_add(r6[k], df_from(sc)));
-      const df va = df_mul(df_from(aLJ), df_mul(inv, inv));
-      const df vb = df_mul(df_from(bLJ), inv);
-      ev += df_to(df_add(va, df_neg(vb)));
-    }
-    vdw[s] = ev;
-  }
+// 1/sqrt(a) and 1/a in FP64: the hardware estimate and two Newton steps
+// (each squares the relative error, so the result is within a few ulp)
+__device__ __forceinline__ double rsqrt_d(double r2) {
+  const double h = 0.5 * r2;
+  double y = __builtin_amdgcn_rsq(r2);
+  y = fma(y, fma(-(h * y), y, 0.5), y);
+  y = fma(y, fma(-(h * y), y, 0.5), y);
+  return y;
 }
-
-// df32 energies of one q-atom / solute-atom pair in every state
-__device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int j, bool is14, double* el,
-                             double* vdw) {
-  const double* x = d.x;
-  const int S = W.S;
-  const df dx = df_dx(x[3 * j], x[3 * qa]);
-  const df dy = df_dx(x[3 * j + 1], x[3 * qa + 1]);
-  const df dz = df_dx(x[3 * j + 2], x[3 * qa + 2]);
-  const df r2 = df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz));
-  const df rinv = df_rsqrt(r2);
-  const df r6 = df_mul(df_mul(r2, r2), r2);
-  const float qj = d.x32[j].w;
-  const int iacj = d.iac[j];
-  const double* tl = &d.t_lj[(W.t_off + iacj) * 4];
-  const int k = is14 ? 2 : 0;
-  for (int s = 0; s < S; ++s) {
-    const float qq = d.q_crg[W.qs_off + iq * S + s] * qj;
-    df vel = df_mulf(rinv, qq);
-    if (is14) vel = df_mul(vel, df_from((double)W.el14));
-    el[s] = df_to(vel);
-    const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq * S + s) * 4];
-    double aLJ, bLJ;
-    lj_ab(ql[k], ql[k + 1], tl[k], tl[k + 1], W.arith != 0, &aLJ, &bLJ);
-    const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq * W.natyps + iacj) * S + s] : 0.0;
-    const df inv = df_rcp(df_add(r6, df_from(sc)));
-    const df va = df_mul(df_from(aLJ), df_mul(inv, inv));
-    const df vb = df_mul(df_from(bLJ), inv);
-    vdw[s] = df_to(df_add(va, df_neg(vb)));
-  }
+__device__ __forceinline__ double rcp_d(double a) {
+  double y = __builtin_amdgcn_rcp(a);
+  y = fma(y, fma(-a, y, 1.0), y);
+  y = fma(y, fma(-a, y, 1.0), y);
+  return y;
 }
 
 // Q-atom tiles: lane (a, b) of a work item takes member a of the i-cluster
@@ -794,7 +664,7 @@ __device__ void qp_energy_df(const Dev& d, const WinDev& W, int iq, int qa, int 
 // the work-item size; the q-atoms' share of each tile is summed over the
 // tile's rows and stored in the tile's own slots (q_reaction), which
 // k_q_gather adds up per q-atom in entry order.  On energy steps the q side
-// makes its own pass for the per-state energies in df32.
+// makes its own pass for the per-state energies in FP64.
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // q-atom share of kQGroup tiles starting at entry e0 of a non-q-side list:
@@ -871,10 +741,9 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
     ow = d.wc_o[ic * kCl + a];
   }
   const float aO = W.aO1, bO = W.bO1, qow = W.crg_ow, qhw = W.crg_hw;
-  // the water side has an instance for 3-atom solvent windows (W3); the q
-  // side's energy pass handles both
+  // instances for 3-atom solvent windows (W3) and for SPC windows
   constexpr bool wat3 = W3;
-  if (!Q_SIDE && (W.wat3 != 0) != W3) return;
+  if ((W.wat3 != 0) != W3) return;
   float lam[kMaxS];
   for (int s = 0; s < kMaxS; ++s) lam[s] = s < S ? (float)W.lambda[s] : 0.f;
   // per-state parameters of the lane's q-atom (fixed on the q side)
@@ -1015,19 +884,60 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
       for (int p = 0; p < 3; ++p) add_grad_fix(d.f, ow + p, gs[3 * p], gs[3 * p + 1], gs[3 * p + 2]);
     return;
   }
-  // energy pass (df32), q side only
+  // energy pass (FP64), q side only: the lane's q-atom is fixed, so its
+  // per-state charge products and, per LJ site of the water (the oxygen, or
+  // all three atoms of a 3-atom solvent, nonbond_qw_3atom), A, B and the
+  // soft-core term are formed once before the tile loop
+  constexpr int NLJ = W3 ? 3 : 1;
+  float qqk[kMaxS][3];
+  double A[kMaxS][NLJ], B[kMaxS][NLJ], SC[kMaxS][NLJ];
+#pragma unroll
+  for (int s = 0; s < kMaxS; ++s) {
+    const int ss = s < S ? s : 0;
+    const int iqs = iq_l >= 0 ? iq_l : 0;
+    const float qc = d.q_crg[W.qs_off + iqs * S + ss];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) qqk[s][k] = W.crg_w[k] * qc;   // real(4) products as in the reference
+    const double* ql = &d.q_ljd[(size_t)(W.qs_off + iqs * S + ss) * 4];
+#pragma unroll
+    for (int k = 0; k < NLJ; ++k) {
+      const double* tl = &d.t_lj[(W.t_off + W.iac_w[k]) * 4];
+      lj_ab(ql[0], ql[1], tl[0], tl[1], W.arith != 0, &A[s][k], &B[s][k]);
+      SC[s][k] = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iqs * W.natyps + W.iac_w[k]) * S + ss] : 0.0;
+    }
+  }
   long long efix[kMaxS][2] = {};
   double eel[kMaxS] = {}, evdw[kMaxS] = {};
+  const double* x = d.x;
   for (int t = 0; t < n; ++t) {
     const int jc = L.jc[s0 + t];
     const uint2 m = L.mask[s0 + t];
     if (lane_bit(m, lane)) {
       const int o = d.wc_o[jc * kCl + b];
-      double e_el[kMaxS], e_vdw[kMaxS];
-      qw_energy_df(d, W, iq_l, qa, o, e_el, e_vdw);
-      for (int s = 0; s < S; ++s) {
-        eel[s] += e_el[s];
-        evdw[s] += e_vdw[s];
+      double rinv[3], r6[NLJ];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double dx = x[3 * (o + k)] - x[3 * qa];
+        const double dy = x[3 * (o + k) + 1] - x[3 * qa + 1];
+        const double dz = x[3 * (o + k) + 2] - x[3 * qa + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        rinv[k] = rsqrt_d(r2);
+        if (k < NLJ) r6[k < NLJ ? k : 0] = r2 * r2 * r2;
+      }
+#pragma unroll
+      for (int s = 0; s < kMaxS; ++s) {
+        if (s >= S) break;
+        double e = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) e += (double)qqk[s][k] * rinv[k];
+        eel[s] += e;
+        double ev = 0.0;
+#pragma unroll
+        for (int k = 0; k < NLJ; ++k) {
+          const double inv = rcp_d(r6[k] + SC[s][k]);
+          ev += A[s][k] * inv * inv - B[s][k] * inv;
+        }
+        evdw[s] += ev;
       }
     }
     if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1)
@@ -1161,19 +1071,37 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
     if (b == 0 && pa >= 0) add_grad_fix(d.f, pa, sx, sy, sz);
     return;
   }
+  // energy pass (FP64); the per-state parameters are loaded in the tile
+  // loop (holding them in registers costs more occupancy than it saves)
   long long efix[kMaxS][2] = {};
   double eel[kMaxS] = {}, evdw[kMaxS] = {};
+  const double* x = d.x;
   for (int t = 0; t < n; ++t) {
     const int jc = L.jc[s0 + t];
     const uint2 m = L.mask[s0 + t];
     if (lane_bit(m, lane)) {
       const bool is14 = lane_bit(L.m14[s0 + t], lane);
       const int j = d.sc_a[jc * kCl + b];
-      double e_el[kMaxS], e_vdw[kMaxS];
-      qp_energy_df(d, W, iq_l, qa, j, is14, e_el, e_vdw);
+      // nonbond_qp_qvdw (md.f90:11138-11204): per state
+      const double dx = x[3 * j] - x[3 * qa], dy = x[3 * j + 1] - x[3 * qa + 1], dz = x[3 * j + 2] - x[3 * qa + 2];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double rinv = rsqrt_d(r2);
+      const double r6 = r2 * r2 * r2;
+      const float qj = d.x32[j].w;
+      const int iacj = d.iac[j];
+      const double* tl = &d.t_lj[(W.t_off + iacj) * 4];
+      const int k = is14 ? 2 : 0;
+      const double ta = tl[k], tb = tl[k + 1];
       for (int s = 0; s < S; ++s) {
-        eel[s] += e_el[s];
-        evdw[s] += e_vdw[s];
+        double vel = (double)(d.q_crg[W.qs_off + iq_l * S + s] * qj) * rinv;
+        if (is14) vel *= (double)W.el14;
+        eel[s] += vel;
+        const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq_l * S + s) * 4];
+        double aLJ, bLJ;
+        lj_ab(ql[k], ql[k + 1], ta, tb, W.arith != 0, &aLJ, &bLJ);
+        const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq_l * W.natyps + iacj) * S + s] : 0.0;
+        const double inv = rcp_d(r6 + sc);
+        evdw[s] += aLJ * inv * inv - bLJ * inv;
       }
     }
     if (((s0 + t - lst) % kQGroup) == kQGroup - 1 || t == n - 1)
@@ -1236,21 +1164,18 @@ __global__ void __launch_bounds__(128) k_qq(Dev d) {
       put_slot(d, sl, -(double)(dv * dx), -(double)(dv * dy), -(double)(dv * dz));
       put_slot(d, sl + 1, (double)(dv * dx), (double)(dv * dy), (double)(dv * dz));
       if (en) {
-        df ddx = df_dx(d.x[3 * j], d.x[3 * i]), ddy = df_dx(d.x[3 * j + 1], d.x[3 * i + 1]),
-           ddz = df_dx(d.x[3 * j + 2], d.x[3 * i + 2]);
-        df rr2 = df_add(df_add(df_mul(ddx, ddx), df_mul(ddy, ddy)), df_mul(ddz, ddz));
-        df rinv = df_rsqrt(rr2);
-        df qq = df_2p((float)qi, (float)qj);
-        df v = df_mul(df_mul(qq, rinv), df_from((double)es));
-        if (iLJ == 3) v = df_mul(v, df_from((double)W.el14));
-        e_el = df_to(v);
+        // FP64 energies; qi qj is exact in FP64, as the reference's real(8) product
+        const double ddx = d.x[3 * j] - d.x[3 * i], ddy = d.x[3 * j + 1] - d.x[3 * i + 1],
+                     ddz = d.x[3 * j + 2] - d.x[3 * i + 2];
+        const double rr2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        double v = qi * qj * rsqrt_d(rr2) * (double)es;
+        if (iLJ == 3) v *= (double)W.el14;
+        e_el = v;
         if (expo) {
-          e_vdw = aLJ * exp(-bLJ * sqrt(df_to(rr2)));
+          e_vdw = aLJ * exp(-bLJ * sqrt(rr2));
         } else {
-          df inv = df_rcp(df_add(df_mul(df_mul(rr2, rr2), rr2), df_from(sc)));
-          df va = df_mul(df_from(aLJ), df_mul(inv, inv));
-          df vb = df_mul(df_from(bLJ), inv);
-          e_vdw = df_to(df_add(va, df_neg(vb)));
+          const double inv = rcp_d(rr2 * rr2 * rr2 + sc);
+          e_vdw = aLJ * inv * inv - bLJ * inv;
         }
         base = jq ? 4 : 6;  // qq or qp slot
       }
@@ -1294,15 +1219,6 @@ __device__ __forceinline__ double r2_pos(double ax, double ay, double az, double
   return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
 }
 
-// 1/sqrt(r2) in FP64: the hardware estimate and two Newton steps (each
-// squares the relative error, so the result is within a few ulp)
-__device__ __forceinline__ double rsqrt_d(double r2) {
-  const double h = 0.5 * r2;
-  double y = __builtin_amdgcn_rsq(r2);
-  y = fma(y, fma(-(h * y), y, 0.5), y);
-  y = fma(y, fma(-(h * y), y, 0.5), y);
-  return y;
-}
 
 // Taylor terms of one source charge q at d = x(source) - centre: potential,
 // field, field gradient and its derivative (the sums of nbpw/nbww lists with
@@ -2398,6 +2314,8 @@ void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
 // window computes energies)
 void launch_q_tiles(const Dev& d, hipStream_t s, bool q_energy) {
   if (q_energy && d.tl[T_QW].nwork) k_qw<true, false><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
+  if (q_energy && d.tl[T_QW].nwork && d.any_wat3)
+    k_qw<true, true><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_WQ].nwork) k_qw<false, false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_WQ].nwork && d.any_wat3)
     k_qw<false, true><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
