@@ -204,10 +204,10 @@ def main():
         },
         "autotune": tuning,
         "roofline": {
-            "bound": "mfma", "kernel": "k_ww (water-water tiles, FP32 vector ALU; compute roof = FP32 peak)",
+            "bound": "mfma", "kernel": "k_ww<false> (SPC water-water rows, FP32 vector ALU; compute roof = FP32 peak)",
             "achieved": round(achieved, 4) if achieved else None, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 5) if achieved else None,
-            "traffic": traffic_per_launch("k_ww"),
+            "traffic": traffic_per_launch("k_ww<false>"),
             "work": {"ww_molecule_pairs_per_step": n_ww_total, "flop_per_pair": 200,
                      "kernel_us_per_launch": round(fam[4], 2) if fam else None},
         },
