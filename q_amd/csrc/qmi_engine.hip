@@ -109,6 +109,10 @@ struct qmi_engine {
   // host memory while the next chunk runs on stream
   hipStream_t copy_stream = nullptr;
   hipEvent_t chunk_ev[2] = {nullptr, nullptr};
+  // the captured step's other streams (LRF, bonded, restraints and water
+  // polarisation; the Q tiles) and its fork/join events
+  hipStream_t side_stream[2] = {nullptr, nullptr};
+  hipEvent_t step_ev[4] = {nullptr, nullptr, nullptr, nullptr};
   long long* h_er = nullptr;
   double* h_or = nullptr;                     // EVB off-diagonal ring, pinned
   double* h_tr = nullptr;
@@ -144,6 +148,8 @@ qmi_engine::~qmi_engine() {
   for (auto& x : ev) if (x) (void)hipEventDestroy(x);
   for (void* p : bufs) (void)hipFree(p);
   for (auto& x : chunk_ev) if (x) (void)hipEventDestroy(x);
+  for (auto& x : step_ev) if (x) (void)hipEventDestroy(x);
+  for (auto& x : side_stream) if (x) (void)hipStreamDestroy(x);
   if (h_er) (void)hipHostFree(h_er);
   if (h_or) (void)hipHostFree(h_or);
   if (h_tr) (void)hipHostFree(h_tr);
@@ -1444,6 +1450,10 @@ int qmi_prepare(qmi_engine* e) {
   HIP_OK(hipSetDevice(e->device));
   if (!e->stream) HIP_OK(hipStreamCreate(&e->stream));
   if (!e->copy_stream) HIP_OK(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
+  for (auto& x : e->side_stream)
+    if (!x) HIP_OK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+  for (auto& x : e->step_ev)
+    if (!x) HIP_OK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
   for (auto& x : e->chunk_ev)
     if (!x) HIP_OK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
   int rc = build_device(e);
@@ -1631,10 +1641,10 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   auto launch_step = [&](int variant) {
     const bool lists = variant & 1, qen = variant & 2;
     launch_begin(e->d, e->stream, 0, 0, variant);
-    if (lists) launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
-    launch_ww(e->d, e->stream);
-    launch_nonbonded(e->d, e->stream, false, qen);
-    launch_other_forces(e->d, e->stream);
+    // lists and forces on three streams; a failure here fails the capture
+    if (launch_step_forces(e->d, e->stream, e->side_stream[0], e->side_stream[1], e->step_ev, lists, e->ngrp, e->nwc,
+                           e->nsc, qen) != hipSuccess)
+      return;
     launch_integrate(e->d, e->stream, e->shake_any);
   };
   if (!e->timing_on && !e->step_exec[0] && !e->graph_failed) {

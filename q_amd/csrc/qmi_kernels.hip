@@ -1665,9 +1665,8 @@ __global__ void __launch_bounds__(256) k_bonded_gather(Dev d) {
         gz += d.bslot[3 * (size_t)m + 2];
       }
     }
-    d.f[3 * atom] += gx;
-    d.f[3 * atom + 1] += gy;
-    d.f[3 * atom + 2] += gz;
+    // atomic: the nonbonded kernels may add to the same atoms concurrently
+    add_grad_fix(d.f, atom, gx, gy, gz);
   }
 }
 
@@ -2300,13 +2299,21 @@ void launch_bonded(const Dev& d, hipStream_t s) {
   if (nb) k_bonded<<<blocks(nb, d.bs_bonded), d.bs_bonded, 0, s>>>(d);
 }
 
-void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
+// list update: group centres and cluster spheres, then the neighbour rows
+// and the LRF expansions (independent of each other)
+static void launch_list_prep(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
   k_centres<<<blocks(ngrp, 128), 128, 0, s>>>(d, ngrp);
   k_bounds<<<blocks(nwc + nsc, 128), 128, 0, s>>>(d, nwc, nsc);
+}
+static void launch_list_rows(const Dev& d, hipStream_t s) {
   if (d.rl[T_WW].nicl) k_build_rows<T_WW><<<blocks((long long)d.rl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.rl[T_WP].nicl) k_build_rows<T_WP><<<blocks((long long)d.rl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.rl[T_PW].nicl) k_build_rows<T_PW><<<blocks((long long)d.rl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.rl[T_PP].nicl) k_build_rows<T_PP><<<blocks((long long)d.rl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
+}
+void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
+  launch_list_prep(d, s, ngrp, nwc, nsc);
+  launch_list_rows(d, s);
   launch_lrf_build(d, s);
 }
 
@@ -2330,7 +2337,7 @@ void launch_ww(const Dev& d, hipStream_t s) {
     k_ww<true><<<blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
 }
 
-void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) {
+void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy, bool with_qq, bool with_q) {
   if (with_ww) launch_ww(d, s);
   if (d.rl[T_WP].nicl) k_pw<true, false><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
   if (d.rl[T_PW].nicl) k_pw<false, false><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
@@ -2339,8 +2346,43 @@ void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) 
   if (d.rl[T_PW].nicl && d.any_wat3)
     k_pw<false, true><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
   if (d.rl[T_PP].nicl) k_pp<<<blocks((long long)d.rl[T_PP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
-  launch_q_tiles(d, s, q_energy);
-  if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
+  if (with_q) launch_q_tiles(d, s, q_energy);
+  if (with_qq && d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
+}
+
+// The lists and forces of an MD step on three streams (after k_begin on s):
+//   s3: the Q tiles (static lists: they need only the step's coordinates);
+//   s2, after the group centres: on list steps the LRF expansions, then the
+//       q-q pairs, the bonded terms with their slot gather (which also
+//       collects the q-q slots), the LRF evaluation, atom and sequence
+//       restraints and water polarisation;
+//   s:  on list steps the neighbour rows, then the water-water,
+//       solute-water and solute-solute rows; it then waits for s2 and s3.
+// All of them write the gradient and the energies only through integer
+// atomic adds, so the result does not depend on how they overlap.
+hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[4],
+                              bool lists, int ngrp, int nwc, int nsc, bool q_energy) {
+  hipError_t err = hipEventRecord(ev[0], s);
+  if (err == hipSuccess) err = hipStreamWaitEvent(s3, ev[0], 0);
+  if (err != hipSuccess) return err;
+  launch_q_tiles(d, s3, q_energy);
+  err = hipEventRecord(ev[1], s3);
+  if (err != hipSuccess) return err;
+  if (lists) launch_list_prep(d, s, ngrp, nwc, nsc);
+  err = hipEventRecord(ev[2], s);
+  if (err == hipSuccess) err = hipStreamWaitEvent(s2, ev[2], 0);
+  if (err != hipSuccess) return err;
+  if (lists) launch_lrf_build(d, s2);
+  if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s2>>>(d);
+  launch_other_forces(d, s2);
+  err = hipEventRecord(ev[3], s2);
+  if (err != hipSuccess) return err;
+  if (lists) launch_list_rows(d, s);
+  launch_ww(d, s);
+  launch_nonbonded(d, s, false, q_energy, false, false);
+  err = hipStreamWaitEvent(s, ev[1], 0);
+  if (err == hipSuccess) err = hipStreamWaitEvent(s, ev[3], 0);
+  return err;
 }
 
 void launch_other_forces(const Dev& d, hipStream_t s) {
