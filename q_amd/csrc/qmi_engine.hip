@@ -112,7 +112,7 @@ struct qmi_engine {
   // the captured step's other streams (LRF, bonded, restraints and water
   // polarisation; the Q tiles) and its fork/join events
   hipStream_t side_stream[2] = {nullptr, nullptr};
-  hipEvent_t step_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t step_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   long long* h_er = nullptr;
   double* h_or = nullptr;                     // EVB off-diagonal ring, pinned
   double* h_tr = nullptr;

@@ -13,11 +13,10 @@ void launch_ww(const Dev& d, hipStream_t s);
 void launch_q_tiles(const Dev& d, hipStream_t s, bool q_energy = true);
 void launch_lrf_build(const Dev& d, hipStream_t s);
 void launch_bonded(const Dev& d, hipStream_t s);
-void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww = true, bool q_energy = true, bool with_qq = true,
-                      bool with_q = true);
+void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww = true, bool q_energy = true);
 void launch_other_forces(const Dev& d, hipStream_t s);
 // the lists and forces of an MD step on three streams (see the definition)
-hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[4],
+hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[5],
                               bool lists, int ngrp, int nwc, int nsc, bool q_energy);
 void launch_integrate(const Dev& d, hipStream_t s, int shake_any);
 void launch_temperature(const Dev& d, hipStream_t s);

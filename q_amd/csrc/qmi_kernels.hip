@@ -2337,37 +2337,43 @@ void launch_ww(const Dev& d, hipStream_t s) {
     k_ww<true><<<blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
 }
 
-void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy, bool with_qq, bool with_q) {
-  if (with_ww) launch_ww(d, s);
+static void launch_pw(const Dev& d, hipStream_t s) {
   if (d.rl[T_WP].nicl) k_pw<true, false><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
   if (d.rl[T_PW].nicl) k_pw<false, false><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
   if (d.rl[T_WP].nicl && d.any_wat3)
     k_pw<true, true><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
   if (d.rl[T_PW].nicl && d.any_wat3)
     k_pw<false, true><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+}
+static void launch_pp(const Dev& d, hipStream_t s) {
   if (d.rl[T_PP].nicl) k_pp<<<blocks((long long)d.rl[T_PP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
-  if (with_q) launch_q_tiles(d, s, q_energy);
-  if (with_qq && d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
+}
+
+void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) {
+  if (with_ww) launch_ww(d, s);
+  launch_pw(d, s);
+  launch_pp(d, s);
+  launch_q_tiles(d, s, q_energy);
+  if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
 }
 
 // The lists and forces of an MD step on three streams (after k_begin on s):
-//   s3: the Q tiles (static lists: they need only the step's coordinates);
+//   s3: the Q tiles (static lists: they need only the step's coordinates),
+//       then, once the neighbour rows are there, the solute-solute rows;
 //   s2, after the group centres: on list steps the LRF expansions, then the
 //       q-q pairs, the bonded terms with their slot gather (which also
 //       collects the q-q slots), the LRF evaluation, atom and sequence
 //       restraints and water polarisation;
-//   s:  on list steps the neighbour rows, then the water-water,
-//       solute-water and solute-solute rows; it then waits for s2 and s3.
+//   s:  on list steps the neighbour rows, then the water-water and
+//       solute-water rows; it then waits for s2 and s3.
 // All of them write the gradient and the energies only through integer
 // atomic adds, so the result does not depend on how they overlap.
-hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[4],
+hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[5],
                               bool lists, int ngrp, int nwc, int nsc, bool q_energy) {
   hipError_t err = hipEventRecord(ev[0], s);
   if (err == hipSuccess) err = hipStreamWaitEvent(s3, ev[0], 0);
   if (err != hipSuccess) return err;
   launch_q_tiles(d, s3, q_energy);
-  err = hipEventRecord(ev[1], s3);
-  if (err != hipSuccess) return err;
   if (lists) launch_list_prep(d, s, ngrp, nwc, nsc);
   err = hipEventRecord(ev[2], s);
   if (err == hipSuccess) err = hipStreamWaitEvent(s2, ev[2], 0);
@@ -2378,8 +2384,14 @@ hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipSt
   err = hipEventRecord(ev[3], s2);
   if (err != hipSuccess) return err;
   if (lists) launch_list_rows(d, s);
+  err = hipEventRecord(ev[4], s);
+  if (err == hipSuccess) err = hipStreamWaitEvent(s3, ev[4], 0);
+  if (err != hipSuccess) return err;
+  launch_pp(d, s3);
+  err = hipEventRecord(ev[1], s3);
+  if (err != hipSuccess) return err;
   launch_ww(d, s);
-  launch_nonbonded(d, s, false, q_energy, false, false);
+  launch_pw(d, s);
   err = hipStreamWaitEvent(s, ev[1], 0);
   if (err == hipSuccess) err = hipStreamWaitEvent(s, ev[3], 0);
   return err;
