@@ -2319,16 +2319,27 @@ void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
 
 // q_energy false: the q-side energy passes are left out (a step where no
 // window computes energies)
-void launch_q_tiles(const Dev& d, hipStream_t s, bool q_energy) {
-  if (q_energy && d.tl[T_QW].nwork) k_qw<true, false><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
-  if (q_energy && d.tl[T_QW].nwork && d.any_wat3)
+// the q-side energy passes (energy steps only): q-water, q-solute
+static void launch_q_energy_qw(const Dev& d, hipStream_t s) {
+  if (d.tl[T_QW].nwork) k_qw<true, false><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
+  if (d.tl[T_QW].nwork && d.any_wat3)
     k_qw<true, true><<<blocks((long long)d.tl[T_QW].nwork * 64, 256), 256, 0, s>>>(d);
+}
+static void launch_q_energy_qp(const Dev& d, hipStream_t s) {
+  if (d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
+}
+// the Q tiles' forces (non-q side) and the q-atom gather
+static void launch_q_forces(const Dev& d, hipStream_t s) {
   if (d.tl[T_WQ].nwork) k_qw<false, false><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_WQ].nwork && d.any_wat3)
     k_qw<false, true><<<blocks((long long)d.tl[T_WQ].nwork * 64, 256), 256, 0, s>>>(d);
-  if (q_energy && d.tl[T_QP].nwork) k_qp<true><<<blocks((long long)d.tl[T_QP].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.tl[T_PQ].nwork) k_qp<false><<<blocks((long long)d.tl[T_PQ].nwork * 64, 256), 256, 0, s>>>(d);
   if (d.nqc_all) k_q_gather<<<blocks((long long)d.nqc_all * kQGatherParts * 64, 256), 256, 0, s>>>(d);
+}
+void launch_q_tiles(const Dev& d, hipStream_t s, bool q_energy) {
+  if (q_energy) launch_q_energy_qw(d, s);
+  if (q_energy) launch_q_energy_qp(d, s);
+  launch_q_forces(d, s);
 }
 
 void launch_ww(const Dev& d, hipStream_t s) {
@@ -2358,14 +2369,17 @@ void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) 
 }
 
 // The lists and forces of an MD step on three streams (after k_begin on s):
-//   s3: the Q tiles (static lists: they need only the step's coordinates),
-//       then, once the neighbour rows are there, the solute-solute rows;
+//   s3: the Q tiles' forces (static lists: they need only the step's
+//       coordinates), then, once the neighbour rows are there, the
+//       solute-solute rows;
 //   s2, after the group centres: on list steps the LRF expansions, then the
 //       q-q pairs, the bonded terms with their slot gather (which also
 //       collects the q-q slots), the LRF evaluation, atom and sequence
-//       restraints and water polarisation;
+//       restraints and water polarisation, and on energy steps the
+//       q-water energy pass;
 //   s:  on list steps the neighbour rows, then the water-water and
-//       solute-water rows; it then waits for s2 and s3.
+//       solute-water rows and on energy steps the q-solute energy pass; it
+//       then waits for s2 and s3.
 // All of them write the gradient and the energies only through integer
 // atomic adds, so the result does not depend on how they overlap.
 hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[5],
@@ -2373,7 +2387,7 @@ hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipSt
   hipError_t err = hipEventRecord(ev[0], s);
   if (err == hipSuccess) err = hipStreamWaitEvent(s3, ev[0], 0);
   if (err != hipSuccess) return err;
-  launch_q_tiles(d, s3, q_energy);
+  launch_q_forces(d, s3);
   if (lists) launch_list_prep(d, s, ngrp, nwc, nsc);
   err = hipEventRecord(ev[2], s);
   if (err == hipSuccess) err = hipStreamWaitEvent(s2, ev[2], 0);
@@ -2381,6 +2395,7 @@ hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipSt
   if (lists) launch_lrf_build(d, s2);
   if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s2>>>(d);
   launch_other_forces(d, s2);
+  if (q_energy) launch_q_energy_qw(d, s2);
   err = hipEventRecord(ev[3], s2);
   if (err != hipSuccess) return err;
   if (lists) launch_list_rows(d, s);
@@ -2392,6 +2407,7 @@ hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipSt
   if (err != hipSuccess) return err;
   launch_ww(d, s);
   launch_pw(d, s);
+  if (q_energy) launch_q_energy_qp(d, s);
   err = hipStreamWaitEvent(s, ev[1], 0);
   if (err == hipSuccess) err = hipStreamWaitEvent(s, ev[3], 0);
   return err;
