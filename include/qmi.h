@@ -71,10 +71,15 @@ int qmi_window_set_log(qmi_engine* e, int w, const char* path);
 int qmi_prepare(qmi_engine* e);
 
 /* Advance every window by `nsteps` MD steps (each window stops at its own
- * [MD] steps), writing its .en/.re/log at the intervals of its .inp; the
- * step loop of md_run (md.f90:4003-4182).  With `finalize` != 0 the final
- * pair lists, energies, summary and restart of md_run (md.f90:4207-4213)
- * are produced for windows that have completed. */
+ * [MD] steps), writing its .en/log at the intervals of its .inp and its
+ * backup restart after every step that is a multiple of 1000; the step loop
+ * of md_run (md.f90:4003-4182).  With `finalize` != 0 the final pair lists,
+ * energies, summary and restart of md_run (md.f90:4207-4213) are produced
+ * for windows that have completed.  A window whose SHAKE does not converge
+ * (or whose neighbour rows overflow) stops at that step, as die does
+ * (md.f90:400-455): its log gets the termination record and its restart
+ * file the state it stopped in; the other windows run on and are finalized,
+ * then the call returns -60 (-61) naming the failed windows. */
 int qmi_run(qmi_engine* e, long long nsteps, int finalize);
 
 /* Evaluate pair lists (if `rebuild_lists`) and the potential energy and

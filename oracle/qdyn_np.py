@@ -333,12 +333,25 @@ def read_fep(path, nq_states, nat_solute, natyps, vdw_rule=1):
     fep = s.get("FEP", [])
     if _onoff(fep, "softcore_use_max_potential", False):
         raise ValueError("softcore_use_max_potential is not restated")
+    # q-q pairs with topology charges (qatom.f90:297; md.f90:13387-13397)
+    f["libq"] = _onoff(fep, "qq_use_library_charges", False)
+    # soft pairs, electrostatic scaling and special exclusions among q-atoms
+    # (qatom.f90:1049-1135); q-atom numbers and topology atoms 0-based
+    f["soft"] = {tuple(sorted((int(v[0]) - 1, int(v[1]) - 1))) for v in (l.split() for l in s.get("SOFT_PAIRS", []))}
+    f["el_scale"] = {}
+    for l in s.get("EL_SCALE", []):
+        v = l.split()
+        f["el_scale"][tuple(sorted((int(v[0]) - 1, int(v[1]) - 1)))] = [float(t) for t in v[2:2 + S]]
+    f["exspec"] = [(int(v[0]) - 1, int(v[1]) - 1, [int(t) == 1 for t in v[2:2 + S]])
+                   for v in (l.split() for l in s.get("EXCLUDED_PAIRS", []))]
+    # extra SHAKE constraints (qatom.f90:1498-1512), topology atoms 0-based
+    f["shake"] = [(int(v[0]) - 1, int(v[1]) - 1) for v in (l.split() for l in s.get("SHAKE_CONSTRAINTS", []))]
     if vdw_rule == 2 and f["qvdw"]:
         # square roots of the q-type epsilons, normal and 1-4 columns only
         # (get_fep, md.f90:1518-1523)
         f["qbvdw"][:, 0] = np.sqrt(f["qbvdw"][:, 0])
         f["qbvdw"][:, 2] = np.sqrt(f["qbvdw"][:, 2])
-    for sec in ("SOFT_PAIRS", "EL_SCALE", "EXCLUDED_PAIRS", "OFF_DIAGONALS", "COUPLINGS"):
+    for sec in ("OFF_DIAGONALS", "COUPLINGS"):
         if s.get(sec):
             raise ValueError(f"[{sec.lower()}] is not restated")
 
@@ -552,6 +565,12 @@ class Window:
                     (p["shake_solute"] and i < ns) or (p["shake_solvent"] and i >= ns)):
                 self.shake_pairs.append((i, j))
                 self.bnd_cod[k] = -1
+        # the .fep file's constraints join them (a pair already constrained
+        # gets the new distance; its bond term stays), md.f90:2416-2441
+        if S:
+            for (i, j) in f["shake"]:
+                if (i, j) not in self.shake_pairs and (j, i) not in self.shake_pairs:
+                    self.shake_pairs.append((i, j))
         for (i, j) in self.shake_pairs:
             a = t["ang"]
             hit = np.nonzero(((a[:, 0] == i) & (a[:, 2] == j)) | ((a[:, 0] == j) & (a[:, 2] == i)))[0]
@@ -591,6 +610,17 @@ class Window:
                     front = nxt
                 for a, lv in dist.items():
                     qc[s, a, q] = lv
+        # special exclusions among q-atoms, per state (make_qconn,
+        # md.f90:7330-7348); with a non-q atom they change the topology
+        # exclusions instead, which is not restated
+        iq_of = {a: q for q, a in enumerate(f["iqseq"])}
+        for (i, j, flags) in f["exspec"]:
+            if i not in iq_of or j not in iq_of:
+                raise ValueError("excluded pairs with non-q atoms are not restated")
+            for s in range(S):
+                if flags[s]:
+                    qc[s, j, iq_of[i]] = 0
+                    qc[s, i, iq_of[j]] = 0
         self.qconn = qc
 
     # ---------------------------------------------------------- pair lists --
@@ -859,7 +889,9 @@ class Window:
                 for b in range(a + 1, nq):
                     lv = qc[s, iq_at[b], a]
                     if lv > 3:
-                        pairs.append((a, iq_at[b], b + 1, 3 if lv == 4 else 1))
+                        # LJ code 2 marks a soft pair (nbqqlist, md.f90:7468-7477)
+                        lj = 3 if lv == 4 else (2 if qv and (a, b) in f["soft"] else 1)
+                        pairs.append((a, iq_at[b], b + 1, lj))
             conn = (qc <= 3).any(axis=(0, 2))
             for j in range(self.ns):
                 if self.iqatom[j] or not conn[j]:
@@ -870,7 +902,13 @@ class Window:
             for (a, j, jq, lj) in pairs:
                 i = iq_at[a]
                 col = lj - 1
+                qi = float(self.qcrg[a, s])
                 qj = float(self.qcrg[jq - 1, s]) if jq else float(self.crg[j])
+                if f["libq"] and jq:
+                    # nonbond_qq_lib_charges (md.f90:10896-10900, 10916)
+                    qi, qj = float(self.crg[i]), float(self.crg[j])
+                es = f["el_scale"].get((a, jq - 1), [1.0] * S)[s] if jq else 1.0
+                expo = qv and jq and lj == 2
                 if not qv:
                     # topology types for both atoms (nonbond_qq, md.f90:10786-10796)
                     aLJ, bLJ = lj_ab(A[iac[i], col], B[iac[i], col], A[iac[j], col], B[iac[j], col], rule)
@@ -878,8 +916,10 @@ class Window:
                     qt = f["qiac"][a, s]
                     if jq:
                         qtj = f["qiac"][jq - 1, s]
+                        # the exponential repulsion takes plain products
+                        # under either rule (nonbon2_qq, md.f90:9424-9429)
                         aLJ, bLJ = lj_ab(f["qavdw"][qt, col], f["qbvdw"][qt, col], f["qavdw"][qtj, col],
-                                         f["qbvdw"][qtj, col], rule)
+                                         f["qbvdw"][qtj, col], 1 if expo else rule)
                     else:
                         aLJ, bLJ = lj_ab(f["qavdw"][qt, col], f["qbvdw"][qt, col], A[iac[j], col],
                                          B[iac[j], col], rule)
@@ -889,9 +929,16 @@ class Window:
                 r6 = 1.0 / (r6hc + self.sc[a, nty + jq - 1, s])
                 r2i = 1.0 / r2
                 r = math.sqrt(r2i)
-                vel = float(self.qcrg[a, s]) * qj * r * (el14 if lj == 3 else 1.0)
-                va, vb = aLJ * r6 * r6, bLJ * r6
-                dv = r2i * (-vel - (12. * va - 6. * vb) * r6hc * r6) * lam[s]
+                vel = qi * qj * r * es
+                if lj == 3:
+                    vel *= el14
+                if expo:
+                    # soft pair: V = A exp(-B r) (nonbond_qq, md.f90:10838-10842)
+                    va, vb = aLJ * math.exp(-bLJ / r), 0.0
+                    dv = r2i * (-vel - bLJ * va / r) * lam[s]
+                else:
+                    va, vb = aLJ * r6 * r6, bLJ * r6
+                    dv = r2i * (-vel - (12. * va - 6. * vb) * r6hc * r6) * lam[s]
                 d[i] -= dv * dx
                 d[j] += dv * dx
                 EQ[s, 8 if jq else 10] += vel

@@ -78,7 +78,10 @@ struct WinDev {
 struct WinState {
   int istep;          // current step (counts from 0 at the start of the run)
   int stop_at;        // the current qmi_run stops this window at this step
+  int ring_base;      // step at the start of the current qmi_run: ring slot = (istep - ring_base) % ring_n
+  int run_epoch;      // number of the current qmi_run, stored with each ring record
   int list_now, energy_now, active;
+  int wpol_acc;       // this evaluation is a pot_energy call of the run: water-polarisation shell averages advance
   double tscale_solute, tscale_solvent;
   double temp, tfree, tfree_solute, tfree_solvent, temp_solute, temp_solvent, ekin;
   double ekin_parts[4];  // solute all, solute free, solvent all, solvent free
@@ -107,6 +110,9 @@ struct TileList {
   const int4* work;
   int nwork;
 };
+// temperature ring record per window and step: T_tot, T_free, Ekin, step tag,
+// T_free solute, T_free solvent
+constexpr int kTRing = 6;
 constexpr int kTempParts = 8;      // blocks per window of the kinetic-energy sum
 constexpr int kGatherAtoms = 256;  // atoms per block of the bonded gather
 constexpr int kGatherSlots = 2048; // slots staged in LDS per gather block
@@ -223,6 +229,7 @@ struct Dev {
   const int* qq_win;
   const float* qq_sc;    // soft-core term of the pair
   const double2* qq_ab;  // A, B of the pair in its state
+  const float2* qq_q;    // charges (scaled, real(4)) of the pair's two atoms in its state
   // bonded terms
   int nbond, nang, ntor, nimp;
   const int2* b_ij; const double2* b_par; const int* b_win; const uint8_t* b_solv;

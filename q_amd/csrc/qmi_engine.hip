@@ -67,7 +67,11 @@ struct HostWin {
   int offd_off = 0;                           // first of the window's EVB off-diagonal functions
   std::vector<double> offd;                   // Hij, r_kl per function at the last drained step
   int istep = 0;                              // host mirror of the device step
+  int E_step = -1;                            // step of the energies in E / EQ
+  double tlast = 0;                           // temperature of the last temperature line (md.f90:3974, 4171)
   bool done = false;
+  bool died = false;                          // the failure record and restart of a stopped window were written
+  double loop_s = 0;                          // wall time of the qmi_run calls that advanced this window
 };
 
 }  // namespace
@@ -113,6 +117,7 @@ struct qmi_engine {
   // polarisation; the Q tiles) and its fork/join events
   hipStream_t side_stream[2] = {nullptr, nullptr};
   hipEvent_t step_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int run_epoch = 0;                          // qmi_run calls so far (tags the ring records)
   long long* h_er = nullptr;
   double* h_or = nullptr;                     // EVB off-diagonal ring, pinned
   double* h_tr = nullptr;
@@ -202,6 +207,7 @@ static int build_device(qmi_engine* e) {
   // q-q
   std::vector<int4> qq_ij;
   std::vector<double2> qq_ab;
+  std::vector<float2> qq_q;
   std::vector<float> qq_es, qq_sc;
   std::vector<int> qq_st, qq_win;
   // bonded
@@ -452,6 +458,20 @@ static int build_device(qmi_engine* e) {
             }
           }
           qq_ab.push_back(make_double2(aLJ, bLJ));
+        }
+        {
+          // charges: the state's FEP charges (nonbond_qq, md.f90:10781-10791),
+          // or with qq_use_library_charges the topology charges of both atoms
+          // of a q-q pair (nonbond_qq_lib_charges, md.f90:10896-10900, 10916)
+          const int i_at = fp.iqseq[pq.iq];
+          float qi = fp.qcrg[pq.iq * S + s], qj;
+          if (pq.jq) qj = fp.qcrg[(pq.jq - 1) * S + s];
+          else qj = t.crg[pq.j];
+          if (fp.qq_use_library_charges && pq.jq) {
+            qi = t.crg[i_at];
+            qj = t.crg[pq.j];
+          }
+          qq_q.push_back(make_float2(qi, qj));
         }
         qq_es.push_back((float)pq.el_scale);
         qq_st.push_back(s);
@@ -891,6 +911,7 @@ static int build_device(qmi_engine* e) {
   UP(ip, qq_win); d.qq_win = ip;
   UP(fp_, qq_sc); d.qq_sc = fp_;
   double2* qab; UP(qab, qq_ab); d.qq_ab = qab;
+  float2* qqq; UP(qqq, qq_q); d.qq_q = qqq;
   d.nbond = (int)b_ij.size(); d.nang = (int)a_ijk.size(); d.ntor = (int)t_ijkl.size(); d.nimp = (int)i_ijkl.size();
   int2* i2; double2* d2; double4* d4; float2* f2;
   UP(i2, b_ij); d.b_ij = i2; UP(d2, b_par); d.b_par = d2; UP(ip, b_win); d.b_win = ip; UP(up, b_solv); d.b_solv = up;
@@ -974,9 +995,9 @@ static int build_device(qmi_engine* e) {
   // two chunks of kChunk steps: one drains while the next runs
   d.ring_n = 2 * kChunk;
   if ((rc = dev_alloc(e, &d.e_ring, (size_t)d.ring_n * NW * E_NSLOT)) != 0) return rc;
-  if ((rc = dev_alloc(e, &d.t_ring, (size_t)d.ring_n * NW * 4)) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.t_ring, (size_t)d.ring_n * NW * kTRing)) != 0) return rc;
   HIP_OK(hipHostMalloc((void**)&e->h_er, (size_t)d.ring_n * NW * E_NSLOT * sizeof(long long), hipHostMallocDefault));
-  HIP_OK(hipHostMalloc((void**)&e->h_tr, (size_t)d.ring_n * NW * 4 * sizeof(double), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&e->h_tr, (size_t)d.ring_n * NW * kTRing * sizeof(double), hipHostMallocDefault));
   if ((rc = dev_alloc(e, &d.t_part, (size_t)NW * kTempParts * 4)) != 0) return rc;
   // EVB off-diagonal functions and their ring
   {
@@ -1289,6 +1310,15 @@ static void write_summary(HostWin& h, int istep, bool final_) {
   fflush(f);
 }
 
+// centered_heading (misc.f90:31-42): n = (78 - len)/2 fill characters, the
+// text between single blanks, then n - 1 fill characters
+static std::string centered_heading(const std::string& text, char ch) {
+  const int n = (78 - (int)text.size()) / 2;
+  return std::string(std::max(n, 0), ch) + " " + text + " " + std::string(std::max(n - 1, 0), ch);
+}
+
+static int evaluate(qmi_engine* e, int rebuild_lists, int mode);
+
 // ------------------------------------------------------------ C-ABI --
 extern "C" {
 
@@ -1474,6 +1504,24 @@ int qmi_prepare(qmi_engine* e) {
   launch_temperature(e->d, e->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(e->stream));
+  const int NW = (int)e->wins.size();
+  std::vector<WinState> ws(NW);
+  HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
+  for (int wi = 0; wi < NW; ++wi) {
+    HostWin& h = e->wins[wi];
+    h.tlast = ws[wi].temp;
+    h.temp = ws[wi].temp;
+    h.tfree = ws[wi].tfree;
+    h.ekin = ws[wi].ekin;
+    if (!h.log) continue;
+    // md.f90:3978-3987
+    fprintf(h.log, "\nInitial temperatures are : Ttot =%10.2f Tfree =%10.2f\n", ws[wi].temp, ws[wi].tfree);
+    if (h.w->detail_temps) {
+      fprintf(h.log, "Solvent temperatures are : Ttot =%10.2f Tfree =%10.2f\n", ws[wi].temp_solvent, ws[wi].tfree_solvent);
+      fprintf(h.log, " Solute temperatures are : Ttot =%10.2f Tfree =%10.2f\n", ws[wi].temp_solute, ws[wi].tfree_solute);
+    }
+    fprintf(h.log, "\n");
+  }
   e->prepared = true;
   return 0;
 }
@@ -1533,18 +1581,22 @@ static int write_frame(HostWin& h, const double* x) {
   return 0;
 }
 
-// Records of `nsteps_done` steps from `first_step`, once the event `done`
-// (recorded after their launches) has passed: the ring slots are copied on
-// the copy stream, then .en records, summaries and temperature lines are
-// written.  The main stream may run the next chunk meanwhile (it writes the
-// other half of the ring).
-static int drain(qmi_engine* e, int first_step, int nsteps_done, hipEvent_t done) {
+// Records of run steps k0 .. k0 + n - 1 (window wi's step start[wi] + k),
+// once the event `done` (recorded after their launches) has passed: the ring
+// slots are copied on the copy stream, then .en records, summaries and
+// temperature lines are written.  The main stream may run the next chunk
+// meanwhile (it writes the other half of the ring).  A record is used only
+// if it carries the step and run number expected for its slot, so the slots
+// of a window that stopped earlier (its own [MD] steps, or a failure) are
+// never read as its later steps.
+static int drain(qmi_engine* e, const std::vector<int>& start, const std::vector<int>& stop, int k0, int n,
+                 hipEvent_t done) {
   const int NW = (int)e->wins.size();
   const int R = e->d.ring_n;
   HIP_OK(hipStreamWaitEvent(e->copy_stream, done, 0));
-  const int s0 = first_step % R;
-  const int n1 = std::min(nsteps_done, R - s0), n2 = nsteps_done - n1;
-  const size_t er_row = (size_t)NW * E_NSLOT, tr_row = (size_t)NW * 4;
+  const int s0 = k0 % R;
+  const int n1 = std::min(n, R - s0), n2 = n - n1;
+  const size_t er_row = (size_t)NW * E_NSLOT, tr_row = (size_t)NW * kTRing;
   HIP_OK(hipMemcpyAsync(e->h_er + s0 * er_row, e->d.e_ring + s0 * er_row, n1 * er_row * 8, hipMemcpyDeviceToHost,
                         e->copy_stream));
   HIP_OK(hipMemcpyAsync(e->h_tr + s0 * tr_row, e->d.t_ring + s0 * tr_row, n1 * tr_row * 8, hipMemcpyDeviceToHost,
@@ -1563,15 +1615,16 @@ static int drain(qmi_engine* e, int first_step, int nsteps_done, hipEvent_t done
   HIP_OK(hipStreamSynchronize(e->copy_stream));
   const long long* er = e->h_er;
   const double* tr = e->h_tr;
+  const double epoch = 4294967296.0 * (double)e->run_epoch;
   for (int wi = 0; wi < NW; ++wi) {
     HostWin& h = e->wins[wi];
     Window& W = *h.w;
-    for (int k = 0; k < nsteps_done; ++k) {
-      const int istep = first_step + k;
-      if (istep >= W.inp.nsteps || h.done) continue;
-      const int slot = istep % R;
-      const double* t = &tr[((size_t)slot * NW + wi) * 4];
-      if ((int)t[3] != istep) continue;
+    for (int k = k0; k < k0 + n; ++k) {
+      const int istep = start[wi] + k;
+      if (istep >= stop[wi] || h.done) continue;
+      const int slot = k % R;
+      const double* t = &tr[((size_t)slot * NW + wi) * kTRing];
+      if (t[3] != (double)istep + epoch) continue;
       h.temp = t[0]; h.tfree = t[1]; h.ekin = t[2];
       if (!h.offd.empty())
         std::copy_n(&e->h_or[(size_t)slot * or_row + 2 * (size_t)h.offd_off], h.offd.size(), h.offd.begin());
@@ -1580,34 +1633,51 @@ static int drain(qmi_engine* e, int first_step, int nsteps_done, hipEvent_t done
       if (en_step || out_step) {
         assemble_energies(h, &er[((size_t)slot * NW + wi) * E_NSLOT]);
         h.E[1] = h.ekin;
+        h.E_step = istep;
         if (en_step) write_en(h);
         if (out_step) write_summary(h, istep, false);
       }
-      if (W.inp.itemp_cycle > 0 && istep % W.inp.itemp_cycle == 0 && istep > 0 && h.log)
+      // temperature lines (md.f90:4168-4176): at the interval, or when the
+      // temperature moved by more than 2 % since the last line
+      if (h.log && (std::fabs(h.temp - h.tlast) / h.temp > (double)0.02f ||
+                    (W.inp.itemp_cycle > 0 && istep % W.inp.itemp_cycle == 0 && istep > 0))) {
+        h.tlast = h.temp;
         fprintf(h.log, "Temperature at step%8d:         T_tot=%10.1f         T_free=%10.1f\n", istep, h.temp, h.tfree);
+        if (W.detail_temps)
+          fprintf(h.log, "                             T_free_solute=%10.1f T_free_solvent=%10.1f\n", t[4], t[5]);
+      }
     }
   }
   return 0;
 }
 
-static int sync_state_to_host(qmi_engine* e) {
+// Copies coordinates, velocities and water-polarisation theta_corr of the
+// windows in `which` (all windows if null) from the device into their host
+// Window, after the engine's stream has finished.
+static int sync_state_to_host(qmi_engine* e, const std::vector<int>* which) {
   const int NW = (int)e->wins.size();
-  std::vector<double> x((size_t)3 * e->ntot), v((size_t)3 * e->ntot);
-  HIP_OK(hipMemcpyAsync(x.data(), e->d.x, x.size() * 8, hipMemcpyDeviceToHost, e->stream));
-  HIP_OK(hipMemcpyAsync(v.data(), e->d.v, v.size() * 8, hipMemcpyDeviceToHost, e->stream));
-  std::vector<ShellDev> sh;
-  size_t nsh = 0;
-  for (auto& h : e->wins) nsh += h.w->wshell.size();
-  sh.resize(nsh);
-  if (nsh) HIP_OK(hipMemcpyAsync(sh.data(), e->d.shells, nsh * sizeof(ShellDev), hipMemcpyDeviceToHost, e->stream));
-  HIP_OK(hipStreamSynchronize(e->stream));
-  size_t ao = 0, so = 0;
+  std::vector<int> all;
+  if (!which) {
+    all.resize(NW);
+    std::iota(all.begin(), all.end(), 0);
+    which = &all;
+  }
+  std::vector<size_t> ao(NW + 1, 0), so(NW + 1, 0);
   for (int wi = 0; wi < NW; ++wi) {
+    ao[wi + 1] = ao[wi] + e->wins[wi].w->natom;
+    so[wi + 1] = so[wi] + e->wins[wi].w->wshell.size();
+  }
+  HIP_OK(hipStreamSynchronize(e->stream));
+  std::vector<ShellDev> sh;
+  for (int wi : *which) {
     Window& W = *e->wins[wi].w;
-    std::copy(x.begin() + 3 * ao, x.begin() + 3 * (ao + W.natom), W.x.begin());
-    std::copy(v.begin() + 3 * ao, v.begin() + 3 * (ao + W.natom), W.v.begin());
-    for (auto& s : W.wshell) { s.theta_corr = sh[so].theta_corr; ++so; }
-    ao += W.natom;
+    HIP_OK(hipMemcpy(W.x.data(), e->d.x + 3 * ao[wi], (size_t)3 * W.natom * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(W.v.data(), e->d.v + 3 * ao[wi], (size_t)3 * W.natom * 8, hipMemcpyDeviceToHost));
+    if (!W.wshell.empty()) {
+      sh.resize(W.wshell.size());
+      HIP_OK(hipMemcpy(sh.data(), e->d.shells + so[wi], sh.size() * sizeof(ShellDev), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < sh.size(); ++k) W.wshell[k].theta_corr = sh[k].theta_corr;
+    }
   }
   return 0;
 }
@@ -1616,18 +1686,25 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   if (!e || !e->prepared) return set_err(-2, "engine not prepared");
   HIP_OK(hipSetDevice(e->device));
   const int NW = (int)e->wins.size();
-  // stop_at per window
   std::vector<WinState> ws(NW);
   HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
-  int first = ws[0].istep;
   // no window runs past its own [MD] steps: stop when the last one is done
   long long remaining = 0;
   for (int wi = 0; wi < NW; ++wi)
-    remaining = std::max<long long>(remaining, (long long)e->wins[wi].w->inp.nsteps - ws[wi].istep);
+    if (e->wins[wi].status == 0)
+      remaining = std::max<long long>(remaining, (long long)e->wins[wi].w->inp.nsteps - ws[wi].istep);
   nsteps = std::max<long long>(0, std::min(nsteps, remaining));
+  // each window runs its steps start[wi] .. stop[wi] - 1 in this call; its
+  // ring records go to slot (step - start) % ring_n, tagged with the run number
+  ++e->run_epoch;
+  std::vector<int> start(NW), stop(NW);
   for (int wi = 0; wi < NW; ++wi) {
-    ws[wi].stop_at = (int)std::min<long long>(e->wins[wi].w->inp.nsteps, (long long)ws[wi].istep + nsteps);
-    HIP_OK(hipMemcpy(&e->d.ws[wi].stop_at, &ws[wi].stop_at, sizeof(int), hipMemcpyHostToDevice));
+    start[wi] = ws[wi].istep;
+    stop[wi] = (int)std::min<long long>(e->wins[wi].w->inp.nsteps, (long long)ws[wi].istep + nsteps);
+    ws[wi].stop_at = stop[wi];
+    ws[wi].ring_base = start[wi];
+    ws[wi].run_epoch = e->run_epoch;
+    HIP_OK(hipMemcpy(&e->d.ws[wi].stop_at, &ws[wi].stop_at, 3 * sizeof(int), hipMemcpyHostToDevice));
   }
   if (e->timing_on && !e->ev[0])
     for (auto& x : e->ev) HIP_OK(hipEventCreate(&x));
@@ -1696,6 +1773,17 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     }
     return false;
   };
+  // backup restart files (md.f90:4165-4167): after window step is, for every
+  // is that is a multiple of 1000, step 0 included
+  auto backup_after = [&](int wi, long long k) -> bool {
+    const long long is = (long long)ws[wi].istep + k;
+    return is < ws[wi].stop_at && is % 1000 == 0 && !e->wins[wi].w->inp.xfin_file.empty();
+  };
+  auto any_backup = [&](long long k) {
+    for (int wi = 0; wi < NW; ++wi)
+      if (backup_after(wi, k)) return true;
+    return false;
+  };
   // trajectory frames (md.f90:4151-4154 in the loop, 4199-4203 after it):
   // the positions after window step is, for is > 0 a multiple of the
   // interval, and after the last step when the interval divides nsteps
@@ -1725,9 +1813,8 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   int pend_first = 0, pend_n = 0, cidx = 0;
   while (done < nsteps) {
     int chunk = (int)std::min<long long>(kChunk, nsteps - done);
-    if (any_trj)
-      for (int k = 0; k < chunk; ++k)
-        if (any_frame(done + k)) { chunk = k + 1; break; }
+    for (int k = 0; k < chunk; ++k)
+      if ((any_trj && any_frame(done + k)) || any_backup(done + k)) { chunk = k + 1; break; }
     if (!e->timing_on && e->step_exec[0]) {
       for (int k = 0; k < chunk; ++k)
         HIP_OK(hipGraphLaunch(e->step_exec[(lists_at(done + k) ? 1 : 0) | (energy_at(done + k) ? 2 : 0)], e->stream));
@@ -1767,18 +1854,30 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
           if (rc) return rc;
         }
     }
+    if (any_backup(done + chunk - 1)) {
+      std::vector<int> which;
+      for (int wi = 0; wi < NW; ++wi)
+        if (backup_after(wi, done + chunk - 1)) which.push_back(wi);
+      int rc = sync_state_to_host(e, &which);
+      if (rc) return rc;
+      for (int wi : which) {
+        const Window& W = *e->wins[wi].w;
+        Error err{0, ""};
+        if (!write_restart(W.inp.dir + "/" + W.inp.xfin_file, W, &err)) return set_err(err.code, err.msg);
+      }
+    }
     HIP_OK(hipEventRecord(e->chunk_ev[cidx & 1], e->stream));
     if (pend_n > 0) {
-      int rc = drain(e, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
+      int rc = drain(e, start, stop, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
       if (rc) return rc;
     }
-    pend_first = first + (int)done;
+    pend_first = (int)done;
     pend_n = chunk;
     ++cidx;
     done += chunk;
   }
   if (pend_n > 0) {
-    int rc = drain(e, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
+    int rc = drain(e, start, stop, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
     if (rc) return rc;
   }
   HIP_OK(hipStreamSynchronize(e->stream));
@@ -1788,8 +1887,11 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   for (int f = 0; f < 5; ++f) e->timings[f] = nsteps > 0 ? fam[f] / (double)nsteps : 0.0;
   e->timings[5] = nsteps > 0 ? us / (double)nsteps : 0.0;
   HIP_OK(hipMemcpy(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost));
+  for (int wi = 0; wi < NW; ++wi)
+    if (ws[wi].istep > start[wi]) e->wins[wi].loop_s += us * 1e-6;
   // a failed window stops at its failing step (as qdyn ends that run); the
-  // others run to the end, and the error names every failed window
+  // others run to the end, get their final evaluation, summary and restart,
+  // and then the error names every failed window
   std::string failed_shake, failed_list;
   for (int wi = 0; wi < NW; ++wi) {
     if (ws[wi].list_missed)
@@ -1804,31 +1906,71 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     }
     e->wins[wi].istep = ws[wi].istep;
   }
-  if (!failed_shake.empty()) return set_err(-60, "shake failure in window(s) " + failed_shake);
-  if (!failed_list.empty()) return set_err(-61, "neighbour list capacity exceeded in window(s) " + failed_list);
-  int rc = sync_state_to_host(e);
+  int rc = sync_state_to_host(e, nullptr);
   if (rc) return rc;
+  // die (md.f90:400-455) for each newly failed window: its last energies, a
+  // restart file of the state it stopped in, and the termination record
+  for (int wi = 0; wi < NW; ++wi) {
+    HostWin& h = e->wins[wi];
+    if (h.status == 0 || h.died) continue;
+    h.died = true;
+    h.done = true;
+    // the step counter has moved past the failing step
+    const int fstep = h.istep - 1;
+    const InputParams& p = h.w->inp;
+    if (h.log) {
+      fprintf(h.log, "\n%s\n", centered_heading("ABNORMAL TERMINATION", '!').c_str());
+      if (fstep > 0 && h.E_step >= 0 && p.iout_cycle > 0 && fstep % p.iout_cycle != 1) write_summary(h, h.E_step, false);
+    }
+    Error err{0, ""};
+    if (!p.xfin_file.empty()) {
+      if (h.log) fprintf(h.log, " restart file written at step%12d\n", fstep);
+      if (!write_restart(p.dir + "/" + p.xfin_file, *h.w, &err)) return set_err(err.code, err.msg);
+    }
+    if (h.log) {
+      fprintf(h.log, "%s\nABNORMAL TERMINATION of qdyn\nTerminating due to %s\n%s\n", std::string(79, '!').c_str(),
+              h.status == 1 ? "shake failure" : "neighbour list capacity exceeded", std::string(79, '!').c_str());
+      fflush(h.log);
+    }
+  }
   if (finalize) {
     // final pair lists, energies, summary and restart (md.f90:4207-4213)
-    rc = qmi_evaluate(e, 1);
+    rc = evaluate(e, 1, 3);
     if (rc) return rc;
     for (int wi = 0; wi < NW; ++wi) {
       HostWin& h = e->wins[wi];
-      if (h.istep < h.w->inp.nsteps) continue;
+      if (h.status != 0 || h.done || h.istep < h.w->inp.nsteps) continue;
       h.E[1] = h.ekin;
+      h.E_step = h.istep;
+      if (h.log) fprintf(h.log, "\n");
       write_summary(h, h.istep, true);
       Error err{0, ""};
       if (!write_restart(h.w->inp.dir + "/" + h.w->inp.xfin_file, *h.w, &err)) return set_err(err.code, err.msg);
+      if (h.log) {
+        fprintf(h.log, "Total time of main loop:                     %15.1f(s)\n", h.loop_s);
+        fflush(h.log);
+      }
       h.done = true;
     }
   }
+  if (!failed_shake.empty()) return set_err(-60, "shake failure in window(s) " + failed_shake);
+  if (!failed_list.empty()) return set_err(-61, "neighbour list capacity exceeded in window(s) " + failed_list);
   return 0;
 }
 
 int qmi_evaluate(qmi_engine* e, int rebuild_lists) {
   if (!e || !e->prepared) return set_err(-2, "engine not prepared");
+  return evaluate(e, rebuild_lists, 1);
+}
+
+// Pair lists (if `rebuild_lists`), energies and gradient of every window at
+// its current coordinates.  mode 1: a query that leaves the run's state
+// (water-polarisation shell averages) as it is; mode 3: the final
+// pot_energy call of md_run (md.f90:4207-4208) for the windows that completed,
+// windows stopped by a failure left out.
+static int evaluate(qmi_engine* e, int rebuild_lists, int mode) {
   HIP_OK(hipSetDevice(e->device));
-  launch_begin(e->d, e->stream, 1, 1, rebuild_lists ? 1 : 0);
+  launch_begin(e->d, e->stream, 1, mode, rebuild_lists ? 1 : 0);
   launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
   launch_nonbonded(e->d, e->stream);
   launch_other_forces(e->d, e->stream);
@@ -1838,6 +1980,7 @@ int qmi_evaluate(qmi_engine* e, int rebuild_lists) {
   HIP_OK(hipMemcpyAsync(ws.data(), e->d.ws, NW * sizeof(WinState), hipMemcpyDeviceToHost, e->stream));
   HIP_OK(hipStreamSynchronize(e->stream));
   for (int wi = 0; wi < NW; ++wi) {
+    if (mode == 3 && e->wins[wi].status != 0) continue;
     if (ws[wi].list_overflow)
       return set_err(-61, "neighbour list capacity exceeded in window " + std::to_string(wi));
     assemble_energies(e->wins[wi], ws[wi].E);
@@ -2005,7 +2148,8 @@ int qmi_write_restart(qmi_engine* e, int w, const char* path) {
   HostWin* h = getw(e, w);
   if (!h || !path) return set_err(-1, "invalid argument");
   if (e->prepared) {
-    int rc = sync_state_to_host(e);
+    std::vector<int> one{w};
+    int rc = sync_state_to_host(e, &one);
     if (rc) return rc;
   }
   Error err{0, ""};

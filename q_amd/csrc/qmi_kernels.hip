@@ -147,14 +147,19 @@ __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
     WinState& s = d.ws[i];
     const WinDev& W = d.win[i];
     if (eval_only) {
-      // 1: energy evaluation; 2: forces only (the autotuner's timing runs)
-      s.active = 1;
+      // 1: energy evaluation; 2: forces only (the autotuner's timing runs);
+      // 3: the energy evaluation after the last step of a run (md.f90:4207-4208),
+      // which is a pot_energy call of the run for windows that completed;
+      // a window stopped by a failure is left as it is
+      s.active = eval_only != 3 || !(s.shake_fail || s.list_overflow);
       s.list_now = rebuild;
-      s.energy_now = eval_only == 1 ? 1 : 0;
+      s.energy_now = eval_only != 2 ? 1 : 0;
+      s.wpol_acc = eval_only == 3 && s.istep >= W.nsteps ? 1 : 0;
     } else {
       // MD step; rebuild bit 0: the list kernels were launched, bit 1: the
       // q-side energy passes were
       s.active = window_runs(s) ? 1 : 0;
+      s.wpol_acc = s.active;
       s.list_now = (s.istep % W.nbcycle) == 0;
       if (s.active && s.list_now && !(rebuild & 1)) s.list_missed = 1;
       bool e = (W.iene > 0 && s.istep % W.iene == 0 && s.istep > 0) || (W.iout > 0 && s.istep % W.iout == 0) ||
@@ -178,7 +183,7 @@ __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
                  dz = d.x[3 * kl.y + 2] - d.x[3 * kl.x + 2];
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     const double2 p = d.offd_par[i];
-    double* o = &d.o_ring[((size_t)(s.istep % d.ring_n) * d.noffd_all + i) * 2];
+    double* o = &d.o_ring[((size_t)((s.istep - s.ring_base) % d.ring_n) * d.noffd_all + i) * 2];
     o[0] = p.x * exp(-p.y * r);
     o[1] = r;
   }
@@ -1130,14 +1135,13 @@ __global__ void __launch_bounds__(128) k_qq(Dev d) {
       valid = true;
       en = st.energy_now;
       const WinDev& W = d.win[w];
-      const int S = W.S;
       const int4 e = d.qq_ij[k];
       s = d.qq_st[k];
       const int i = d.q_atom[W.q_off + e.x], j = e.y, jq = e.z, iLJ = e.w;
       const float es = d.qq_es[k];
       // parameters (double, as the reference)
-      const double qi = d.q_crg[W.qs_off + e.x * S + s];
-      const double qj = jq ? (double)d.q_crg[W.qs_off + (jq - 1) * S + s] : (double)d.x32[j].w;
+      const float2 qq = d.qq_q[k];
+      const double qi = (double)qq.x, qj = (double)qq.y;
       const double2 ab = d.qq_ab[k];
       const double aLJ = ab.x, bLJ = ab.y;
       const double sc = d.qq_sc[k];
@@ -1921,9 +1925,11 @@ __global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
   __shared__ int s_iw[kWpolLds];
   __shared__ int s_n;
   __shared__ double s_red[4][2];
-  // theta_corr follows the shell's mean theta every 100 steps
+  // theta_corr follows the shell's mean theta every 100 steps; only the
+  // pot_energy calls of the run (MD steps and the final evaluation) advance
+  // the shell averages, so extra evaluations (tuning, queries) leave them
   if (threadIdx.x == 0) {
-    if (st.istep != 0 && st.istep % 100 == 0) {
+    if (st.wpol_acc && st.istep != 0 && st.istep % 100 == 0) {
       sh.avtheta = sh.avtheta / 100.0f;
       sh.avn_insh = sh.avn_insh / 100.0f;
       sh.theta_corr = sh.theta_corr + sh.avtheta - acosf(sh.cstb);
@@ -2027,7 +2033,7 @@ __global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
   if (lane == 0) { s_red[wv][0] = t_w; s_red[wv][1] = e_w; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (n > 0) {
+    if (n > 0 && st.wpol_acc) {
       const double avtdum = (s_red[0][0] + s_red[1][0]) + (s_red[2][0] + s_red[3][0]);
       // real(4) + real(8) quotient, rounded once (md.f90:15285)
       sh.avtheta = (float)((double)sh.avtheta + avtdum / (double)(float)n);
@@ -2267,9 +2273,11 @@ __global__ void __launch_bounds__(64) k_temperature(Dev d, int advance) {
       st.tscale_solute = st.tscale_solvent;
     }
     if (advance) {
-      const int slot = st.istep % d.ring_n;
-      double* tr = &d.t_ring[((size_t)slot * d.nwin + w) * 4];
-      tr[0] = st.temp; tr[1] = st.tfree; tr[2] = st.ekin; tr[3] = (double)st.istep;
+      const int slot = (st.istep - st.ring_base) % d.ring_n;
+      double* tr = &d.t_ring[((size_t)slot * d.nwin + w) * kTRing];
+      tr[0] = st.temp; tr[1] = st.tfree; tr[2] = st.ekin;
+      tr[3] = (double)st.istep + 4294967296.0 * (double)st.run_epoch;
+      tr[4] = st.tfree_solute; tr[5] = st.tfree_solvent;
       if (st.energy_now) {
         long long* er = &d.e_ring[((size_t)slot * d.nwin + w) * E_NSLOT];
         for (int k = 0; k < E_NSLOT; ++k) er[k] = st.E[k];

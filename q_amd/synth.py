@@ -115,8 +115,14 @@ class System:
     qimpropers: list = field(default_factory=list)
     bond_lib: list = field(default_factory=lambda: list(BOND_LIB))
     angle_lib: list = field(default_factory=lambda: list(ANGLE_LIB))
-    qbond_lib: list = field(default_factory=list)    # (fk, r0) per q-bond type
+    qbond_lib: list = field(default_factory=list)    # (fk, r0) harmonic or (Dmz, amz, r0) Morse per q-bond type
     qangle_lib: list = field(default_factory=list)   # (fk, theta0 deg) per q-angle type
+    # further .fep sections (qatom.f90:297, 1049-1135, 1498-1512):
+    # qq_use_library_charges (bool), el_scale [(iq, jq, s1, s2)], soft_pairs
+    # [(iq, jq)] with exp_repulsion (A, B) columns for the q types,
+    # excluded_pairs [(i, j, f1, f2)], shake [(i, j, d1, d2)]
+    fep_extra: dict = field(default_factory=dict)
+    inp_extra: list = field(default_factory=list)    # restraint sections appended to the .inp
 
     @property
     def natom(self) -> int:
@@ -391,6 +397,58 @@ def _add_water_lattice(rng, sysm: System, radius: float, grid: _Grid | None, spa
         sysm.molecules.append(base)
 
 
+def _fill_water_gaps(rng, sysm: System, radius: float, grid: _Grid, heavy_grid: _Grid, first_water: int,
+                     step: float = 0.9, oo_min: float = 2.55, hv_min: float = 2.35, any_min: float = 1.75,
+                     h_min: float = 1.35) -> None:
+    """Add waters in the voids the lattice left next to the solute: points of
+    a finer grid, visited in random order, take a water when its oxygen is at
+    least ``oo_min`` from every other water oxygen, ``hv_min`` from solute
+    heavy atoms and ``any_min`` from any solute atom, and its (randomly
+    oriented) hydrogens at least ``h_min`` from every other atom (close
+    contacts, as hydrogen bonds have them)."""
+    t_o, t_h, q_o = T_OW, T_HW, SPC_QO
+    half = math.radians(SPC_HOH / 2)
+    h1 = np.array([SPC_ROH * math.sin(half), SPC_ROH * math.cos(half), 0.0])
+    h2 = np.array([-SPC_ROH * math.sin(half), SPC_ROH * math.cos(half), 0.0])
+    qh = -q_o / 2
+    wo = _Grid()
+    wall = _Grid()
+    for a in range(first_water - 1, sysm.natom, 3):
+        wo.add(sysm.x[a], a)
+        for k in range(3):
+            wall.add(sysm.x[a + k], a + k)
+    n = int(math.ceil(radius / step))
+    pts = []
+    r_in = radius - 0.5
+    for i in range(-n, n + 1):
+        for j in range(-n, n + 1):
+            for k in range(-n, n + 1):
+                p = (np.array([i, j, k], float) + 0.5) * step
+                if np.dot(p, p) <= r_in * r_in:
+                    pts.append(p)
+    order = rng.permutation(len(pts))
+    for idx in order:
+        p = pts[idx]
+        if wo.min_dist(p) < oo_min or heavy_grid.min_dist(p) < hv_min or grid.min_dist(p) < any_min:
+            continue
+        rot = _rand_rot(rng)
+        hs = [p + rot @ h1, p + rot @ h2]
+        if any(grid.min_dist(h) < h_min or wall.min_dist(h) < h_min for h in hs):
+            continue
+        base = sysm.natom + 1
+        sysm.x += [p, hs[0], hs[1]]
+        sysm.iac += [t_o, t_h, t_h]
+        sysm.crg += [q_o, qh, qh]
+        sysm.bonds += [(base, base + 1, B_OW_H), (base, base + 2, B_OW_H), (base + 1, base + 2, B_HW_HW)]
+        sysm.angles += [(base + 1, base, base + 2, A_WAT)]
+        sysm.cgps.append((base, [base, base + 1, base + 2]))
+        sysm.residues.append((base, "HOH"))
+        sysm.molecules.append(base)
+        wo.add(p, base - 1)
+        for k, q in enumerate([p] + hs):
+            wall.add(q, base - 1 + k)
+
+
 def _strain_free(s: System) -> None:
     """Give every solute bond and angle its own equilibrium parameter taken
     from the built geometry, so the bonded terms start unstrained."""
@@ -515,9 +573,110 @@ def water_droplet(radius: float = 15.0, seed: int = 0) -> System:
     return s
 
 
+def _graph_distances(nb: list) -> np.ndarray:
+    """Bond-count distances between all atoms of a molecule graph."""
+    n = len(nb)
+    dist = np.full((n, n), 99, int)
+    for a in range(n):
+        dist[a, a] = 0
+        front = [a]
+        while front:
+            nxt = []
+            for u in front:
+                for v in nb[u]:
+                    if dist[a, v] == 99:
+                        dist[a, v] = dist[a, u] + 1
+                        nxt.append(v)
+            front = nxt
+    return dist
+
+
+def _fep_sections(s: System, nb: list, heavy: list, seed: int) -> None:
+    """The FEP sections beyond charges, types and bonded changes on the
+    ligand (all of whose atoms are q-atoms, numbered as the topology atoms):
+    electrostatic scaling of three q-q pairs, two soft (exponential
+    repulsion) pairs, three excluded pairs (one per state, one in both), a
+    Morse q-bond in state 1, and two extra SHAKE constraints with
+    state-dependent distances (an X-H bond and an H...H distance across an
+    angle).  Pairs are nonbonded (more than three bonds apart) and 3-7 A
+    apart, so every term is non-trivial."""
+    rng = np.random.default_rng(seed)
+    x = np.array(s.x)
+    dist = _graph_distances(nb)
+    n = len(nb)
+    cand = [(a, b) for a in range(n) for b in range(a + 1, n)
+            if dist[a, b] > 3 and 3.0 < float(np.linalg.norm(x[a] - x[b])) < 7.0]
+    pick = [cand[k] for k in rng.choice(len(cand), size=8, replace=False)]
+    s.fep_extra["el_scale"] = [(a + 1, b + 1, f1, f2) for (a, b), (f1, f2) in
+                               zip(pick[:3], [(0.5, 0.8), (0.2, 1.0), (1.0, 0.3)])]
+    s.fep_extra["soft_pairs"] = [(a + 1, b + 1) for (a, b) in pick[3:5]]
+    s.fep_extra["exp_repulsion"] = (30.0, 1.2)
+    s.fep_extra["excluded_pairs"] = [(a + 1, b + 1, f1, f2) for (a, b), (f1, f2) in
+                                     zip(pick[5:8], [(1, 0), (0, 1), (1, 1)])]
+    # a Morse bond for state 1 of the first q-bond (qbond, md.f90:14060-14114)
+    i, j, c1, c2 = s.qbonds[0]
+    r0 = s.qbond_lib[c1 - 1][-1]
+    s.qbond_lib.append((70.0, 2.0, r0))
+    s.qbonds[0] = (i, j, len(s.qbond_lib), c2)
+    # SHAKE: one X-H bond and the H...H distance of an H-X-H angle
+    xh = [(a, b) for a in range(n) for b in nb[a] if heavy[a] and not heavy[b]]
+    hh = [(a, b) for c in range(n) for a in nb[c] for b in nb[c] if a < b and not heavy[a] and not heavy[b]]
+    sh = []
+    for (a, b) in [xh[0]] + hh[:1]:
+        d0 = round(float(np.linalg.norm(x[a] - x[b])), 4)
+        sh.append((a + 1, b + 1, d0, round(d0 + 0.02, 4)))
+    s.fep_extra["shake"] = sh
+
+
+def _library_charges(s: System, n_lig: int) -> None:
+    """qq_use_library_charges on (qatom.f90:297): q-q pairs take the topology
+    charges (nonbond_qq_lib_charges, md.f90:10872-10975), which are made to
+    differ from both states' FEP charges (0.9 of state 1, kept neutral)."""
+    q = [round(0.9 * s.crg[k], 4) for k in range(n_lig)]
+    q[-1] = round(q[-1] - sum(q), 4)
+    s.crg[:n_lig] = q
+    s.fep_extra["qq_use_library_charges"] = True
+
+
+def _restraint_sections(s: System, heavy: list, seed: int) -> list:
+    """.inp restraint sections (md.f90:3118-3259) over the ligand and a few
+    waters: sequence restraints to each atom's topology position and to the
+    mass centre, position restraints for all states and for each state,
+    flat-bottomed distance restraints (one below d1, one above d2, one
+    state-dependent), angle restraints, and a wall whose radius leaves some
+    ligand atoms outside (harmonic) and some inside (Morse)."""
+    rng = np.random.default_rng(seed)
+    x = np.array(s.x)
+    n = s.nat_solute
+    wo = n + 1 + 3 * 7                      # an oxygen of a water (1-based)
+    out = ["", "[sequence_restraints]", f"1 {n // 2} 2.0 0 0", f"{n // 2 + 1} {n} 1.5 1 2",
+           "", "[atom_restraints]"]
+    for k, ipsi in ((3, 0), (11, 1), (27, 2), (wo, 0)):
+        p = x[k - 1] + rng.normal(0.0, 0.3, 3)
+        out.append(f"{k} {p[0]:.4f} {p[1]:.4f} {p[2]:.4f} 2.0 3.0 1.5 {ipsi}")
+    out += ["", "[distance_restraints]"]
+    hv = [k + 1 for k in range(n) if heavy[k]]
+    pairs = [(hv[0], hv[-1]), (hv[2], hv[12]), (hv[5], wo), (hv[8], hv[20])]
+    for m, (a, b) in enumerate(pairs):
+        r = float(np.linalg.norm(x[a - 1] - x[b - 1]))
+        d1, d2 = ((r + 0.5, r + 1.0), (r - 1.0, r - 0.5), (r - 0.2, r + 0.2), (r + 0.3, r + 0.6))[m]
+        out.append(f"{a} {b} {d1:.3f} {d2:.3f} {4.0 + m:.1f} {(0, 0, 1, 2)[m]}")
+    out += ["", "[angle_restraints]"]
+    for m, (a, b, c) in enumerate([(hv[1], hv[4], hv[9]), (hv[3], hv[7], hv[15]), (hv[6], wo, hv[11])]):
+        u, w = x[a - 1] - x[b - 1], x[c - 1] - x[b - 1]
+        th = math.degrees(math.acos(float(np.dot(u, w) / np.linalg.norm(u) / np.linalg.norm(w))))
+        out.append(f"{a} {b} {c} {th + 7.0 * (m + 1):.2f} {5.0 + m:.1f} {m}")
+    out += ["", "[wall_restraints]", f"1 {n} 3.0 10.0 2.0 0.5 1"]
+    return out
+
+
 def ligand_in_water(radius: float = 20.0, seed: int = 0, n_heavy: int = 30, n_h: int = 20,
                     n_pert: int = 6, alpha: float = 10.0, vdw_rule: int = 1,
-                    sc_max_potential: bool = False, offdiag: tuple = (), water: str = "spc") -> System:
+                    sc_max_potential: bool = False, offdiag: tuple = (), water: str = "spc",
+                    extras: tuple = ()) -> System:
+    """Seeded ligand in an SPC (or 3-atom) water sphere.  ``extras`` adds, on
+    the same system: "fep" the further FEP sections, "libq" library charges
+    for q-q pairs, "restr" the restraint sections of the .inp."""
     rng = np.random.default_rng(seed)
     s = System(name="ligwat", radius=radius, vdw_rule=vdw_rule, sc_max_potential=sc_max_potential,
                offdiag=list(offdiag), water=water)
@@ -548,6 +707,12 @@ def ligand_in_water(radius: float = 20.0, seed: int = 0, n_heavy: int = 30, n_h:
     _strain_free(s)
     _ligand_fep(rng, s, 1, n, types, n_pert, alpha, nb, heavy)
     _add_water_lattice(rng, s, radius, grid)
+    if "fep" in extras:
+        _fep_sections(s, nb, heavy, seed + 1000)
+    if "libq" in extras:
+        _library_charges(s, n)
+    if "restr" in extras:
+        s.inp_extra = _restraint_sections(s, heavy, seed + 2000)
     return s
 
 
@@ -570,9 +735,11 @@ def _unit_template():
 
 
 def protein_leg(radius: float = 25.0, seed: int = 2, n_units: int = 160, n_lig: int = 40,
-                n_pert: int = 6, alpha: float = 10.0, core: float = 21.5, vdw_rule: int = 1) -> System:
+                n_pert: int = 6, alpha: float = 10.0, core: float = 21.5, vdw_rule: int = 1,
+                fill: bool = True) -> System:
     """Synthetic chain of ``n_units`` 16-atom units wrapped around a central
-    two-state ligand, solvated by SPC water up to ``radius``."""
+    two-state ligand, solvated by SPC water up to ``radius`` (a lattice, then,
+    with ``fill``, the voids it leaves next to the chain)."""
     rng = np.random.default_rng(seed)
     s = System(name="protlig", radius=radius, vdw_rule=vdw_rule)
     grid = _Grid()
@@ -747,7 +914,14 @@ def protein_leg(radius: float = 25.0, seed: int = 2, n_units: int = 160, n_lig: 
     s.ncgp_solute = len(s.cgps)
     _strain_free(s)
     _ligand_fep(rng, s, 1, lig_n, ltypes, n_pert, alpha, lnb, lheavy)
-    _add_water_lattice(rng, s, radius, grid, heavy_grid=hgrid)
+    first_water = s.natom + 1
+    if fill:
+        # a 3.0 A lattice, then the voids next to the chain: ~1700 waters in
+        # the 25 A sphere around the 2600 solute atoms
+        _add_water_lattice(rng, s, radius, grid, spacing=3.0, heavy_grid=hgrid)
+        _fill_water_gaps(rng, s, radius, grid, hgrid, first_water)
+    else:
+        _add_water_lattice(rng, s, radius, grid, heavy_grid=hgrid)
     return s
 
 
@@ -910,9 +1084,12 @@ def write_top(s: System, path: str) -> None:
 def write_fep(s: System, path: str) -> None:
     if not s.qatoms:
         return
+    fx = s.fep_extra
     out = ["!synthetic two-state perturbation", "[FEP]", "states 2",
-           f"softcore_use_max_potential {'on' if s.sc_max_potential else 'off'}", "",
-           "[atoms]"]
+           f"softcore_use_max_potential {'on' if s.sc_max_potential else 'off'}"]
+    if fx.get("qq_use_library_charges"):
+        out.append("qq_use_library_charges on")
+    out += ["", "[atoms]"]
     for k, a in enumerate(s.qatoms, 1):
         out.append(f"{k:5d}{a:6d}")
     out += ["", "[change_charges]"]
@@ -921,6 +1098,10 @@ def write_fep(s: System, path: str) -> None:
     out += ["", "[atom_types]"]
     for (nm, sig, eps, mass) in LJ_TYPES:
         (a, b), (ap, bp), (a14, b14) = lj_columns(sig, eps, s.vdw_rule)
+        if "exp_repulsion" in fx:
+            # the second column of q types gives soft pairs their repulsion
+            # A exp(-B r) (nonbond_qq, md.f90:10838-10842)
+            ap, bp = fx["exp_repulsion"]
         out.append(f"{nm:<6s}{a:12.4f}{b:10.5f}{ap:12.4f}{bp:10.5f}{a14:12.4f}{b14:10.5f}{mass:9.4f}")
     out += ["", "[change_atoms]"]
     for k, (t1, t2) in enumerate(s.qtypes, 1):
@@ -928,10 +1109,18 @@ def write_fep(s: System, path: str) -> None:
     out += ["", "[softcore]"]
     for k, (a1, a2) in enumerate(s.qalpha, 1):
         out.append(f"{k:5d}{a1:8.2f}{a2:8.2f}")
+    for sec, key, fmt in (("soft_pairs", "soft_pairs", "{:5d}{:5d}"),
+                          ("el_scale", "el_scale", "{:5d}{:5d}{:8.3f}{:8.3f}"),
+                          ("excluded_pairs", "excluded_pairs", "{:6d}{:6d}{:4d}{:4d}")):
+        if fx.get(key):
+            out += ["", f"[{sec}]"] + [fmt.format(*r) for r in fx[key]]
     if s.qbonds:
         out += ["", "[bond_types]"]
-        for c, (fk, r0) in enumerate(s.qbond_lib, 1):
-            out.append(f"{c:6d} {fk:10.2f} {r0:10.5f}")
+        for c, par in enumerate(s.qbond_lib, 1):
+            if len(par) == 3:   # Morse: E_diss, alpha, b0
+                out.append(f"{c:6d} {par[0]:10.2f} {par[1]:10.4f} {par[2]:10.5f}")
+            else:
+                out.append(f"{c:6d} {par[0]:10.2f} {par[1]:10.5f}")
         out += ["", "[change_bonds]"]
         for (i, j, c1, c2) in s.qbonds:
             out.append(f"{i:6d} {j:6d} {c1:5d} {c2:5d}")
@@ -949,6 +1138,8 @@ def write_fep(s: System, path: str) -> None:
         out += ["", "[change_torsions]"]
         for (i, j, k, l, c1, c2) in s.qtorsions:
             out.append(f"{i:6d}{j:6d}{k:6d}{l:6d}{c1:4d}{c2:4d}")
+    if fx.get("shake"):
+        out += ["", "[shake_constraints]"] + [f"{i:6d}{j:6d}{d1:10.4f}{d2:10.4f}" for (i, j, d1, d2) in fx["shake"]]
     if s.offdiag:
         out += ["", "[off_diagonals]"]
         for (i, j, k, l, a, mu) in s.offdiag:
@@ -970,7 +1161,8 @@ def write_inp(path: str, top: str, fep: str | None, final: str, *, steps: int, s
               output: int = 5, energy: int = 10, energy_file: str | None = None,
               restart: str | None = None, bath_coupling: float = 10.0,
               separate_scaling: bool = True, polarization: bool = True, trajectory: int = 0,
-              trajectory_file: str | None = None, trajectory_atoms: list | None = None) -> None:
+              trajectory_file: str | None = None, trajectory_atoms: list | None = None,
+              extra_lines: list | None = None) -> None:
     onoff = lambda v: "on" if v else "off"  # noqa: E731
     out = ["[MD]", f"steps {steps}", f"stepsize {stepsize}", f"temperature {temperature}",
            f"bath_coupling {bath_coupling}"]
@@ -1003,6 +1195,8 @@ def write_inp(path: str, top: str, fep: str | None, final: str, *, steps: int, s
         out += ["", "[lambdas]", " ".join(f"{v:.6f}" for v in lambdas)]
     if trajectory_file and trajectory_atoms:
         out += ["", "[trajectory_atoms]"] + list(trajectory_atoms)
+    if extra_lines:
+        out += list(extra_lines)
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
 
@@ -1025,6 +1219,15 @@ CONFIGS = {
                {"lrf": True}),
     # the protein leg with the arithmetic LJ combination rule
     "protarith": (protein_leg, {"radius": 25.0, "vdw_rule": 2}, {"lrf": True, "shake_hydrogens": True}),
+    # the ligand in water with the further FEP sections: el_scale,
+    # soft_pairs, excluded_pairs, a Morse q-bond, shake_constraints
+    "ligfepx": (ligand_in_water, {"radius": 20.0, "extras": ("fep",)}, {"lrf": True}),
+    # library charges for q-q pairs under either LJ rule
+    # (nonbond_qq_lib_charges / nonbon2_qq_lib_charges)
+    "liglibq": (ligand_in_water, {"radius": 20.0, "extras": ("libq",)}, {"lrf": True}),
+    "ligarithlibq": (ligand_in_water, {"radius": 20.0, "vdw_rule": 2, "extras": ("libq",)}, {"lrf": True}),
+    # sequence, position, distance, angle and wall restraints (p_restrain)
+    "ligrestr": (ligand_in_water, {"radius": 20.0, "extras": ("restr",)}, {"lrf": True}),
 }
 
 
@@ -1042,14 +1245,21 @@ def make_window(config: str, outdir: str, *, seed: int = 0, md_seed: int = 1, la
     fep = f"{config}_s{seed}.fep"
     if system is None and not os.path.exists(os.path.join(outdir, top)):
         system = builder(seed=seed, **bkw)
+    # restraint sections of the system's .inp files, kept beside the topology
+    rst = os.path.join(outdir, f"{config}_s{seed}.rst")
     if system is not None:
         write_top(system, os.path.join(outdir, top))
         write_fep(system, os.path.join(outdir, fep))
+        if system.inp_extra:
+            with open(rst, "w") as f:
+                f.write("\n".join(system.inp_extra) + "\n")
+    extra = open(rst).read().splitlines() if os.path.exists(rst) else None
     has_fep = os.path.exists(os.path.join(outdir, fep))
     kw = dict(ikw)
     kw.update(inp_overrides)
     inp = os.path.join(outdir, f"{tag}.inp")
     write_inp(inp, top, fep if has_fep else None, f"{tag}.re", steps=steps, seed=md_seed,
               lambdas=(1.0 - lam, lam) if has_fep else None, nb_cycle=nb_cycle, output=output,
-              energy=energy, energy_file=f"{tag}.en" if has_fep else None, restart=restart, **kw)
+              energy=energy, energy_file=f"{tag}.en" if has_fep else None, restart=restart,
+              extra_lines=extra, **kw)
     return inp

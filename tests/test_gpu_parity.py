@@ -21,10 +21,14 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # ligand in water, one state, sequence restraint) and the protein leg under
 # the arithmetic LJ combination rule (nonbon2_*, md.f90:8972-10227), and
 # the ligand in water with soft-core maximum potentials under either rule
-# (sc_lookup, qatom.f90:1600-1664), and the ligand in a 3-atom water with LJ
-# on the hydrogens (nonbond_3atomsolvent, nonbond_qw_3atom)
+# (sc_lookup, qatom.f90:1600-1664), the ligand in a 3-atom water with LJ
+# on the hydrogens (nonbond_3atomsolvent, nonbond_qw_3atom), the ligand with
+# el_scale, soft pairs, excluded pairs, a Morse q-bond and FEP SHAKE
+# constraints, with library charges for q-q pairs under either LJ rule
+# (nonbond_qq_lib_charges, nonbon2_qq_lib_charges), and with sequence,
+# position, distance, angle and wall restraints (p_restrain)
 CASES = {"water": 0, "ligwat": 0, "protein": 2, "kat": 0, "protarith": 2, "ligmaxpot": 0, "ligarithmax": 0,
-         "ligtip3": 0}
+         "ligtip3": 0, "ligfepx": 0, "liglibq": 0, "ligarithlibq": 0, "ligrestr": 0}
 
 
 def _rel(a, b, floor=1.0):
@@ -77,7 +81,8 @@ def test_step0_energies(evaluated, cfg):
         assert _rel(E[k], ref[k]) < 1e-6, k
 
 
-@pytest.mark.parametrize("cfg", ["ligwat", "protein", "kat", "protarith", "ligmaxpot", "ligarithmax", "ligtip3"])
+@pytest.mark.parametrize("cfg", ["ligwat", "protein", "kat", "protarith", "ligmaxpot", "ligarithmax", "ligtip3",
+                                 "ligfepx", "liglibq", "ligarithlibq", "ligrestr"])
 def test_step0_q_energies(evaluated, cfg):
     e, w, dump = evaluated[cfg]
     EQ = e.q_energies(w)

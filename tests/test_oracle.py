@@ -18,10 +18,14 @@ from oracle import qdyn_np as O
 from tests.golden.make_golden import E_NAMES, canonical_lists, file_sha, input_files, sha
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-# five seeded synthetic windows (the protein leg under the arithmetic LJ
-# rule, the ligand in a 3-atom water with LJ on the hydrogens) and the
-# reference's own test system
-CASES = [("water", 0), ("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2), ("ligtip3", 0)]
+# seeded synthetic windows (the protein leg under the arithmetic LJ rule,
+# the ligand in a 3-atom water with LJ on the hydrogens, the ligand with the
+# further FEP sections -- el_scale, soft pairs, excluded pairs, a Morse
+# q-bond, FEP SHAKE constraints --, with library charges for q-q pairs under
+# either rule, and with sequence, position, distance, angle and wall
+# restraints) and the reference's own test system
+CASES = [("water", 0), ("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2), ("ligtip3", 0), ("ligfepx", 0),
+         ("liglibq", 0), ("ligarithlibq", 0), ("ligrestr", 0)]
 
 
 def load_gold(cfg, seed):
