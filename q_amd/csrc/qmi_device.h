@@ -68,6 +68,9 @@ struct WinDev {
   int iac_w[3];                     // 0-based types of the solvent atoms
   float crg_w[3], aw[3], bw[3];     // charges and LJ columns of the solvent atoms
   float ww3_a[9], ww3_b[9];         // A, B of solvent atom pairs (p, q) at [3p + q]
+  // the same in FP64, for the energy terms of energy steps
+  double el14d, aO1d, bO1d, ww_ad, ww_bd;
+  double awd[3], bwd[3], ww3_ad[9], ww3_bd[9];
   double dt, tau_t, temp0;
   double rcpp2, rcpw2, rcww2, rclrf2;
   double fk_pshell, fk_wsphere, fkwpol, dwmz, awmz, rwat;

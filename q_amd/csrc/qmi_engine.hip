@@ -256,13 +256,16 @@ static int build_device(qmi_engine* e) {
     D.ndegf = W.ndegf; D.ndegfree = W.ndegfree; D.ndegf_solute = W.ndegf_solute;
     D.ndegfree_solute = W.ndegfree_solute; D.ndegf_solvent = W.ndegf_solvent; D.ndegfree_solvent = W.ndegfree_solvent;
     D.el14 = (float)t.el14_scale; D.crg_ow = W.crg_ow; D.crg_hw = W.crg_hw;
+    D.el14d = t.el14_scale;
     D.arith = t.ivdw_rule == 2 ? 1 : 0;
     D.wat3 = (W.nwat > 0 && t.solvent_type == 1) ? 1 : 0;
     for (int k = 0; k < 3; ++k) {
       D.iac_w[k] = W.nwat > 0 ? t.iac[NS + k] - 1 : 0;
       D.crg_w[k] = W.nwat > 0 ? (D.wat3 ? t.crg[NS + k] : (k == 0 ? W.crg_ow : W.crg_hw)) : 0.f;
-      D.aw[k] = W.nwat > 0 ? (float)t.iaclib[t.iac[NS + k]].avdw[0] : 0.f;
-      D.bw[k] = W.nwat > 0 ? (float)t.iaclib[t.iac[NS + k]].bvdw[0] : 0.f;
+      D.awd[k] = W.nwat > 0 ? t.iaclib[t.iac[NS + k]].avdw[0] : 0.0;
+      D.bwd[k] = W.nwat > 0 ? t.iaclib[t.iac[NS + k]].bvdw[0] : 0.0;
+      D.aw[k] = (float)D.awd[k];
+      D.bw[k] = (float)D.bwd[k];
     }
     if (D.wat3) {
       // A, B of the nine solvent atom pairs (nonbond_3atomsolvent,
@@ -273,6 +276,8 @@ static int build_device(qmi_engine* e) {
           const AtomType& tq = t.iaclib[t.iac[NS + q]];
           double A, B;
           lj_ab(tp.avdw[0], tp.bvdw[0], tq.avdw[0], tq.bvdw[0], D.arith != 0, &A, &B);
+          D.ww3_ad[3 * p + q] = A;
+          D.ww3_bd[3 * p + q] = B;
           D.ww3_a[3 * p + q] = (float)A;
           D.ww3_b[3 * p + q] = (float)B;
         }
@@ -281,11 +286,14 @@ static int build_device(qmi_engine* e) {
       const AtomType& o = t.iaclib[t.iac[NS]];
       D.aO1 = (float)o.avdw[0];
       D.bO1 = (float)o.bvdw[0];
+      D.aO1d = o.avdw[0];
+      D.bO1d = o.bvdw[0];
+      // A_OO, B_OO (nonbond_ww_spc, md.f90:11839-11843; nonbon2_ww for the
+      // arithmetic rule) in FP64, and the FP32 pair arithmetic's copies
+      lj_ab(o.avdw[0], o.bvdw[0], o.avdw[0], o.bvdw[0], D.arith != 0, &D.ww_ad, &D.ww_bd);
       if (D.arith) {
-        double A, B;
-        lj_ab(o.avdw[0], o.bvdw[0], o.avdw[0], o.bvdw[0], true, &A, &B);
-        D.ww_a = (float)A;
-        D.ww_b = (float)B;
+        D.ww_a = (float)D.ww_ad;
+        D.ww_b = (float)D.ww_bd;
       } else {
         D.ww_a = D.aO1 * D.aO1;
         D.ww_b = D.bO1 * D.bO1;
@@ -1405,7 +1413,7 @@ static int autotune(qmi_engine* e) {
     double bt = 1e30;
     for (int b : bs) {
       *field[g] = b;
-      const double t = g == 0 ? time_us([&] { launch_nonbonded(d, e->stream, true); })
+      const double t = g == 0 ? time_us([&] { launch_nonbonded(d, e->stream, true, false); })
                               : time_us([&] { launch_bonded(d, e->stream); });
       if (t < bt) { bt = t; bb = b; }
     }
@@ -1439,8 +1447,8 @@ static int autotune(qmi_engine* e) {
     for (int x = 0; x < 2; ++x) {
       d.xcd = x;
       tx[x] = time_us([&] {
-        launch_nonbonded(d, e->stream, true);
-        launch_q_tiles(d, e->stream);
+        launch_nonbonded(d, e->stream, true, false);
+        launch_q_tiles(d, e->stream, false);
         launch_lrf_build(d, e->stream);
       });
     }
@@ -1454,7 +1462,7 @@ static int autotune(qmi_engine* e) {
     for (int a = 0; a < 2; ++a) {
       d.bond_atomic = a;
       ta[a] = time_us([&] {
-        launch_nonbonded(d, e->stream, false);
+        launch_nonbonded(d, e->stream, false, false);
         launch_other_forces(d, e->stream);
       });
     }
@@ -1820,13 +1828,16 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
         HIP_OK(hipGraphLaunch(e->step_exec[(lists_at(done + k) ? 1 : 0) | (energy_at(done + k) ? 2 : 0)], e->stream));
     }
     for (int k = 0; k < chunk && (e->timing_on || !e->step_exec[0]); ++k) {
-      launch_begin(e->d, e->stream, 0, 0, 3);
+      // the same kernel instances as the step graph of this step: those
+      // with energy terms only where some window computes energies
+      const bool qen = energy_at(done + k);
+      launch_begin(e->d, e->stream, 0, 0, 1 | (qen ? 2 : 0));
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[0], e->stream));
       launch_lists(e->d, e->stream, e->ngrp, e->nwc, e->nsc);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[1], e->stream));
-      launch_ww(e->d, e->stream);
+      launch_ww(e->d, e->stream, qen);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[2], e->stream));
-      launch_nonbonded(e->d, e->stream, false);
+      launch_nonbonded(e->d, e->stream, false, qen);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[3], e->stream));
       launch_other_forces(e->d, e->stream);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[4], e->stream));
@@ -1937,6 +1948,13 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     // final pair lists, energies, summary and restart (md.f90:4207-4213)
     rc = evaluate(e, 1, 3);
     if (rc) return rc;
+    // the final pot_energy updates the water-polarisation theta_corr at a
+    // multiple of 100 steps (md.f90:15198-15210) before write_xfin
+    std::vector<int> fin;
+    for (int wi = 0; wi < NW; ++wi)
+      if (e->wins[wi].status == 0 && !e->wins[wi].done && e->wins[wi].istep >= e->wins[wi].w->inp.nsteps)
+        fin.push_back(wi);
+    if (!fin.empty() && (rc = sync_state_to_host(e, &fin)) != 0) return rc;
     for (int wi = 0; wi < NW; ++wi) {
       HostWin& h = e->wins[wi];
       if (h.status != 0 || h.done || h.istep < h.w->inp.nsteps) continue;

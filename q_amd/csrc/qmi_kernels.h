@@ -7,8 +7,10 @@
 namespace qmi {
 void launch_begin(const Dev& d, hipStream_t s, int force_energy, int eval_only, int rebuild);
 void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc);
-// water-water tiles alone, and all nonbonded tile kernels (with or without them)
-void launch_ww(const Dev& d, hipStream_t s);
+// water-water tiles alone, and all nonbonded tile kernels (with or without
+// them); `energy` / `q_energy`: the kernel instances that also evaluate the
+// energy terms of windows at an energy step (false: forces only)
+void launch_ww(const Dev& d, hipStream_t s, bool energy = true);
 // single families, for the autotuner's timing runs
 void launch_q_tiles(const Dev& d, hipStream_t s, bool q_energy = true);
 void launch_lrf_build(const Dev& d, hipStream_t s);

@@ -35,6 +35,22 @@ __device__ __forceinline__ int block_id(int xcd) {
 
 __device__ __forceinline__ long long fix(double v, double scale) { return __double2ll_rn(v * scale); }
 
+// 1/sqrt(a) and 1/a in FP64: the hardware estimate and two Newton steps
+// (each squares the relative error, so the result is within a few ulp)
+__device__ __forceinline__ double rsqrt_d(double r2) {
+  const double h = 0.5 * r2;
+  double y = __builtin_amdgcn_rsq(r2);
+  y = fma(y, fma(-(h * y), y, 0.5), y);
+  y = fma(y, fma(-(h * y), y, 0.5), y);
+  return y;
+}
+__device__ __forceinline__ double rcp_d(double a) {
+  double y = __builtin_amdgcn_rcp(a);
+  y = fma(y, fma(-a, y, 1.0), y);
+  y = fma(y, fma(-a, y, 1.0), y);
+  return y;
+}
+
 __device__ __forceinline__ void add_grad(long long* f, int atom, double gx, double gy, double gz) {
   atomicAdd((unsigned long long*)&f[3 * atom + 0], (unsigned long long)fix(gx, kFScale));
   atomicAdd((unsigned long long*)&f[3 * atom + 1], (unsigned long long)fix(gy, kFScale));
@@ -67,22 +83,6 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
-// energy of a term into slot `slot` of window w: lanes of a wave that agree
-// on (w, slot) are summed first and added with one atomic
-__device__ __forceinline__ void wave_energy(WinState* ws, int w, int slot, double e, bool valid) {
-  const unsigned long long vm = __ballot(valid);
-  if (vm == 0ull) return;
-  const int first = __ffsll((long long)vm) - 1;
-  const int key = valid ? w * E_NSLOT + slot : -1;
-  const int key0 = __shfl(key, first, 64);
-  const bool uniform = __ballot(valid && key != key0) == 0ull;
-  if (uniform) {
-    const double sum = wave_sum(valid ? e : 0.0);
-    if ((int)(threadIdx.x & 63) == first) add_energy(ws, w, slot, sum);
-  } else if (valid) {
-    add_energy(ws, w, slot, e);
-  }
-}
 
 // integer forms: exact, so the order of the sums does not matter
 __device__ __forceinline__ long long wave_sum_i(long long v) {
@@ -102,6 +102,26 @@ __device__ __forceinline__ void add_grad_fix(long long* f, int atom, long long g
 }
 __device__ __forceinline__ void add_energy_fix(WinState* ws, int w, int slot, long long e) {
   if (e != 0) atomicAdd((unsigned long long*)&ws[w].E[slot], (unsigned long long)e);
+}
+// energy of a term into slot `slot` of window w: each lane's term goes to
+// fixed point, then lanes of a wave that agree on (w, slot) are summed and
+// added with one atomic.  The integer sums are exact, so the result does not
+// depend on which terms share a wave (it is the same for a window alone or
+// batched with others).
+__device__ __forceinline__ void wave_energy(WinState* ws, int w, int slot, double e, bool valid) {
+  const unsigned long long vm = __ballot(valid);
+  if (vm == 0ull) return;
+  const int first = __ffsll((long long)vm) - 1;
+  const int key = valid ? w * E_NSLOT + slot : -1;
+  const int key0 = __shfl(key, first, 64);
+  const bool uniform = __ballot(valid && key != key0) == 0ull;
+  const long long ef = valid ? fix(e, kEScale) : 0ll;
+  if (uniform) {
+    const long long sum = wave_sum_i(ef);
+    if ((int)(threadIdx.x & 63) == first) add_energy_fix(ws, w, slot, sum);
+  } else if (valid) {
+    add_energy_fix(ws, w, slot, ef);
+  }
 }
 
 // sum over the 8 lanes that share lane>>3
@@ -384,6 +404,11 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   double eel = 0.0, evdw = 0.0;
+  // energy steps: the energy terms in FP64 from the FP64 coordinates, as
+  // nonbond_ww_spc / nonbond_3atomsolvent evaluate them
+  double xi[9];
+  if (en)
+    for (int k = 0; k < 9; ++k) xi[k] = oa >= 0 ? d.x[3 * oa + k] : 0.0;
   const int* ent = L.ent + (size_t)row * L.cap;
   for (int t = 0; t < nch; ++t) {
     const int k = t * kCl + b;
@@ -395,6 +420,22 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
       pj[s] = make_float3(q.x, q.y, q.z);
     }
     const bool cnt = en && oa < ob;   // each unordered pair counted once
+    if (cnt) {
+      double xj[9];
+      for (int m = 0; m < 9; ++m) xj[m] = d.x[3 * ob + m];
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) {
+          const double dx = xj[3 * q] - xi[3 * p], dy = xj[3 * q + 1] - xi[3 * p + 1],
+                       dz = xj[3 * q + 2] - xi[3 * p + 2];
+          const double rinv = rsqrt_d(dx * dx + dy * dy + dz * dz);
+          eel += (double)qq[p][q] * rinv;
+          if (W3 || (p == 0 && q == 0)) {
+            const double r2i = rinv * rinv, r6 = r2i * r2i * r2i;
+            const double a_ = W3 ? W.ww3_ad[3 * p + q] : W.ww_ad, b_ = W3 ? W.ww3_bd[3 * p + q] : W.ww_bd;
+            evdw += a_ * r6 * r6 - b_ * r6;
+          }
+        }
+    }
     for (int p = 0; p < 3; ++p) {
       for (int q = 0; q < 3; ++q) {
         float dx = pj[q].x - pi[p].x, dy = pj[q].y - pi[p].y, dz = pj[q].z - pi[p].z;
@@ -408,11 +449,9 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
           const float a_ = W3 ? W.ww3_a[3 * p + q] : A, b_ = W3 ? W.ww3_b[3 * p + q] : B;
           float va = a_ * r6 * r6, vb = b_ * r6;
           dv = r2inv * (-vel - 12.f * va + 6.f * vb);
-          if (cnt) evdw += (double)(va - vb);
         } else {
           dv = r2inv * (-vel);
         }
-        if (cnt) eel += (double)vel;
         g[3 * p + 0] -= dv * dx;
         g[3 * p + 1] -= dv * dy;
         g[3 * p + 2] -= dv * dz;
@@ -432,13 +471,16 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
 
 // W3: the instance for windows with a 3-atom solvent (launched only when
 // there are any); each instance skips the other windows' rows
-template <bool W3>
+// EK: the instance launched on steps where some window computes energies
+// (it also serves that step's other windows); the instance without energy
+// terms keeps the smaller register footprint of the force-only loop
+template <bool W3, bool EK>
 __global__ void __launch_bounds__(256) k_ww(Dev d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   if (wave >= d.rl[T_WW].nicl) return;
   const int w = d.wc_win[wave];
   if ((d.win[w].wat3 != 0) != W3) return;
-  if (d.ws[w].energy_now) ww_impl<true, W3>(d);
+  if (EK && d.ws[w].energy_now) ww_impl<EK, W3>(d);
   else ww_impl<false, W3>(d);
 }
 
@@ -476,6 +518,15 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
   double eel = 0.0, evdw = 0.0;
+  // energy steps: the energy terms in FP64 from the FP64 coordinates and
+  // LJ columns, as nonbond_pw / nonbond_3atomsolvent evaluate them
+  double xs[3] = {0, 0, 0}, Ad[3] = {0, 0, 0}, Bd[3] = {0, 0, 0};
+  if (en && ea >= 0) {
+    for (int c = 0; c < 3; ++c) xs[c] = d.x[3 * ea + c];
+    const double* tl = &d.t_lj[(size_t)(W.t_off + d.iac[ea]) * 4];
+    for (int k = 0; k < (wat3 ? 3 : 1); ++k)
+      lj_ab(tl[0], tl[1], k == 0 ? W.aO1d : W.awd[k], k == 0 ? W.bO1d : W.bwd[k], arith, &Ad[k], &Bd[k]);
+  }
   const int* ent = L.ent + (size_t)row * L.cap;
   // kUnroll entries per pass, loads first (as in k_pp)
   for (int t0 = 0; t0 < nch; t0 += kUnroll) {
@@ -529,7 +580,17 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
           dv = r2inv * (-vel);
         }
         if (!v[u]) continue;
-        if (en) { eel += (double)vel; if (k == 0 || wat3) evdw += (double)ev; }
+        if (en) {
+          const double ddx = d.x[3 * (e[u] + k)] - xs[0], ddy = d.x[3 * (e[u] + k) + 1] - xs[1],
+                       ddz = d.x[3 * (e[u] + k) + 2] - xs[2];
+          const double rinv_d = rsqrt_d(ddx * ddx + ddy * ddy + ddz * ddz);
+          eel += (double)(ps.w * pw[k].w) * rinv_d;
+          if (k == 0 || wat3) {
+            const double r2i = rinv_d * rinv_d, r6d = r2i * r2i * r2i;
+            evdw += Ad[k] * r6d * r6d - Bd[k] * r6d;
+          }
+        }
+        (void)ev;
         if (WATER_SIDE) {
           g[3 * k + 0] += dv * dx; g[3 * k + 1] += dv * dy; g[3 * k + 2] += dv * dz;
         } else {
@@ -550,13 +611,13 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
   }
 }
 
-template <bool WATER_SIDE, bool W3>
+template <bool WATER_SIDE, bool W3, bool EK>
 __global__ void __launch_bounds__(256) k_pw(Dev d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   if (wave >= d.rl[WATER_SIDE ? T_WP : T_PW].nicl) return;
   const int w = WATER_SIDE ? d.wc_win[wave] : d.sc_win[wave];
   if ((d.win[w].wat3 != 0) != W3) return;
-  if (!WATER_SIDE && d.ws[w].energy_now) pw_impl<WATER_SIDE, true, W3>(d);
+  if (EK && !WATER_SIDE && d.ws[w].energy_now) pw_impl<WATER_SIDE, EK, W3>(d);
   else pw_impl<WATER_SIDE, false, W3>(d);
 }
 
@@ -584,6 +645,14 @@ __device__ __forceinline__ void pp_impl(const Dev& d) {
   constexpr bool en = EN;
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   double eel = 0.0, evdw = 0.0;
+  // energy steps: the energy terms in FP64 from the FP64 coordinates and LJ
+  // columns, as nonbond_pp evaluates them
+  double xs[3] = {0, 0, 0};
+  const double* tli = nullptr;
+  if (en && ea >= 0) {
+    for (int c = 0; c < 3; ++c) xs[c] = d.x[3 * ea + c];
+    tli = &d.t_lj[(size_t)(W.t_off + d.iac[ea]) * 4];
+  }
   const int* ent = L.ent + (size_t)row * L.cap;
   // kUnroll entries per pass: all their loads are issued before the
   // arithmetic; entries past the row end read atom 0 and are masked, and
@@ -621,7 +690,22 @@ __device__ __forceinline__ void pp_impl(const Dev& d) {
       float dv = r2inv * (-vel - 12.f * va + 6.f * vb);
       if (v[u]) {
         g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz;
-        if (en && ea < eb) { eel += (double)vel; evdw += (double)(va - vb); }
+        if (en && ea < eb) {
+          const double ddx = d.x[3 * eb] - xs[0], ddy = d.x[3 * eb + 1] - xs[1], ddz = d.x[3 * eb + 2] - xs[2];
+          const double rinv_d = rsqrt_d(ddx * ddx + ddy * ddy + ddz * ddz);
+          const double r2i = rinv_d * rinv_d, r6d = r2i * r2i * r2i;
+          const double* tlj = &d.t_lj[(size_t)(W.t_off + d.iac[eb]) * 4];
+          double ad, bd;
+          double vd = (double)(pi.w * pj[u].w) * rinv_d;
+          if (is14) {
+            vd *= W.el14d;
+            lj_ab(tli[2], tli[3], tlj[2], tlj[3], arith, &ad, &bd);
+          } else {
+            lj_ab(tli[0], tli[1], tlj[0], tlj[1], arith, &ad, &bd);
+          }
+          eel += vd;
+          evdw += ad * r6d * r6d - bd * r6d;
+        }
       }
     }
   }
@@ -634,10 +718,11 @@ __device__ __forceinline__ void pp_impl(const Dev& d) {
   }
 }
 
+template <bool EK>
 __global__ void __launch_bounds__(256) k_pp(Dev d) {
   const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
   if (wave >= d.rl[T_PP].nicl) return;
-  if (d.ws[d.sc_win[wave]].energy_now) pp_impl<true>(d);
+  if (EK && d.ws[d.sc_win[wave]].energy_now) pp_impl<EK>(d);
   else pp_impl<false>(d);
 }
 
@@ -645,21 +730,6 @@ __global__ void __launch_bounds__(256) k_pp(Dev d) {
 // q-atom vs solute atom (nonbond_qp_qvdw / nonbond_qp, md.f90:11138-11204,
 // 10980-11045), all states; gradients are lambda-weighted over states.
 
-// 1/sqrt(a) and 1/a in FP64: the hardware estimate and two Newton steps
-// (each squares the relative error, so the result is within a few ulp)
-__device__ __forceinline__ double rsqrt_d(double r2) {
-  const double h = 0.5 * r2;
-  double y = __builtin_amdgcn_rsq(r2);
-  y = fma(y, fma(-(h * y), y, 0.5), y);
-  y = fma(y, fma(-(h * y), y, 0.5), y);
-  return y;
-}
-__device__ __forceinline__ double rcp_d(double a) {
-  double y = __builtin_amdgcn_rcp(a);
-  y = fma(y, fma(-a, y, 1.0), y);
-  y = fma(y, fma(-a, y, 1.0), y);
-  return y;
-}
 
 // Q-atom tiles: lane (a, b) of a work item takes member a of the i-cluster
 // and member b of each listed j-cluster.  Forces are computed once per pair,
@@ -1099,7 +1169,7 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
       const double ta = tl[k], tb = tl[k + 1];
       for (int s = 0; s < S; ++s) {
         double vel = (double)(d.q_crg[W.qs_off + iq_l * S + s] * qj) * rinv;
-        if (is14) vel *= (double)W.el14;
+        if (is14) vel *= W.el14d;
         eel[s] += vel;
         const double* ql = &d.q_ljd[(size_t)(W.qs_off + iq_l * S + s) * 4];
         double aLJ, bLJ;
@@ -1173,7 +1243,7 @@ __global__ void __launch_bounds__(128) k_qq(Dev d) {
                      ddz = d.x[3 * j + 2] - d.x[3 * i + 2];
         const double rr2 = ddx * ddx + ddy * ddy + ddz * ddz;
         double v = qi * qj * rsqrt_d(rr2) * (double)es;
-        if (iLJ == 3) v *= (double)W.el14;
+        if (iLJ == 3) v *= W.el14d;
         e_el = v;
         if (expo) {
           e_vdw = aLJ * exp(-bLJ * sqrt(rr2));
@@ -2350,28 +2420,41 @@ void launch_q_tiles(const Dev& d, hipStream_t s, bool q_energy) {
   launch_q_forces(d, s);
 }
 
-void launch_ww(const Dev& d, hipStream_t s) {
-  if (d.rl[T_WW].nicl) k_ww<false><<<blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
-  if (d.rl[T_WW].nicl && d.any_wat3)
-    k_ww<true><<<blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+template <bool EK>
+static void launch_ww_k(const Dev& d, hipStream_t s) {
+  const int nb = blocks((long long)d.rl[T_WW].nicl * 64, d.bs_rows);
+  if (d.rl[T_WW].nicl) k_ww<false, EK><<<nb, d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_WW].nicl && d.any_wat3) k_ww<true, EK><<<nb, d.bs_rows, 0, s>>>(d);
+}
+void launch_ww(const Dev& d, hipStream_t s, bool energy) {
+  if (energy) launch_ww_k<true>(d, s);
+  else launch_ww_k<false>(d, s);
 }
 
-static void launch_pw(const Dev& d, hipStream_t s) {
-  if (d.rl[T_WP].nicl) k_pw<true, false><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
-  if (d.rl[T_PW].nicl) k_pw<false, false><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
-  if (d.rl[T_WP].nicl && d.any_wat3)
-    k_pw<true, true><<<blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
-  if (d.rl[T_PW].nicl && d.any_wat3)
-    k_pw<false, true><<<blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+template <bool EK>
+static void launch_pw_k(const Dev& d, hipStream_t s) {
+  const int nwp = blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows);
+  const int npw = blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows);
+  if (d.rl[T_WP].nicl) k_pw<true, false, EK><<<nwp, d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_PW].nicl) k_pw<false, false, EK><<<npw, d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_WP].nicl && d.any_wat3) k_pw<true, true, EK><<<nwp, d.bs_rows, 0, s>>>(d);
+  if (d.rl[T_PW].nicl && d.any_wat3) k_pw<false, true, EK><<<npw, d.bs_rows, 0, s>>>(d);
 }
-static void launch_pp(const Dev& d, hipStream_t s) {
-  if (d.rl[T_PP].nicl) k_pp<<<blocks((long long)d.rl[T_PP].nicl * 64, d.bs_rows), d.bs_rows, 0, s>>>(d);
+static void launch_pw(const Dev& d, hipStream_t s, bool energy) {
+  if (energy) launch_pw_k<true>(d, s);
+  else launch_pw_k<false>(d, s);
+}
+static void launch_pp(const Dev& d, hipStream_t s, bool energy) {
+  if (!d.rl[T_PP].nicl) return;
+  const int nb = blocks((long long)d.rl[T_PP].nicl * 64, d.bs_rows);
+  if (energy) k_pp<true><<<nb, d.bs_rows, 0, s>>>(d);
+  else k_pp<false><<<nb, d.bs_rows, 0, s>>>(d);
 }
 
 void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) {
-  if (with_ww) launch_ww(d, s);
-  launch_pw(d, s);
-  launch_pp(d, s);
+  if (with_ww) launch_ww(d, s, q_energy);
+  launch_pw(d, s, q_energy);
+  launch_pp(d, s, q_energy);
   launch_q_tiles(d, s, q_energy);
   if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
 }
@@ -2410,11 +2493,11 @@ hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipSt
   err = hipEventRecord(ev[4], s);
   if (err == hipSuccess) err = hipStreamWaitEvent(s3, ev[4], 0);
   if (err != hipSuccess) return err;
-  launch_pp(d, s3);
+  launch_pp(d, s3, q_energy);
   err = hipEventRecord(ev[1], s3);
   if (err != hipSuccess) return err;
-  launch_ww(d, s);
-  launch_pw(d, s);
+  launch_ww(d, s, q_energy);
+  launch_pw(d, s, q_energy);
   if (q_energy) launch_q_energy_qp(d, s);
   err = hipStreamWaitEvent(s, ev[1], 0);
   if (err == hipSuccess) err = hipStreamWaitEvent(s, ev[3], 0);

@@ -11,6 +11,9 @@ into the local spread of the work (the width of Gaussian work with that
 overlap), which integrates to a thermodynamic length along lambda; the
 proposed ladder puts windows at equal steps of that length, with as many
 windows (16 to the time budget) as the predicted neighbour overlap needs.
+When the predicted interval overlaps disagree with the ones the pilots
+measure by more than FALLBACK_DISAGREEMENT on average, the uniform ladder
+with as many windows runs instead (the fallback).
 The free energy of the ladder (BAR, by q_amd.qfep) is judged against a
 uniform 51-window ladder; the uniform ladder with as many windows is the
 method without the network.
@@ -159,9 +162,33 @@ def measured_overlaps(lams, dEs) -> np.ndarray:
                      for k in range(len(lams) - 1)])
 
 
+# mean |predicted - measured| interval overlap of the pilot grid beyond which
+# the network's ladder is not used and the uniform ladder with as many windows
+# runs instead (the held-out pilot grids of the shipped network disagree by
+# 0.03-0.20, 0.10 on average)
+FALLBACK_DISAGREEMENT = 0.15
+
+
+def learned_schedule(predict, lams, dEs, alpha, nmin=16, nmax=22, bound=FALLBACK_DISAGREEMENT):
+    """The schedule of one perturbation from its pilot windows: the ladder
+    at equal thermodynamic length from the network's interval overlaps, or,
+    when those disagree with the overlaps the pilots measure by more than
+    `bound` on average, the uniform ladder with as many windows (the
+    fallback).  Returns the ladder and a record of the decision."""
+    pred = predicted_overlaps(predict, lams, dEs, alpha)
+    meas = measured_overlaps(lams, dEs)
+    sched, implied = ladder_from_overlaps(lams, pred, nmin, nmax)
+    dis = float(np.mean(np.abs(pred - meas)))
+    fallback = dis > bound
+    if fallback:
+        sched = np.linspace(0.0, 1.0, len(sched))
+    return sched, dict(disagreement=dis, fallback=fallback, implied_neighbour_overlap=implied,
+                       predicted_overlap=pred, pilot_overlap=meas)
+
+
 def propose_n(predict, lams, dEs, alpha, nmin=16, nmax=22):
     """The learned ladder of one perturbation from its pilot windows."""
-    return ladder_from_overlaps(lams, predicted_overlaps(predict, lams, dEs, alpha), nmin, nmax)[0]
+    return learned_schedule(predict, lams, dEs, alpha, nmin, nmax)[0]
 
 
 # ------------------------------------------------------------ running --
@@ -299,9 +326,8 @@ def evaluate(p: Perturbation, system, predict, workdir, pilot_steps, prod_steps,
         pilots = [energy_gap(en, pilot_skip) for en in ens]
     lams = [a for a, _ in pilots]
     dEs = [b for _, b in pilots]
-    pred = predicted_overlaps(predict, lams, dEs, p.alpha)
-    meas = measured_overlaps(lams, dEs)
-    sched, implied = ladder_from_overlaps(lams, pred, nmin, nmax)
+    sched, dec = learned_schedule(predict, lams, dEs, p.alpha, nmin, nmax)
+    pred, meas, implied = dec["predicted_overlap"], dec["pilot_overlap"], dec["implied_neighbour_overlap"]
     n = len(sched)
     ladders = dict(ref51=np.linspace(0.0, 1.0, 51), learned=sched, uniform=np.linspace(0.0, 1.0, n),
                    direct=ladder_from_overlaps(lams, meas, n=n)[0])
@@ -317,7 +343,8 @@ def evaluate(p: Perturbation, system, predict, workdir, pilot_steps, prod_steps,
     ref = st["ref51"]["dG"]
     return dict(perturbation=p.seed, leg=p.leg, n_pert=p.n_pert, windows=n,
                 schedule=[round(float(v), 4) for v in sched], predicted_overlap=[round(float(v), 3) for v in pred],
-                pilot_overlap=[round(float(v), 3) for v in meas], implied_neighbour_overlap=implied, stats=st,
+                pilot_overlap=[round(float(v), 3) for v in meas], implied_neighbour_overlap=implied,
+                overlap_disagreement=dec["disagreement"], fallback=dec["fallback"], stats=st,
                 err_learned=abs(st["learned"]["dG"] - ref), err_uniform=abs(st["uniform"]["dG"] - ref),
                 err_direct=abs(st["direct"]["dG"] - ref),
                 simulated_fraction=simulated_fraction(n, prod_steps, pilot_steps))
@@ -389,7 +416,8 @@ def evaluate_replicas(args):
         dEs = [b for _, b in pilots]
         nmax = max(16, min(24, int((0.5 * 51 * args.prod_steps - len(PILOT_LAMBDAS) * args.pilot_steps)
                                    // args.prod_steps)))
-        sched, implied = ladder_from_overlaps(lams, predicted_overlaps(net.forward, lams, dEs, p.alpha), 16, nmax)
+        sched, dec = learned_schedule(net.forward, lams, dEs, p.alpha, 16, nmax)
+        implied = dec["implied_neighbour_overlap"]
         n = len(sched)
         ladders = dict(ref51=np.linspace(0.0, 1.0, 51), learned=sched, uniform=np.linspace(0.0, 1.0, n),
                        direct=ladder_from_overlaps(lams, measured_overlaps(lams, dEs), n=n)[0])
@@ -413,7 +441,8 @@ def evaluate_replicas(args):
                 stats[name].append(st)
         mean = {k: float(np.mean(v)) for k, v in dG.items()}
         row = dict(perturbation=p.seed, leg=p.leg, n_pert=p.n_pert, windows=n,
-                   schedule=[round(float(v), 4) for v in sched], implied_neighbour_overlap=implied, dG=dG,
+                   schedule=[round(float(v), 4) for v in sched], implied_neighbour_overlap=implied,
+                   overlap_disagreement=dec["disagreement"], fallback=dec["fallback"], dG=dG,
                    dG_mean=mean, replica_spread={k: float(np.std(v)) for k, v in dG.items()},
                    min_overlap={k: float(np.mean([s["min_overlap"] for s in v])) for k, v in stats.items()},
                    sigma={k: float(np.mean([s["sigma"] for s in v])) for k, v in stats.items()},
@@ -429,6 +458,7 @@ def evaluate_replicas(args):
         rep["mean_min_overlap_" + k] = float(np.mean([r["min_overlap"][k] for r in rows]))
         rep["mean_sigma_" + k] = float(np.mean([r["sigma"][k] for r in rows]))
     rep["mean_ref51_replica_spread"] = float(np.mean([r["replica_spread"]["ref51"] for r in rows]))
+    rep["fallbacks"] = int(sum(1 for r in rows if r["fallback"]))
     rep["mean_simulated_fraction"] = float(np.mean([r["simulated_fraction"] for r in rows]))
     json.dump(rep, open(os.path.join(args.out, "schedule_eval.json"), "w"), indent=1)
     log({k: v for k, v in rep.items() if k != "cases"})

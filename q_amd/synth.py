@@ -716,6 +716,63 @@ def ligand_in_water(radius: float = 20.0, seed: int = 0, n_heavy: int = 30, n_h:
     return s
 
 
+def ligand_dual(radius: float = 20.0, seed: int = 0, n_heavy: int = 30, n_h: int = 20, vmax: float = 20.0,
+                vdw_rule: int = 1) -> System:
+    """A QligFEP-style dual-topology perturbation (qligfep.py:311-362) in an
+    SPC sphere: ligand A and ligand B (A's graph with some heavy-atom types
+    changed, new charges, coordinates displaced by up to 0.4 A, so the two
+    overlap as aligned ligands do) are both in the topology and all their
+    atoms are q-atoms.  State 1 has A with its charges and types and B as
+    dummies (charge 0, the zero-LJ type DU); state 2 the reverse.  Soft-core
+    is given as maximum potentials: 0 / vmax for A's atoms, vmax / 0 for B's
+    (softcore_use_max_potential on)."""
+    rng = np.random.default_rng(seed)
+    s = System(name="ligdual", radius=radius, vdw_rule=vdw_rule, sc_max_potential=True)
+    grid = _Grid()
+    xyz, types, nb, heavy = _grow_ligand(rng, n_heavy, n_h, np.zeros(3), grid, 1)
+    c = np.mean(xyz, axis=0)
+    xa = [p - c for p in xyz]
+    n = len(xa)
+    xb = [p + rng.uniform(-0.4, 0.4, 3) for p in xa]
+    types_b = [int(rng.choice([u for u in HEAVY_TYPES if u != t])) if heavy[k] and rng.random() < 0.3 else t
+               for k, t in enumerate(types)]
+    qa = _neutral_charges(rng, n)
+    qb = list(np.round(np.array(qa) + rng.normal(0.0, 0.1, n), 4))
+    qb[-1] = round(qb[-1] - sum(qb), 4)
+    qb = [float(v) for v in qb]
+    grid = _Grid()
+    for p in xa + xb:
+        grid.add(p, "lig")
+    s.x = list(xa) + list(xb)
+    s.iac = list(types) + list(types_b)
+    s.crg = list(qa) + list(qb)
+    for off in (0, n):
+        b, a, t, i = _topology_from_graph(nb, 1 + off, heavy)
+        s.bonds += b
+        s.angles += a
+        s.torsions += t
+        s.impropers += i
+        for k in range(n):
+            if heavy[k]:
+                members = [k + 1 + off] + [h + 1 + off for h in nb[k] if not heavy[h]]
+                s.cgps.append((k + 1 + off, sorted(members)))
+        s.residues.append((1 + off, "LIG" if off == 0 else "LID"))
+        s.molecules.append(1 + off)
+    s.nat_solute = 2 * n
+    s.nbonds_solute = len(s.bonds)
+    s.nangles_solute = len(s.angles)
+    s.ncgp_solute = len(s.cgps)
+    _strain_free(s)
+    names = {t: LJ_TYPES[t - 1][0] for t in range(1, len(LJ_TYPES) + 1)}
+    du = names[T_DU]
+    s.qatoms = list(range(1, 2 * n + 1))
+    s.qcharges = [(qa[k], 0.0) for k in range(n)] + [(0.0, qb[k]) for k in range(n)]
+    s.qtypes = [(names[types[k]], du) for k in range(n)] + [(du, names[types_b[k]]) for k in range(n)]
+    s.qalpha = [(0.0, vmax)] * n + [(vmax, 0.0)] * n
+    _add_water_lattice(rng, s, radius, grid)
+    return s
+
+
 # residue-like unit: 7 heavy atoms + 9 hydrogens = 16 atoms
 # heavy (0-based within unit): N0-C1, C1-C2, C2-C3(=O4), C1-C5-C6 side chain;
 # C3 of one unit bonds N0 of the next.  Hydrogens: N0 1, C1 1, C2 2, C5 2, C6 3.
@@ -1228,6 +1285,9 @@ CONFIGS = {
     "ligarithlibq": (ligand_in_water, {"radius": 20.0, "vdw_rule": 2, "extras": ("libq",)}, {"lrf": True}),
     # sequence, position, distance, angle and wall restraints (p_restrain)
     "ligrestr": (ligand_in_water, {"radius": 20.0, "extras": ("restr",)}, {"lrf": True}),
+    # a QligFEP dual-topology perturbation: both ligands, all atoms q-atoms,
+    # dummies of the zero-LJ type, soft-core maximum potentials 0 / 20
+    "ligdual": (ligand_dual, {"radius": 20.0}, {"lrf": True}),
 }
 
 

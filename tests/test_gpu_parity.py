@@ -3,8 +3,8 @@
 The reference side is the golden vectors of tests/golden/ (written by
 tests/golden/make_golden.py from the unmodified Q6 modules, src/q6/md.f90
 make_pair_lists + pot_energy at step 0) for the same seeded input files.
-Step-0 per-state Q energies must agree within 1e-6 relative, FP64 terms
-within 1e-6, bulk FP32 nonbonded sums within 2e-6 relative + 2e-2 kcal/mol,
+Step-0 energies (per-state Q energies and every term of the potential)
+must agree within 1e-6 relative, the FP32 gradient within 1e-3 of its rms,
 pair lists exactly (as sets of unordered atom pairs).
 """
 import os
@@ -26,9 +26,10 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # el_scale, soft pairs, excluded pairs, a Morse q-bond and FEP SHAKE
 # constraints, with library charges for q-q pairs under either LJ rule
 # (nonbond_qq_lib_charges, nonbon2_qq_lib_charges), and with sequence,
-# position, distance, angle and wall restraints (p_restrain)
+# position, distance, angle and wall restraints (p_restrain), and a
+# QligFEP-style dual-topology perturbation (qligfep.py:311-362)
 CASES = {"water": 0, "ligwat": 0, "protein": 2, "kat": 0, "protarith": 2, "ligmaxpot": 0, "ligarithmax": 0,
-         "ligtip3": 0, "ligfepx": 0, "liglibq": 0, "ligarithlibq": 0, "ligrestr": 0}
+         "ligtip3": 0, "ligfepx": 0, "liglibq": 0, "ligarithlibq": 0, "ligrestr": 0, "ligdual": 0}
 
 
 def _rel(a, b, floor=1.0):
@@ -64,25 +65,22 @@ def test_step0_energies(evaluated, cfg):
               "restraint.total", "restraint.fix", "restraint.shell", "restraint.solvent_radial",
               "restraint.water_pol", "potential"]:
         print(f"  {k:18s} gpu {E[k]: .10f} ref {ref[k]: .10f} rel {_rel(E[k], ref[k]):.2e}")
-    # bulk nonbonded sums: FP32 pair arithmetic (FP32 coordinates, hardware
-    # rsqrt).  Budget: 2e-6 relative plus 1e-7 kcal/mol per pair term, at
-    # least 2e-2 (the terms are up to ~100 kcal/mol with ~3e-7 relative
-    # error each and add up like a random walk: an FP32 emulation of the
-    # water-water Coulomb sum on the CPU is off by 0.007 (SPC) and 0.032
-    # (the 3-atom water) kcal/mol over 5e5 terms)
-    g = dump["gold"]
-    nterm = {"ww": 9 * int(g["n_ww"]), "pw": int(g["n_pw"]), "pp": int(g["n_pp"])}
-    for k in ["ww.el", "ww.vdw", "pw.el", "pw.vdw", "pp.el", "pp.vdw"]:
-        assert abs(E[k] - ref[k]) < 2e-6 * abs(ref[k]) + max(2e-2, 1e-7 * nterm[k[:2]]), k
-    # FP64 terms
-    for k in ["LRF", "p.bond", "p.angle", "p.torsion", "p.improper", "q.bond", "q.angle", "q.torsion", "q.improper",
-              "restraint.total", "restraint.solvent_radial", "restraint.water_pol", "restraint.shell",
-              "restraint.protein"]:
-        assert _rel(E[k], ref[k]) < 1e-6, k
+    # every energy term: on energy steps the pair kernels evaluate the
+    # energy terms in FP64 from the FP64 coordinates (the forces stay FP32),
+    # so the bulk nonbonded sums are held to the FP64 gate as well.  The
+    # totals cancel to far less than the sum of the terms' magnitudes (e.g.
+    # ww.el 152 kcal/mol of 7e6 for the ligand in water), so a relative gate
+    # on the total is strict; the 1e-6 kcal/mol floor covers the fixed-point
+    # quantum (2^-36 kcal/mol per wave sum)
+    for k in ["ww.el", "ww.vdw", "pw.el", "pw.vdw", "pp.el", "pp.vdw", "LRF", "p.bond", "p.angle", "p.torsion",
+              "p.improper", "q.bond", "q.angle", "q.torsion", "q.improper", "restraint.total",
+              "restraint.solvent_radial", "restraint.water_pol", "restraint.shell", "restraint.protein",
+              "potential"]:
+        assert abs(E[k] - ref[k]) < 1e-6 * abs(ref[k]) + 1e-6, (k, E[k], ref[k])
 
 
 @pytest.mark.parametrize("cfg", ["ligwat", "protein", "kat", "protarith", "ligmaxpot", "ligarithmax", "ligtip3",
-                                 "ligfepx", "liglibq", "ligarithlibq", "ligrestr"])
+                                 "ligfepx", "liglibq", "ligarithlibq", "ligrestr", "ligdual"])
 def test_step0_q_energies(evaluated, cfg):
     e, w, dump = evaluated[cfg]
     EQ = e.q_energies(w)

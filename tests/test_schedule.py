@@ -94,6 +94,30 @@ def test_learned_ladder_follows_the_predictor():
     assert steps[:3].mean() > 2 * steps[-3:].mean()
 
 
+def test_fallback_to_uniform_when_the_network_disagrees_with_the_pilots():
+    # pilots whose measured neighbour overlaps the predictor reproduces keep
+    # the learned spacing; a predictor far from them (overlap 0.3 where the
+    # pilots measure far more) makes the schedule fall back to the uniform
+    # ladder with as many windows, and the decision is recorded
+    lams = list(S.PILOT_LAMBDAS)
+    rng = np.random.default_rng(2)
+    dEs = [rng.normal(0, 1 + 9 * lam ** 4, 400) for lam in lams]
+    meas = S.measured_overlaps(lams, dEs)
+
+    def faithful(F):
+        # per window: the mean of the measured overlaps of its intervals
+        m = np.concatenate([[meas[0]], meas])
+        m2 = np.concatenate([meas, [meas[-1]]])
+        return 0.5 * (m + m2)[:len(F)]
+
+    s, dec = S.learned_schedule(faithful, lams, dEs, 10.0, 16, 22)
+    assert not dec["fallback"] and dec["disagreement"] < S.FALLBACK_DISAGREEMENT
+    assert not np.allclose(np.diff(s), np.diff(s)[0])
+    s2, dec2 = S.learned_schedule(lambda F: np.full(len(F), 0.3), lams, dEs, 10.0, 16, 22)
+    assert dec2["fallback"] and dec2["disagreement"] > S.FALLBACK_DISAGREEMENT
+    assert np.allclose(s2, np.linspace(0.0, 1.0, len(s2)))
+
+
 def test_bar_variance_matches_replica_spread():
     # Bennett's variance against the spread of BAR over independent replicas
     rng = np.random.default_rng(3)
