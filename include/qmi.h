@@ -169,6 +169,17 @@ int qmi_mlp_grad(qmi_mlp* m, const float* x, const float* t, int batch, float* l
  * minibatches of `batch`; final_loss = mean loss over the last pass */
 int qmi_mlp_train(qmi_mlp* m, const float* x, const float* t, int n, int epochs, int batch, float lr,
                   unsigned long long seed, float* final_loss);
+/* The forward pass of qmi_mlp_forward on device buffers, allocating nothing:
+ * d_x [batch x n_in] and d_y [batch] are device pointers of the model's
+ * device, the kernels are queued on `stream` (a hipStream_t; NULL = the
+ * model's own stream) and the call returns without waiting, so it can be
+ * captured into a HIP graph.  No counterpart in the reference; it is for
+ * callers that cannot allocate (code under graph capture, serving loops).
+ * The activations go to one fixed scratch array per device (4 MiB), so
+ * batch x (4 n_hidden + 1) must not exceed 1048576 floats (2047 rows for
+ * n_hidden 128; larger batches are refused), and calls that overlap on one
+ * device race on it. */
+int qmi_mlp_forward_noalloc(qmi_mlp* m, const float* d_x, int batch, float* d_y, void* stream);
 const char* qmi_mlp_last_error(void);
 void qmi_mlp_destroy(qmi_mlp* m);
 

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -215,19 +216,52 @@ void gemm(hipStream_t s, int Mr, int N, int K, float alpha, const TA_* A, int ld
   k_gemm<TA_, TB_, TA, TB><<<grid, 256, 0, s>>>(Mr, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
-// forward pass of B rows in M->x; with targets (M->t) also the loss and dz3
-void forward(qmi_mlp* M, int B, bool with_t) {
+// activations of one forward pass: B x H (z1, a1, z2, a2) and B (z3)
+struct Act {
+  float *z1, *a1, *z2, *a2, *z3;
+};
+
+// forward pass of B rows x[B x n_in] into y[B] on stream s, activations in
+// `a`; with targets t also the loss terms and dz3
+void forward_on(qmi_mlp* M, hipStream_t s, int B, const float* x, const Act& a, float* y, const float* t,
+                float* dz3, float* lossv) {
   const int H = M->H, I = M->n_in;
-  hipStream_t s = M->s;
   const hip_bfloat16* W1 = M->pb + M->o_w1;
   const hip_bfloat16* W2 = M->pb + M->o_w2;
   const hip_bfloat16* W3 = M->pb + M->o_w3;
-  gemm<float, hip_bfloat16, false, true>(s, B, H, I, 1.f, M->x, I, W1, I, 0.f, M->z1, H);   // x W1^T
-  k_bias_silu<<<nblk((long long)B * H, 256), 256, 0, s>>>(B, H, M->z1, M->p + M->o_b1, M->a1);
-  gemm<float, hip_bfloat16, false, true>(s, B, H, H, 1.f, M->a1, H, W2, H, 0.f, M->z2, H);  // a1 W2^T
-  k_bias_silu<<<nblk((long long)B * H, 256), 256, 0, s>>>(B, H, M->z2, M->p + M->o_b2, M->a2);
-  gemm<float, hip_bfloat16, false, true>(s, B, 1, H, 1.f, M->a2, H, W3, H, 0.f, M->z3, 1);  // a2 W3^T
-  k_out<<<nblk(B, 256), 256, 0, s>>>(B, M->z3, M->p + M->o_b3, M->y, with_t ? M->t : nullptr, M->dz3, M->lossv);
+  gemm<float, hip_bfloat16, false, true>(s, B, H, I, 1.f, x, I, W1, I, 0.f, a.z1, H);   // x W1^T
+  k_bias_silu<<<nblk((long long)B * H, 256), 256, 0, s>>>(B, H, a.z1, M->p + M->o_b1, a.a1);
+  gemm<float, hip_bfloat16, false, true>(s, B, H, H, 1.f, a.a1, H, W2, H, 0.f, a.z2, H);  // a1 W2^T
+  k_bias_silu<<<nblk((long long)B * H, 256), 256, 0, s>>>(B, H, a.z2, M->p + M->o_b2, a.a2);
+  gemm<float, hip_bfloat16, false, true>(s, B, 1, H, 1.f, a.a2, H, W3, H, 0.f, a.z3, 1);  // a2 W3^T
+  k_out<<<nblk(B, 256), 256, 0, s>>>(B, a.z3, M->p + M->o_b3, y, t, dz3, lossv);
+}
+
+// forward pass of B rows in M->x; with targets (M->t) also the loss and dz3
+void forward(qmi_mlp* M, int B, bool with_t) {
+  forward_on(M, M->s, B, M->x, Act{M->z1, M->a1, M->z2, M->a2, M->z3}, M->y, with_t ? M->t : nullptr, M->dz3,
+             M->lossv);
+}
+
+// Scratch of qmi_mlp_forward_noalloc: the activations of one forward pass,
+// B x (4H + 1) floats, in a fixed device array (4 MiB on every device the
+// library runs on, held until the process ends).  Host code never passes the
+// array's name to a kernel: it uses the address hipGetSymbolAddress returns
+// for the current device, kept per device in scratch_ptr.
+constexpr long long kScratchFloats = 1LL << 20;
+__device__ __align__(256) float scratch[kScratchFloats];
+constexpr int kMaxDevices = 64;
+float* scratch_ptr[kMaxDevices] = {};
+
+// set up scratch_ptr for `device`, which must be current
+int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return mlp_err(-1, "device number beyond the scratch table");
+  if (scratch_ptr[device]) return 0;
+  void* ptr = nullptr;
+  MLP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return mlp_err(-100, "no address for the scratch array");
+  scratch_ptr[device] = (float*)ptr;
+  return 0;
 }
 
 // gradient of the mean squared error into M->g (after forward with targets)
@@ -303,6 +337,12 @@ int qmi_mlp_create(int hip_device, int n_in, int n_hidden, unsigned long long se
   }
   k_to_bf16<<<nblk(M->np, 256), 256, 0, M->s>>>(M->np, M->p, M->pb);
   if (hipStreamSynchronize(M->s) != hipSuccess) { delete M; return mlp_err(-100, "init"); }
+  {
+    // the fixed scratch array of qmi_mlp_forward_noalloc on this device
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((rc = scratch_init(hip_device)) != 0) { delete M; return rc; }
+  }
   *out = M;
   return 0;
 }
@@ -337,6 +377,22 @@ int qmi_mlp_forward(qmi_mlp* M, const float* x, int batch, float* y) {
   MLP_OK(hipGetLastError());
   MLP_OK(hipMemcpyAsync(y, M->y, (size_t)batch * 4, hipMemcpyDeviceToHost, M->s));
   MLP_OK(hipStreamSynchronize(M->s));
+  return 0;
+}
+
+int qmi_mlp_forward_noalloc(qmi_mlp* M, const float* d_x, int batch, float* d_y, void* stream) {
+  if (!M || !d_x || !d_y || batch < 1) return mlp_err(-1, "invalid argument");
+  const long long per_row = 4LL * M->H + 1;
+  if ((long long)batch * per_row > kScratchFloats)
+    return mlp_err(-3, "batch of " + std::to_string(batch) + " rows exceeds the scratch of " +
+                           std::to_string(kScratchFloats / per_row) + " rows for this network");
+  float* base = M->device >= 0 && M->device < kMaxDevices ? scratch_ptr[M->device] : nullptr;
+  if (!base) return mlp_err(-2, "scratch not set up for this device");
+  MLP_OK(hipSetDevice(M->device));
+  const long long BH = (long long)batch * M->H;
+  const Act a{base, base + BH, base + 2 * BH, base + 3 * BH, base + 4 * BH};
+  forward_on(M, stream ? (hipStream_t)stream : M->s, batch, d_x, a, d_y, nullptr, nullptr, nullptr);
+  MLP_OK(hipGetLastError());
   return 0;
 }
 

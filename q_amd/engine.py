@@ -92,7 +92,7 @@ EXPORTED = [
     "qmi_set_state", "qmi_get_pairlist", "qmi_window_info", "qmi_write_restart", "qmi_timings",
     "qmi_set_timing", "qmi_set_autotune", "qmi_tuning", "qmi_last_error", "qmi_engine_destroy",
     "qmi_mlp_create", "qmi_mlp_num_params", "qmi_mlp_get_params", "qmi_mlp_set_params", "qmi_mlp_forward",
-    "qmi_mlp_grad", "qmi_mlp_train", "qmi_mlp_last_error", "qmi_mlp_destroy",
+    "qmi_mlp_grad", "qmi_mlp_train", "qmi_mlp_last_error", "qmi_mlp_destroy", "qmi_mlp_forward_noalloc",
 ]
 
 

@@ -22,6 +22,7 @@ def _bind(lib):
         "qmi_mlp_get_params": (C.c_int, [p, _FP, C.c_longlong]),
         "qmi_mlp_set_params": (C.c_int, [p, _FP, C.c_longlong]),
         "qmi_mlp_forward": (C.c_int, [p, _FP, C.c_int, _FP]),
+        "qmi_mlp_forward_noalloc": (C.c_int, [p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
         "qmi_mlp_grad": (C.c_int, [p, _FP, _FP, C.c_int, _FP, _FP]),
         "qmi_mlp_train": (C.c_int, [p, _FP, _FP, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _FP]),
         "qmi_mlp_last_error": (C.c_char_p, []),
@@ -95,6 +96,13 @@ class MLP:
         y = np.zeros(len(x), np.float32)
         self._check(self.lib.qmi_mlp_forward(self._h, px, len(x), y.ctypes.data_as(_FP)))
         return y
+
+    def forward_device(self, x_ptr: int, batch: int, y_ptr: int, stream: int = 0) -> None:
+        """qmi_mlp_forward_noalloc: the forward pass on device buffers (device
+        addresses x_ptr [batch x n_in], y_ptr [batch]) queued on `stream`
+        (0: the model's stream), without allocating and without waiting."""
+        self._check(self.lib.qmi_mlp_forward_noalloc(self._h, C.c_void_p(x_ptr), batch, C.c_void_p(y_ptr),
+                                                     C.c_void_p(stream) if stream else None))
 
     def grad(self, x, t):
         x, px = _f32(x)
