@@ -60,6 +60,10 @@ struct WinDev {
   int ndegf, ndegfree, ndegf_solute, ndegfree_solute, ndegf_solvent, ndegfree_solvent;
   float el14, crg_ow, crg_hw, aO1, bO1;   // aO1, bO1: LJ columns of the water oxygen type
   int arith;                        // arithmetic LJ combination rule (topology vdW rule 2)
+  // the Q-atom lists are rebuilt at list updates (q_atom cut-off within the
+  // exclusion radius, nbqplist / nbqwlist md.f90:7875-7964, 8066-8118)
+  int qlist_dynamic;
+  double rcq2;
   float ww_a, ww_b;                 // A, B of the water O-O pair
   // 3-atom solvent (topology solvent type 1): LJ on all three atoms and the
   // charges of the first solvent molecule; crg_w is (crg_ow, crg_hw, crg_hw)
@@ -150,6 +154,11 @@ struct Dev {
   const int* iac;       // 0-based window-local atom type
   const int* grp;       // charge group (global) of atom
   const uint8_t* aflag; // bit0 excl, bit1 shell, bit2 heavy, bit3 q-atom
+  // Q-atom lists: whether a non-q atom is within the q_atom cut-off of the
+  // Q centre (the switching atom of its group, or its water's oxygen) at
+  // the last list update; always 1 where the lists are static
+  uint8_t* qin;
+  const int* qref;      // the atom whose distance decides qin, -1: never updated
   // charge groups
   const int* g_sw;      // switch atom (global); water groups: O
   const int* g_first;   // into g_atoms

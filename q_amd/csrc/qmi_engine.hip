@@ -190,8 +190,8 @@ static int build_device(qmi_engine* e) {
   std::vector<double> x, v, xtop, mass;
   std::vector<float4> x32, ljp;
   std::vector<float> winv;
-  std::vector<int> wid, iac, grp;
-  std::vector<uint8_t> aflag;
+  std::vector<int> wid, iac, grp, qref;
+  std::vector<uint8_t> aflag, qin;
   // groups
   std::vector<int> g_sw, g_first, g_n, g_atoms, g_win;
   // special partners
@@ -310,8 +310,10 @@ static int build_device(qmi_engine* e) {
     D.rwat = W.rwat;
     for (int k = 0; k < 3; ++k) { D.xwcent[k] = t.xwcent[k]; D.xpcent[k] = t.xpcent[k]; }
     for (int s = 0; s < S; ++s) D.lambda[s] = p.lambda[s];
-    if (fp.nqat > 0 && p.rcq <= t.rexcl_o)
-      return set_err(-10, "q_atom cut-off must exceed the exclusion radius (static Q-atom lists)");
+    // Q-atom lists built once (q_atom cut-off beyond the exclusion radius) or
+    // at every list update (md.f90:7903, 8086)
+    D.qlist_dynamic = fp.nqat > 0 && p.rcq <= t.rexcl_o;
+    D.rcq2 = p.rcq * p.rcq;
 
     // --- atoms
     for (int i = 0; i < N; ++i) {
@@ -331,6 +333,35 @@ static int build_device(qmi_engine* e) {
       if (W.iqatom[i]) fl |= 8;
       aflag.push_back(fl);
       e->h_atom_win.push_back(wi);
+      qref.push_back(-1);
+      qin.push_back(1);
+    }
+    if (D.qlist_dynamic) {
+      // the atom whose distance to the Q centre puts an atom in the Q lists:
+      // its group's switching atom (nbqplist) or its water's oxygen
+      // (nbqwlist); excluded groups and waters are never listed
+      auto within = [&](int a) {
+#pragma clang fp contract(off)
+        const double dx = W.x[3 * a] - t.xpcent[0], dy = W.x[3 * a + 1] - t.xpcent[1],
+                     dz = W.x[3 * a + 2] - t.xpcent[2];
+        return dx * dx + dy * dy + dz * dz <= D.rcq2;
+      };
+      for (int g = 0; g < t.ncgp_solute; ++g) {
+        const Cgp& c = t.cgp[g];
+        for (int k = c.first; k <= c.last; ++k) {
+          const int i = t.cgpatom[k];
+          if (W.iqatom[i] || t.excl[c.iswitch]) { qin[ao + i] = 0; continue; }
+          qref[ao + i] = ao + c.iswitch;
+          qin[ao + i] = within(c.iswitch) ? 1 : 0;
+        }
+      }
+      for (int k = 0; k < W.nwat; ++k) {
+        const int o = NS + 3 * k;
+        for (int m = 0; m < 3; ++m) {
+          qref[ao + o + m] = t.excl[o] ? -1 : ao + o;
+          qin[ao + o + m] = !t.excl[o] && within(o) ? 1 : 0;
+        }
+      }
     }
     for (int ty = 1; ty <= t.natyps; ++ty) {
       const AtomType& at = t.iaclib[ty];
@@ -498,10 +529,13 @@ static int build_device(qmi_engine* e) {
       hw.qw_waters.clear();
       hw.qp_pairs.clear();
       std::vector<char> wlisted(W.nwat, 0);
+      // dynamic lists: every water and group that may come within the
+      // cut-off is tiled, and the device flags (qin) select them
+      const bool dyn = D.qlist_dynamic != 0;
       for (int k = 0; k < W.nwat; ++k) {
         int o = NS + 3 * k;
         if (t.excl[o]) continue;
-        if (r2c(o) <= rcq2) { wlisted[k] = 1; hw.qw_waters.push_back(o); }
+        if (dyn || r2c(o) <= rcq2) { wlisted[k] = 1; hw.qw_waters.push_back(o); }
       }
       std::vector<char> plisted(NS, 0);
       const int nq = fp.nqat;
@@ -509,7 +543,7 @@ static int build_device(qmi_engine* e) {
       for (int g = 0; g < t.ncgp_solute; ++g) {
         const Cgp& c = t.cgp[g];
         if (t.excl[c.iswitch]) continue;
-        if (r2c(c.iswitch) > rcq2) continue;
+        if (!dyn && r2c(c.iswitch) > rcq2) continue;
         for (int k = c.first; k <= c.last; ++k) {
           int i = t.cgpatom[k];
           bool conn = false;
@@ -772,6 +806,8 @@ static int build_device(qmi_engine* e) {
   UP(ip, grp); d.grp = ip;
   uint8_t* up;
   UP(up, aflag); d.aflag = up;
+  UP(up, qin); d.qin = up;
+  UP(ip, qref); d.qref = ip;
   UP(ip, g_sw); d.g_sw = ip;
   UP(ip, g_first); d.g_first = ip;
   UP(ip, g_n); d.g_n = ip;
@@ -2114,10 +2150,17 @@ int qmi_get_pairlist(qmi_engine* e, int w, int kind, int32_t* pi, int32_t* pj, i
     return qq == 0.f && a.avdw[lj - 1] * b.avdw[lj - 1] == 0. && a.bvdw[lj - 1] * b.bvdw[lj - 1] == 0.;
   };
   auto ljc = [&](int i, int j) { return W.ljcod[(t.iac[i] - 1) * t.natyps + (t.iac[j] - 1)]; };
-  if (kind == QMI_LIST_QW) {
-    for (int o : h->qw_waters) out.push_back({0, o + 1, 0});
-  } else if (kind == QMI_LIST_QP) {
-    for (auto& p : h->qp_pairs) out.push_back({p[0] + 1, p[1] + 1, p[2]});
+  if (kind == QMI_LIST_QW || kind == QMI_LIST_QP) {
+    // the tiled superset, less what the last list update put out of range
+    std::vector<uint8_t> in(W.natom, 1);
+    HIP_OK(hipMemcpy(in.data(), e->d.qin + ao, W.natom, hipMemcpyDeviceToHost));
+    if (kind == QMI_LIST_QW) {
+      for (int o : h->qw_waters)
+        if (in[o]) out.push_back({0, o + 1, 0});
+    } else {
+      for (auto& p : h->qp_pairs)
+        if (in[p[1]]) out.push_back({p[0] + 1, p[1] + 1, p[2]});
+    }
   } else {
     TileType tt = kind == QMI_LIST_WW ? T_WW : kind == QMI_LIST_PW ? T_PW : kind == QMI_LIST_PP ? T_PP : T_NTYPES;
     if (tt == T_NTYPES) return set_err(-3, "invalid list kind");

@@ -44,6 +44,14 @@ __device__ __forceinline__ double rsqrt_d(double r2) {
   y = fma(y, fma(-(h * y), y, 0.5), y);
   return y;
 }
+// 1/sqrt(a) for the energy terms of the bulk pair kernels: the FP32 hardware
+// estimate (relative error below 2^-22) and one FP64 Newton step, which
+// leaves a relative error below 1e-12
+__device__ __forceinline__ double rsqrt_e(double r2) {
+  const double h = 0.5 * r2;
+  const double y = (double)__builtin_amdgcn_rsqf((float)r2);
+  return fma(y, fma(-(h * y), y, 0.5), y);
+}
 __device__ __forceinline__ double rcp_d(double a) {
   double y = __builtin_amdgcn_rcp(a);
   y = fma(y, fma(-a, y, 1.0), y);
@@ -192,6 +200,21 @@ __global__ void k_begin(Dev d, int force_energy, int eval_only, int rebuild) {
   }
   if (i < d.ntot && (eval_only || window_runs(d.ws[d.wid[i]]))) {
     d.f[3 * i] = 0; d.f[3 * i + 1] = 0; d.f[3 * i + 2] = 0;
+  }
+  // dynamic Q-atom lists (nbqplist / nbqwlist, md.f90:7875-7964, 8066-8118):
+  // at a list update, an atom is in the lists while the switching atom of
+  // its group (the oxygen of its water) lies within the q_atom cut-off of
+  // the Q centre; the Q tiles cover every candidate and test this flag
+  if (i < d.ntot && d.qref[i] >= 0) {
+    const WinState& s = d.ws[d.wid[i]];
+    const WinDev& W = d.win[d.wid[i]];
+    const bool upd = eval_only ? (rebuild & 1) != 0 : window_runs(s) && (s.istep % W.nbcycle) == 0;
+    if (upd) {
+      const int r = d.qref[i];
+      const double dx = __dsub_rn(d.x[3 * r], W.xpcent[0]), dy = __dsub_rn(d.x[3 * r + 1], W.xpcent[1]),
+                   dz = __dsub_rn(d.x[3 * r + 2], W.xpcent[2]);
+      d.qin[i] = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)) <= W.rcq2 ? 1 : 0;
+    }
   }
   // EVB off-diagonal elements at the step's starting coordinates (offdiag,
   // md.f90:12911-12937), into the step's ring slot; the ring is read for
@@ -410,15 +433,24 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
   if (en)
     for (int k = 0; k < 9; ++k) xi[k] = oa >= 0 ? d.x[3 * oa + k] : 0.0;
   const int* ent = L.ent + (size_t)row * L.cap;
+  // the next entry's index and coordinates are loaded while the current one
+  // is evaluated (entries past the row end read water 0 and are skipped)
+  int ob_next = ent[min(b, L.cap - 1)];
+  ob_next = b < n ? ob_next : 0;
+  float4 nx[3];
+  for (int s = 0; s < 3; ++s) nx[s] = d.x32[ob_next + s];
   for (int t = 0; t < nch; ++t) {
     const int k = t * kCl + b;
-    if (k >= n) continue;
-    const int ob = ent[k];
+    const int ob = ob_next;
     float3 pj[3];
-    for (int s = 0; s < 3; ++s) {
-      float4 q = d.x32[ob + s];
-      pj[s] = make_float3(q.x, q.y, q.z);
+    for (int s = 0; s < 3; ++s) pj[s] = make_float3(nx[s].x, nx[s].y, nx[s].z);
+    {
+      const int kn = k + kCl;
+      const int e = ent[min(kn, L.cap - 1)];   // in the row, so the load needs no branch
+      ob_next = kn < n ? e : 0;
+      for (int s = 0; s < 3; ++s) nx[s] = d.x32[ob_next + s];
     }
+    if (k >= n) continue;
     const bool cnt = en && oa < ob;   // each unordered pair counted once
     if (cnt) {
       double xj[9];
@@ -427,7 +459,7 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
         for (int q = 0; q < 3; ++q) {
           const double dx = xj[3 * q] - xi[3 * p], dy = xj[3 * q + 1] - xi[3 * p + 1],
                        dz = xj[3 * q + 2] - xi[3 * p + 2];
-          const double rinv = rsqrt_d(dx * dx + dy * dy + dz * dz);
+          const double rinv = rsqrt_e(dx * dx + dy * dy + dz * dz);
           eel += (double)qq[p][q] * rinv;
           if (W3 || (p == 0 && q == 0)) {
             const double r2i = rinv * rinv, r6 = r2i * r2i * r2i;
@@ -583,7 +615,7 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
         if (en) {
           const double ddx = d.x[3 * (e[u] + k)] - xs[0], ddy = d.x[3 * (e[u] + k) + 1] - xs[1],
                        ddz = d.x[3 * (e[u] + k) + 2] - xs[2];
-          const double rinv_d = rsqrt_d(ddx * ddx + ddy * ddy + ddz * ddz);
+          const double rinv_d = rsqrt_e(ddx * ddx + ddy * ddy + ddz * ddz);
           eel += (double)(ps.w * pw[k].w) * rinv_d;
           if (k == 0 || wat3) {
             const double r2i = rinv_d * rinv_d, r6d = r2i * r2i * r2i;
@@ -692,7 +724,7 @@ __device__ __forceinline__ void pp_impl(const Dev& d) {
         g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz;
         if (en && ea < eb) {
           const double ddx = d.x[3 * eb] - xs[0], ddy = d.x[3 * eb + 1] - xs[1], ddz = d.x[3 * eb + 2] - xs[2];
-          const double rinv_d = rsqrt_d(ddx * ddx + ddy * ddy + ddz * ddz);
+          const double rinv_d = rsqrt_e(ddx * ddx + ddy * ddy + ddz * ddz);
           const double r2i = rinv_d * rinv_d, r6d = r2i * r2i * r2i;
           const double* tlj = &d.t_lj[(size_t)(W.t_off + d.iac[eb]) * 4];
           double ad, bd;
@@ -834,6 +866,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
   const float4 pq_fixed = (Q_SIDE && qa >= 0) ? d.x32[qa] : make_float4(0, 0, 0, 0);
   float4 pw_fixed[3];
   for (int k = 0; k < 3; ++k) pw_fixed[k] = (!Q_SIDE && ow >= 0) ? d.x32[ow + k] : make_float4(0, 0, 0, 0);
+  const bool w_in = Q_SIDE || (ow >= 0 && d.qin[ow]);   // the lane's water is in the Q lists
   const int s0 = wk.y, n = wk.z, lst = L.start[ic];
   float g[9];
   for (int k = 0; k < 9; ++k) g[k] = 0.f;
@@ -851,7 +884,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
     for (int u = 0; u < kQGroup; ++u) {
       const int tt = s0 + min(t0 + u, n - 1);
       const int jc = L.jc[tt];
-      on[u] = t0 + u < n && lane_bit(L.mask[tt], lane);
+      on[u] = t0 + u < n && lane_bit(L.mask[tt], lane) && w_in;
       // water O atom (q side), or the q-cluster member index (water side)
       jx[u] = Q_SIDE ? d.wc_o[jc * kCl + b] : jc * kCl + b;
     }
@@ -987,8 +1020,8 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
   for (int t = 0; t < n; ++t) {
     const int jc = L.jc[s0 + t];
     const uint2 m = L.mask[s0 + t];
-    if (lane_bit(m, lane)) {
-      const int o = d.wc_o[jc * kCl + b];
+    const int o = d.wc_o[jc * kCl + b];
+    if (lane_bit(m, lane) && d.qin[o]) {
       double rinv[3], r6[NLJ];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -1066,6 +1099,7 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   const float4 pp_fixed = (!Q_SIDE && pa >= 0) ? d.x32[pa] : make_float4(0, 0, 0, 0);
   const float4 lj_fixed = (!Q_SIDE && pa >= 0) ? d.ljp[pa] : make_float4(0, 0, 0, 0);
   const int iac_fixed = (!Q_SIDE && pa >= 0) ? d.iac[pa] : 0;
+  const bool p_in = Q_SIDE || (pa >= 0 && d.qin[pa]);   // the lane's solute atom is in the Q lists
   const int s0 = wk.y, n = wk.z, lst = L.start[ic];
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   long long gx = 0, gy = 0, gz = 0;
@@ -1078,7 +1112,7 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
     for (int u = 0; u < kQGroup; ++u) {
       const int tt = s0 + min(t0 + u, n - 1);
       const int jc = L.jc[tt];
-      on[u] = t0 + u < n && lane_bit(L.mask[tt], lane);
+      on[u] = t0 + u < n && lane_bit(L.mask[tt], lane) && p_in;
       o14[u] = lane_bit(L.m14[tt], lane);
       jx[u] = Q_SIDE ? d.sc_a[jc * kCl + b] : d.qc_q[jc * kCl + b];
     }
@@ -1154,9 +1188,9 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   for (int t = 0; t < n; ++t) {
     const int jc = L.jc[s0 + t];
     const uint2 m = L.mask[s0 + t];
-    if (lane_bit(m, lane)) {
+    const int j = d.sc_a[jc * kCl + b];
+    if (lane_bit(m, lane) && j >= 0 && d.qin[j]) {
       const bool is14 = lane_bit(L.m14[s0 + t], lane);
-      const int j = d.sc_a[jc * kCl + b];
       // nonbond_qp_qvdw (md.f90:11138-11204): per state
       const double dx = x[3 * j] - x[3 * qa], dy = x[3 * j + 1] - x[3 * qa + 1], dz = x[3 * j + 2] - x[3 * qa + 2];
       const double r2 = dx * dx + dy * dy + dz * dz;
@@ -1299,7 +1333,7 @@ __device__ __forceinline__ double r2_pos(double ax, double ay, double az, double
 // LRF, md.f90:5589-5700), with the shared factors hoisted
 __device__ __forceinline__ void lrf_add(double acc[20], double q, double dx, double dy, double dz) {
   const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-  const double ir = rsqrt_d(r2);
+  const double ir = rsqrt_e(r2);
   const double ir2 = ir * ir;
   const double qr = q * ir;            // q / r
   const double f0 = qr * ir2;          // q / r^3

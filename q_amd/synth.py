@@ -1288,6 +1288,10 @@ CONFIGS = {
     # a QligFEP dual-topology perturbation: both ligands, all atoms q-atoms,
     # dummies of the zero-LJ type, soft-core maximum potentials 0 / 20
     "ligdual": (ligand_dual, {"radius": 20.0}, {"lrf": True}),
+    # Q-atom lists rebuilt at every list update: a q_atom cut-off within the
+    # sphere's exclusion radius (nbqplist / nbqwlist, md.f90:7903, 8086)
+    "ligqcut": (ligand_in_water, {"radius": 20.0}, {"lrf": True, "rc_q": 9.0}),
+    "protqcut": (protein_leg, {"radius": 25.0}, {"lrf": True, "shake_hydrogens": True, "rc_q": 12.0}),
 }
 
 

@@ -25,11 +25,13 @@ def systems(tmp_path_factory):
         key = (config, seed, steps, tuple(sorted(kw.items())))
         if key not in cache:
             d = base / f"{config}_{seed}_{steps}_{len(cache)}"
-            if (config, seed) not in systems_cache:
-                builder, bkw, _ = synth.CONFIGS[config]
-                systems_cache[(config, seed)] = builder(seed=seed, **bkw)
+            builder, bkw, _ = synth.CONFIGS[config]
+            # configurations that differ only in their .inp share the system
+            skey = (builder.__name__, tuple(sorted(bkw.items())), seed)
+            if skey not in systems_cache:
+                systems_cache[skey] = builder(seed=seed, **bkw)
             cache[key] = synth.make_window(config, str(d), seed=seed, steps=steps,
-                                           system=systems_cache[(config, seed)], **kw)
+                                           system=systems_cache[skey], **kw)
         return cache[key]
 
     return get

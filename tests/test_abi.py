@@ -57,7 +57,7 @@ def test_errors_are_reported_not_fatal(tmp_path):
 
 @pytest.mark.parametrize("cfg,seed", [("water", 0), ("ligwat", 0), ("protein", 2), ("kat", 0), ("ligfepx", 0),
                                       ("liglibq", 0), ("ligarithlibq", 0), ("ligrestr", 0),
-                                      ("ligdual", 0)])
+                                      ("ligdual", 0), ("ligqcut", 0)])
 def test_host_setup_matches_reference_start(golden_inp, cfg, seed):
     from q_amd.engine import Engine
     g = dict(np.load(os.path.join(GOLD, f"{cfg}_s{seed}.npz")))

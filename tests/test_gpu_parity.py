@@ -27,9 +27,12 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # constraints, with library charges for q-q pairs under either LJ rule
 # (nonbond_qq_lib_charges, nonbon2_qq_lib_charges), and with sequence,
 # position, distance, angle and wall restraints (p_restrain), and a
-# QligFEP-style dual-topology perturbation (qligfep.py:311-362)
+# QligFEP-style dual-topology perturbation (qligfep.py:311-362), and both
+# legs with Q-atom lists rebuilt at list updates (q_atom cut-off inside the
+# exclusion radius, nbqplist / nbqwlist)
 CASES = {"water": 0, "ligwat": 0, "protein": 2, "kat": 0, "protarith": 2, "ligmaxpot": 0, "ligarithmax": 0,
-         "ligtip3": 0, "ligfepx": 0, "liglibq": 0, "ligarithlibq": 0, "ligrestr": 0, "ligdual": 0}
+         "ligtip3": 0, "ligfepx": 0, "liglibq": 0, "ligarithlibq": 0, "ligrestr": 0, "ligdual": 0,
+         "ligqcut": 0, "protqcut": 2}
 
 
 def _rel(a, b, floor=1.0):
@@ -80,7 +83,8 @@ def test_step0_energies(evaluated, cfg):
 
 
 @pytest.mark.parametrize("cfg", ["ligwat", "protein", "kat", "protarith", "ligmaxpot", "ligarithmax", "ligtip3",
-                                 "ligfepx", "liglibq", "ligarithlibq", "ligrestr", "ligdual"])
+                                 "ligfepx", "liglibq", "ligarithlibq", "ligrestr", "ligdual", "ligqcut",
+                                 "protqcut"])
 def test_step0_q_energies(evaluated, cfg):
     e, w, dump = evaluated[cfg]
     EQ = e.q_energies(w)

@@ -22,10 +22,11 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # the ligand in a 3-atom water with LJ on the hydrogens, the ligand with the
 # further FEP sections -- el_scale, soft pairs, excluded pairs, a Morse
 # q-bond, FEP SHAKE constraints --, with library charges for q-q pairs under
-# either rule, and with sequence, position, distance, angle and wall
-# restraints) and the reference's own test system
+# either rule, with sequence, position, distance, angle and wall restraints,
+# and with a q_atom cut-off that selects the Q-atom lists) and the
+# reference's own test system
 CASES = [("water", 0), ("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2), ("ligtip3", 0), ("ligfepx", 0),
-         ("liglibq", 0), ("ligarithlibq", 0), ("ligrestr", 0)]
+         ("liglibq", 0), ("ligarithlibq", 0), ("ligrestr", 0), ("ligqcut", 0)]
 
 
 def load_gold(cfg, seed):
