@@ -245,8 +245,8 @@ def main():
         "family_us_per_step": {
             "lists_lrf_build": round(fam[0], 2) if fam else None,
             "nonbonded_tiles": round(fam[1], 2) if fam else None,
-            "lrf_bonded_restraints": round(fam[2], 2) if fam else None,
-            "integrate_shake_thermostat": round(fam[3], 2) if fam else None,
+            "bonded_restraints_polarisation": round(fam[2], 2) if fam else None,
+            "lrf_eval_integrate_shake_thermostat": round(fam[3], 2) if fam else None,
             "ww_kernel": round(fam[4], 2) if fam else None,
             "wall_per_step_timed_pass": round(fam[5], 2) if fam else None,
         },

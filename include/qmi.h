@@ -140,8 +140,9 @@ int qmi_set_timing(qmi_engine* e, int on);
 
 /* Device time per kernel family of the last qmi_run, in microseconds per
  * step (no reference counterpart).  Order: pair-list + LRF build,
- * nonbonded tiles, LRF evaluation + bonded + restraints, leapfrog + SHAKE +
- * thermostat, the water-water tile kernel alone, wall-clock total.  All but
+ * nonbonded tiles, bonded + sequence restraints + water polarisation, LRF
+ * evaluation + atom restraints + leapfrog + SHAKE + thermostat, the
+ * water-water tile kernel alone, wall-clock total.  All but
  * the last are zero unless qmi_set_timing was on.  Returns the number of
  * values written. */
 int qmi_timings(qmi_engine* e, double* out, int cap);

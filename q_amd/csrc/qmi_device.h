@@ -146,6 +146,13 @@ struct Dev {
   // atoms
   double *x, *v, *xx, *xtop;
   float4* x32;          // x, y, z, scaled charge
+  // the water atoms' x32 again, in water-cluster slot order: slot
+  // c*8 + m (cluster c, member m) holds O, H1, H2 at 3*slot + 0..2, so the
+  // members of a cluster are contiguous; water-water and solute-water row
+  // entries name slots.  wslot: per atom, its index in xw (3*slot + site),
+  // or -1 for solute atoms
+  float4* xw;
+  const int* wslot;
   float4* ljp;          // avdw1, bvdw1, avdw3, bvdw3
   long long* f;         // gradient, 3 per atom, fixed point 2^-32
   const float* winv;

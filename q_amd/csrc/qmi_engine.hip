@@ -733,7 +733,7 @@ static int build_device(qmi_engine* e) {
             sc_loc.push_back((uint8_t)(li < 255 ? li : 255));
             sc_loc.push_back((uint8_t)(lj < 255 ? lj : 255));
           }
-        // units of up to kShakeRegAtoms atoms are iterated in registers
+        // units of up to 4 atoms are iterated in registers (k_shake)
         su_atoms.push_back((int)su_atom_list.size());
         su_natom.push_back((int)atoms.size());
         for (int at : atoms) su_atom_list.push_back(ao + at);
@@ -863,6 +863,22 @@ static int build_device(qmi_engine* e) {
     UP(rp, sp_rng); d.sp_rng = rp;
   }
   UP(ip, wc_o); d.wc_o = ip;
+  // water coordinates in cluster-slot order (the row kernels' gathers of
+  // neighbouring waters then read contiguous memory)
+  {
+    std::vector<int> wslot(x32.size(), -1);
+    std::vector<float4> xw(std::max<size_t>(3, 3 * wc_o.size()), make_float4(0.f, 0.f, 0.f, 0.f));
+    for (size_t m = 0; m < wc_o.size(); ++m) {
+      const int o = wc_o[m];
+      if (o < 0) continue;
+      for (int k = 0; k < 3; ++k) {
+        wslot[o + k] = 3 * (int)m + k;
+        xw[3 * m + k] = x32[o + k];
+      }
+    }
+    UP(d.xw, xw);
+    UP(ip, wslot); d.wslot = ip;
+  }
   UP(ip, sc_a); d.sc_a = ip;
   UP(ip, qc_q); d.qc_q = ip;
   UP(ip, wc_win); d.wc_win = ip;
@@ -1752,7 +1768,8 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   }
   if (e->timing_on && !e->ev[0])
     for (auto& x : e->ev) HIP_OK(hipEventCreate(&x));
-  // per family: 0 lists+LRF build, 1 nonbonded tiles, 2 LRF/bonded/restraints, 3 integrate
+  // per family: 0 lists+LRF build, 1 nonbonded tiles, 2 bonded/sequence restraints/polarisation,
+  // 3 per-atom terms + integrate
   double fam[5] = {0, 0, 0, 0, 0};
   auto t0 = std::chrono::steady_clock::now();
   long long done = 0;
@@ -1766,7 +1783,7 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     if (launch_step_forces(e->d, e->stream, e->side_stream[0], e->side_stream[1], e->step_ev, lists, e->ngrp, e->nwc,
                            e->nsc, qen) != hipSuccess)
       return;
-    launch_integrate(e->d, e->stream, e->shake_any);
+    launch_integrate(e->d, e->stream, e->shake_any, true);
   };
   if (!e->timing_on && !e->step_exec[0] && !e->graph_failed) {
     bool ok = true;
@@ -1875,9 +1892,9 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[2], e->stream));
       launch_nonbonded(e->d, e->stream, false, qen);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[3], e->stream));
-      launch_other_forces(e->d, e->stream);
+      launch_other_forces(e->d, e->stream, false);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[4], e->stream));
-      launch_integrate(e->d, e->stream, e->shake_any);
+      launch_integrate(e->d, e->stream, e->shake_any, true);
       if (e->timing_on) {
         HIP_OK(hipEventRecord(e->ev[5], e->stream));
         HIP_OK(hipEventSynchronize(e->ev[5]));
@@ -1885,8 +1902,8 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
         for (int f = 0; f < 5; ++f) HIP_OK(hipEventElapsedTime(&ms[f], e->ev[f], e->ev[f + 1]));
         fam[0] += ms[0] * 1000.0;                 // lists + LRF build
         fam[1] += (ms[1] + ms[2]) * 1000.0;       // all nonbonded tiles
-        fam[2] += ms[3] * 1000.0;                 // LRF evaluation, bonded, restraints
-        fam[3] += ms[4] * 1000.0;                 // leapfrog, SHAKE, thermostat
+        fam[2] += ms[3] * 1000.0;                 // bonded, sequence restraints, polarisation
+        fam[3] += ms[4] * 1000.0;                 // LRF evaluation, atom restraints, leapfrog, SHAKE, thermostat
         fam[4] += ms[1] * 1000.0;                 // water-water tiles alone
       }
     }
@@ -2178,7 +2195,8 @@ int qmi_get_pairlist(qmi_engine* e, int w, int kind, int32_t* pi, int32_t* pj, i
       for (int k = 0; k < cnt[row]; ++k) {
         const int v = ent[row * L.cap + k];
         const bool is14 = (tt == T_PP) && (v & kRow14);
-        const int eb = (v & ~kRow14) - (int)ao;
+        // water-water and solute-water entries name water slots
+        const int eb = (tt == T_PP ? (v & ~kRow14) : e->h_wc_o[v]) - (int)ao;
         if (tt == T_WW) {
           if (ea < eb) out.push_back({ea + 1, eb + 1, 1});
         } else if (tt == T_PP) {

@@ -52,6 +52,11 @@ __device__ __forceinline__ double rsqrt_e(double r2) {
   const double y = (double)__builtin_amdgcn_rsqf((float)r2);
   return fma(y, fma(-(h * y), y, 0.5), y);
 }
+// FP32 1/sqrt of a squared interatomic distance: the hardware instruction
+// alone.  rsqrtf wraps the same instruction in a rescaling for denormal
+// arguments (a compare and two selects); the squared distance of a listed
+// pair is never that small, and on normal arguments both return the same bits
+__device__ __forceinline__ float rsq_r2(float r2) { return __builtin_amdgcn_rsqf(r2); }
 __device__ __forceinline__ double rcp_d(double a) {
   double y = __builtin_amdgcn_rcp(a);
   y = fma(y, fma(-a, y, 1.0), y);
@@ -63,6 +68,20 @@ __device__ __forceinline__ void add_grad(long long* f, int atom, double gx, doub
   atomicAdd((unsigned long long*)&f[3 * atom + 0], (unsigned long long)fix(gx, kFScale));
   atomicAdd((unsigned long long*)&f[3 * atom + 1], (unsigned long long)fix(gy, kFScale));
   atomicAdd((unsigned long long*)&f[3 * atom + 2], (unsigned long long)fix(gz, kFScale));
+}
+// the gradient of one atom from a per-atom pass: added to the gradient array
+// with atomics (LOCAL false), or summed in the lane's own fixed-point
+// registers (LOCAL true) for a kernel that owns the atom.  The integers are
+// the same either way.
+template <bool LOCAL>
+__device__ __forceinline__ void atom_grad(const Dev& d, long long* acc, int atom, double gx, double gy, double gz) {
+  if (LOCAL) {
+    acc[0] += fix(gx, kFScale);
+    acc[1] += fix(gy, kFScale);
+    acc[2] += fix(gz, kFScale);
+  } else {
+    add_grad(d.f, atom, gx, gy, gz);
+  }
 }
 
 // gradient of term atom k (in term order): into its own slot (fixed point, no
@@ -87,20 +106,86 @@ __device__ __forceinline__ void add_energy(WinState* ws, int w, int slot, double
   if (e != 0.0) atomicAdd((unsigned long long*)&ws[w].E[slot], (unsigned long long)fix(e, kEScale));
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// ------------------------------------------------ cross-lane sums --
+// Partner values without the LDS crossbar: DPP moves inside rows of 16
+// lanes and the gfx950 permlane swaps between rows.  Every sum below is a
+// butterfly whose step k adds the value of lane l ^ m_k; at each step the
+// values are already uniform over the lane groups of the earlier steps, so
+// a DPP pattern that lands in the right group reads the same value as the
+// xor partner (row_ror:8 for m = 8, row_ror:4 for m = 4 once the values are
+// symmetric under l ^ 8, row_half_mirror for m = 4 once they are uniform over
+// quads).  The sums are therefore the same, bit for bit, as the butterflies
+// over __shfl_xor.  All 64 lanes must be active.
+constexpr int kDppXor1 = 0xB1;      // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;      // quad_perm [2,3,0,1]
+constexpr int kDppRor4 = 0x124;     // row_ror:4
+constexpr int kDppRor8 = 0x128;     // row_ror:8
+constexpr int kDppHalfMirror = 0x141;
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_u(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+// the value of lane l ^ 16 (row swap) and of lane l ^ 32 (half swap)
+__device__ __forceinline__ unsigned xor16_u(unsigned v) {
+  const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+  return (threadIdx.x & 16) ? r[0] : r[1];
+}
+__device__ __forceinline__ unsigned xor32_u(unsigned v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return (threadIdx.x & 32) ? r[0] : r[1];
+}
+// the same moves for 32- and 64-bit values: M is a DPP control, or 16 / 32
+// for the row and half swaps
+template <int M>
+__device__ __forceinline__ unsigned mv_u(unsigned v) {
+  if constexpr (M == 16) return xor16_u(v);
+  else if constexpr (M == 32) return xor32_u(v);
+  else return dpp_u<M>(v);
+}
+template <int M>
+__device__ __forceinline__ float mv(float v) { return __builtin_bit_cast(float, mv_u<M>(__builtin_bit_cast(unsigned, v))); }
+template <int M>
+__device__ __forceinline__ int mv(int v) { return (int)mv_u<M>((unsigned)v); }
+template <int M>
+__device__ __forceinline__ unsigned long long mv(unsigned long long v) {
+  return (unsigned long long)mv_u<M>((unsigned)v) | ((unsigned long long)mv_u<M>((unsigned)(v >> 32)) << 32);
+}
+template <int M>
+__device__ __forceinline__ long long mv(long long v) { return (long long)mv<M>((unsigned long long)v); }
+template <int M>
+__device__ __forceinline__ double mv(double v) {
+  return __builtin_bit_cast(double, mv<M>(__builtin_bit_cast(unsigned long long, v)));
 }
 
-// integer forms: exact, so the order of the sums does not matter
-__device__ __forceinline__ long long wave_sum_i(long long v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+// sum over the wave: partners l^32, l^16, l^8, l^4, l^2, l^1
+template <class T>
+__device__ __forceinline__ T wave_sum_t(T v) {
+  v += mv<32>(v);
+  v += mv<16>(v);
+  v += mv<kDppRor8>(v);
+  v += mv<kDppRor4>(v);
+  v += mv<kDppXor2>(v);
+  v += mv<kDppXor1>(v);
   return v;
 }
-__device__ __forceinline__ long long row8_sum_i(long long v) {
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 4, 64);
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_t(v); }
+// integer forms: exact, so the order of the sums does not matter
+__device__ __forceinline__ long long wave_sum_i(long long v) { return wave_sum_t(v); }
+// sum over the 8 lanes that share lane>>3: partners l^1, l^2, l^4
+template <class T>
+__device__ __forceinline__ T row8_sum_t(T v) {
+  v += mv<kDppXor1>(v);
+  v += mv<kDppXor2>(v);
+  v += mv<kDppHalfMirror>(v);
+  return v;
+}
+__device__ __forceinline__ long long row8_sum_i(long long v) { return row8_sum_t(v); }
+// sum over the 8 lanes that share lane&7: partners l^8, l^16, l^32
+template <class T>
+__device__ __forceinline__ T col8_sum_t(T v) {
+  v += mv<kDppRor8>(v);
+  v += mv<16>(v);
+  v += mv<32>(v);
   return v;
 }
 __device__ __forceinline__ void add_grad_fix(long long* f, int atom, long long gx, long long gy, long long gz) {
@@ -121,7 +206,7 @@ __device__ __forceinline__ void wave_energy(WinState* ws, int w, int slot, doubl
   if (vm == 0ull) return;
   const int first = __ffsll((long long)vm) - 1;
   const int key = valid ? w * E_NSLOT + slot : -1;
-  const int key0 = __shfl(key, first, 64);
+  const int key0 = __builtin_amdgcn_readlane(key, first);   // first is uniform over the wave
   const bool uniform = __ballot(valid && key != key0) == 0ull;
   const long long ef = valid ? fix(e, kEScale) : 0ll;
   if (uniform) {
@@ -132,20 +217,8 @@ __device__ __forceinline__ void wave_energy(WinState* ws, int w, int slot, doubl
   }
 }
 
-// sum over the 8 lanes that share lane>>3
-__device__ __forceinline__ double row8_sum(double v) {
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 4, 64);
-  return v;
-}
-// sum over the 8 lanes that share lane&7
-__device__ __forceinline__ double col8_sum(double v) {
-  v += __shfl_xor(v, 8, 64);
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
-  return v;
-}
+__device__ __forceinline__ double row8_sum(double v) { return row8_sum_t(v); }
+__device__ __forceinline__ double col8_sum(double v) { return col8_sum_t(v); }
 
 __device__ __forceinline__ bool lane_bit(uint2 m, int lane) {
   return lane < 32 ? ((m.x >> lane) & 1u) : ((m.y >> (lane - 32)) & 1u);
@@ -365,7 +438,9 @@ __global__ void __launch_bounds__(256) k_build_rows(Dev d) {
       const unsigned rowbits = (unsigned)((kb >> (8 * a)) & 0xffull);
       if (keep) {
         const int idx = cnt + __popc(rowbits & ((1u << b) - 1u));
-        if (idx < L.cap) L.ent[(size_t)row * L.cap + idx] = (TT == T_PP && sp == 2) ? (eb | kRow14) : eb;
+        // water entries name the member's cluster slot
+        const int ev = jw ? (j0 + base + k) * kCl + b : (TT == T_PP && sp == 2) ? (eb | kRow14) : eb;
+        if (idx < L.cap) L.ent[(size_t)row * L.cap + idx] = ev;
       }
       cnt += __popc(rowbits);
       k = kn;
@@ -384,9 +459,9 @@ constexpr int kUnroll = 4;   // row entries per pass of the pair kernels
 
 __device__ __forceinline__ int row_chunks(int n) {
   int m = n;
-  m = max(m, __shfl_xor(m, 8, 64));
-  m = max(m, __shfl_xor(m, 16, 64));
-  m = max(m, __shfl_xor(m, 32, 64));
+  m = max(m, mv<kDppRor8>(m));
+  m = max(m, mv<16>(m));
+  m = max(m, mv<32>(m));
   return (m + kCl - 1) / kCl;
 }
 
@@ -433,12 +508,14 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
   if (en)
     for (int k = 0; k < 9; ++k) xi[k] = oa >= 0 ? d.x[3 * oa + k] : 0.0;
   const int* ent = L.ent + (size_t)row * L.cap;
-  // the next entry's index and coordinates are loaded while the current one
-  // is evaluated (entries past the row end read water 0 and are skipped)
+  // entries are water slots: the members of a neighbouring cluster are
+  // contiguous in xw, so the 8 lanes of a row mostly read the same lines.
+  // The next entry's slot and coordinates are loaded while the current one
+  // is evaluated (entries past the row end read slot 0 and are skipped)
   int ob_next = ent[min(b, L.cap - 1)];
   ob_next = b < n ? ob_next : 0;
   float4 nx[3];
-  for (int s = 0; s < 3; ++s) nx[s] = d.x32[ob_next + s];
+  for (int s = 0; s < 3; ++s) nx[s] = d.xw[3 * ob_next + s];
   for (int t = 0; t < nch; ++t) {
     const int k = t * kCl + b;
     const int ob = ob_next;
@@ -448,13 +525,14 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
       const int kn = k + kCl;
       const int e = ent[min(kn, L.cap - 1)];   // in the row, so the load needs no branch
       ob_next = kn < n ? e : 0;
-      for (int s = 0; s < 3; ++s) nx[s] = d.x32[ob_next + s];
+      for (int s = 0; s < 3; ++s) nx[s] = d.xw[3 * ob_next + s];
     }
     if (k >= n) continue;
-    const bool cnt = en && oa < ob;   // each unordered pair counted once
+    const int ob_atom = en ? d.wc_o[ob] : 0;
+    const bool cnt = en && oa < ob_atom;   // each unordered pair counted once
     if (cnt) {
       double xj[9];
-      for (int m = 0; m < 9; ++m) xj[m] = d.x[3 * ob + m];
+      for (int m = 0; m < 9; ++m) xj[m] = d.x[3 * ob_atom + m];
       for (int p = 0; p < 3; ++p)
         for (int q = 0; q < 3; ++q) {
           const double dx = xj[3 * q] - xi[3 * p], dy = xj[3 * q + 1] - xi[3 * p + 1],
@@ -472,7 +550,7 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
       for (int q = 0; q < 3; ++q) {
         float dx = pj[q].x - pi[p].x, dy = pj[q].y - pi[p].y, dz = pj[q].z - pi[p].z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        float rinv = rsqrtf(r2);
+        float rinv = rsq_r2(r2);
         float r2inv = rinv * rinv;
         float vel = qq[p][q] * rinv;
         float dv;
@@ -518,7 +596,7 @@ __global__ void __launch_bounds__(256) k_ww(Dev d) {
 
 // solute atom - water: LJ on O (on all three sites for a 3-atom solvent),
 // Coulomb on all three sites (nonbond_pw, md.f90:10631-10682).  WATER_SIDE: rows are waters and entries solute atoms
-// (W<-P); otherwise rows are solute atoms and entries water O atoms (P<-W),
+// (W<-P); otherwise rows are solute atoms and entries water slots (P<-W),
 // which also counts the energies.
 template <bool WATER_SIDE, bool EN, bool W3>
 __device__ __forceinline__ void pw_impl(const Dev& d) {
@@ -574,11 +652,12 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
     float4 ld[kUnroll][3];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      ld[u][0] = d.x32[e[u]];
       if (WATER_SIDE) {
+        ld[u][0] = d.x32[e[u]];
         ld[u][1] = d.ljp[e[u]];
       } else {
-        ld[u][1] = d.x32[e[u] + 1]; ld[u][2] = d.x32[e[u] + 2];
+        // water slots (contiguous per cluster)
+        ld[u][0] = d.xw[3 * e[u]]; ld[u][1] = d.xw[3 * e[u] + 1]; ld[u][2] = d.xw[3 * e[u] + 2];
       }
     }
 #pragma unroll
@@ -597,7 +676,7 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
         // dx = x(water atom) - x(solute atom), as nonbond_pw with i solute, j water
         float dx = pw[k].x - ps.x, dy = pw[k].y - ps.y, dz = pw[k].z - ps.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        float rinv = rsqrtf(r2);
+        float rinv = rsq_r2(r2);
         float r2inv = rinv * rinv;
         float vel = (ps.w * pw[k].w) * rinv;
         float dv, ev = 0.f;
@@ -613,8 +692,8 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
         }
         if (!v[u]) continue;
         if (en) {
-          const double ddx = d.x[3 * (e[u] + k)] - xs[0], ddy = d.x[3 * (e[u] + k) + 1] - xs[1],
-                       ddz = d.x[3 * (e[u] + k) + 2] - xs[2];
+          const int wa = d.wc_o[e[u]] + k;   // the water atom of the slot
+          const double ddx = d.x[3 * wa] - xs[0], ddy = d.x[3 * wa + 1] - xs[1], ddz = d.x[3 * wa + 2] - xs[2];
           const double rinv_d = rsqrt_e(ddx * ddx + ddy * ddy + ddz * ddz);
           eel += (double)(ps.w * pw[k].w) * rinv_d;
           if (k == 0 || wat3) {
@@ -711,7 +790,7 @@ __device__ __forceinline__ void pp_impl(const Dev& d) {
       const int eb = e[u] & ~kRow14;
       float dx = pj[u].x - pi.x, dy = pj[u].y - pi.y, dz = pj[u].z - pi.z;
       float r2 = dx * dx + dy * dy + dz * dz;
-      float rinv = rsqrtf(r2);
+      float rinv = rsq_r2(r2);
       float r2inv = rinv * rinv;
       float r6 = r2inv * r2inv * r2inv;
       float vel = (pi.w * pj[u].w) * rinv;
@@ -782,12 +861,7 @@ __device__ __forceinline__ void q_reaction(const Dev& d, int slot0, int lane, in
                                            const float* ry, const float* rz) {
 #pragma unroll
   for (int u = 0; u < kQGroup; ++u) {
-    float x = rx[u], y = ry[u], z = rz[u];
-    for (int o = 8; o < 64; o <<= 1) {
-      x += __shfl_xor(x, o, 64);
-      y += __shfl_xor(y, o, 64);
-      z += __shfl_xor(z, o, 64);
-    }
+    const float x = col8_sum_t(rx[u]), y = col8_sum_t(ry[u]), z = col8_sum_t(rz[u]);
     if (lane < kCl && t0 + u < n) {
       long long* q = d.qslot + 3 * ((size_t)d.ent_slot[slot0 + e0 + u] * kCl + lane);
       q[0] = fix(-(double)x, kFScale);
@@ -815,11 +889,9 @@ __global__ void __launch_bounds__(256) k_q_gather(Dev d) {
     const long long* q = d.qslot + 3 * ((size_t)m * kCl + b);
     gx += q[0]; gy += q[1]; gz += q[2];
   }
-  for (int o = 8; o < 64; o <<= 1) {
-    gx += __shfl_xor(gx, o, 64);
-    gy += __shfl_xor(gy, o, 64);
-    gz += __shfl_xor(gz, o, 64);
-  }
+  gx = col8_sum_t(gx);
+  gy = col8_sum_t(gy);
+  gz = col8_sum_t(gz);
   const int iq = d.qc_q[qc * kCl + b];
   if (k == 0 && iq >= 0) add_grad_fix(d.f, d.q_atom[d.win[w].q_off + iq], gx, gy, gz);
 }
@@ -910,9 +982,9 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
         dxv[k] = pw.x - pq.x; dyv[k] = pw.y - pq.y; dzv[k] = pw.z - pq.z;
         r2v[k] = dxv[k] * dxv[k] + dyv[k] * dyv[k] + dzv[k] * dzv[k];
       }
-      const float rO = rsqrtf(r2v[0]), r2O = rO * rO;
-      const float rH1 = rsqrtf(r2v[1]), r2H1 = rH1 * rH1;
-      const float rH2 = rsqrtf(r2v[2]), r2H2 = rH2 * rH2;
+      const float rO = rsq_r2(r2v[0]), r2O = rO * rO;
+      const float rH1 = rsq_r2(r2v[1]), r2H1 = rH1 * rH1;
+      const float rH2 = rsq_r2(r2v[2]), r2H2 = rH2 * rH2;
       const float r6hc = r2v[0] * r2v[0] * r2v[0];
       float dvO = 0.f, dvH1 = 0.f, dvH2 = 0.f;
       for (int s = 0; s < S; ++s) {
@@ -1141,7 +1213,7 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
       const int iacj = Q_SIDE ? iacv[u] : iac_fixed;
       const float dx = pj.x - pq.x, dy = pj.y - pq.y, dz = pj.z - pq.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
-      const float rinv = rsqrtf(r2), r2inv = rinv * rinv;
+      const float rinv = rsq_r2(r2), r2inv = rinv * rinv;
       const float r6hc = r2 * r2 * r2;
       float dv = 0.f;
       for (int s = 0; s < S; ++s) {
@@ -1432,7 +1504,8 @@ __global__ void __launch_bounds__(256) k_lrf_reduce(Dev d) {
 }
 
 // Evaluation per atom (lrf_taylor, md.f90:3646-3712), FP64.
-__device__ __forceinline__ void lrf_taylor_atom(const Dev& d, int i) {
+template <bool LOCAL>
+__device__ __forceinline__ void lrf_taylor_atom(const Dev& d, int i, long long* acc) {
   bool on = false;
   int w = 0;
   double e = 0.0;
@@ -1461,7 +1534,7 @@ __device__ __forceinline__ void lrf_taylor_atom(const Dev& d, int i) {
                         0.5 * (t_xyz * r0 + t_yyz * r1 + t_yzz * r2) * r1 +
                         0.5 * (t_xzz * r0 + t_yzz * r1 + t_zzz * r2) * r2;
       const double q = (double)d.x32[i].w;
-      add_grad(d.f, i, -q * f0, -q * f1, -q * f2);
+      atom_grad<LOCAL>(d, acc, i, -q * f0, -q * f1, -q * f2);
       e = 0.5 * q * vij;
       on = st.energy_now;
     }
@@ -1780,7 +1853,8 @@ __global__ void __launch_bounds__(256) k_bonded_gather(Dev d) {
 
 // fixed/shell restraints (fix_shell, md.f90:1414-1450) and the radial
 // solvent restraint (restrain_solvent, 14910-14957), per atom
-__device__ __forceinline__ void restr_atom(const Dev& d, int i) {
+template <bool LOCAL>
+__device__ __forceinline__ void restr_atom(const Dev& d, int i, long long* acc) {
   int w = 0;
   bool en = false, is_fix = false, is_shell = false, is_rad = false;
   double e_fs = 0.0, e_rad = 0.0;
@@ -1795,7 +1869,7 @@ __device__ __forceinline__ void restr_atom(const Dev& d, int i) {
         const double fk = (fl & 1) ? 200.0 : W.fk_pshell;
         double dr[3];
         for (int c = 0; c < 3; ++c) dr[c] = d.x[3 * i + c] - d.xtop[3 * i + c];
-        add_grad(d.f, i, fk * dr[0], fk * dr[1], fk * dr[2]);
+        atom_grad<LOCAL>(d, acc, i, fk * dr[0], fk * dr[1], fk * dr[2]);
         e_fs = 0.5 * fk * (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
         is_fix = fl & 1;
         is_shell = fl & 2;
@@ -1818,7 +1892,7 @@ __device__ __forceinline__ void restr_atom(const Dev& d, int i) {
         } else {
           dv = 0.0;
         }
-        add_grad(d.f, i, dv * dr[0], dv * dr[1], dv * dr[2]);
+        atom_grad<LOCAL>(d, acc, i, dv * dr[0], dv * dr[1], dv * dr[2]);
         is_rad = true;
       }
     }
@@ -1833,8 +1907,8 @@ __device__ __forceinline__ void restr_atom(const Dev& d, int i) {
 // reaches the wave energy sums of both)
 __global__ void __launch_bounds__(256) k_atom_terms(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  lrf_taylor_atom(d, i);
-  restr_atom(d, i);
+  lrf_taylor_atom<false>(d, i, nullptr);
+  restr_atom<false>(d, i, nullptr);
 }
 
 // p_restrain (md.f90:12940-13274): one 64-lane block per restraint record.
@@ -2016,7 +2090,9 @@ __global__ void __launch_bounds__(256) k_watpol_theta(Dev d) {
 // that order gives its target angle theta0.
 constexpr int kWpolMaxShell = 8;
 constexpr int kWpolLds = 4096;       // waters of one shell held (and sorted) in LDS
-__global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
+constexpr int kWpolRank = 1024;      // shells up to this size are ordered by counting
+constexpr int kWpolBlock = 256;      // threads per (window, shell) block
+__global__ void __launch_bounds__(kWpolBlock) k_watpol_shell(Dev d) {
   const int w = blockIdx.x / kWpolMaxShell, is = blockIdx.x % kWpolMaxShell;
   if (w >= d.nwin) return;
   const WinState& st = d.ws[w];
@@ -2051,7 +2127,33 @@ __global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
   __syncthreads();
   const int n = s_n;
   const bool in_lds = n <= kWpolLds;
-  if (in_lds) {
+  if (n <= kWpolRank) {
+    // place of each member by counting the members before it in the order
+    // (theta, water index), then a scatter to that place: the sorted order
+    // the bitonic network gives, in n^2/blockDim compares instead of its
+    // log^2 barrier stages
+    double th[kWpolRank / kWpolBlock];
+    int iw[kWpolRank / kWpolBlock], rank[kWpolRank / kWpolBlock];
+#pragma unroll
+    for (int m = 0; m < kWpolRank / kWpolBlock; ++m) {
+      const int k = threadIdx.x + kWpolBlock * m;
+      rank[m] = -1;
+      if (k >= n) continue;
+      th[m] = s_th[k];
+      iw[m] = s_iw[k];
+      int r = 0;
+      for (int j = 0; j < n; ++j) {
+        const double tj = s_th[j];
+        r += (tj < th[m] || (tj == th[m] && s_iw[j] < iw[m])) ? 1 : 0;
+      }
+      rank[m] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kWpolRank / kWpolBlock; ++m)
+      if (rank[m] >= 0) { s_th[rank[m]] = th[m]; s_iw[rank[m]] = iw[m]; }
+    __syncthreads();
+  } else if (in_lds) {
     // bitonic sort of (theta, water index) ascending; padding keys sort last
     int np = 1;
     while (np < n) np <<= 1;
@@ -2148,15 +2250,26 @@ __global__ void __launch_bounds__(256) k_watpol_shell(Dev d) {
   }
 }
 
+// the FP32 coordinates of atom i from its FP64 ones, into x32 and, for a
+// water atom, into its cluster slot
+__device__ __forceinline__ void store_x32(const Dev& d, int i) {
+  const float4 q = d.x32[i];
+  const float4 n = make_float4((float)d.x[3 * i], (float)d.x[3 * i + 1], (float)d.x[3 * i + 2], q.w);
+  d.x32[i] = n;
+  const int ws = d.wslot[i];
+  if (ws >= 0) d.xw[ws] = n;
+}
+
 // ------------------------------------------------------- integrator --
 // leapfrog (md.f90:4081-4111): v = (v - d*winv*dt)*Tscale; xx = x; x += v*dt
-__device__ __forceinline__ void leapfrog_atom(const Dev& d, const WinState& st, const WinDev& W, int i) {
+__device__ __forceinline__ void leapfrog_atom(const Dev& d, const WinState& st, const WinDev& W, int i,
+                                              const long long gfix[3]) {
 #pragma clang fp contract(off)
   const double ts = (i - W.ao) < W.nat_solute ? st.tscale_solute : st.tscale_solvent;
   const double wi = (double)d.winv[i];
   for (int c = 0; c < 3; ++c) {
     const int k = 3 * i + c;
-    const double g = (double)d.f[k] * (1.0 / kFScale);
+    const double g = (double)gfix[c] * (1.0 / kFScale);
     double v = (d.v[k] - g * wi * W.dt) * ts;
     d.v[k] = v;
     d.xx[k] = d.x[k];
@@ -2164,30 +2277,48 @@ __device__ __forceinline__ void leapfrog_atom(const Dev& d, const WinState& st, 
   }
 }
 
-__global__ void k_leapfrog(Dev d) {
+// ATOM_TERMS: the step's per-atom terms (LRF evaluation, fixed/shell and
+// radial solvent restraints) are evaluated here, summed with the gradient of
+// the other kernels in the lane's registers and stored back, so the step's
+// gradient array ends complete without per-atom atomics
+template <bool ATOM_TERMS>
+__global__ void __launch_bounds__(256) k_leapfrog(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  long long g[3] = {0, 0, 0};
+  if (ATOM_TERMS) {
+    // every lane reaches the wave energy sums of both
+    lrf_taylor_atom<true>(d, i, g);
+    restr_atom<true>(d, i, g);
+  }
   if (i >= d.ntot) return;
   const int w = d.wid[i];
   const WinState& st = d.ws[w];
   if (!st.active) return;
-  leapfrog_atom(d, st, d.win[w], i);
+  for (int c = 0; c < 3; ++c) {
+    g[c] += d.f[3 * i + c];
+    if (ATOM_TERMS) d.f[3 * i + c] = g[c];
+  }
+  leapfrog_atom(d, st, d.win[w], i, g);
 }
 
 // SHAKE (md.f90:14445-14566): one thread per connected component of a
 // molecule's constraint graph; components share no atoms, so iterating them
 // separately gives the reference's result for the whole molecule.
 // Constraints are swept in order, Gauss-Seidel, until every one is within
-// tol*d0^2 (at most 1000 sweeps).  Units of up to kShakeRegAtoms atoms
-// hold their coordinates in registers during the sweeps; larger ones work on
-// the coordinates in memory.  Both do the same arithmetic in the same order.
-// a unit of NA atoms held in registers (constant indices after unrolling;
-// the constraint's atoms are picked with selects)
+// tol*d0^2 (at most 1000 sweeps).  Units of up to 4 atoms with at most one
+// constraint per atom pair hold their coordinates and constraints in
+// registers during the sweeps; others work on the coordinates in memory.
+// Both do the same arithmetic in the same order.
+// a unit of NA atoms held in registers, with its at most NA(NA-1)/2
+// constraints (atom slots and d0^2) loaded once before the sweeps (constant
+// indices after unrolling; the constraint's atoms are picked with selects)
 template <int NA>
 __device__ __forceinline__ void shake_regs(const Dev& d, WinState& st, int c0, int nc, const int* al) {
 #pragma clang fp contract(off)
+  constexpr int NC = NA * (NA - 1) / 2;
   const double tol = (double)0.0001f;
   unsigned long long ready = 0ull;
-  const unsigned long long all = nc >= 64 ? ~0ull : ((1ull << nc) - 1ull);
+  const unsigned long long all = (1ull << nc) - 1ull;
   double px[NA][3], pxx[NA][3];
   float wv[NA];
 #pragma unroll
@@ -2197,11 +2328,21 @@ __device__ __forceinline__ void shake_regs(const Dev& d, WinState& st, int c0, i
     for (int c = 0; c < 3; ++c) { px[k][c] = d.x[3 * at + c]; pxx[k][c] = d.xx[3 * at + c]; }
     wv[k] = d.winv[at];
   }
+  int cli[NC], clj[NC];
+  double cd2[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const bool on = c < nc;
+    cli[c] = on ? d.sc_loc[2 * (c0 + c)] : 0;
+    clj[c] = on ? d.sc_loc[2 * (c0 + c) + 1] : 0;
+    cd2[c] = on ? d.sc_d2[c0 + c] : 0.0;
+  }
   int nits = 0;
   while (true) {
-    for (int c = 0; c < nc; ++c) {
-      if (ready & (1ull << c)) continue;
-      const int li = d.sc_loc[2 * (c0 + c)], lj = d.sc_loc[2 * (c0 + c) + 1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c >= nc || (ready & (1ull << c))) continue;
+      const int li = cli[c], lj = clj[c];
       double xi[3], xj[3], xxi[3], xxj[3];
       float wvi = 0.f, wvj = 0.f;
 #pragma unroll
@@ -2212,7 +2353,7 @@ __device__ __forceinline__ void shake_regs(const Dev& d, WinState& st, int c0, i
       double xij[3], xxij[3];
       for (int q = 0; q < 3; ++q) xij[q] = xi[q] - xj[q];
       const double xij2 = xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2];
-      const double d2 = d.sc_d2[c0 + c];
+      const double d2 = cd2[c];
       const double diff = d2 - xij2;
       if (fabs(diff) < tol * d2) ready |= 1ull << c;
       for (int q = 0; q < 3; ++q) xxij[q] = xxi[q] - xxj[q];
@@ -2248,9 +2389,10 @@ __global__ void __launch_bounds__(64) k_shake(Dev d) {
   const int c0 = d.su_start[u], nc = d.su_n[u];
   const int na = d.su_natom[u];
   const int* al = d.su_atom_list + d.su_atoms[u];
-  if (na == 3) { shake_regs<3>(d, st, c0, nc, al); return; }
-  if (na == 2) { shake_regs<2>(d, st, c0, nc, al); return; }
-  if (na == 4) { shake_regs<4>(d, st, c0, nc, al); return; }
+  // register units: at most one constraint per atom pair
+  if (na == 3 && nc <= 3) { shake_regs<3>(d, st, c0, nc, al); return; }
+  if (na == 2 && nc <= 1) { shake_regs<2>(d, st, c0, nc, al); return; }
+  if (na == 4 && nc <= 6) { shake_regs<4>(d, st, c0, nc, al); return; }
   const double tol = (double)0.0001f;
   unsigned long long ready = 0ull;  // up to 64 constraints per component
   const unsigned long long all = nc >= 64 ? ~0ull : ((1ull << nc) - 1ull);
@@ -2297,8 +2439,7 @@ __global__ void k_post(Dev d, int shake_any) {
     const int k = 3 * i + c;
     if (shake_any) d.v[k] = (d.x[k] - d.xx[k]) / W.dt;
   }
-  float4 q = d.x32[i];
-  d.x32[i] = make_float4((float)d.x[3 * i], (float)d.x[3 * i + 1], (float)d.x[3 * i + 2], q.w);
+  store_x32(d, i);
 }
 
 // temperature and Berendsen scaling factors (md.f90:3813-3914): one block per
@@ -2322,8 +2463,7 @@ __global__ void __launch_bounds__(256) k_ekin_part(Dev d, int advance, int post)
     if (post >= 0) {
       if (post)
         for (int c = 0; c < 3; ++c) d.v[3 * i + c] = (d.x[3 * i + c] - d.xx[3 * i + c]) / W.dt;
-      const float4 q = d.x32[i];
-      d.x32[i] = make_float4((float)d.x[3 * i], (float)d.x[3 * i + 1], (float)d.x[3 * i + 2], q.w);
+      store_x32(d, i);
     }
     const double* v = &d.v[3 * i];
     const double ek = 0.5 * d.mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -2496,17 +2636,17 @@ void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) 
 // The lists and forces of an MD step on three streams (after k_begin on s):
 //   s3: the Q tiles' forces (static lists: they need only the step's
 //       coordinates), then, once the neighbour rows are there, the
-//       solute-solute rows;
+//       solute-solute rows, and on energy steps the q-water energy pass;
 //   s2, after the group centres: on list steps the LRF expansions, then the
-//       q-q pairs, the bonded terms with their slot gather (which also
-//       collects the q-q slots), the LRF evaluation, atom and sequence
-//       restraints and water polarisation, and on energy steps the
-//       q-water energy pass;
+//       q-q pairs, water polarisation, the bonded terms with their slot
+//       gather (which also collects the q-q slots) and sequence restraints;
 //   s:  on list steps the neighbour rows, then the water-water and
 //       solute-water rows and on energy steps the q-solute energy pass; it
 //       then waits for s2 and s3.
 // All of them write the gradient and the energies only through integer
-// atomic adds, so the result does not depend on how they overlap.
+// atomic adds, so the result does not depend on how they overlap.  The
+// per-atom terms (LRF evaluation, atom restraints) follow in the leapfrog
+// kernel of launch_integrate(..., true).
 hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[5],
                               bool lists, int ngrp, int nwc, int nsc, bool q_energy) {
   hipError_t err = hipEventRecord(ev[0], s);
@@ -2519,8 +2659,7 @@ hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipSt
   if (err != hipSuccess) return err;
   if (lists) launch_lrf_build(d, s2);
   if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s2>>>(d);
-  launch_other_forces(d, s2);
-  if (q_energy) launch_q_energy_qw(d, s2);
+  launch_other_forces(d, s2, false);
   err = hipEventRecord(ev[3], s2);
   if (err != hipSuccess) return err;
   if (lists) launch_list_rows(d, s);
@@ -2528,6 +2667,7 @@ hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipSt
   if (err == hipSuccess) err = hipStreamWaitEvent(s3, ev[4], 0);
   if (err != hipSuccess) return err;
   launch_pp(d, s3, q_energy);
+  if (q_energy) launch_q_energy_qw(d, s3);
   err = hipEventRecord(ev[1], s3);
   if (err != hipSuccess) return err;
   launch_ww(d, s, q_energy);
@@ -2538,21 +2678,24 @@ hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipSt
   return err;
 }
 
-void launch_other_forces(const Dev& d, hipStream_t s) {
-  k_atom_terms<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+// water polarisation first: its few latency-bound blocks then run beside the
+// large pair kernels of the other streams instead of after them
+void launch_other_forces(const Dev& d, hipStream_t s, bool atom_terms) {
+  if (d.nwat_total) {
+    k_watpol_theta<<<blocks(d.nwat_total, 256), 256, 0, s>>>(d);
+    k_watpol_shell<<<d.nwin * kWpolMaxShell, kWpolBlock, 0, s>>>(d);
+  }
+  if (atom_terms) k_atom_terms<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
   if (nb) k_bonded<<<blocks(nb, d.bs_bonded), d.bs_bonded, 0, s>>>(d);
   // the gather also collects the q-q pair slots written by k_qq
   if ((nb || d.nqq) && !d.bond_atomic && d.ngw) k_bonded_gather<<<d.ngw, 256, 0, s>>>(d);
   if (d.nrst) k_prestr<<<d.nrst, 64, 0, s>>>(d);
-  if (d.nwat_total) {
-    k_watpol_theta<<<blocks(d.nwat_total, 256), 256, 0, s>>>(d);
-    k_watpol_shell<<<d.nwin * kWpolMaxShell, 256, 0, s>>>(d);
-  }
 }
 
-void launch_integrate(const Dev& d, hipStream_t s, int shake_any) {
-  k_leapfrog<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+void launch_integrate(const Dev& d, hipStream_t s, int shake_any, bool atom_terms) {
+  if (atom_terms) k_leapfrog<true><<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+  else k_leapfrog<false><<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   if (d.nsu) k_shake<<<blocks(d.nsu, 64), 64, 0, s>>>(d);
   k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d, 1, shake_any ? 1 : 0);
   k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 1);
