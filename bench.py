@@ -129,12 +129,13 @@ def cpu_reference(inps_water, inps_protein, water_steps, protein_steps, cores):
 def traffic_per_launch(kernel: str):
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), as recorded
-    in profiles/traffic.txt ("kernel bytes" lines); None when not measured."""
+    in profiles/traffic.txt (tab-separated "kernel, bytes, ..." lines); None
+    when not measured."""
     path = os.path.join(ROOT, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        t = line.split()
+        t = line.rstrip("\n").split("\t")
         if len(t) >= 2 and t[0] == kernel and not line.startswith("#"):
             return float(t[1])
     return None
@@ -198,6 +199,9 @@ def main():
     e.set_timing(True)
     e.run(n_timing, finalize=False)
     fam = e.timings()
+    # the water-water force kernel alone: 50 launches in a row on the
+    # engine's stream between two HIP events, at the timed run's lists
+    us_ww = e.time_kernel(0, 50)
     e.close()
 
     # work counts for the roofline: water-water molecule pairs of every window
@@ -212,8 +216,8 @@ def main():
     n_ww_total = nw * n_ww[0] + (len(inps) - nw) * n_ww[1]
     # water-water kernel: 200 FP32 FLOP per listed molecule pair (1 LJ +
     # Coulomb and 8 Coulomb-only site pairs) over the kernel's own device
-    # time, measured with HIP events on the engine's stream
-    t_ww = fam[4] * 1e-6 if len(fam) > 4 and fam[4] > 0 else None
+    # time per launch, measured with HIP events on the engine's stream
+    t_ww = us_ww * 1e-6 if us_ww > 0 else None
     achieved = (200.0 * n_ww_total / t_ww / 1e12) if t_ww else None
 
     result = {
@@ -253,13 +257,14 @@ def main():
         "autotune": tuning,
         "roofline": {
             "bound": "valu_fp32",
-            "kernel": "k_ww<false> (SPC water-water rows, FP32 vector ALU; compute roof = FP32 vector peak)",
+            "kernel": "k_ww<false, false> (SPC water-water rows, force-only instance; FP32 vector ALU; compute roof = FP32 vector peak)",
             "achieved": round(achieved, 4) if achieved else None, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 5) if achieved else None,
             "frac_nonpacked": round(achieved / FP32_NONPACKED_TFLOPS, 5) if achieved else None,
-            "traffic": traffic_per_launch("k_ww<false>"),
+            "traffic": traffic_per_launch("k_ww<false, false>"),
             "work": {"ww_molecule_pairs_per_step": n_ww_total, "flop_per_pair": 200,
-                     "kernel_us_per_launch": round(fam[4], 2) if fam else None},
+                     "kernel_us_per_launch": round(us_ww, 2),
+                     "timing": "50 back-to-back launches at the timed run's lists, HIP events"},
         },
     }
 

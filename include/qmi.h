@@ -129,7 +129,8 @@ int qmi_set_autotune(qmi_engine* e, int on);
 
 /* Choices of the autotuner: Q tiles per wave, threads per block of the row,
  * LRF-build and bonded kernels, whether blocks are grouped by XCD (0/1),
- * whether bonded gradients use atomics (0/1), the number of source parts
+ * the bonded mode (0: per-term slots and a per-atom gather, 1: atomics, 2:
+ * LDS sums per tile of terms), the number of source parts
  * of the LRF build, then the measured microseconds of the first six chosen
  * configurations (13 values).  Returns the number written. */
 int qmi_tuning(qmi_engine* e, double* out, int cap);
@@ -146,6 +147,15 @@ int qmi_set_timing(qmi_engine* e, int on);
  * the last are zero unless qmi_set_timing was on.  Returns the number of
  * values written. */
 int qmi_timings(qmi_engine* e, double* out, int cap);
+
+/* Average device time of one launch of a step kernel in microseconds, over
+ * `reps` launches in a row on the engine's stream between two HIP events (no
+ * reference counterpart; for measurement).  which 0: the water-water force
+ * kernel.  Call between qmi_run calls while the windows are still running:
+ * the launches add to the gradient, so the engine then evaluates the
+ * gradient and energies again at the current coordinates, as qmi_evaluate
+ * does without rebuilding lists. */
+int qmi_time_kernel(qmi_engine* e, int which, int reps, double* us);
 
 /* Last error message of the calling thread. */
 const char* qmi_last_error(void);

@@ -123,14 +123,16 @@ constexpr int kTRing = 6;
 constexpr int kTempParts = 8;      // blocks per window of the kinetic-energy sum
 constexpr int kGatherAtoms = 256;  // atoms per block of the bonded gather
 constexpr int kGatherSlots = 2048; // slots staged in LDS per gather block
+constexpr int kBondTileTerms = 512; // bonded terms per tile
+constexpr int kBondSpan = 1024;     // atoms of a tile's LDS gradient sums
 constexpr int kQGroup = 4;        // Q-tile sums go to fixed point every 4 tiles of a list
 constexpr int kQChunkN = 4;
 constexpr int kQChunks[kQChunkN] = {4, 8, 16, 32};   // autotuner candidates, tiles per wave
 
 // Neighbour rows of the dynamic pair lists (WW, W<-P, P<-W, P<-P): row
 // ic*8 + a belongs to member a of i-cluster ic and holds the j entries of
-// that member (water: O atom; solute: atom; P<-P entries carry the 1-4 flag
-// in bit 30), in a fixed capacity `cap` per row.
+// that member (water: its slot in xw; solute: atom; P<-P entries carry the
+// 1-4 flag in bit 30), in a fixed capacity `cap` per row.
 struct RowList {
   int* cnt;        // entries per row
   int* ent;        // [row * cap + k]
@@ -153,6 +155,7 @@ struct Dev {
   // or -1 for solute atoms
   float4* xw;
   const int* wslot;
+  double4* xwd;         // the same slots in FP64 (x, y, z, unused), for the energy terms
   float4* ljp;          // avdw1, bvdw1, avdw3, bvdw3
   long long* f;         // gradient, 3 per atom, fixed point 2^-32
   const float* winv;
@@ -236,10 +239,23 @@ struct Dev {
   int bs_rows, bs_lrf, bs_bonded;
   // 1: blocks on one XCD take consecutive work (block_id in the kernels)
   int xcd;
-  // 1: bonded terms and q-q pairs add their gradients with atomics; 0: they
-  // write per-term slots that k_bonded_gather sums per atom.  Both give the
-  // same int64 sums.
+  // bonded mode.  0: bonded terms and q-q pairs write per-term slots that
+  // k_bonded_gather sums per atom; 1: both add their gradients with atomics;
+  // 2: the bonded terms sum in LDS per tile (k_bonded_tiles) and the q-q
+  // pairs use atomics.  All give the same int64 sums.
   int bond_atomic;
+  // bonded tiles: (first entry of btile_terms, terms, first atom, atoms
+  // spanned); btile_terms: term indices, grouped by kind within a tile
+  int nbtile;
+  const int4* btile;
+  const int* btile_terms;
+  // bonded mode 2: the q-q pair slots (slot k >= qq_slot0 at qq_map[k -
+  // qq_slot0] of qqslot), contiguous per atom: atoms qqg_atom, runs qqg_start
+  long long* qqslot;
+  const int* qq_map;
+  int nqqg;
+  const int* qqg_atom;
+  const int* qqg_start;
   // q-q pairs (static)
   int nqq;
   const int4* qq_ij;     // iq (window-local), j (global), jq (1-based q or 0), ljcod
@@ -290,6 +306,7 @@ struct Dev {
   ShellDev* shells;
   double* wtheta;        // per water (global water index), theta
   int* wshell;           // shell index (-1 if none)
+  int* wrank;            // place in its shell's order (theta, water index), from 0
   const int* w_off;      // per window: first global water index
   const int* wat_win;    // per global water: its window
   int nwat_total;

@@ -22,6 +22,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/qmi.h"
@@ -113,10 +114,11 @@ struct qmi_engine {
   // host memory while the next chunk runs on stream
   hipStream_t copy_stream = nullptr;
   hipEvent_t chunk_ev[2] = {nullptr, nullptr};
+  hipEvent_t copy_ev = nullptr;               // the last drain's ring copies done
   // the captured step's other streams (LRF, bonded, restraints and water
   // polarisation; the Q tiles) and its fork/join events
   hipStream_t side_stream[2] = {nullptr, nullptr};
-  hipEvent_t step_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t step_ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int run_epoch = 0;                          // qmi_run calls so far (tags the ring records)
   long long* h_er = nullptr;
   double* h_or = nullptr;                     // EVB off-diagonal ring, pinned
@@ -153,6 +155,7 @@ qmi_engine::~qmi_engine() {
   for (auto& x : ev) if (x) (void)hipEventDestroy(x);
   for (void* p : bufs) (void)hipFree(p);
   for (auto& x : chunk_ev) if (x) (void)hipEventDestroy(x);
+  if (copy_ev) (void)hipEventDestroy(copy_ev);
   for (auto& x : step_ev) if (x) (void)hipEventDestroy(x);
   for (auto& x : side_stream) if (x) (void)hipStreamDestroy(x);
   if (h_er) (void)hipHostFree(h_er);
@@ -788,7 +791,7 @@ static int build_device(qmi_engine* e) {
   d.nwin = NW;
   d.bs_rows = d.bs_lrf = d.bs_bonded = 256;
   d.xcd = 0;
-  d.bond_atomic = 0;
+  d.bond_atomic = 2;   // bonded tiles until the autotuner picks
   d.ntot = ao;
   int rc = 0;
 #define UP(dst, src) if ((rc = dev_upload(e, &dst, src)) != 0) return rc
@@ -868,15 +871,18 @@ static int build_device(qmi_engine* e) {
   {
     std::vector<int> wslot(x32.size(), -1);
     std::vector<float4> xw(std::max<size_t>(3, 3 * wc_o.size()), make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<double4> xwd(xw.size(), make_double4(0.0, 0.0, 0.0, 0.0));
     for (size_t m = 0; m < wc_o.size(); ++m) {
       const int o = wc_o[m];
       if (o < 0) continue;
       for (int k = 0; k < 3; ++k) {
         wslot[o + k] = 3 * (int)m + k;
         xw[3 * m + k] = x32[o + k];
+        xwd[3 * m + k] = make_double4(x[3 * (o + k)], x[3 * (o + k) + 1], x[3 * (o + k) + 2], 0.0);
       }
     }
     UP(d.xw, xw);
+    UP(d.xwd, xwd);
     UP(ip, wslot); d.wslot = ip;
   }
   UP(ip, sc_a); d.sc_a = ip;
@@ -1035,6 +1041,71 @@ static int build_device(qmi_engine* e) {
     UP(ip, bg_start); d.bg_start = ip;
     UP(ip, slot_map); d.slot_map = ip;
     UP(ip, slot_atom); d.slot_atom = ip;
+    // bonded tiles: terms ordered by their first (lowest) atom, cut into
+    // tiles of at most kBondTileTerms terms spanning at most kBondSpan atoms,
+    // each tile's terms grouped by kind
+    {
+      const int nat_kind[8] = {2, 3, 4, 4, 2, 3, 4, 4};
+      const int nk[8] = {(int)b_ij.size(), (int)a_ijk.size(), (int)t_ijkl.size(), (int)i_ijkl.size(),
+                         (int)qb_ij.size(), (int)qa_ijk.size(), (int)qt_ijkl.size(), (int)qi_ijkl.size()};
+      struct TermRef { int lo, hi, kind, t; };
+      std::vector<TermRef> terms;
+      int t = 0, sl = 0;
+      for (int kd = 0; kd < 8; ++kd)
+        for (int m = 0; m < nk[kd]; ++m, ++t) {
+          int lo = slot_atom[sl], hi = lo;
+          for (int a = 1; a < nat_kind[kd]; ++a) {
+            lo = std::min(lo, slot_atom[sl + a]);
+            hi = std::max(hi, slot_atom[sl + a]);
+          }
+          terms.push_back({lo, hi, kd, t});
+          sl += nat_kind[kd];
+        }
+      std::stable_sort(terms.begin(), terms.end(), [](const TermRef& a, const TermRef& b) { return a.lo < b.lo; });
+      std::vector<int4> btile;
+      std::vector<int> bterms;
+      for (size_t k0 = 0; k0 < terms.size();) {
+        size_t k1 = k0 + 1;
+        const int lo = terms[k0].lo;
+        int hi = terms[k0].hi;
+        while (k1 < terms.size() && (int)(k1 - k0) < kBondTileTerms && std::max(hi, terms[k1].hi) - lo < kBondSpan) {
+          hi = std::max(hi, terms[k1].hi);
+          ++k1;
+        }
+        std::stable_sort(terms.begin() + k0, terms.begin() + k1,
+                         [](const TermRef& a, const TermRef& b) { return a.kind < b.kind; });
+        btile.push_back(make_int4((int)bterms.size(), (int)(k1 - k0), lo, std::min(hi - lo + 1, kBondSpan)));
+        for (size_t k = k0; k < k1; ++k) bterms.push_back(terms[k].t);
+        k0 = k1;
+      }
+      d.nbtile = (int)btile.size();
+      int4* bt;
+      UP(bt, btile); d.btile = bt;
+      UP(ip, bterms); d.btile_terms = ip;
+    }
+    // the q-q pair slots alone, grouped by atom, for bonded mode 2
+    {
+      const int nqs = (int)slot_atom.size() - d.qq_slot0;
+      std::vector<int> qcount(ao, 0);
+      for (int k = 0; k < nqs; ++k) qcount[slot_atom[d.qq_slot0 + k]]++;
+      std::vector<int> qqg_atom, qqg_start(1, 0), qpos(ao, -1);
+      for (int a = 0; a < ao; ++a)
+        if (qcount[a]) {
+          qpos[a] = (int)qqg_atom.size();
+          qqg_atom.push_back(a);
+          qqg_start.push_back(qqg_start.back() + qcount[a]);
+        }
+      std::vector<int> qfill(qqg_atom.size(), 0), qq_map(std::max(1, nqs), 0);
+      for (int k = 0; k < nqs; ++k) {
+        const int p = qpos[slot_atom[d.qq_slot0 + k]];
+        qq_map[k] = qqg_start[p] + qfill[p]++;
+      }
+      d.nqqg = (int)qqg_atom.size();
+      UP(ip, qqg_atom); d.qqg_atom = ip;
+      UP(ip, qqg_start); d.qqg_start = ip;
+      UP(ip, qq_map); d.qq_map = ip;
+      if ((rc = dev_alloc(e, &d.qqslot, 3 * (size_t)std::max(1, nqs))) != 0) return rc;
+    }
     if ((rc = dev_alloc(e, &d.bslot, 3 * std::max<size_t>(1, slot_atom.size()))) != 0) return rc;
   }
   d.nrst = (int)rs_kwij.size();
@@ -1049,6 +1120,7 @@ static int build_device(qmi_engine* e) {
   UP(sd, shells); d.shells = sd;
   if ((rc = dev_alloc(e, &d.wtheta, (size_t)e->nwat_total)) != 0) return rc;
   if ((rc = dev_alloc(e, &d.wshell, (size_t)e->nwat_total)) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.wrank, (size_t)e->nwat_total)) != 0) return rc;
   UP(ip, w_off); d.w_off = ip;
   UP(ip, wat_win); d.wat_win = ip;
   d.nwat_total = e->nwat_total;
@@ -1444,15 +1516,20 @@ static int autotune(qmi_engine* e) {
   launch_begin(d, e->stream, 0, 2, 1);
   launch_lists(d, e->stream, e->ngrp, e->nwc, e->nsc);
   HIP_OK(hipStreamSynchronize(e->stream));
-  const int bs[3] = {64, 128, 256};
+  const int bs[3] = {256, 128, 64};
+  // Candidates are tried from the coarsest work grouping down, and a later
+  // one replaces the best so far only when it is more than 2 % faster:
+  // timed alone, close candidates differ by noise, and in the step graph,
+  // where the kernels share the GPU, the coarser groupings fare better.
+  auto faster = [](double t, double best) { return t < 0.98 * best; };
   // 0: Q-tile work-item size
   {
-    int bi = 2;
+    int bi = kQChunkN - 1;
     double bt = 1e30;
-    for (int ci = 0; ci < kQChunkN; ++ci) {
+    for (int ci = kQChunkN - 1; ci >= 0; --ci) {
       for (int tt : {T_PQ, T_WQ}) { d.tl[tt].work = e->qwork[tt][ci]; d.tl[tt].nwork = e->nqwork[tt][ci]; }
       const double t = time_us([&] { launch_q_tiles(d, e->stream); });
-      if (t < bt) { bt = t; bi = ci; }
+      if (faster(t, bt)) { bt = t; bi = ci; }
     }
     for (int tt : {T_PQ, T_WQ}) { d.tl[tt].work = e->qwork[tt][bi]; d.tl[tt].nwork = e->nqwork[tt][bi]; }
     e->tuned[0] = bi;
@@ -1467,7 +1544,7 @@ static int autotune(qmi_engine* e) {
       *field[g] = b;
       const double t = g == 0 ? time_us([&] { launch_nonbonded(d, e->stream, true, false); })
                               : time_us([&] { launch_bonded(d, e->stream); });
-      if (t < bt) { bt = t; bb = b; }
+      if (faster(t, bt)) { bt = t; bb = b; }
     }
     *field[g] = bb;
     e->tuned[1 + g] = bb;
@@ -1475,7 +1552,7 @@ static int autotune(qmi_engine* e) {
   }
   // 2, 6: threads per block and source parts of the LRF build
   {
-    int bv = 8;
+    int bv = 11;
     double bt = 1e30;
     auto set_lrf = [&](int vi) {
       d.bs_lrf = 64 << (vi / 4);
@@ -1483,10 +1560,10 @@ static int autotune(qmi_engine* e) {
       d.lrf_work = e->lrfwork[vi];
       d.nlrfw = e->nlrfwork[vi];
     };
-    for (int vi = 0; vi < 12; ++vi) {
+    for (int vi = 11; vi >= 0; --vi) {
       set_lrf(vi);
       const double t = time_us([&] { launch_lrf_build(d, e->stream); });
-      if (t < bt) { bt = t; bv = vi; }
+      if (faster(t, bt)) { bt = t; bv = vi; }
     }
     set_lrf(bv);
     e->tuned[2] = d.bs_lrf;
@@ -1498,29 +1575,37 @@ static int autotune(qmi_engine* e) {
     double tx[2];
     for (int x = 0; x < 2; ++x) {
       d.xcd = x;
+      // the pair and Q-tile kernels, whose gathers the grouping keeps in
+      // one XCD's L2 (the LRF build, timed with them, would drown the
+      // difference in its own run-to-run spread)
       tx[x] = time_us([&] {
         launch_nonbonded(d, e->stream, true, false);
         launch_q_tiles(d, e->stream, false);
-        launch_lrf_build(d, e->stream);
       });
     }
-    d.xcd = tx[1] < tx[0] ? 1 : 0;
+    // ungrouped unless grouped is clearly faster (grouping keeps each XCD's
+    // share of the water slots in its own L2, but in the step graph it was
+    // measured slower for the water-water and LRF kernels)
+    d.xcd = faster(tx[1], tx[0]) ? 1 : 0;
     e->tuned[4] = d.xcd;
     e->tune_us[4] = tx[d.xcd];
   }
-  // 5: bonded and q-q gradients by atomics or by slots and a per-atom gather
+  // 5: bonded and q-q gradients: LDS tiles (2), slots and a per-atom gather
+  // (0) or atomics (1)
   {
-    double ta[2];
-    for (int a = 0; a < 2; ++a) {
+    int bm = 2;
+    double bt = 1e30;
+    for (int a : {2, 0, 1}) {
       d.bond_atomic = a;
-      ta[a] = time_us([&] {
+      const double t = time_us([&] {
         launch_nonbonded(d, e->stream, false, false);
         launch_other_forces(d, e->stream);
       });
+      if (faster(t, bt)) { bt = t; bm = a; }
     }
-    d.bond_atomic = ta[1] < ta[0] ? 1 : 0;
-    e->tuned[5] = d.bond_atomic;
-    e->tune_us[5] = ta[d.bond_atomic];
+    d.bond_atomic = bm;
+    e->tuned[5] = bm;
+    e->tune_us[5] = bt;
   }
   HIP_OK(hipGetLastError());
   (void)hipEventDestroy(t0);
@@ -1546,6 +1631,7 @@ int qmi_prepare(qmi_engine* e) {
     if (!x) HIP_OK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
   for (auto& x : e->chunk_ev)
     if (!x) HIP_OK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+  if (!e->copy_ev) HIP_OK(hipEventCreateWithFlags(&e->copy_ev, hipEventDisableTiming));
   int rc = build_device(e);
   if (rc) return rc;
   if (e->autotune && (rc = autotune(e)) != 0) return rc;
@@ -1649,8 +1735,7 @@ static int write_frame(HostWin& h, const double* x) {
 // if it carries the step and run number expected for its slot, so the slots
 // of a window that stopped earlier (its own [MD] steps, or a failure) are
 // never read as its later steps.
-static int drain(qmi_engine* e, const std::vector<int>& start, const std::vector<int>& stop, int k0, int n,
-                 hipEvent_t done) {
+static int drain_copy(qmi_engine* e, int k0, int n, hipEvent_t done) {
   const int NW = (int)e->wins.size();
   const int R = e->d.ring_n;
   HIP_OK(hipStreamWaitEvent(e->copy_stream, done, 0));
@@ -1672,7 +1757,22 @@ static int drain(qmi_engine* e, const std::vector<int>& start, const std::vector
                           e->copy_stream));
     if (n2 > 0) HIP_OK(hipMemcpyAsync(e->h_or, e->d.o_ring, n2 * or_row * 8, hipMemcpyDeviceToHost, e->copy_stream));
   }
-  HIP_OK(hipStreamSynchronize(e->copy_stream));
+  // the chunk after next writes these ring slots again: the engine's stream
+  // waits for the copies
+  HIP_OK(hipEventRecord(e->copy_ev, e->copy_stream));
+  HIP_OK(hipStreamWaitEvent(e->stream, e->copy_ev, 0));
+  return 0;
+}
+
+// After drain_copy: the chunk's .en records, log summaries and temperature
+// lines, written from the host copies of the rings.  Runs beside the next
+// chunk (see qmi_run), touching only the windows' output state.
+static int drain_write(qmi_engine* e, const std::vector<int>& start, const std::vector<int>& stop, int k0, int n) {
+  const int NW = (int)e->wins.size();
+  const int R = e->d.ring_n;
+  const size_t or_row = (size_t)e->d.noffd_all * 2;
+  HIP_OK(hipSetDevice(e->device));
+  HIP_OK(hipEventSynchronize(e->copy_ev));
   const long long* er = e->h_er;
   const double* tr = e->h_tr;
   const double epoch = 4294967296.0 * (double)e->run_epoch;
@@ -1728,9 +1828,30 @@ static int sync_state_to_host(qmi_engine* e, const std::vector<int>* which) {
     so[wi + 1] = so[wi] + e->wins[wi].w->wshell.size();
   }
   HIP_OK(hipStreamSynchronize(e->stream));
+  // one copy per array when most windows are wanted (each small copy costs
+  // tens of microseconds), else one per window
+  const bool bulk = which->size() * 2 > (size_t)NW;
+  std::vector<double> xs, vs;
+  std::vector<ShellDev> sha;
+  if (bulk) {
+    xs.resize(3 * ao[NW]);
+    vs.resize(3 * ao[NW]);
+    HIP_OK(hipMemcpy(xs.data(), e->d.x, xs.size() * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(vs.data(), e->d.v, vs.size() * 8, hipMemcpyDeviceToHost));
+    if (so[NW]) {
+      sha.resize(so[NW]);
+      HIP_OK(hipMemcpy(sha.data(), e->d.shells, sha.size() * sizeof(ShellDev), hipMemcpyDeviceToHost));
+    }
+  }
   std::vector<ShellDev> sh;
   for (int wi : *which) {
     Window& W = *e->wins[wi].w;
+    if (bulk) {
+      std::copy(xs.begin() + 3 * ao[wi], xs.begin() + 3 * ao[wi + 1], W.x.begin());
+      std::copy(vs.begin() + 3 * ao[wi], vs.begin() + 3 * ao[wi + 1], W.v.begin());
+      for (size_t k = 0; k < W.wshell.size(); ++k) W.wshell[k].theta_corr = sha[so[wi] + k].theta_corr;
+      continue;
+    }
     HIP_OK(hipMemcpy(W.x.data(), e->d.x + 3 * ao[wi], (size_t)3 * W.natom * 8, hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(W.v.data(), e->d.v + 3 * ao[wi], (size_t)3 * W.natom * 8, hipMemcpyDeviceToHost));
     if (!W.wshell.empty()) {
@@ -1764,8 +1885,9 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     ws[wi].stop_at = stop[wi];
     ws[wi].ring_base = start[wi];
     ws[wi].run_epoch = e->run_epoch;
-    HIP_OK(hipMemcpy(&e->d.ws[wi].stop_at, &ws[wi].stop_at, 3 * sizeof(int), hipMemcpyHostToDevice));
   }
+  // the device is idle: the whole state array goes back in one copy
+  HIP_OK(hipMemcpy(e->d.ws, ws.data(), NW * sizeof(WinState), hipMemcpyHostToDevice));
   if (e->timing_on && !e->ev[0])
     for (auto& x : e->ev) HIP_OK(hipEventCreate(&x));
   // per family: 0 lists+LRF build, 1 nonbonded tiles, 2 bonded/sequence restraints/polarisation,
@@ -1789,11 +1911,17 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     bool ok = true;
     for (int k = 0; k < 4 && ok; ++k) {
       hipGraph_t g = nullptr;
+      // energy steps keep the bonded slots and their gather even where the
+      // tuner picked the LDS tiles: beside the FP64 energy passes the tile
+      // kernel (164 VGPRs) finds few wave slots, and the result is the same
+      const int bond_mode = e->d.bond_atomic;
+      if ((k & 2) && bond_mode == 2) e->d.bond_atomic = 0;
       ok = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
       if (ok) {
         launch_step(k);
         ok = hipStreamEndCapture(e->stream, &g) == hipSuccess && g != nullptr;
       }
+      e->d.bond_atomic = bond_mode;
       if (ok) ok = hipGraphInstantiate(&e->step_exec[k], g, nullptr, nullptr, 0) == hipSuccess;
       if (ok) {
         e->step_graph[k] = g;
@@ -1869,9 +1997,32 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
   bool any_trj = false;
   for (auto& h : e->wins) any_trj = any_trj || h.trj != nullptr;
   std::vector<double> xh;
-  // chunk c is drained while chunk c + 1 runs; a chunk ends after a step
-  // with trajectory frames, which are written from the positions then
+  // chunk c is drained while chunk c + 1 runs: its ring copies are queued
+  // behind it, and its records are written on a host thread while the next
+  // chunk's steps are launched (writing a chunk's log summaries takes about
+  // as long as a dozen steps, which the GPU would otherwise wait for).  A
+  // chunk ends after a step with trajectory frames, which are written from
+  // the positions then.
   int pend_first = 0, pend_n = 0, cidx = 0;
+  std::thread writer;
+  int writer_rc = 0;
+  std::string writer_err;
+  auto join_writer = [&]() -> int {
+    if (writer.joinable()) writer.join();
+    if (writer_rc) return set_err(writer_rc, writer_err);
+    return 0;
+  };
+  auto start_writer = [&](int first, int n) {
+    writer = std::thread([e, &start, &stop, first, n, &writer_rc, &writer_err] {
+      writer_rc = drain_write(e, start, stop, first, n);
+      if (writer_rc) writer_err = qmi_last_error();
+    });
+  };
+  // the writer thread must end before this function returns on any path
+  struct WriterGuard {
+    std::thread& t;
+    ~WriterGuard() { if (t.joinable()) t.join(); }
+  } writer_guard{writer};
   while (done < nsteps) {
     int chunk = (int)std::min<long long>(kChunk, nsteps - done);
     for (int k = 0; k < chunk; ++k)
@@ -1908,6 +2059,12 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
       }
     }
     HIP_OK(hipGetLastError());
+    // frames and backups read and write the windows' host state: the
+    // previous chunk's writer finishes first
+    if ((any_trj && any_frame(done + chunk - 1)) || any_backup(done + chunk - 1)) {
+      int rc = join_writer();
+      if (rc) return rc;
+    }
     if (any_trj && any_frame(done + chunk - 1)) {
       xh.resize((size_t)3 * e->ntot);
       HIP_OK(hipStreamSynchronize(e->stream));
@@ -1932,16 +2089,20 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     }
     HIP_OK(hipEventRecord(e->chunk_ev[cidx & 1], e->stream));
     if (pend_n > 0) {
-      int rc = drain(e, start, stop, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
+      int rc = join_writer();
+      if (rc == 0) rc = drain_copy(e, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
       if (rc) return rc;
+      start_writer(pend_first, pend_n);
     }
     pend_first = (int)done;
     pend_n = chunk;
     ++cidx;
     done += chunk;
   }
-  if (pend_n > 0) {
-    int rc = drain(e, start, stop, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
+  {
+    int rc = join_writer();
+    if (rc == 0 && pend_n > 0) rc = drain_copy(e, pend_first, pend_n, e->chunk_ev[(cidx - 1) & 1]);
+    if (rc == 0 && pend_n > 0) rc = drain_write(e, start, stop, pend_first, pend_n);
     if (rc) return rc;
   }
   HIP_OK(hipStreamSynchronize(e->stream));
@@ -2258,6 +2419,34 @@ int qmi_set_timing(qmi_engine* e, int on) {
   if (!e) return set_err(-1, "null engine");
   e->timing_on = on != 0;
   return 0;
+}
+
+int qmi_time_kernel(qmi_engine* e, int which, int reps, double* us) {
+  if (!e || !e->prepared) return set_err(-2, "engine not prepared");
+  if (which != 0 || reps < 1 || reps > 100000 || !us) return set_err(-3, "invalid kernel or repeat count");
+  HIP_OK(hipSetDevice(e->device));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipError_t err = hipEventCreate(&ev[0]);
+  if (err == hipSuccess) err = hipEventCreate(&ev[1]);
+  if (err == hipSuccess) {
+    launch_ww(e->d, e->stream, false);   // first launch untimed
+    err = hipEventRecord(ev[0], e->stream);
+  }
+  if (err == hipSuccess) {
+    for (int k = 0; k < reps; ++k) launch_ww(e->d, e->stream, false);
+    err = hipEventRecord(ev[1], e->stream);
+  }
+  if (err == hipSuccess) err = hipEventSynchronize(ev[1]);
+  float ms = 0.f;
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
+  if (err != hipSuccess) return set_err(-5, std::string("HIP error: ") + hipGetErrorString(err));
+  *us = 1000.0 * (double)ms / reps;
+  // the repeated launches added to the gradient: recompute it, and the
+  // energies, at the current coordinates
+  return evaluate(e, 0, 1);
 }
 
 int qmi_timings(qmi_engine* e, double* out, int cap) {

@@ -20,7 +20,7 @@ void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww = true, bool q_e
 // radial solvent restraints) are left to launch_integrate(..., true)
 void launch_other_forces(const Dev& d, hipStream_t s, bool atom_terms = true);
 // the lists and forces of an MD step on three streams (see the definition)
-hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[5],
+hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[6],
                               bool lists, int ngrp, int nwc, int nsc, bool q_energy);
 // atom_terms true: the leapfrog kernel first adds the per-atom terms that
 // launch_other_forces(..., false) left out

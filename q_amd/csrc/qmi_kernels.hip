@@ -85,10 +85,19 @@ __device__ __forceinline__ void atom_grad(const Dev& d, long long* acc, int atom
 }
 
 // gradient of term atom k (in term order): into its own slot (fixed point, no
-// atomic), or added to the atom's gradient with atomics; the int64 values are
-// the same either way
+// atomic), or added to the atom's gradient with atomics (bonded mode 1); the
+// int64 values are the same either way
 __device__ __forceinline__ void put_slot(const Dev& d, int k, double gx, double gy, double gz) {
   const long long fx = fix(gx, kFScale), fy = fix(gy, kFScale), fz = fix(gz, kFScale);
+  if (d.bond_atomic == 2) {
+    // bonded mode 2: only the q-q pairs come here, into slots of their own
+    // that k_qq_gather sums per atom (atomics would collide on the q-atoms)
+    const int m = d.qq_map[k - d.qq_slot0];
+    d.qqslot[3 * m + 0] = fx;
+    d.qqslot[3 * m + 1] = fy;
+    d.qqslot[3 * m + 2] = fz;
+    return;
+  }
   if (d.bond_atomic) {
     const int a = d.slot_atom[k];
     atomicAdd((unsigned long long*)&d.f[3 * a + 0], (unsigned long long)fx);
@@ -101,6 +110,12 @@ __device__ __forceinline__ void put_slot(const Dev& d, int k, double gx, double 
   d.bslot[3 * m + 1] = fy;
   d.bslot[3 * m + 2] = fz;
 }
+
+// the gradient of a term atom into its slot, or with atomics (put_slot)
+struct SlotSink {
+  const Dev* d;
+  __device__ void operator()(int k, int, double gx, double gy, double gz) const { put_slot(*d, k, gx, gy, gz); }
+};
 
 __device__ __forceinline__ void add_energy(WinState* ws, int w, int slot, double e) {
   if (e != 0.0) atomicAdd((unsigned long long*)&ws[w].E[slot], (unsigned long long)fix(e, kEScale));
@@ -532,7 +547,10 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
     const bool cnt = en && oa < ob_atom;   // each unordered pair counted once
     if (cnt) {
       double xj[9];
-      for (int m = 0; m < 9; ++m) xj[m] = d.x[3 * ob_atom + m];
+      for (int m = 0; m < 3; ++m) {
+        const double4 q = d.xwd[3 * ob + m];
+        xj[3 * m] = q.x; xj[3 * m + 1] = q.y; xj[3 * m + 2] = q.z;
+      }
       for (int p = 0; p < 3; ++p)
         for (int q = 0; q < 3; ++q) {
           const double dx = xj[3 * q] - xi[3 * p], dy = xj[3 * q + 1] - xi[3 * p + 1],
@@ -692,8 +710,8 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
         }
         if (!v[u]) continue;
         if (en) {
-          const int wa = d.wc_o[e[u]] + k;   // the water atom of the slot
-          const double ddx = d.x[3 * wa] - xs[0], ddy = d.x[3 * wa + 1] - xs[1], ddz = d.x[3 * wa + 2] - xs[2];
+          const double4 xa = d.xwd[3 * e[u] + k];   // the water atom of the slot, FP64
+          const double ddx = xa.x - xs[0], ddy = xa.y - xs[1], ddz = xa.z - xs[2];
           const double rinv_d = rsqrt_e(ddx * ddx + ddy * ddy + ddz * ddz);
           eel += (double)(ps.w * pw[k].w) * rinv_d;
           if (k == 0 || wat3) {
@@ -1088,7 +1106,8 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
   }
   long long efix[kMaxS][2] = {};
   double eel[kMaxS] = {}, evdw[kMaxS] = {};
-  const double* x = d.x;
+  const int qa0 = qa >= 0 ? qa : 0;
+  const double xq = d.x[3 * qa0], yq = d.x[3 * qa0 + 1], zq = d.x[3 * qa0 + 2];
   for (int t = 0; t < n; ++t) {
     const int jc = L.jc[s0 + t];
     const uint2 m = L.mask[s0 + t];
@@ -1097,9 +1116,11 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
       double rinv[3], r6[NLJ];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double dx = x[3 * (o + k)] - x[3 * qa];
-        const double dy = x[3 * (o + k) + 1] - x[3 * qa + 1];
-        const double dz = x[3 * (o + k) + 2] - x[3 * qa + 2];
+        // the water's FP64 coordinates from its cluster slot
+        const double4 pw = d.xwd[3 * (jc * kCl + b) + k];
+        const double dx = pw.x - xq;
+        const double dy = pw.y - yq;
+        const double dz = pw.z - zq;
         const double r2 = dx * dx + dy * dy + dz * dz;
         rinv[k] = rsqrt_d(r2);
         if (k < NLJ) r6[k < NLJ ? k : 0] = r2 * r2 * r2;
@@ -1299,8 +1320,10 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
 
 // q-q and q-(bonded neighbour) pairs (nonbond_qq, md.f90:10763-10868), one
 // thread per (pair, state)
-__global__ void __launch_bounds__(128) k_qq(Dev d) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// q-q pair k, its gradients handed to `put` (slots qq_slot0 + 2k, + 1); k
+// past the last pair does nothing.  Every lane of the wave must call it.
+template <class Sink>
+__device__ __forceinline__ void qq_term(const Dev& d, int k, const Sink& put) {
   bool valid = false, en = false;
   int w = 0, s = 0, base = 6;
   double e_el = 0.0, e_vdw = 0.0;
@@ -1341,8 +1364,8 @@ __global__ void __launch_bounds__(128) k_qq(Dev d) {
         dv = r2inv * (-vel - (12.f * va - 6.f * vb) * r6hc * r6) * (float)W.lambda[s];
       }
       const int sl = d.qq_slot0 + 2 * k;
-      put_slot(d, sl, -(double)(dv * dx), -(double)(dv * dy), -(double)(dv * dz));
-      put_slot(d, sl + 1, (double)(dv * dx), (double)(dv * dy), (double)(dv * dz));
+      put(sl, 0, -(double)(dv * dx), -(double)(dv * dy), -(double)(dv * dz));
+      put(sl + 1, 1, (double)(dv * dx), (double)(dv * dy), (double)(dv * dz));
       if (en) {
         // FP64 energies; qi qj is exact in FP64, as the reference's real(8) product
         const double ddx = d.x[3 * j] - d.x[3 * i], ddy = d.x[3 * j + 1] - d.x[3 * i + 1],
@@ -1363,6 +1386,32 @@ __global__ void __launch_bounds__(128) k_qq(Dev d) {
   }
   wave_energy(d.ws, w, E_Q0 + 8 * s + base, e_el, valid && en);
   wave_energy(d.ws, w, E_Q0 + 8 * s + base + 1, e_vdw, valid && en);
+}
+
+__global__ void __launch_bounds__(128) k_qq(Dev d) {
+  qq_term(d, blockIdx.x * blockDim.x + threadIdx.x, SlotSink{&d});
+}
+
+// bonded mode 2: the q-q pair slots of each atom (contiguous), summed by
+// one thread per atom and added to the gradient
+__global__ void __launch_bounds__(256) k_qq_gather(Dev d) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nqqg) return;
+  const int atom = d.qqg_atom[k];
+  if (!d.ws[d.wid[atom]].active) return;
+  long long gx = 0, gy = 0, gz = 0;
+  for (int m = d.qqg_start[k]; m < d.qqg_start[k + 1]; ++m) {
+    gx += d.qqslot[3 * (size_t)m];
+    gy += d.qqslot[3 * (size_t)m + 1];
+    gz += d.qqslot[3 * (size_t)m + 2];
+  }
+  add_grad_fix(d.f, atom, gx, gy, gz);
+}
+
+static void launch_qq(const Dev& d, hipStream_t s) {
+  if (!d.nqq) return;
+  k_qq<<<(d.nqq + 127) / 128, 128, 0, s>>>(d);
+  if (d.bond_atomic == 2 && d.nqqg) k_qq_gather<<<(d.nqqg + 255) / 256, 256, 0, s>>>(d);
 }
 
 // ------------------------------------------------------------- LRF --
@@ -1402,8 +1451,12 @@ __device__ __forceinline__ double r2_pos(double ax, double ay, double az, double
 
 // Taylor terms of one source charge q at d = x(source) - centre: potential,
 // field, field gradient and its derivative (the sums of nbpw/nbww lists with
-// LRF, md.f90:5589-5700), with the shared factors hoisted
-__device__ __forceinline__ void lrf_add(double acc[20], double q, double dx, double dy, double dz) {
+// LRF, md.f90:5589-5700), with the shared factors hoisted.  The terms of the
+// form u - f0 (diagonal of the gradient) and u - g d (derivative) are split:
+// acc takes the u parts and aux the sums of f0 and of g d_x, g d_y, g d_z,
+// which lrf_finish subtracts once (the sums are linear, so this is the same
+// sum with fewer instructions per source).
+__device__ __forceinline__ void lrf_add(double acc[20], double aux[4], double q, double dx, double dy, double dz) {
   const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
   const double ir = rsqrt_e(r2);
   const double ir2 = ir * ir;
@@ -1414,22 +1467,25 @@ __device__ __forceinline__ void lrf_add(double acc[20], double q, double dx, dou
   acc[0] += qr;
   acc[1] -= f0 * dx; acc[2] -= f0 * dy; acc[3] -= f0 * dz;
   const double ax = f1 * dx, ay = f1 * dy;
-  acc[4] += fma(ax, dx, -f0); acc[5] += ax * dy; acc[6] += ax * dz;
-  acc[7] += fma(ay, dy, -f0); acc[8] += ay * dz; acc[9] += fma(f1 * dz, dz, -f0);
+  acc[4] += ax * dx; acc[5] += ax * dy; acc[6] += ax * dz;
+  acc[7] += ay * dy; acc[8] += ay * dz; acc[9] += f1 * dz * dz;
+  aux[0] += f0;
   // f2 (5 d_a d_b d_c - r2 (delta_ab d_c + delta_ac d_b + delta_bc d_a))
-  const double c5 = 5.0 * f2, g = f2 * r2, g3 = 3.0 * g;
+  const double c5 = 5.0 * f2, g = f2 * r2;
   const double px = c5 * dx, py = c5 * dy;
   const double xx = px * dx, yy = py * dy, zz = c5 * dz * dz;
-  acc[10] += (xx - g3) * dx;
-  acc[11] += (xx - g) * dy;
-  acc[12] += (xx - g) * dz;
-  acc[13] += (yy - g) * dx;
-  acc[14] += px * dy * dz;
-  acc[15] += (zz - g) * dx;
-  acc[16] += (yy - g3) * dy;
-  acc[17] += (yy - g) * dz;
-  acc[18] += (zz - g) * dy;
-  acc[19] += (zz - g3) * dz;
+  acc[10] += xx * dx; acc[11] += xx * dy; acc[12] += xx * dz;
+  acc[13] += yy * dx; acc[14] += px * dy * dz; acc[15] += zz * dx;
+  acc[16] += yy * dy; acc[17] += yy * dz; acc[18] += zz * dy;
+  acc[19] += zz * dz;
+  aux[1] += g * dx; aux[2] += g * dy; aux[3] += g * dz;
+}
+__device__ __forceinline__ void lrf_finish(double acc[20], const double aux[4]) {
+  acc[4] -= aux[0]; acc[7] -= aux[0]; acc[9] -= aux[0];
+  acc[10] -= 3.0 * aux[1]; acc[11] -= aux[2]; acc[12] -= aux[3];
+  acc[13] -= aux[1]; acc[15] -= aux[1];
+  acc[16] -= 3.0 * aux[2]; acc[17] -= aux[3]; acc[18] -= aux[2];
+  acc[19] -= 3.0 * aux[3];
 }
 
 __global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
@@ -1455,7 +1511,7 @@ __global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
   // cut-off against a water source group and against a solute source group
   const double rc_hw = gw ? W.rcww2 : W.rcpw2, rc_hs = gw ? W.rcpw2 : W.rcpp2, rclrf2 = W.rclrf2;
   const int h_sol = W.g_off + W.ngrp_solute;
-  double acc[20];
+  double acc[20], aux[4] = {0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < 20; ++k) acc[k] = 0.0;
   // this block's part of the window's source groups
   const int P = d.lrf_parts;
@@ -1481,12 +1537,13 @@ __global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
       const int f0 = s_first[j], n = s_n[j];
       for (int k = 0; k < n; ++k) {
         const double4 a = xq[f0 + k];
-        lrf_add(acc, a.w, a.x - cx, a.y - cy, a.z - cz);
+        lrf_add(acc, aux, a.w, a.x - cx, a.y - cy, a.z - cz);
       }
     }
     __syncthreads();
   }
   if (!g_ok) return;
+  lrf_finish(acc, aux);
   double* out = P == 1 ? d.g_lrf + 20 * (size_t)g : d.g_lrf_part + 20 * ((size_t)lw.w * d.ngrp_all + g);
   for (int k = 0; k < 20; ++k) out[k] = acc[k];
 }
@@ -1599,11 +1656,12 @@ __device__ __forceinline__ void torsion_geom(const double* x, int i, int j, int 
 // slots in a fixed order.  Integer sums make the result independent of the
 // order, so it equals the atomic form bit for bit.
 
-__device__ __forceinline__ void put4(const Dev& d, int k, double dv, const double dp[12]) {
-  put_slot(d, k + 0, dv * dp[0], dv * dp[1], dv * dp[2]);
-  put_slot(d, k + 1, dv * dp[3], dv * dp[4], dv * dp[5]);
-  put_slot(d, k + 2, dv * dp[6], dv * dp[7], dv * dp[8]);
-  put_slot(d, k + 3, dv * dp[9], dv * dp[10], dv * dp[11]);
+template <class Sink>
+__device__ __forceinline__ void put4(const Sink& put, int k, double dv, const double dp[12]) {
+  put(k + 0, 0, dv * dp[0], dv * dp[1], dv * dp[2]);
+  put(k + 1, 1, dv * dp[3], dv * dp[4], dv * dp[5]);
+  put(k + 2, 2, dv * dp[6], dv * dp[7], dv * dp[8]);
+  put(k + 3, 3, dv * dp[9], dv * dp[10], dv * dp[11]);
 }
 
 // angle (md.f90:814-901): energy and the gradient factors di, dk of the end
@@ -1632,13 +1690,18 @@ __device__ __forceinline__ void angle_geom(const double* x, int i, int j, int k,
 }
 
 
-__global__ void __launch_bounds__(256) k_bonded(Dev d) {
+// One bonded term t of the flat term index (bonds, angles, torsions,
+// impropers, then the q-bonds, q-angles, q-torsions and q-impropers), its
+// gradient handed to `put` per term atom (slot index in term order, and the
+// atom's place in the term); t past the last term does nothing.  Every lane of the wave must call it (the
+// energy goes through a wave sum).
+template <class Sink>
+__device__ __forceinline__ void bonded_term(const Dev& d, int t, const Sink& put) {
   const int n1 = d.nbond, n2 = n1 + d.nang, n3 = n2 + d.ntor, n4 = n3 + d.nimp;
   const int n5 = n4 + d.nqb, n6 = n5 + d.nqa, n7 = n6 + d.nqt, n8 = n7 + d.nqi;
   // first slot of each kind (2, 3, 4, 4, 2, 3, 4, 4 atoms per term)
   const int b1 = 0, b2 = b1 + 2 * d.nbond, b3 = b2 + 3 * d.nang, b4 = b3 + 4 * d.ntor;
   const int b5 = b4 + 4 * d.nimp, b6 = b5 + 2 * d.nqb, b7 = b6 + 3 * d.nqa, b8 = b7 + 4 * d.nqt;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const double* x = d.x;
   bool valid = false;
   int w = 0, eslot = 0;
@@ -1653,8 +1716,8 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
       const double bl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
       const double db = bl - p.y;
       const double dv = p.x * db / bl;
-      put_slot(d, b1 + 2 * t, -r[0] * dv, -r[1] * dv, -r[2] * dv);
-      put_slot(d, b1 + 2 * t + 1, r[0] * dv, r[1] * dv, r[2] * dv);
+      put(b1 + 2 * t, 0, -r[0] * dv, -r[1] * dv, -r[2] * dv);
+      put(b1 + 2 * t + 1, 1, r[0] * dv, r[1] * dv, r[2] * dv);
       valid = true;
       e = 0.5 * p.x * db * db;
       eslot = d.b_solv[t] ? E_W_BOND : E_P_BOND;
@@ -1679,9 +1742,9 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
         const double du = 2 * p.z * ru / dik;
         for (int c = 0; c < 3; ++c) { gk[c] += du * r[c]; gi[c] -= du * r[c]; }
       }
-      put_slot(d, b2 + 3 * a, gi[0], gi[1], gi[2]);
-      put_slot(d, b2 + 3 * a + 1, gj[0], gj[1], gj[2]);
-      put_slot(d, b2 + 3 * a + 2, gk[0], gk[1], gk[2]);
+      put(b2 + 3 * a, 0, gi[0], gi[1], gi[2]);
+      put(b2 + 3 * a + 1, 1, gj[0], gj[1], gj[2]);
+      put(b2 + 3 * a + 2, 2, gk[0], gk[1], gk[2]);
       valid = true;
       eslot = d.a_solv[a] ? E_W_ANG : E_P_ANG;
     }
@@ -1696,7 +1759,7 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
       const double arg = (double)p.y * phi - (double)p.z;
       e = (double)p.x * (1.0 + cos(arg)) * (double)p.w;
       const double dv = -(double)p.y * (double)p.x * sin(arg) * (double)p.w;
-      put4(d, b3 + 4 * a, dv, dp);
+      put4(put, b3 + 4 * a, dv, dp);
       valid = true;
       eslot = d.t_solv[a] ? E_W_TOR : E_P_TOR;
     }
@@ -1720,7 +1783,7 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
         dv = (double)p.x * arg;
         e = 0.5 * dv * arg;
       }
-      put4(d, b4 + 4 * a, dv, dp);
+      put4(put, b4 + 4 * a, dv, dp);
       valid = true;
       eslot = d.i_solv[a] ? E_W_IMP : E_P_IMP;
     }
@@ -1740,8 +1803,8 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
       const double fexp = exp(-p.y * db);
       e = p.x * (fexp * fexp - 2. * fexp) + 0.5 * p.w * db * db;
       const double dv = (2. * p.x * p.y * (fexp - fexp * fexp) + p.w * db) * lam / bl;
-      put_slot(d, b5 + 2 * a, -dv * r[0], -dv * r[1], -dv * r[2]);
-      put_slot(d, b5 + 2 * a + 1, dv * r[0], dv * r[1], dv * r[2]);
+      put(b5 + 2 * a, 0, -dv * r[0], -dv * r[1], -dv * r[2]);
+      put(b5 + 2 * a + 1, 1, dv * r[0], dv * r[1], dv * r[2]);
       valid = true;
       eslot = E_Q0 + 8 * s + 0;
     }
@@ -1773,9 +1836,9 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
         di[c] = f1 * (rjk[c] / (bji * bjk) - scp * rji[c] / (bji * bji));
         dk[c] = f1 * (rji[c] / (bji * bjk) - scp * rjk[c] / (bjk * bjk));
       }
-      put_slot(d, b6 + 3 * a, dv * di[0], dv * di[1], dv * di[2]);
-      put_slot(d, b6 + 3 * a + 1, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
-      put_slot(d, b6 + 3 * a + 2, dv * dk[0], dv * dk[1], dv * dk[2]);
+      put(b6 + 3 * a, 0, dv * di[0], dv * di[1], dv * di[2]);
+      put(b6 + 3 * a + 1, 1, -dv * (di[0] + dk[0]), -dv * (di[1] + dk[1]), -dv * (di[2] + dk[2]));
+      put(b6 + 3 * a + 2, 2, dv * dk[0], dv * dk[1], dv * dk[2]);
       valid = true;
       eslot = E_Q0 + 8 * s + 1;
     }
@@ -1805,13 +1868,71 @@ __global__ void __launch_bounds__(256) k_bonded(Dev d) {
         e = 0.5 * dv * arg;
         dv = dv * lam;
       }
-      put4(d, tors ? b7 + 4 * a : b8 + 4 * a, dv, dp);
+      put4(put, tors ? b7 + 4 * a : b8 + 4 * a, dv, dp);
       valid = true;
       eslot = E_Q0 + 8 * s + (tors ? 2 : 3);
     }
   }
   const bool en = valid && d.ws[w].energy_now;
   wave_energy(d.ws, w, eslot, e, en);
+}
+
+__global__ void __launch_bounds__(256) k_bonded(Dev d) {
+  bonded_term(d, blockIdx.x * blockDim.x + threadIdx.x, SlotSink{&d});
+}
+
+// Bonded terms in tiles: a block takes the terms of one tile (consecutive
+// by their first atom, at most kBondTileTerms of them, grouped by kind so
+// that waves run one kind) and sums their gradients in LDS, per atom of the
+// tile's span, with integer atomics; then one atomic add per atom and
+// component goes to the gradient.  An atom outside the span (a term that
+// reaches further than kBondSpan atoms) goes to the gradient directly.  The
+// integer sums are the same as those of the slots and their gather.
+// The term's gradients are recorded per term atom (pos 0..3, a constant at
+// each call site, so the record stays in registers) and added to the tile's
+// LDS sums after the term, in one loop for every kind.
+struct TermRec {
+  int k[4];
+  long long g[4][3];
+};
+struct TileSink {
+  TermRec* rec;
+  __device__ void operator()(int k, int pos, double gx, double gy, double gz) const {
+    rec->k[pos] = k;
+    rec->g[pos][0] = fix(gx, kFScale);
+    rec->g[pos][1] = fix(gy, kFScale);
+    rec->g[pos][2] = fix(gz, kFScale);
+  }
+};
+
+__global__ void __launch_bounds__(256) k_bonded_tiles(Dev d) {
+  __shared__ unsigned long long s_acc[3 * kBondSpan];
+  const int4 tl = d.btile[blockIdx.x];   // first entry, terms, first atom, atoms spanned
+  for (int m = threadIdx.x; m < 3 * tl.w; m += blockDim.x) s_acc[m] = 0ull;
+  __syncthreads();
+  for (int k0 = 0; k0 < tl.y; k0 += blockDim.x) {
+    const int k = k0 + (int)threadIdx.x;
+    TermRec rec;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) rec.k[p] = -1;
+    bonded_term(d, k < tl.y ? d.btile_terms[tl.x + k] : 0x7fffffff, TileSink{&rec});
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      if (rec.k[p] < 0) continue;
+      const int a = d.slot_atom[rec.k[p]];
+      const int l = a - tl.z;
+      if (l >= 0 && l < kBondSpan) {
+        for (int c = 0; c < 3; ++c) atomicAdd(&s_acc[3 * l + c], (unsigned long long)rec.g[p][c]);
+      } else {
+        for (int c = 0; c < 3; ++c) atomicAdd((unsigned long long*)&d.f[3 * a + c], (unsigned long long)rec.g[p][c]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < tl.w; l += blockDim.x) {
+    const long long gx = (long long)s_acc[3 * l], gy = (long long)s_acc[3 * l + 1], gz = (long long)s_acc[3 * l + 2];
+    if (gx | gy | gz) add_grad_fix(d.f, tl.z + l, gx, gy, gz);
+  }
 }
 
 // sum of each bonded atom's slots (CSR in slot order) into the gradient.  A
@@ -2087,7 +2208,8 @@ __global__ void __launch_bounds__(256) k_watpol_theta(Dev d) {
 // Pass 2, one block per (window, shell): the shell's waters are gathered in
 // LDS with their theta and sorted (theta ascending, ties by water index: the
 // order of the reference's first-minimum selection sort); a water's place in
-// that order gives its target angle theta0.
+// that order, stored per water, gives its target angle theta0 in pass 3.  The
+// block also advances the shell's theta averages.
 constexpr int kWpolMaxShell = 8;
 constexpr int kWpolLds = 4096;       // waters of one shell held (and sorted) in LDS
 constexpr int kWpolRank = 1024;      // shells up to this size are ordered by counting
@@ -2099,7 +2221,6 @@ __global__ void __launch_bounds__(kWpolBlock) k_watpol_shell(Dev d) {
   const WinDev& W = d.win[w];
   if (!st.active || !W.wpol || W.nwat == 0 || is >= W.nshell) return;
   ShellDev& sh = d.shells[W.sh_off + is];
-  const double pi = (double)(4.0f * atanf(1.0f));
   const int wo = d.w_off[w];
   __shared__ double s_th[kWpolLds];
   __shared__ int s_iw[kWpolLds];
@@ -2175,68 +2296,36 @@ __global__ void __launch_bounds__(kWpolBlock) k_watpol_shell(Dev d) {
       }
     }
   }
-  const double cstb = (double)sh.cstb, tc = (double)sh.theta_corr;
-  double eacc = 0.0, tsum = 0.0;
-  // members: sorted in LDS (place = index), or (shell larger than LDS)
-  // straight from the window's waters with their place counted
+  // each member's place in the order, for the per-water pass, and the
+  // shell's theta sum in a fixed order: members sorted in LDS (place =
+  // index), or (shell larger than LDS) straight from the window's waters with
+  // their place counted
+  double tsum = 0.0;
   const int nloop = in_lds ? n : W.nwat;
   for (int k = threadIdx.x; k < nloop; k += blockDim.x) {
-    int iw;
+    int iw, rank = 0;
     double th;
     if (in_lds) {
       iw = s_iw[k];
       th = s_th[k];
+      rank = k;
     } else {
       iw = k;
       if (d.wshell[wo + iw] != is) continue;
       th = d.wtheta[wo + iw];
-    }
-    int rank = 0;
-    if (in_lds) {
-      rank = k;
-    } else {
       for (int jw = 0; jw < W.nwat; ++jw) {
         if (d.wshell[wo + jw] != is) continue;
         const double tj = d.wtheta[wo + jw];
         rank += (tj < th || (tj == th && jw < iw)) ? 1 : 0;
       }
     }
-    const int il = rank + 1;
-    const float argf = 1.0f + (1.0f - 2.0f * (float)il) / (float)n;
-    double th0 = acos((double)argf);
-    th0 = th0 - 3. * sin(th0) * cstb / 2.;
-    th0 = fmin(fmax(th0, 0.0), pi);
+    d.wrank[wo + iw] = rank;
     tsum += th;
-    const double dth = th - th0 + tc;
-    eacc += 0.5 * W.fkwpol * dth * dth;
-    const double dv = W.fkwpol * dth;
-    const int i = W.ao + W.nat_solute + 3 * iw;
-    const double* x = d.x;
-    double rmu[3], rcu[3];
-    for (int c = 0; c < 3; ++c) rmu[c] = x[3 * (i + 1) + c] + x[3 * (i + 2) + c] - 2. * x[3 * i + c];
-    const double rm = sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
-    for (int c = 0; c < 3; ++c) rmu[c] /= rm;
-    for (int c = 0; c < 3; ++c) rcu[c] = x[3 * i + c] - W.xwcent[c];
-    const double rc = sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
-    for (int c = 0; c < 3; ++c) rcu[c] /= rc;
-    double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
-    scp = fmin(1.0, fmax(-1.0, scp));
-    double f0 = sin(acos(scp));
-    if (fabs(f0) < 1.e-12) f0 = 1.e-12;
-    f0 = dv * (-1.0 / f0);
-    double f1[3], f2[3];
-    for (int c = 0; c < 3; ++c) {
-      f1[c] = (rcu[c] - rmu[c] * scp) / rm;
-      f2[c] = (rmu[c] - rcu[c] * scp) / rc;
-    }
-    add_grad(d.f, i, f0 * (-2. * f1[0] + f2[0]), f0 * (-2. * f1[1] + f2[1]), f0 * (-2. * f1[2] + f2[2]));
-    add_grad(d.f, i + 1, f0 * f1[0], f0 * f1[1], f0 * f1[2]);
-    add_grad(d.f, i + 2, f0 * f1[0], f0 * f1[1], f0 * f1[2]);
   }
-  // block sums in a fixed order (waves, then the 4 wave results)
+  // block sum in a fixed order (waves, then the 4 wave results)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const double t_w = wave_sum(tsum), e_w = wave_sum(eacc);
-  if (lane == 0) { s_red[wv][0] = t_w; s_red[wv][1] = e_w; }
+  const double t_w = wave_sum(tsum);
+  if (lane == 0) s_red[wv][0] = t_w;
   __syncthreads();
   if (threadIdx.x == 0) {
     if (n > 0 && st.wpol_acc) {
@@ -2246,18 +2335,75 @@ __global__ void __launch_bounds__(kWpolBlock) k_watpol_shell(Dev d) {
       sh.avn_insh = sh.avn_insh + (float)n;
     }
     sh.n_insh = n;
-    if (st.energy_now) add_energy(d.ws, w, E_WPOL, (s_red[0][1] + s_red[1][1]) + (s_red[2][1] + s_red[3][1]));
   }
 }
 
+// Pass 3, one thread per water in a shell: the target angle theta0 from the
+// water's place in its shell's order, then the restraint's energy and the
+// forces on its three atoms (the loop over il of watpol)
+__global__ void __launch_bounds__(256) k_watpol_force(Dev d) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  int w = 0;
+  bool on = false;
+  double e = 0.0;
+  if (g < d.nwat_total) {
+    w = d.wat_win[g];
+    const WinState& st = d.ws[w];
+    const WinDev& W = d.win[w];
+    const int is = d.wshell[g];   // -1 unless in a shell of an active window with polarisation
+    if (is >= 0) {
+      const ShellDev& sh = d.shells[W.sh_off + is];
+      const double pi = (double)(4.0f * atanf(1.0f));
+      const double cstb = (double)sh.cstb, tc = (double)sh.theta_corr;
+      const int n = sh.n_insh;
+      const int il = d.wrank[g] + 1;
+      const double th = d.wtheta[g];
+      const float argf = 1.0f + (1.0f - 2.0f * (float)il) / (float)n;
+      double th0 = acos((double)argf);
+      th0 = th0 - 3. * sin(th0) * cstb / 2.;
+      th0 = fmin(fmax(th0, 0.0), pi);
+      const double dth = th - th0 + tc;
+      e = 0.5 * W.fkwpol * dth * dth;
+      on = st.energy_now;
+      const double dv = W.fkwpol * dth;
+      const int i = W.ao + W.nat_solute + 3 * (g - d.w_off[w]);
+      const double* x = d.x;
+      double rmu[3], rcu[3];
+      for (int c = 0; c < 3; ++c) rmu[c] = x[3 * (i + 1) + c] + x[3 * (i + 2) + c] - 2. * x[3 * i + c];
+      const double rm = sqrt(rmu[0] * rmu[0] + rmu[1] * rmu[1] + rmu[2] * rmu[2]);
+      for (int c = 0; c < 3; ++c) rmu[c] /= rm;
+      for (int c = 0; c < 3; ++c) rcu[c] = x[3 * i + c] - W.xwcent[c];
+      const double rc = sqrt(rcu[0] * rcu[0] + rcu[1] * rcu[1] + rcu[2] * rcu[2]);
+      for (int c = 0; c < 3; ++c) rcu[c] /= rc;
+      double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+      scp = fmin(1.0, fmax(-1.0, scp));
+      double f0 = sin(acos(scp));
+      if (fabs(f0) < 1.e-12) f0 = 1.e-12;
+      f0 = dv * (-1.0 / f0);
+      double f1[3], f2[3];
+      for (int c = 0; c < 3; ++c) {
+        f1[c] = (rcu[c] - rmu[c] * scp) / rm;
+        f2[c] = (rmu[c] - rcu[c] * scp) / rc;
+      }
+      add_grad(d.f, i, f0 * (-2. * f1[0] + f2[0]), f0 * (-2. * f1[1] + f2[1]), f0 * (-2. * f1[2] + f2[2]));
+      add_grad(d.f, i + 1, f0 * f1[0], f0 * f1[1], f0 * f1[2]);
+      add_grad(d.f, i + 2, f0 * f1[0], f0 * f1[1], f0 * f1[2]);
+    }
+  }
+  wave_energy(d.ws, w, E_WPOL, e, on);
+}
+
 // the FP32 coordinates of atom i from its FP64 ones, into x32 and, for a
-// water atom, into its cluster slot
+// water atom, into its cluster slot (with the FP64 ones)
 __device__ __forceinline__ void store_x32(const Dev& d, int i) {
   const float4 q = d.x32[i];
   const float4 n = make_float4((float)d.x[3 * i], (float)d.x[3 * i + 1], (float)d.x[3 * i + 2], q.w);
   d.x32[i] = n;
   const int ws = d.wslot[i];
-  if (ws >= 0) d.xw[ws] = n;
+  if (ws >= 0) {
+    d.xw[ws] = n;
+    d.xwd[ws] = make_double4(d.x[3 * i], d.x[3 * i + 1], d.x[3 * i + 2], 0.0);
+  }
 }
 
 // ------------------------------------------------------- integrator --
@@ -2548,7 +2694,11 @@ void launch_lrf_build(const Dev& d, hipStream_t s) {
 
 void launch_bonded(const Dev& d, hipStream_t s) {
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
-  if (nb) k_bonded<<<blocks(nb, d.bs_bonded), d.bs_bonded, 0, s>>>(d);
+  if (d.bond_atomic == 2) {
+    if (d.nbtile) k_bonded_tiles<<<d.nbtile, 256, 0, s>>>(d);
+  } else if (nb) {
+    k_bonded<<<blocks(nb, d.bs_bonded), d.bs_bonded, 0, s>>>(d);
+  }
 }
 
 // list update: group centres and cluster spheres, then the neighbour rows
@@ -2557,11 +2707,19 @@ static void launch_list_prep(const Dev& d, hipStream_t s, int ngrp, int nwc, int
   k_centres<<<blocks(ngrp, 128), 128, 0, s>>>(d, ngrp);
   k_bounds<<<blocks(nwc + nsc, 128), 128, 0, s>>>(d, nwc, nsc);
 }
-static void launch_list_rows(const Dev& d, hipStream_t s) {
+// the rows whose i-clusters are waters (water-water, water<-solute), and
+// those whose i-clusters are solute (solute<-water, solute-solute)
+static void launch_rows_water(const Dev& d, hipStream_t s) {
   if (d.rl[T_WW].nicl) k_build_rows<T_WW><<<blocks((long long)d.rl[T_WW].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.rl[T_WP].nicl) k_build_rows<T_WP><<<blocks((long long)d.rl[T_WP].nicl * 64, 256), 256, 0, s>>>(d);
+}
+static void launch_rows_solute(const Dev& d, hipStream_t s) {
   if (d.rl[T_PW].nicl) k_build_rows<T_PW><<<blocks((long long)d.rl[T_PW].nicl * 64, 256), 256, 0, s>>>(d);
   if (d.rl[T_PP].nicl) k_build_rows<T_PP><<<blocks((long long)d.rl[T_PP].nicl * 64, 256), 256, 0, s>>>(d);
+}
+static void launch_list_rows(const Dev& d, hipStream_t s) {
+  launch_rows_water(d, s);
+  launch_rows_solute(d, s);
 }
 void launch_lists(const Dev& d, hipStream_t s, int ngrp, int nwc, int nsc) {
   launch_list_prep(d, s, ngrp, nwc, nsc);
@@ -2605,18 +2763,27 @@ void launch_ww(const Dev& d, hipStream_t s, bool energy) {
   else launch_ww_k<false>(d, s);
 }
 
+// water_side: the rows of water i-clusters (W<-P), else those of solute
+// i-clusters (P<-W)
 template <bool EK>
-static void launch_pw_k(const Dev& d, hipStream_t s) {
+static void launch_pw_k(const Dev& d, hipStream_t s, bool water_side) {
   const int nwp = blocks((long long)d.rl[T_WP].nicl * 64, d.bs_rows);
   const int npw = blocks((long long)d.rl[T_PW].nicl * 64, d.bs_rows);
-  if (d.rl[T_WP].nicl) k_pw<true, false, EK><<<nwp, d.bs_rows, 0, s>>>(d);
-  if (d.rl[T_PW].nicl) k_pw<false, false, EK><<<npw, d.bs_rows, 0, s>>>(d);
-  if (d.rl[T_WP].nicl && d.any_wat3) k_pw<true, true, EK><<<nwp, d.bs_rows, 0, s>>>(d);
-  if (d.rl[T_PW].nicl && d.any_wat3) k_pw<false, true, EK><<<npw, d.bs_rows, 0, s>>>(d);
+  if (water_side) {
+    if (d.rl[T_WP].nicl) k_pw<true, false, EK><<<nwp, d.bs_rows, 0, s>>>(d);
+    if (d.rl[T_WP].nicl && d.any_wat3) k_pw<true, true, EK><<<nwp, d.bs_rows, 0, s>>>(d);
+  } else {
+    if (d.rl[T_PW].nicl) k_pw<false, false, EK><<<npw, d.bs_rows, 0, s>>>(d);
+    if (d.rl[T_PW].nicl && d.any_wat3) k_pw<false, true, EK><<<npw, d.bs_rows, 0, s>>>(d);
+  }
+}
+static void launch_pw_side(const Dev& d, hipStream_t s, bool energy, bool water_side) {
+  if (energy) launch_pw_k<true>(d, s, water_side);
+  else launch_pw_k<false>(d, s, water_side);
 }
 static void launch_pw(const Dev& d, hipStream_t s, bool energy) {
-  if (energy) launch_pw_k<true>(d, s);
-  else launch_pw_k<false>(d, s);
+  launch_pw_side(d, s, energy, true);
+  launch_pw_side(d, s, energy, false);
 }
 static void launch_pp(const Dev& d, hipStream_t s, bool energy) {
   if (!d.rl[T_PP].nicl) return;
@@ -2630,50 +2797,72 @@ void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) 
   launch_pw(d, s, q_energy);
   launch_pp(d, s, q_energy);
   launch_q_tiles(d, s, q_energy);
-  if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s>>>(d);
+  launch_qq(d, s);
 }
 
 // The lists and forces of an MD step on three streams (after k_begin on s):
-//   s3: the Q tiles' forces (static lists: they need only the step's
-//       coordinates), then, once the neighbour rows are there, the
-//       solute-solute rows, and on energy steps the q-water energy pass;
-//   s2, after the group centres: on list steps the LRF expansions, then the
-//       q-q pairs, water polarisation, the bonded terms with their slot
-//       gather (which also collects the q-q slots) and sequence restraints;
-//   s:  on list steps the neighbour rows, then the water-water and
-//       solute-water rows and on energy steps the q-solute energy pass; it
-//       then waits for s2 and s3.
+//   s3: on list steps the solute rows (solute<-water, solute-solute) once
+//       the group centres are there; the Q tiles' forces (static lists:
+//       they need only the step's coordinates); the solute-solute rows, and
+//       on energy steps the q-water energy pass;
+//   s2, after the group centres: the q-q pairs, water polarisation, the
+//       bonded terms with their slot gather (which also collects the q-q
+//       slots; in bonded mode 2 the q-q slots have a gather of their own)
+//       and sequence restraints, then on list steps, once every pair kernel
+//       of the step is done, the LRF expansions;
+//   s:  on list steps the group centres and the water rows, then the
+//       water-water and solute-water rows and on energy steps the q-solute
+//       energy pass; it then waits for s2 and s3.
+// The LRF build comes last because it fills every CU for about two
+// milliseconds of FP64 work: beside the row builds and pair kernels it
+// starves them (they wait for its long-running blocks), and cutting its
+// occupancy so that they fit slows it more than it frees them.  Only the
+// leapfrog kernel reads the expansions.
 // All of them write the gradient and the energies only through integer
 // atomic adds, so the result does not depend on how they overlap.  The
 // per-atom terms (LRF evaluation, atom restraints) follow in the leapfrog
 // kernel of launch_integrate(..., true).
-hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[5],
+hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[6],
                               bool lists, int ngrp, int nwc, int nsc, bool q_energy) {
   hipError_t err = hipEventRecord(ev[0], s);
   if (err == hipSuccess) err = hipStreamWaitEvent(s3, ev[0], 0);
   if (err != hipSuccess) return err;
-  launch_q_forces(d, s3);
   if (lists) launch_list_prep(d, s, ngrp, nwc, nsc);
   err = hipEventRecord(ev[2], s);
   if (err == hipSuccess) err = hipStreamWaitEvent(s2, ev[2], 0);
+  if (err == hipSuccess && lists) err = hipStreamWaitEvent(s3, ev[2], 0);
   if (err != hipSuccess) return err;
-  if (lists) launch_lrf_build(d, s2);
-  if (d.nqq) k_qq<<<blocks(d.nqq, 128), 128, 0, s2>>>(d);
+  if (lists) {
+    launch_rows_solute(d, s3);
+    err = hipEventRecord(ev[5], s3);
+    if (err != hipSuccess) return err;
+  }
+  launch_q_forces(d, s3);
+  launch_qq(d, s2);
   launch_other_forces(d, s2, false);
-  err = hipEventRecord(ev[3], s2);
-  if (err != hipSuccess) return err;
-  if (lists) launch_list_rows(d, s);
-  err = hipEventRecord(ev[4], s);
-  if (err == hipSuccess) err = hipStreamWaitEvent(s3, ev[4], 0);
-  if (err != hipSuccess) return err;
+  if (lists) launch_rows_water(d, s);
   launch_pp(d, s3, q_energy);
   if (q_energy) launch_q_energy_qw(d, s3);
   err = hipEventRecord(ev[1], s3);
   if (err != hipSuccess) return err;
   launch_ww(d, s, q_energy);
-  launch_pw(d, s, q_energy);
+  launch_pw_side(d, s, q_energy, true);
+  if (lists) {
+    err = hipStreamWaitEvent(s, ev[5], 0);
+    if (err != hipSuccess) return err;
+  }
+  launch_pw_side(d, s, q_energy, false);
   if (q_energy) launch_q_energy_qp(d, s);
-  err = hipStreamWaitEvent(s, ev[1], 0);
+  if (lists) {
+    // the LRF build once every pair kernel of the step is done
+    err = hipEventRecord(ev[4], s);
+    if (err == hipSuccess) err = hipStreamWaitEvent(s2, ev[4], 0);
+    if (err == hipSuccess) err = hipStreamWaitEvent(s2, ev[1], 0);
+    if (err != hipSuccess) return err;
+    launch_lrf_build(d, s2);
+  }
+  err = hipEventRecord(ev[3], s2);
+  if (err == hipSuccess) err = hipStreamWaitEvent(s, ev[1], 0);
   if (err == hipSuccess) err = hipStreamWaitEvent(s, ev[3], 0);
   return err;
 }
@@ -2684,10 +2873,11 @@ void launch_other_forces(const Dev& d, hipStream_t s, bool atom_terms) {
   if (d.nwat_total) {
     k_watpol_theta<<<blocks(d.nwat_total, 256), 256, 0, s>>>(d);
     k_watpol_shell<<<d.nwin * kWpolMaxShell, kWpolBlock, 0, s>>>(d);
+    k_watpol_force<<<blocks(d.nwat_total, 256), 256, 0, s>>>(d);
   }
   if (atom_terms) k_atom_terms<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
-  if (nb) k_bonded<<<blocks(nb, d.bs_bonded), d.bs_bonded, 0, s>>>(d);
+  launch_bonded(d, s);
   // the gather also collects the q-q pair slots written by k_qq
   if ((nb || d.nqq) && !d.bond_atomic && d.ngw) k_bonded_gather<<<d.ngw, 256, 0, s>>>(d);
   if (d.nrst) k_prestr<<<d.nrst, 64, 0, s>>>(d);

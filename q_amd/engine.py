@@ -72,6 +72,7 @@ def load_library(path: str = LIB_PATH):
         "qmi_write_restart": (i32, [p, i32, C.c_char_p]),
         "qmi_timings": (i32, [p, dp, i32]),
         "qmi_set_timing": (i32, [p, i32]),
+        "qmi_time_kernel": (i32, [p, i32, i32, dp]),
         "qmi_set_autotune": (i32, [p, i32]),
         "qmi_tuning": (i32, [p, dp, i32]),
         "qmi_last_error": (C.c_char_p, []),
@@ -90,7 +91,7 @@ EXPORTED = [
     "qmi_evaluate",
     "qmi_get_energies", "qmi_get_q_energies", "qmi_get_coords", "qmi_get_velocities", "qmi_get_forces",
     "qmi_set_state", "qmi_get_pairlist", "qmi_window_info", "qmi_write_restart", "qmi_timings",
-    "qmi_set_timing", "qmi_set_autotune", "qmi_tuning", "qmi_last_error", "qmi_engine_destroy",
+    "qmi_set_timing", "qmi_time_kernel", "qmi_set_autotune", "qmi_tuning", "qmi_last_error", "qmi_engine_destroy",
     "qmi_mlp_create", "qmi_mlp_num_params", "qmi_mlp_get_params", "qmi_mlp_set_params", "qmi_mlp_forward",
     "qmi_mlp_grad", "qmi_mlp_train", "qmi_mlp_last_error", "qmi_mlp_destroy", "qmi_mlp_forward_noalloc",
 ]
@@ -205,11 +206,17 @@ class Engine:
         a = np.zeros(13)
         _check(self.lib.qmi_tuning(self._h, a.ctypes.data_as(C.POINTER(C.c_double)), 13))
         return dict(q_tiles_per_wave=int(a[0]), rows_block=int(a[1]), lrf_block=int(a[2]), bonded_block=int(a[3]),
-                    xcd_grouped=int(a[4]), bonded_atomic=int(a[5]), lrf_source_parts=int(a[6]),
+                    xcd_grouped=int(a[4]), bonded_mode=int(a[5]), lrf_source_parts=int(a[6]),
                     us=dict(q_tiles=a[7], rows=a[8], lrf_build=a[9], bonded=a[10], xcd=a[11], bonded_mode=a[12]))
 
     def set_timing(self, on: bool):
         _check(self.lib.qmi_set_timing(self._h, 1 if on else 0))
+
+    def time_kernel(self, which: int = 0, reps: int = 50) -> float:
+        """device microseconds per launch of a step kernel (0: water-water)"""
+        us = C.c_double()
+        _check(self.lib.qmi_time_kernel(self._h, which, reps, C.byref(us)))
+        return us.value
 
     def timings(self) -> list:
         a = np.zeros(8)

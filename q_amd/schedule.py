@@ -26,6 +26,7 @@ import argparse
 import json
 import math
 import os
+import shutil
 import sys
 import time
 from dataclasses import dataclass
@@ -399,11 +400,15 @@ def evaluate_replicas(args):
 
     net = load_predictor(args.weights)
     _, held = perturbation_sets(args.seed, args.n_train, args.n_eval, args.n_scaffolds)
-    system_of = systems_for(held, args.n_scaffolds, args.procs)
-    log(f"built {len(held)} held-out systems")
+    legs = set(getattr(args, "legs", "ligwat,protein").split(","))
+    picks = None if getattr(args, "cases", None) is None else {int(c) for c in args.cases.split(",")}
+    todo = [(j, p) for j, p in enumerate(held) if p.leg in legs and (picks is None or j in picks)]
+    system_of = systems_for([p for _, p in todo], args.n_scaffolds, args.procs)
+    log(f"built {len(todo)} held-out systems")
     os.makedirs(args.out, exist_ok=True)
+    energy, batch = getattr(args, "energy", 5), getattr(args, "batch", 64)
     rows = []
-    for j, p in enumerate(held):
+    for j, p in todo:
         system = system_of(p)
         base = os.path.join(args.out, "work", f"case_{j:03d}")
         inps = write_ladder(p, system, PILOT_LAMBDAS, os.path.join(base, "pilot"), args.pilot_steps, tag="p")
@@ -425,10 +430,10 @@ def evaluate_replicas(args):
         for r in range(args.replicas):
             for name, lamv in ladders.items():
                 w = write_ladder(p, system, lamv, os.path.join(base, f"{name}_r{r}"), args.prod_steps, md_seed=1 + r,
-                                 tag="e")
+                                 tag="e", energy=energy)
                 inps += w
                 owner += [(name, r)] * len(w)
-        ens = run_batched(inps, args.prod_steps, log=log)
+        ens = run_batched(inps, args.prod_steps, batch=batch, log=log)
         if any(en is None for en in ens):
             log(f"{p}: a ladder window was stopped, left out")
             continue
@@ -448,10 +453,24 @@ def evaluate_replicas(args):
                    sigma={k: float(np.mean([s["sigma"] for s in v])) for k, v in stats.items()},
                    err={k: abs(mean[k] - mean["ref51"]) for k in ("learned", "uniform", "direct")},
                    simulated_fraction=simulated_fraction(n, args.prod_steps, args.pilot_steps))
+        row["case"] = j
         rows.append(row)
+        json.dump(dict(prod_steps=args.prod_steps, prod_skip=args.prod_skip, pilot_steps=args.pilot_steps,
+                       pilot_skip=args.pilot_skip, replicas=args.replicas, energy=energy, row=row),
+                  open(os.path.join(args.out, f"case_{j:03d}.json"), "w"), indent=1)
+        if not getattr(args, "keep", True):
+            shutil.rmtree(base, ignore_errors=True)
         log(json.dumps({k: v for k, v in row.items() if k not in ("schedule", "dG")}))
-    rep = dict(prod_steps=args.prod_steps, prod_skip=args.prod_skip, pilot_steps=args.pilot_steps,
-               pilot_skip=args.pilot_skip, replicas=args.replicas, cases=rows)
+    rep = report(rows, dict(prod_steps=args.prod_steps, prod_skip=args.prod_skip, pilot_steps=args.pilot_steps,
+                            pilot_skip=args.pilot_skip, replicas=args.replicas))
+    json.dump(rep, open(os.path.join(args.out, "schedule_eval.json"), "w"), indent=1)
+    log({k: v for k, v in rep.items() if k != "cases"})
+
+
+def report(rows, head: dict) -> dict:
+    """The evaluation summary over the case rows: mean and largest error of
+    each ladder against the 51-window reference, overlaps, spreads, fallbacks."""
+    rep = dict(head, cases=rows)
     for k in ("learned", "uniform", "direct"):
         rep["mae_" + k] = float(np.mean([r["err"][k] for r in rows]))
         rep["max_err_" + k] = float(np.max([r["err"][k] for r in rows]))
@@ -460,8 +479,23 @@ def evaluate_replicas(args):
     rep["mean_ref51_replica_spread"] = float(np.mean([r["replica_spread"]["ref51"] for r in rows]))
     rep["fallbacks"] = int(sum(1 for r in rows if r["fallback"]))
     rep["mean_simulated_fraction"] = float(np.mean([r["simulated_fraction"] for r in rows]))
-    json.dump(rep, open(os.path.join(args.out, "schedule_eval.json"), "w"), indent=1)
-    log({k: v for k, v in rep.items() if k != "cases"})
+    return rep
+
+
+def merge_reports(args):
+    """one report from the per-case files of several evaluation runs"""
+    rows, head = [], None
+    for d in args.dirs:
+        for f in sorted(os.listdir(d)):
+            if f.startswith("case_") and f.endswith(".json"):
+                c = json.load(open(os.path.join(d, f)))
+                rows.append(c["row"])
+                head = head or {k: c[k] for k in ("prod_steps", "prod_skip", "pilot_steps", "pilot_skip", "replicas",
+                                                  "energy")}
+    rows.sort(key=lambda r: (r.get("case", 0), r["leg"]))
+    rep = report(rows, head)
+    json.dump(rep, open(args.out, "w"), indent=1)
+    print({k: v for k, v in rep.items() if k != "cases"})
 
 
 def build(args):
@@ -572,11 +606,21 @@ def main(argv=None):
     v.add_argument("--prod-skip", type=int, default=1000)
     v.add_argument("--procs", type=int, default=16)
     v.add_argument("--seed", type=int, default=0)
+    v.add_argument("--legs", default="ligwat,protein", help="comma-separated legs to evaluate")
+    v.add_argument("--cases", default=None, help="comma-separated held-out case indices (default: all)")
+    v.add_argument("--energy", type=int, default=5, help="energy interval of the production windows")
+    v.add_argument("--batch", type=int, default=64, help="windows per engine")
+    v.add_argument("--keep", action="store_true", help="keep the windows' files (default: removed per case)")
+    m = sub.add_parser("merge", help="one report from the per-case files of evaluation runs")
+    m.add_argument("--out", required=True)
+    m.add_argument("dirs", nargs="+")
     args = ap.parse_args(argv)
     if args.cmd == "build":
         build(args)
     elif args.cmd == "evaluate":
         evaluate_replicas(args)
+    elif args.cmd == "merge":
+        merge_reports(args)
 
 
 if __name__ == "__main__":
