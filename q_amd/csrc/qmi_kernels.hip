@@ -35,18 +35,9 @@ __device__ __forceinline__ int block_id(int xcd) {
 
 __device__ __forceinline__ long long fix(double v, double scale) { return __double2ll_rn(v * scale); }
 
-// 1/sqrt(a) and 1/a in FP64: the hardware estimate and two Newton steps
-// (each squares the relative error, so the result is within a few ulp)
-__device__ __forceinline__ double rsqrt_d(double r2) {
-  const double h = 0.5 * r2;
-  double y = __builtin_amdgcn_rsq(r2);
-  y = fma(y, fma(-(h * y), y, 0.5), y);
-  y = fma(y, fma(-(h * y), y, 0.5), y);
-  return y;
-}
-// 1/sqrt(a) for the energy terms of the bulk pair kernels: the FP32 hardware
-// estimate (relative error below 2^-22) and one FP64 Newton step, which
-// leaves a relative error below 1e-12
+// 1/sqrt(a) for the FP64 energy terms: the FP32 hardware estimate (relative
+// error below 2^-22) and one FP64 Newton step, which squares it to below
+// 1e-12
 __device__ __forceinline__ double rsqrt_e(double r2) {
   const double h = 0.5 * r2;
   const double y = (double)__builtin_amdgcn_rsqf((float)r2);
@@ -57,11 +48,12 @@ __device__ __forceinline__ double rsqrt_e(double r2) {
 // arguments (a compare and two selects); the squared distance of a listed
 // pair is never that small, and on normal arguments both return the same bits
 __device__ __forceinline__ float rsq_r2(float r2) { return __builtin_amdgcn_rsqf(r2); }
-__device__ __forceinline__ double rcp_d(double a) {
-  double y = __builtin_amdgcn_rcp(a);
-  y = fma(y, fma(-a, y, 1.0), y);
-  y = fma(y, fma(-a, y, 1.0), y);
-  return y;
+// 1/a for the Q-atom energy terms, as rsqrt_e: the FP32 hardware estimate
+// and one FP64 Newton step (relative error below 1e-12; a is a sum of
+// powers of interatomic distances, well inside the FP32 range)
+__device__ __forceinline__ double rcp_e(double a) {
+  const double y = (double)__builtin_amdgcn_rcpf((float)a);
+  return fma(y, fma(-a, y, 1.0), y);
 }
 
 __device__ __forceinline__ void add_grad(long long* f, int atom, double gx, double gy, double gz) {
@@ -543,8 +535,11 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
       for (int s = 0; s < 3; ++s) nx[s] = d.xw[3 * ob_next + s];
     }
     if (k >= n) continue;
-    const int ob_atom = en ? d.wc_o[ob] : 0;
-    const bool cnt = en && oa < ob_atom;   // each unordered pair counted once
+    // each unordered pair counted once, from the row with the lower slot: a
+    // row's entries ascend in slot order, so the counted ones are a suffix
+    // of the row and the 8 rows of the wave (consecutive slots) switch at
+    // nearly the same place, which keeps the FP64 block from diverging
+    const bool cnt = en && row < ob;
     if (cnt) {
       double xj[9];
       for (int m = 0; m < 3; ++m) {
@@ -1122,7 +1117,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
         const double dy = pw.y - yq;
         const double dz = pw.z - zq;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        rinv[k] = rsqrt_d(r2);
+        rinv[k] = rsqrt_e(r2);
         if (k < NLJ) r6[k < NLJ ? k : 0] = r2 * r2 * r2;
       }
 #pragma unroll
@@ -1135,7 +1130,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
         double ev = 0.0;
 #pragma unroll
         for (int k = 0; k < NLJ; ++k) {
-          const double inv = rcp_d(r6[k] + SC[s][k]);
+          const double inv = rcp_e(r6[k] + SC[s][k]);
           ev += A[s][k] * inv * inv - B[s][k] * inv;
         }
         evdw[s] += ev;
@@ -1287,7 +1282,7 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
       // nonbond_qp_qvdw (md.f90:11138-11204): per state
       const double dx = x[3 * j] - x[3 * qa], dy = x[3 * j + 1] - x[3 * qa + 1], dz = x[3 * j + 2] - x[3 * qa + 2];
       const double r2 = dx * dx + dy * dy + dz * dz;
-      const double rinv = rsqrt_d(r2);
+      const double rinv = rsqrt_e(r2);
       const double r6 = r2 * r2 * r2;
       const float qj = d.x32[j].w;
       const int iacj = d.iac[j];
@@ -1302,7 +1297,7 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
         double aLJ, bLJ;
         lj_ab(ql[k], ql[k + 1], ta, tb, W.arith != 0, &aLJ, &bLJ);
         const double sc = W.qvdw ? d.q_scd[W.qsc_off + ((size_t)iq_l * W.natyps + iacj) * S + s] : 0.0;
-        const double inv = rcp_d(r6 + sc);
+        const double inv = rcp_e(r6 + sc);
         evdw[s] += aLJ * inv * inv - bLJ * inv;
       }
     }
@@ -1371,13 +1366,13 @@ __device__ __forceinline__ void qq_term(const Dev& d, int k, const Sink& put) {
         const double ddx = d.x[3 * j] - d.x[3 * i], ddy = d.x[3 * j + 1] - d.x[3 * i + 1],
                      ddz = d.x[3 * j + 2] - d.x[3 * i + 2];
         const double rr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        double v = qi * qj * rsqrt_d(rr2) * (double)es;
+        double v = qi * qj * rsqrt_e(rr2) * (double)es;
         if (iLJ == 3) v *= W.el14d;
         e_el = v;
         if (expo) {
           e_vdw = aLJ * exp(-bLJ * sqrt(rr2));
         } else {
-          const double inv = rcp_d(rr2 * rr2 * rr2 + sc);
+          const double inv = rcp_e(rr2 * rr2 * rr2 + sc);
           e_vdw = aLJ * inv * inv - bLJ * inv;
         }
         base = jq ? 4 : 6;  // qq or qp slot
