@@ -156,6 +156,10 @@ struct Dev {
   float4* xw;
   const int* wslot;
   double4* xwd;         // the same slots in FP64 (x, y, z, unused), for the energy terms
+  // the q-atoms' x32 again, per q-cluster member slot (cluster * 8 + member;
+  // zero for empty members); qmslot: per atom, its slot or -1
+  float4* xq;
+  const int* qmslot;
   float4* ljp;          // avdw1, bvdw1, avdw3, bvdw3
   long long* f;         // gradient, 3 per atom, fixed point 2^-32
   const float* winv;

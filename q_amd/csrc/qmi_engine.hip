@@ -927,6 +927,18 @@ static int build_device(qmi_engine* e) {
       }
     }
     UP(ip, qc_atom); d.qc_atom = ip;
+    // q-atom coordinates per q-cluster member slot, kept with x32
+    {
+      std::vector<int> qmslot(x32.size(), -1);
+      std::vector<float4> xq(std::max<size_t>(1, qc_atom.size()), make_float4(0.f, 0.f, 0.f, 0.f));
+      for (size_t m = 0; m < qc_atom.size(); ++m)
+        if (qc_atom[m] >= 0) {
+          qmslot[qc_atom[m]] = (int)m;
+          xq[m] = x32[qc_atom[m]];
+        }
+      UP(d.xq, xq);
+      UP(ip, qmslot); d.qmslot = ip;
+    }
     float4* pp;
     UP(pp, qcw_par); d.qcw_par = pp;
     float* fp;

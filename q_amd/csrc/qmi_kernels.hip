@@ -982,7 +982,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
         const int jj = on[u] ? jx[u] : 0;
         for (int k = 0; k < 3; ++k) pwv[u][k] = d.x32[jj + k];
       } else {
-        pqv[u] = d.x32[on[u] ? d.qc_atom[jx[u]] : 0];
+        pqv[u] = d.xq[on[u] ? jx[u] : 0];   // the q-atom's coordinates from its q-cluster slot
       }
     }
 #pragma unroll
@@ -1195,14 +1195,15 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
   // kQGroup tiles per pass, loads first (as in k_qw)
   for (int t0 = 0; !Q_SIDE && t0 < n; t0 += kQGroup) {
     bool on[kQGroup], o14[kQGroup];
-    int jx[kQGroup];
+    int jx[kQGroup], jm[kQGroup];
 #pragma unroll
     for (int u = 0; u < kQGroup; ++u) {
       const int tt = s0 + min(t0 + u, n - 1);
       const int jc = L.jc[tt];
       on[u] = t0 + u < n && lane_bit(L.mask[tt], lane) && p_in;
       o14[u] = lane_bit(L.m14[tt], lane);
-      jx[u] = Q_SIDE ? d.sc_a[jc * kCl + b] : d.qc_q[jc * kCl + b];
+      jm[u] = jc * kCl + b;   // the member's slot
+      jx[u] = Q_SIDE ? d.sc_a[jm[u]] : d.qc_q[jm[u]];
     }
     float4 pv[kQGroup], lv[kQGroup];
     int iacv[kQGroup];
@@ -1215,7 +1216,8 @@ __global__ void __launch_bounds__(256) k_qp(Dev d) {
         lv[u] = d.ljp[jj];
         iacv[u] = d.iac[jj];
       } else {
-        pv[u] = d.x32[d.q_atom[W.q_off + jj]];
+        // the q-atom's coordinates from its q-cluster slot
+        pv[u] = d.xq[on[u] ? jm[u] : 0];
       }
     }
 #pragma unroll
@@ -2389,7 +2391,8 @@ __global__ void __launch_bounds__(256) k_watpol_force(Dev d) {
 }
 
 // the FP32 coordinates of atom i from its FP64 ones, into x32 and, for a
-// water atom, into its cluster slot (with the FP64 ones)
+// water atom, into its cluster slot (with the FP64 ones), for a q-atom into
+// its q-cluster slot
 __device__ __forceinline__ void store_x32(const Dev& d, int i) {
   const float4 q = d.x32[i];
   const float4 n = make_float4((float)d.x[3 * i], (float)d.x[3 * i + 1], (float)d.x[3 * i + 2], q.w);
@@ -2399,6 +2402,8 @@ __device__ __forceinline__ void store_x32(const Dev& d, int i) {
     d.xw[ws] = n;
     d.xwd[ws] = make_double4(d.x[3 * i], d.x[3 * i + 1], d.x[3 * i + 2], 0.0);
   }
+  const int qs = d.qmslot[i];
+  if (qs >= 0) d.xq[qs] = n;
 }
 
 // ------------------------------------------------------- integrator --
