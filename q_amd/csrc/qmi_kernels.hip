@@ -1390,25 +1390,39 @@ __global__ void __launch_bounds__(128) k_qq(Dev d) {
 }
 
 // bonded mode 2: the q-q pair slots of each atom (contiguous), summed by
-// one thread per atom and added to the gradient
+// eight lanes per atom (a q-atom has a slot per partner and state, so one
+// lane alone would walk a long dependent run) and added to the gradient.
+// Every lane stays to the row sum.
+constexpr int kQqGatherLanes = 8;
 __global__ void __launch_bounds__(256) k_qq_gather(Dev d) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.nqqg) return;
-  const int atom = d.qqg_atom[k];
-  if (!d.ws[d.wid[atom]].active) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t / kQqGatherLanes, l = t % kQqGatherLanes;
+  int atom = -1;
   long long gx = 0, gy = 0, gz = 0;
-  for (int m = d.qqg_start[k]; m < d.qqg_start[k + 1]; ++m) {
-    gx += d.qqslot[3 * (size_t)m];
-    gy += d.qqslot[3 * (size_t)m + 1];
-    gz += d.qqslot[3 * (size_t)m + 2];
+  if (k < d.nqqg) {
+    atom = d.qqg_atom[k];
+    if (d.ws[d.wid[atom]].active) {
+      const int m1 = d.qqg_start[k + 1];
+      for (int m = d.qqg_start[k] + l; m < m1; m += kQqGatherLanes) {
+        gx += d.qqslot[3 * (size_t)m];
+        gy += d.qqslot[3 * (size_t)m + 1];
+        gz += d.qqslot[3 * (size_t)m + 2];
+      }
+    } else {
+      atom = -1;
+    }
   }
-  add_grad_fix(d.f, atom, gx, gy, gz);
+  gx = row8_sum_i(gx);
+  gy = row8_sum_i(gy);
+  gz = row8_sum_i(gz);
+  if (l == 0 && atom >= 0) add_grad_fix(d.f, atom, gx, gy, gz);
 }
 
 static void launch_qq(const Dev& d, hipStream_t s) {
   if (!d.nqq) return;
   k_qq<<<(d.nqq + 127) / 128, 128, 0, s>>>(d);
-  if (d.bond_atomic == 2 && d.nqqg) k_qq_gather<<<(d.nqqg + 255) / 256, 256, 0, s>>>(d);
+  if (d.bond_atomic == 2 && d.nqqg)
+    k_qq_gather<<<((size_t)d.nqqg * kQqGatherLanes + 255) / 256, 256, 0, s>>>(d);
 }
 
 // ------------------------------------------------------------- LRF --
