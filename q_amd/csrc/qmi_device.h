@@ -91,7 +91,10 @@ struct WinState {
   int wpol_acc;       // this evaluation is a pot_energy call of the run: water-polarisation shell averages advance
   double tscale_solute, tscale_solvent;
   double temp, tfree, tfree_solute, tfree_solvent, temp_solute, temp_solvent, ekin;
-  double ekin_parts[4];  // solute all, solute free, solvent all, solvent free
+  // kinetic energy of the step's atoms, fixed point (kEScale): solute all,
+  // solute free, solvent all, solvent free; k_integrate adds, k_temperature
+  // reads and clears
+  long long ekin_fix[4];
   int shake_fail;
   int list_overflow;  // a neighbour row exceeded its capacity
   int list_missed;    // 1 (2): a step needing pair lists (energies) was launched without those kernels
@@ -121,6 +124,10 @@ struct TileList {
 // T_free solute, T_free solvent
 constexpr int kTRing = 6;
 constexpr int kTempParts = 8;      // blocks per window of the kinetic-energy sum
+// the LRF build splits a window's source groups in this many parts, summed in
+// part order; fixed, so the expansions do not depend on launch shapes
+constexpr int kLrfParts = 8;
+constexpr int kIntBlock = 256;     // atoms per block of the integrator
 constexpr int kGatherAtoms = 256;  // atoms per block of the bonded gather
 constexpr int kGatherSlots = 2048; // slots staged in LDS per gather block
 constexpr int kBondTileTerms = 512; // bonded terms per tile
@@ -184,8 +191,7 @@ struct Dev {
   int ngrp_all;         // groups of all windows
   int nlrfw;            // LRF build blocks: (window, first target group, count, source part)
   const int4* lrf_work;
-  int lrf_parts;         // source groups of a window split in this many parts (1: no split)
-  double* g_lrf_part;    // [part][group][20] partial sums when lrf_parts > 1
+  double* g_lrf_part;    // [part][group][20] partial sums over the kLrfParts source parts
   // LRF sources, packed at every build: switching atom of each group
   // (x, y, z, excluded) and the group's non-q atoms (x, y, z, charge)
   double4* ls_sw;
@@ -300,12 +306,19 @@ struct Dev {
   // (sequence, wall) or ipsi (position, distance, angle); 8 parameters each
   int nrst;
   const int4* rs_kwij; const int2* rs_kf; const double* rs_par;
-  // shake units (independent constraint components)
+  // shake units (independent constraint components): their constraints
   int nsu;
-  const int* su_start; const int* su_n; const int* su_win;
-  const int* sc_i; const int* sc_j; const double* sc_d2;
-  const int* su_atoms; const int* su_natom; const int* su_atom_list;   // atoms of each unit
+  const int* su_start; const int* su_n;
+  const double* sc_d2;     // per constraint: d0^2
+  const float2* sc_w;      // per constraint: inverse masses of i and j
   const uint8_t* sc_loc;   // per constraint: local atom numbers of i and j within the unit
+  // integration units of k_integrate, each a SHAKE component or one
+  // unconstrained atom: (window, first entry in iu_atom, number of atoms,
+  // SHAKE unit or -1); a component's atoms are listed in the order its
+  // sc_loc numbers them.  Blocks of k_integrate: runs of whole units of at
+  // most kIntBlock atoms, (first iu_atom entry, atoms, first unit, units)
+  int niu, nib;
+  const int4* iu; const int* iu_atom; const int4* ib;
   // water shells
   ShellDev* shells;
   double* wtheta;        // per water (global water index), theta

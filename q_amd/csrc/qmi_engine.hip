@@ -94,10 +94,9 @@ struct qmi_engine {
   // autotuner: work-item variants of the static Q lists and the choices made
   int4* qwork[T_NTYPES][4] = {};
   int nqwork[T_NTYPES][4] = {};
-  // LRF build blocks for 64, 128, 256 threads (index 4 * b) and 1, 2, 4, 8
-  // source parts (+ p)
-  int4* lrfwork[12] = {};
-  int nlrfwork[12] = {};
+  // LRF build blocks for 64, 128, 256 threads, each over kLrfParts source parts
+  int4* lrfwork[3] = {};
+  int nlrfwork[3] = {};
   bool autotune = true;
   // Q chunk index, row / LRF / bonded block sizes, XCD map, atomic bonded
   // gradients, LRF source parts
@@ -226,9 +225,10 @@ static int build_device(qmi_engine* e) {
   // restraints
   std::vector<int4> rs_kwij; std::vector<int2> rs_kf; std::vector<double> rs_par;
   // shake
-  std::vector<int> su_start, su_n, su_win, sc_i, sc_j, su_atoms, su_natom, su_atom_list;
+  std::vector<int> su_start, su_n, su_win, su_atoms, su_natom, su_atom_list;
   std::vector<uint8_t> sc_loc;
   std::vector<double> sc_d2;
+  std::vector<float2> sc_w;
   // shells
   std::vector<ShellDev> shells;
   std::vector<int> w_off(NW), wat_win;
@@ -721,7 +721,7 @@ static int build_device(qmi_engine* e) {
       std::vector<int> roots;
       for (int a = 0; a < nc; ++a) if (find(a) == a) roots.push_back(a);
       for (int r : roots) {
-        su_start.push_back((int)sc_i.size());
+        su_start.push_back((int)sc_d2.size());
         int cnt = 0;
         std::vector<int> atoms;   // the unit's atoms in first-use order
         auto local = [&](int at) {
@@ -731,12 +731,14 @@ static int build_device(qmi_engine* e) {
         };
         for (int a = 0; a < nc; ++a)
           if (find(a) == r) {
-            sc_i.push_back(ao + m.i[a]); sc_j.push_back(ao + m.j[a]); sc_d2.push_back(m.dist2[a]); ++cnt;
+            sc_d2.push_back(m.dist2[a]);
+            sc_w.push_back(make_float2(winv[ao + m.i[a]], winv[ao + m.j[a]]));
+            ++cnt;
             const int li = local(m.i[a]), lj = local(m.j[a]);
             sc_loc.push_back((uint8_t)(li < 255 ? li : 255));
             sc_loc.push_back((uint8_t)(lj < 255 ? lj : 255));
           }
-        // units of up to 4 atoms are iterated in registers (k_shake)
+        // units of up to 4 atoms are iterated in registers (k_integrate)
         su_atoms.push_back((int)su_atom_list.size());
         su_natom.push_back((int)atoms.size());
         for (int at : atoms) su_atom_list.push_back(ao + at);
@@ -821,9 +823,9 @@ static int build_device(qmi_engine* e) {
   d.ngrp_all = goff;
   {
     // LRF build: one block per (window, block-size consecutive targets,
-    // part of the sources), a variant per candidate of the autotuner
-    for (int vi = 0; vi < 12; ++vi) {
-      const int bsz = 64 << (vi / 4), parts = 1 << (vi % 4);
+    // part of the sources), a variant per block size of the autotuner
+    for (int vi = 0; vi < 3; ++vi) {
+      const int bsz = 64 << vi, parts = kLrfParts;
       std::vector<int4> lw;
       for (int wi = 0; wi < NW; ++wi)
         for (int g0 = 0; g0 < wd[wi].ngrp; g0 += bsz)
@@ -834,10 +836,9 @@ static int build_device(qmi_engine* e) {
       e->lrfwork[vi] = lwp;
       e->nlrfwork[vi] = (int)lw.size();
     }
-    d.lrf_work = e->lrfwork[8];
-    d.nlrfw = e->nlrfwork[8];
-    d.lrf_parts = 1;
-    if ((rc = dev_alloc(e, &d.g_lrf_part, 8 * 20 * (size_t)std::max(goff, 1))) != 0) return rc;
+    d.lrf_work = e->lrfwork[2];
+    d.nlrfw = e->nlrfwork[2];
+    if ((rc = dev_alloc(e, &d.g_lrf_part, kLrfParts * 20 * (size_t)std::max(goff, 1))) != 0) return rc;
     // packed sources: non-q atoms of every group, in group order
     std::vector<int> ls_first(goff), ls_n(goff), ls_atom;
     for (int g = 0; g < goff; ++g) {
@@ -1123,11 +1124,52 @@ static int build_device(qmi_engine* e) {
   d.nrst = (int)rs_kwij.size();
   UP(i4, rs_kwij); d.rs_kwij = i4; UP(i2, rs_kf); d.rs_kf = i2; UP(dp, rs_par); d.rs_par = dp;
   d.nsu = (int)su_n.size();
-  UP(ip, su_start); d.su_start = ip; UP(ip, su_n); d.su_n = ip; UP(ip, su_win); d.su_win = ip;
-  UP(ip, sc_i); d.sc_i = ip; UP(ip, sc_j); d.sc_j = ip;
-  UP(ip, su_atoms); d.su_atoms = ip; UP(ip, su_natom); d.su_natom = ip; UP(ip, su_atom_list); d.su_atom_list = ip;
+  UP(ip, su_start); d.su_start = ip; UP(ip, su_n); d.su_n = ip;
   UP(up, sc_loc); d.sc_loc = up;
+  {
+    // integration units, per window: the three-atom SHAKE components (the
+    // waters), the other components, then the unconstrained atoms, so that
+    // the waves of k_integrate mostly hold units of one shape
+    std::vector<int4> iu;
+    std::vector<int> iu_atom;
+    std::vector<char> constrained(ao, 0);
+    std::vector<std::vector<int>> units_of(NW);
+    for (size_t u = 0; u < su_n.size(); ++u) units_of[su_win[u]].push_back((int)u);
+    for (int wi = 0; wi < NW; ++wi) {
+      for (int pass = 0; pass < 2; ++pass)
+        for (int u : units_of[wi]) {
+          if ((su_natom[u] == 3) != (pass == 0)) continue;
+          iu.push_back(make_int4(wi, (int)iu_atom.size(), su_natom[u], u));
+          for (int k = 0; k < su_natom[u]; ++k) {
+            const int a = su_atom_list[su_atoms[u] + k];
+            iu_atom.push_back(a);
+            constrained[a] = 1;
+          }
+        }
+      for (int a = wd[wi].ao; a < wd[wi].ao + wd[wi].natom; ++a)
+        if (!constrained[a]) {
+          iu.push_back(make_int4(wi, (int)iu_atom.size(), 1, -1));
+          iu_atom.push_back(a);
+        }
+    }
+    if ((long long)iu_atom.size() != (long long)ao) return set_err(-10, "integration units do not cover every atom once");
+    // blocks of k_integrate: runs of whole units, at most kIntBlock atoms
+    std::vector<int4> ib;
+    for (int u = 0; u < (int)iu.size(); ++u) {
+      if (iu[u].z > kIntBlock || iu[u].z > 255)
+        return set_err(-10, "a SHAKE component with more than 255 atoms is not supported");
+      if (ib.empty() || ib.back().y + iu[u].z > kIntBlock) ib.push_back(make_int4(iu[u].y, 0, u, 0));
+      ib.back().y += iu[u].z;
+      ib.back().w += 1;
+    }
+    d.niu = (int)iu.size();
+    d.nib = (int)ib.size();
+    UP(i4, iu); d.iu = i4;
+    UP(ip, iu_atom); d.iu_atom = ip;
+    UP(i4, ib); d.ib = i4;
+  }
   UP(dp, sc_d2); d.sc_d2 = dp;
+  UP(f2, sc_w); d.sc_w = f2;
   ShellDev* sd;
   UP(sd, shells); d.shells = sd;
   if ((rc = dev_alloc(e, &d.wtheta, (size_t)e->nwat_total)) != 0) return rc;
@@ -1562,17 +1604,17 @@ static int autotune(qmi_engine* e) {
     e->tuned[1 + g] = bb;
     e->tune_us[1 + g] = bt;
   }
-  // 2, 6: threads per block and source parts of the LRF build
+  // 2, 6: threads per block of the LRF build (its source parts are fixed:
+  // they set the order of its sums)
   {
-    int bv = 11;
+    int bv = 2;
     double bt = 1e30;
     auto set_lrf = [&](int vi) {
-      d.bs_lrf = 64 << (vi / 4);
-      d.lrf_parts = 1 << (vi % 4);
+      d.bs_lrf = 64 << vi;
       d.lrf_work = e->lrfwork[vi];
       d.nlrfw = e->nlrfwork[vi];
     };
-    for (int vi = 11; vi >= 0; --vi) {
+    for (int vi = 2; vi >= 0; --vi) {
       set_lrf(vi);
       const double t = time_us([&] { launch_lrf_build(d, e->stream); });
       if (faster(t, bt)) { bt = t; bv = vi; }
@@ -1580,7 +1622,7 @@ static int autotune(qmi_engine* e) {
     set_lrf(bv);
     e->tuned[2] = d.bs_lrf;
     e->tune_us[2] = bt;
-    e->tuned[6] = d.lrf_parts;
+    e->tuned[6] = kLrfParts;
   }
   // 4: XCD-grouped block order of the pair, Q-tile, LRF-build and gather kernels
   {

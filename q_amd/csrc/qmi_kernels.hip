@@ -33,6 +33,13 @@ __device__ __forceinline__ int block_id(int xcd) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// index of the calling wave in the grid (blocks are whole waves), read from
+// the first lane so the compiler keeps it, and the list loads it indexes, in
+// scalar registers
+__device__ __forceinline__ int wave_id(const Dev& d) {
+  return __builtin_amdgcn_readfirstlane((int)((block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6));
+}
+
 __device__ __forceinline__ long long fix(double v, double scale) { return __double2ll_rn(v * scale); }
 
 // 1/sqrt(a) for the FP64 energy terms: the FP32 hardware estimate (relative
@@ -387,7 +394,7 @@ __device__ __forceinline__ int special_type(const Dev& d, int i, int j) {
 // members are appended, in j order, to row a.
 template <int TT>
 __global__ void __launch_bounds__(256) k_build_rows(Dev d) {
-  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  const int wave = wave_id(d);
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[TT];
   if (wave >= L.nicl) return;
@@ -483,7 +490,7 @@ __device__ __forceinline__ int row_chunks(int n) {
 // loop for the wave's window (uniform over the wave).
 template <bool EN, bool W3>
 __device__ __forceinline__ void ww_impl(const Dev& d) {
-  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  const int wave = wave_id(d);
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[T_WW];
   if (wave >= L.nicl) return;
@@ -599,7 +606,7 @@ __device__ __forceinline__ void ww_impl(const Dev& d) {
 // terms keeps the smaller register footprint of the force-only loop
 template <bool W3, bool EK>
 __global__ void __launch_bounds__(256) k_ww(Dev d) {
-  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  const int wave = wave_id(d);
   if (wave >= d.rl[T_WW].nicl) return;
   const int w = d.wc_win[wave];
   if ((d.win[w].wat3 != 0) != W3) return;
@@ -613,7 +620,7 @@ __global__ void __launch_bounds__(256) k_ww(Dev d) {
 // which also counts the energies.
 template <bool WATER_SIDE, bool EN, bool W3>
 __device__ __forceinline__ void pw_impl(const Dev& d) {
-  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  const int wave = wave_id(d);
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[WATER_SIDE ? T_WP : T_PW];
   if (wave >= L.nicl) return;
@@ -737,7 +744,7 @@ __device__ __forceinline__ void pw_impl(const Dev& d) {
 
 template <bool WATER_SIDE, bool W3, bool EK>
 __global__ void __launch_bounds__(256) k_pw(Dev d) {
-  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  const int wave = wave_id(d);
   if (wave >= d.rl[WATER_SIDE ? T_WP : T_PW].nicl) return;
   const int w = WATER_SIDE ? d.wc_win[wave] : d.sc_win[wave];
   if ((d.win[w].wat3 != 0) != W3) return;
@@ -749,7 +756,7 @@ __global__ void __launch_bounds__(256) k_pw(Dev d) {
 // LJ column and the el14 scale
 template <bool EN>
 __device__ __forceinline__ void pp_impl(const Dev& d) {
-  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  const int wave = wave_id(d);
   const int lane = threadIdx.x & 63;
   const RowList L = d.rl[T_PP];
   if (wave >= L.nicl) return;
@@ -844,7 +851,7 @@ __device__ __forceinline__ void pp_impl(const Dev& d) {
 
 template <bool EK>
 __global__ void __launch_bounds__(256) k_pp(Dev d) {
-  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  const int wave = wave_id(d);
   if (wave >= d.rl[T_PP].nicl) return;
   if (EK && d.ws[d.sc_win[wave]].energy_now) pp_impl<EK>(d);
   else pp_impl<false>(d);
@@ -912,7 +919,7 @@ __global__ void __launch_bounds__(256) k_q_gather(Dev d) {
 // q-atom vs water (nonbond_qw_spc, md.f90:11302-11411)
 template <bool Q_SIDE, bool W3>
 __global__ void __launch_bounds__(256) k_qw(Dev d) {
-  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  const int wave = wave_id(d);
   const int lane = threadIdx.x & 63;
   const TileList L = d.tl[Q_SIDE ? T_QW : T_WQ];
   if (wave >= L.nwork) return;
@@ -1152,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_qw(Dev d) {
 // 10980-11045)
 template <bool Q_SIDE>
 __global__ void __launch_bounds__(256) k_qp(Dev d) {
-  const int wave = (block_id(d.xcd) * blockDim.x + threadIdx.x) >> 6;
+  const int wave = wave_id(d);
   const int lane = threadIdx.x & 63;
   const TileList L = d.tl[Q_SIDE ? T_QP : T_PQ];
   if (wave >= L.nwork) return;
@@ -1525,7 +1532,7 @@ __global__ void __launch_bounds__(256) k_lrf_build(Dev d) {
   double acc[20], aux[4] = {0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < 20; ++k) acc[k] = 0.0;
   // this block's part of the window's source groups
-  const int P = d.lrf_parts;
+  constexpr int P = kLrfParts;
   const int h_beg = W.g_off + (int)(((long long)lw.w * W.ngrp) / P);
   const int h_end = W.g_off + (int)(((long long)(lw.w + 1) * W.ngrp) / P);
   const double4* __restrict__ xq = d.ls_xq;
@@ -1567,7 +1574,7 @@ __global__ void __launch_bounds__(256) k_lrf_reduce(Dev d) {
   const int w = d.g_win[g];
   if (!d.ws[w].active || !d.ws[w].list_now || !d.win[w].use_lrf || d.ls_sw[g].w != 0.0) return;
   double v = 0.0;
-  for (int p = 0; p < d.lrf_parts; ++p) v += d.g_lrf_part[(size_t)p * 20 * d.ngrp_all + i];
+  for (int p = 0; p < kLrfParts; ++p) v += d.g_lrf_part[(size_t)p * 20 * d.ngrp_all + i];
   d.g_lrf[i] = v;
 }
 
@@ -2407,176 +2414,135 @@ __global__ void __launch_bounds__(256) k_watpol_force(Dev d) {
 // the FP32 coordinates of atom i from its FP64 ones, into x32 and, for a
 // water atom, into its cluster slot (with the FP64 ones), for a q-atom into
 // its q-cluster slot
-__device__ __forceinline__ void store_x32(const Dev& d, int i) {
+__device__ __forceinline__ void store_x32_xyz(const Dev& d, int i, double x, double y, double z) {
   const float4 q = d.x32[i];
-  const float4 n = make_float4((float)d.x[3 * i], (float)d.x[3 * i + 1], (float)d.x[3 * i + 2], q.w);
+  const float4 n = make_float4((float)x, (float)y, (float)z, q.w);
   d.x32[i] = n;
   const int ws = d.wslot[i];
   if (ws >= 0) {
     d.xw[ws] = n;
-    d.xwd[ws] = make_double4(d.x[3 * i], d.x[3 * i + 1], d.x[3 * i + 2], 0.0);
+    d.xwd[ws] = make_double4(x, y, z, 0.0);
   }
   const int qs = d.qmslot[i];
   if (qs >= 0) d.xq[qs] = n;
 }
+__device__ __forceinline__ void store_x32(const Dev& d, int i) {
+  store_x32_xyz(d, i, d.x[3 * i], d.x[3 * i + 1], d.x[3 * i + 2]);
+}
 
 // ------------------------------------------------------- integrator --
-// leapfrog (md.f90:4081-4111): v = (v - d*winv*dt)*Tscale; xx = x; x += v*dt
-__device__ __forceinline__ void leapfrog_atom(const Dev& d, const WinState& st, const WinDev& W, int i,
-                                              const long long gfix[3]) {
+// The end of a step in one kernel, k_integrate: the leapfrog of every atom,
+// the SHAKE sweeps of every constraint component, v = (x - xx)/dt and the
+// kinetic energy.  Every atom's arithmetic is that of the reference's three
+// loops in their order; only the kinetic-energy sum goes to fixed point.
+
+// leapfrog (md.f90:4081-4111) of atom i with gradient gfix (fixed point):
+// v = (v - d*winv*dt)*Tscale; xx = x; x += v*dt, into x[], xx[], v[]
+__device__ __forceinline__ void leapfrog_vals(const Dev& d, const WinState& st, const WinDev& W, int i,
+                                              const long long gfix[3], double x[3], double xx[3], double v[3]) {
 #pragma clang fp contract(off)
   const double ts = (i - W.ao) < W.nat_solute ? st.tscale_solute : st.tscale_solvent;
   const double wi = (double)d.winv[i];
   for (int c = 0; c < 3; ++c) {
     const int k = 3 * i + c;
     const double g = (double)gfix[c] * (1.0 / kFScale);
-    double v = (d.v[k] - g * wi * W.dt) * ts;
-    d.v[k] = v;
-    d.xx[k] = d.x[k];
-    d.x[k] = d.x[k] + v * W.dt;
+    v[c] = (d.v[k] - g * wi * W.dt) * ts;
+    xx[c] = d.x[k];
+    x[c] = xx[c] + v[c] * W.dt;
   }
 }
 
-// ATOM_TERMS: the step's per-atom terms (LRF evaluation, fixed/shell and
-// radial solvent restraints) are evaluated here, summed with the gradient of
-// the other kernels in the lane's registers and stored back, so the step's
-// gradient array ends complete without per-atom atomics
+// gradient of atom i for the integrator: the step's per-atom terms (LRF
+// evaluation, fixed/shell and radial solvent restraints) when ATOM_TERMS,
+// summed with the other kernels' gradient in registers and stored back, so
+// the gradient array ends complete without per-atom atomics.  Every lane
+// calls it (wave energy sums inside); i = d.ntot does nothing.
 template <bool ATOM_TERMS>
-__global__ void __launch_bounds__(256) k_leapfrog(Dev d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  long long g[3] = {0, 0, 0};
+__device__ __forceinline__ void step_gradient(const Dev& d, int i, bool valid, long long g[3]) {
+  g[0] = g[1] = g[2] = 0;
   if (ATOM_TERMS) {
-    // every lane reaches the wave energy sums of both
     lrf_taylor_atom<true>(d, i, g);
     restr_atom<true>(d, i, g);
   }
-  if (i >= d.ntot) return;
-  const int w = d.wid[i];
-  const WinState& st = d.ws[w];
-  if (!st.active) return;
+  if (!valid) return;
   for (int c = 0; c < 3; ++c) {
     g[c] += d.f[3 * i + c];
     if (ATOM_TERMS) d.f[3 * i + c] = g[c];
   }
-  leapfrog_atom(d, st, d.win[w], i, g);
 }
 
 // SHAKE (md.f90:14445-14566): one thread per connected component of a
 // molecule's constraint graph; components share no atoms, so iterating them
 // separately gives the reference's result for the whole molecule.
 // Constraints are swept in order, Gauss-Seidel, until every one is within
-// tol*d0^2 (at most 1000 sweeps).  Units of up to 4 atoms with at most one
-// constraint per atom pair hold their coordinates and constraints in
-// registers during the sweeps; others work on the coordinates in memory.
-// Both do the same arithmetic in the same order.
-// a unit of NA atoms held in registers, with its at most NA(NA-1)/2
-// constraints (atom slots and d0^2) loaded once before the sweeps (constant
-// indices after unrolling; the constraint's atoms are picked with selects)
-template <int NA>
-__device__ __forceinline__ void shake_regs(const Dev& d, WinState& st, int c0, int nc, const int* al) {
-#pragma clang fp contract(off)
-  constexpr int NC = NA * (NA - 1) / 2;
-  const double tol = (double)0.0001f;
-  unsigned long long ready = 0ull;
-  const unsigned long long all = (1ull << nc) - 1ull;
-  double px[NA][3], pxx[NA][3];
-  float wv[NA];
+// tol*d0^2 (at most 1000 sweeps).
+// a unit whose coordinates are in LDS (sx new, sxx old, its atoms numbered
+// as sc_loc numbers them), any size.  Units of at most kShakeHeld
+// constraints (the waters' three) take theirs from a ShakeHeld loaded
+// before (at the start of the integrator, so the loads overlap its first
+// phase) and hold the old-coordinate differences in registers over the
+// sweeps; larger ones load each constraint per sweep.  The sums are the same
+// either way.
+constexpr int kShakeHeld = 3;
+struct ShakeHeld {
+  int li[kShakeHeld], lj[kShakeHeld];
+  double d2[kShakeHeld];
+  float2 w[kShakeHeld];
+};
+__device__ __forceinline__ void shake_hold(const Dev& d, int c0, int nc, ShakeHeld& h) {
 #pragma unroll
-  for (int k = 0; k < NA; ++k) {
-    const int at = al[k];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) { px[k][c] = d.x[3 * at + c]; pxx[k][c] = d.xx[3 * at + c]; }
-    wv[k] = d.winv[at];
-  }
-  int cli[NC], clj[NC];
-  double cd2[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
+  for (int c = 0; c < kShakeHeld; ++c) {
     const bool on = c < nc;
-    cli[c] = on ? d.sc_loc[2 * (c0 + c)] : 0;
-    clj[c] = on ? d.sc_loc[2 * (c0 + c) + 1] : 0;
-    cd2[c] = on ? d.sc_d2[c0 + c] : 0.0;
+    h.li[c] = on ? d.sc_loc[2 * (c0 + c)] : 0;
+    h.lj[c] = on ? d.sc_loc[2 * (c0 + c) + 1] : 0;
+    h.d2[c] = on ? d.sc_d2[c0 + c] : 0.0;
+    h.w[c] = on ? d.sc_w[c0 + c] : make_float2(0.f, 0.f);
   }
-  int nits = 0;
-  while (true) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      if (c >= nc || (ready & (1ull << c))) continue;
-      const int li = cli[c], lj = clj[c];
-      double xi[3], xj[3], xxi[3], xxj[3];
-      float wvi = 0.f, wvj = 0.f;
-#pragma unroll
-      for (int k = 0; k < NA; ++k) {
-        if (k == li) { for (int q = 0; q < 3; ++q) { xi[q] = px[k][q]; xxi[q] = pxx[k][q]; } wvi = wv[k]; }
-        if (k == lj) { for (int q = 0; q < 3; ++q) { xj[q] = px[k][q]; xxj[q] = pxx[k][q]; } wvj = wv[k]; }
-      }
-      double xij[3], xxij[3];
-      for (int q = 0; q < 3; ++q) xij[q] = xi[q] - xj[q];
-      const double xij2 = xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2];
-      const double d2 = cd2[c];
-      const double diff = d2 - xij2;
-      if (fabs(diff) < tol * d2) ready |= 1ull << c;
-      for (int q = 0; q < 3; ++q) xxij[q] = xxi[q] - xxj[q];
-      const double scp = xij[0] * xxij[0] + xij[1] * xxij[1] + xij[2] * xxij[2];
-      const float wsum = wvi + wvj;
-      const double corr = diff / (2. * scp * (double)wsum);
-      const double wi = (double)wvi, wj = (double)wvj;
-#pragma unroll
-      for (int k = 0; k < NA; ++k) {
-        if (k == li)
-          for (int q = 0; q < 3; ++q) px[k][q] = px[k][q] + xxij[q] * corr * wi;
-        if (k == lj)
-          for (int q = 0; q < 3; ++q) px[k][q] = px[k][q] - xxij[q] * corr * wj;
-      }
-    }
-    ++nits;
-    if ((ready & all) == all) break;
-    if (nits >= 1000) { st.shake_fail = 1; break; }
-  }
-#pragma unroll
-  for (int k = 0; k < NA; ++k)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) d.x[3 * al[k] + c] = px[k][c];
 }
-
-__global__ void __launch_bounds__(64) k_shake(Dev d) {
+template <bool HELD>
+__device__ __forceinline__ void shake_lds(const Dev& d, WinState& st, int c0, int nc, double (*sx)[3],
+                                          const double (*sxx)[3], const ShakeHeld& h) {
 #pragma clang fp contract(off)
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= d.nsu) return;
-  const int w = d.su_win[u];
-  WinState& st = d.ws[w];
-  if (!st.active) return;
-  const int c0 = d.su_start[u], nc = d.su_n[u];
-  const int na = d.su_natom[u];
-  const int* al = d.su_atom_list + d.su_atoms[u];
-  // register units: at most one constraint per atom pair
-  if (na == 3 && nc <= 3) { shake_regs<3>(d, st, c0, nc, al); return; }
-  if (na == 2 && nc <= 1) { shake_regs<2>(d, st, c0, nc, al); return; }
-  if (na == 4 && nc <= 6) { shake_regs<4>(d, st, c0, nc, al); return; }
   const double tol = (double)0.0001f;
   unsigned long long ready = 0ull;  // up to 64 constraints per component
   const unsigned long long all = nc >= 64 ? ~0ull : ((1ull << nc) - 1ull);
-  const double* xx = d.xx;
+  double hxx[HELD ? kShakeHeld : 1][3];
+  if (HELD) {
+#pragma unroll
+    for (int c = 0; c < kShakeHeld; ++c)
+      for (int k = 0; k < 3; ++k) hxx[c][k] = c < nc ? sxx[h.li[c]][k] - sxx[h.lj[c]][k] : 0.0;
+  }
+  // one constraint: the correction of the Gauss-Seidel sweep
+  auto fix_pair = [&](int c, int li, int lj, double d2, const double xxij[3], float wvi, float wvj) {
+#pragma clang fp contract(off)
+    double xij[3];
+    for (int k = 0; k < 3; ++k) xij[k] = sx[li][k] - sx[lj][k];
+    const double xij2 = xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2];
+    const double diff = d2 - xij2;
+    if (fabs(diff) < tol * d2) ready |= 1ull << c;
+    const double scp = xij[0] * xxij[0] + xij[1] * xxij[1] + xij[2] * xxij[2];
+    const float wsum = wvi + wvj;
+    const double corr = diff / (2. * scp * (double)wsum);
+    const double wi = (double)wvi, wj = (double)wvj;
+    for (int k = 0; k < 3; ++k) {
+      sx[li][k] = sx[li][k] + xxij[k] * corr * wi;
+      sx[lj][k] = sx[lj][k] - xxij[k] * corr * wj;
+    }
+  };
   int nits = 0;
-  double* x = d.x;
   while (true) {
-    for (int c = 0; c < nc; ++c) {
-      if (ready & (1ull << c)) continue;
-      const int i = d.sc_i[c0 + c], j = d.sc_j[c0 + c];
-      double xij[3], xxij[3];
-      for (int k = 0; k < 3; ++k) xij[k] = x[3 * i + k] - x[3 * j + k];
-      const double xij2 = xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2];
-      const double d2 = d.sc_d2[c0 + c];
-      const double diff = d2 - xij2;
-      if (fabs(diff) < tol * d2) ready |= 1ull << c;
-      for (int k = 0; k < 3; ++k) xxij[k] = xx[3 * i + k] - xx[3 * j + k];
-      const double scp = xij[0] * xxij[0] + xij[1] * xxij[1] + xij[2] * xxij[2];
-      const float wsum = d.winv[i] + d.winv[j];
-      const double corr = diff / (2. * scp * (double)wsum);
-      const double wi = (double)d.winv[i], wj = (double)d.winv[j];
-      for (int k = 0; k < 3; ++k) {
-        x[3 * i + k] = x[3 * i + k] + xxij[k] * corr * wi;
-        x[3 * j + k] = x[3 * j + k] - xxij[k] * corr * wj;
+    if constexpr (HELD) {
+#pragma unroll
+      for (int c = 0; c < kShakeHeld; ++c)
+        if (c < nc && !(ready & (1ull << c))) fix_pair(c, h.li[c], h.lj[c], h.d2[c], hxx[c], h.w[c].x, h.w[c].y);
+    } else {
+      for (int c = 0; c < nc; ++c) {
+        if (ready & (1ull << c)) continue;
+        const int li = d.sc_loc[2 * (c0 + c)], lj = d.sc_loc[2 * (c0 + c) + 1];
+        const float2 wv = d.sc_w[c0 + c];
+        double xxij[3];
+        for (int k = 0; k < 3; ++k) xxij[k] = sxx[li][k] - sxx[lj][k];
+        fix_pair(c, li, lj, d.sc_d2[c0 + c], xxij, wv.x, wv.y);
       }
     }
     ++nits;
@@ -2585,46 +2551,130 @@ __global__ void __launch_bounds__(64) k_shake(Dev d) {
   }
 }
 
-// after SHAKE: v = (x - xx)/dt for every atom (md.f90:4117-4120), refresh the
-// FP32 coordinates and clear the gradient for the next step
-__global__ void k_post(Dev d, int shake_any) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.ntot) return;
-  const int w = d.wid[i];
-  const WinState& st = d.ws[w];
-  if (!st.active && shake_any >= 0) return;
-  if (shake_any < 0) shake_any = 0;
-  const WinDev& W = d.win[w];
-  for (int c = 0; c < 3; ++c) {
-    const int k = 3 * i + c;
-    if (shake_any) d.v[k] = (d.x[k] - d.xx[k]) / W.dt;
+// kinetic energy of the step (md.f90:3813-3830): the lanes' fixed-point sums
+// added to their window's accumulators, one wave sum and one atomic each when
+// all valid lanes hold the same window.  Integer sums: the result does not
+// depend on the order.  Every lane calls it.
+__device__ __forceinline__ void wave_ekin(const Dev& d, int w, const long long ek[4], bool valid) {
+  const unsigned long long vm = __ballot(valid);
+  if (vm == 0ull) return;
+  const int first = __ffsll((long long)vm) - 1;
+  const int w0 = __builtin_amdgcn_readlane(valid ? w : -1, first);   // first is uniform over the wave
+  const bool uniform = __ballot(valid && w != w0) == 0ull;
+  for (int c = 0; c < 4; ++c) {
+    if (uniform) {
+      const long long s = wave_sum_i(valid ? ek[c] : 0ll);
+      if ((int)(threadIdx.x & 63) == first && s != 0)
+        atomicAdd((unsigned long long*)&d.ws[w0].ekin_fix[c], (unsigned long long)s);
+    } else if (valid && ek[c] != 0) {
+      atomicAdd((unsigned long long*)&d.ws[w].ekin_fix[c], (unsigned long long)ek[c]);
+    }
   }
-  store_x32(d, i);
 }
 
-// temperature and Berendsen scaling factors (md.f90:3813-3914): one block per
-// window, fixed-order reduction; advances the window's step counter
-// Kinetic energy per window in two launches: kTempParts blocks per window
-// sum strided atoms (solute, solute free, solvent, solvent free) and store
-// the block's partials; k_temperature adds the partials in part order, so
-// the sum does not depend on scheduling.
-// With post >= 0 (at the end of a step) each atom is first finished as
-// k_post does it: post = 1, v = (x - xx)/dt after SHAKE; then the FP32
-// coordinates.
-__global__ void __launch_bounds__(256) k_ekin_part(Dev d, int advance, int post) {
+// v = (x - xx)/dt after SHAKE when `post` (md.f90:4117-4120), the FP32
+// copies of x, and the atom's kinetic energy into ek (solute all, solute
+// free, solvent all, solvent free)
+__device__ __forceinline__ void finish_atom(const Dev& d, const WinDev& W, int i, const double x[3],
+                                            const double xx[3], const double vlf[3], int post, long long ek[4]) {
+  double v[3];
+  for (int c = 0; c < 3; ++c) {
+    v[c] = post ? (x[c] - xx[c]) / W.dt : vlf[c];
+    d.v[3 * i + c] = v[c];
+  }
+  store_x32_xyz(d, i, x[0], x[1], x[2]);
+  const long long e = fix(0.5 * d.mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), kEScale);
+  const bool solute = (i - W.ao) < W.nat_solute, freeat = !(d.aflag[i] & 1);
+  ek[0] += solute ? e : 0ll;
+  ek[1] += solute && freeat ? e : 0ll;
+  ek[2] += solute ? 0ll : e;
+  ek[3] += !solute && freeat ? e : 0ll;
+}
+
+// k_integrate: one block per run of whole integration units (Dev::ib), in
+// three phases.  1: one atom per thread, its gradient (with the per-atom
+// terms when ATOM_TERMS) and leapfrog, the new and old coordinates into LDS.
+// 2: one unit per thread, the SHAKE sweeps on the unit's coordinates in
+// LDS.  3: one atom per thread again, finish_atom.
+template <bool ATOM_TERMS>
+__global__ void __launch_bounds__(kIntBlock) k_integrate(Dev d, int post) {
+#pragma clang fp contract(off)
+  __shared__ double s_x[kIntBlock][3], s_xx[kIntBlock][3];
+  const int4 B = d.ib[blockIdx.x];   // first iu_atom entry, atoms, first unit, units
+  const int t = threadIdx.x;
+  // the thread's SHAKE unit of the second phase and its constraints
+  int4 U = make_int4(0, 0, 0, -1);   // window, first iu_atom entry, atoms, SHAKE unit
+  int c0 = 0, nc = 0;
+  ShakeHeld h;
+  if (t < B.w) {
+    U = d.iu[B.z + t];
+    if (U.w >= 0) {
+      c0 = d.su_start[U.w];
+      nc = d.su_n[U.w];
+    }
+  }
+  shake_hold(d, c0, nc, h);
+  int i = d.ntot, w = 0;
+  bool on = false;
+  if (t < B.y) {
+    const int a = d.iu_atom[B.x + t];
+    w = d.wid[a];
+    if (d.ws[w].active) {
+      i = a;
+      on = true;
+    }
+  }
+  WinState& st = d.ws[w];
+  const WinDev& W = d.win[w];
+  long long g[3];
+  step_gradient<ATOM_TERMS>(d, i, on, g);
+  double x[3], xx[3], v[3];
+  if (on) {
+    leapfrog_vals(d, st, W, i, g, x, xx, v);
+    for (int c = 0; c < 3; ++c) {
+      s_x[t][c] = x[c];
+      s_xx[t][c] = xx[c];
+    }
+  }
+  __syncthreads();
+  if (nc > 0 && d.ws[U.x].active) {
+    WinState& su = d.ws[U.x];
+    const int l0 = U.y - B.x;
+    if (nc <= kShakeHeld) shake_lds<true>(d, su, c0, nc, &s_x[l0], &s_xx[l0], h);
+    else shake_lds<false>(d, su, c0, nc, &s_x[l0], &s_xx[l0], h);
+  }
+  __syncthreads();
+  long long ek[4] = {0, 0, 0, 0};
+  if (on) {
+    for (int c = 0; c < 3; ++c) {
+      x[c] = s_x[t][c];
+      d.x[3 * i + c] = x[c];
+    }
+    finish_atom(d, W, i, x, xx, v, post, ek);
+  }
+  wave_ekin(d, w, ek, on);
+}
+
+// FP32 coordinates (and their slot copies) from the FP64 ones, after a
+// state upload
+__global__ void k_refresh32(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.ntot) store_x32(d, i);
+}
+
+// temperature and Berendsen scaling factors (md.f90:3813-3914) outside the
+// step (at the start of a run, after a state upload): kTempParts blocks per
+// window sum strided atoms (solute, solute free, solvent, solvent free) and
+// store the block's partials; k_temperature adds the partials in part order,
+// so the sum does not depend on scheduling.  Inside the step k_integrate
+// sums the kinetic energy instead.
+__global__ void __launch_bounds__(256) k_ekin_part(Dev d) {
   const int w = blockIdx.x / kTempParts, part = blockIdx.x % kTempParts;
   if (w >= d.nwin) return;
-  const WinState& st = d.ws[w];
   const WinDev& W = d.win[w];
-  if (!st.active && advance) return;
   double acc[4] = {0, 0, 0, 0};  // solute, solute free, solvent, solvent free
   for (int li = part * blockDim.x + threadIdx.x; li < W.natom; li += kTempParts * blockDim.x) {
     const int i = W.ao + li;
-    if (post >= 0) {
-      if (post)
-        for (int c = 0; c < 3; ++c) d.v[3 * i + c] = (d.x[3 * i + c] - d.xx[3 * i + c]) / W.dt;
-      store_x32(d, i);
-    }
     const double* v = &d.v[3 * i];
     const double ek = 0.5 * d.mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     const bool solute = li < W.nat_solute;
@@ -2646,8 +2696,11 @@ __global__ void __launch_bounds__(256) k_ekin_part(Dev d, int advance, int post)
   }
 }
 
-// temperatures and Berendsen factors from the partial sums (one thread per
-// window); with `advance`, the ring records of the step and the step count
+// temperatures and Berendsen factors (one thread per window) from the
+// kinetic-energy partials of k_ekin_part, or with `advance` (the end of a
+// step) from the fixed-point sums of k_integrate, which it clears for the
+// next step; with `advance` also the ring records of the step and the step
+// count
 __global__ void __launch_bounds__(64) k_temperature(Dev d, int advance) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= d.nwin) return;
@@ -2655,8 +2708,15 @@ __global__ void __launch_bounds__(64) k_temperature(Dev d, int advance) {
   const WinDev& W = d.win[w];
   if (!st.active && advance) return;
   double t[4] = {0, 0, 0, 0};
-  for (int p = 0; p < kTempParts; ++p)
-    for (int k = 0; k < 4; ++k) t[k] += d.t_part[((size_t)w * kTempParts + p) * 4 + k];
+  if (advance) {
+    for (int k = 0; k < 4; ++k) {
+      t[k] = (double)st.ekin_fix[k] * (1.0 / kEScale);
+      st.ekin_fix[k] = 0;
+    }
+  } else {
+    for (int p = 0; p < kTempParts; ++p)
+      for (int k = 0; k < 4; ++k) t[k] += d.t_part[((size_t)w * kTempParts + p) * 4 + k];
+  }
   {
     const double boltz = (double)0.001986f;
     const double tfree_e = t[1] + t[3];
@@ -2703,7 +2763,7 @@ void launch_lrf_build(const Dev& d, hipStream_t s) {
   if (!d.nlrfw) return;
   k_lrf_pack<<<blocks(std::max(d.ngrp_all, d.nls), 256), 256, 0, s>>>(d);
   k_lrf_build<<<d.nlrfw, d.bs_lrf, 0, s>>>(d);
-  if (d.lrf_parts > 1) k_lrf_reduce<<<blocks(20LL * d.ngrp_all, 256), 256, 0, s>>>(d);
+  k_lrf_reduce<<<blocks(20LL * d.ngrp_all, 256), 256, 0, s>>>(d);
 }
 
 void launch_bonded(const Dev& d, hipStream_t s) {
@@ -2898,19 +2958,18 @@ void launch_other_forces(const Dev& d, hipStream_t s, bool atom_terms) {
 }
 
 void launch_integrate(const Dev& d, hipStream_t s, int shake_any, bool atom_terms) {
-  if (atom_terms) k_leapfrog<true><<<blocks(d.ntot, 256), 256, 0, s>>>(d);
-  else k_leapfrog<false><<<blocks(d.ntot, 256), 256, 0, s>>>(d);
-  if (d.nsu) k_shake<<<blocks(d.nsu, 64), 64, 0, s>>>(d);
-  k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d, 1, shake_any ? 1 : 0);
+  if (!d.nib) return;
+  if (atom_terms) k_integrate<true><<<d.nib, kIntBlock, 0, s>>>(d, shake_any ? 1 : 0);
+  else k_integrate<false><<<d.nib, kIntBlock, 0, s>>>(d, shake_any ? 1 : 0);
   k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 1);
 }
 
 void launch_temperature(const Dev& d, hipStream_t s) {
-  k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d, 0, -1);
+  k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d);
   k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 0);
 }
 
-void launch_refresh32(const Dev& d, hipStream_t s) { k_post<<<blocks(d.ntot, 256), 256, 0, s>>>(d, -1); }
+void launch_refresh32(const Dev& d, hipStream_t s) { k_refresh32<<<blocks(d.ntot, 256), 256, 0, s>>>(d); }
 
 
 
