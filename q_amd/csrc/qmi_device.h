@@ -153,7 +153,7 @@ struct Dev {
   const WinDev* win;
   WinState* ws;
   // atoms
-  double *x, *v, *xx, *xtop;
+  double *x, *v, *xtop;
   float4* x32;          // x, y, z, scaled charge
   // the water atoms' x32 again, in water-cluster slot order: slot
   // c*8 + m (cluster c, member m) holds O, H1, H2 at 3*slot + 0..2, so the
@@ -169,6 +169,7 @@ struct Dev {
   const int* qmslot;
   float4* ljp;          // avdw1, bvdw1, avdw3, bvdw3
   long long* f;         // gradient, 3 per atom, fixed point 2^-32
+  long long* fterm;     // per-atom terms of the step's gradient (LRF evaluation, atom restraints), fixed point
   const float* winv;
   const double* mass;
   const int* wid;       // window of atom

@@ -800,9 +800,9 @@ static int build_device(qmi_engine* e) {
   WinDev* wdp; UP(wdp, wd); d.win = wdp;
   UP(d.ws, wst);
   UP(d.x, x); UP(d.v, v); UP(d.xtop, xtop);
-  if ((rc = dev_alloc(e, &d.xx, x.size())) != 0) return rc;
   UP(d.x32, x32); UP(d.ljp, ljp);
   if ((rc = dev_alloc(e, &d.f, x.size())) != 0) return rc;
+  if ((rc = dev_alloc(e, &d.fterm, x.size())) != 0) return rc;
   float* fwinv; UP(fwinv, winv); d.winv = fwinv;
   double* fmass; UP(fmass, mass); d.mass = fmass;
   int* ip;
@@ -1959,7 +1959,7 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
     if (launch_step_forces(e->d, e->stream, e->side_stream[0], e->side_stream[1], e->step_ev, lists, e->ngrp, e->nwc,
                            e->nsc, qen) != hipSuccess)
       return;
-    launch_integrate(e->d, e->stream, e->shake_any, true);
+    launch_integrate(e->d, e->stream, e->shake_any);
   };
   if (!e->timing_on && !e->step_exec[0] && !e->graph_failed) {
     bool ok = true;
@@ -2099,7 +2099,8 @@ int qmi_run(qmi_engine* e, long long nsteps, int finalize) {
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[3], e->stream));
       launch_other_forces(e->d, e->stream, false);
       if (e->timing_on) HIP_OK(hipEventRecord(e->ev[4], e->stream));
-      launch_integrate(e->d, e->stream, e->shake_any, true);
+      launch_atom_terms(e->d, e->stream);
+      launch_integrate(e->d, e->stream, e->shake_any);
       if (e->timing_on) {
         HIP_OK(hipEventRecord(e->ev[5], e->stream));
         HIP_OK(hipEventSynchronize(e->ev[5]));

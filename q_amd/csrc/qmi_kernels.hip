@@ -2043,11 +2043,23 @@ __device__ __forceinline__ void restr_atom(const Dev& d, int i, long long* acc) 
 
 // the per-atom terms of a step in one pass: LRF evaluation, then the
 // fixed, shell and solvent restraints (one thread per atom; every lane
-// reaches the wave energy sums of both)
+// reaches the wave energy sums of both).  STORE: the atom's sum of them is
+// stored in fterm, which k_integrate adds to the gradient (an MD step, where
+// it runs beside the pair kernels); else it is added to the gradient with
+// atomics (an evaluation)
+template <bool STORE>
 __global__ void __launch_bounds__(256) k_atom_terms(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  lrf_taylor_atom<false>(d, i, nullptr);
-  restr_atom<false>(d, i, nullptr);
+  if (STORE) {
+    long long g[3] = {0, 0, 0};
+    lrf_taylor_atom<true>(d, i, g);
+    restr_atom<true>(d, i, g);
+    if (i < d.ntot)
+      for (int c = 0; c < 3; ++c) d.fterm[3 * i + c] = g[c];
+  } else {
+    lrf_taylor_atom<false>(d, i, nullptr);
+    restr_atom<false>(d, i, nullptr);
+  }
 }
 
 // p_restrain (md.f90:12940-13274): one 64-lane block per restraint record.
@@ -2453,21 +2465,13 @@ __device__ __forceinline__ void leapfrog_vals(const Dev& d, const WinState& st, 
 }
 
 // gradient of atom i for the integrator: the step's per-atom terms (LRF
-// evaluation, fixed/shell and radial solvent restraints) when ATOM_TERMS,
-// summed with the other kernels' gradient in registers and stored back, so
-// the gradient array ends complete without per-atom atomics.  Every lane
-// calls it (wave energy sums inside); i = d.ntot does nothing.
-template <bool ATOM_TERMS>
-__device__ __forceinline__ void step_gradient(const Dev& d, int i, bool valid, long long g[3]) {
-  g[0] = g[1] = g[2] = 0;
-  if (ATOM_TERMS) {
-    lrf_taylor_atom<true>(d, i, g);
-    restr_atom<true>(d, i, g);
-  }
-  if (!valid) return;
+// evaluation, fixed/shell and radial solvent restraints, stored by
+// k_atom_terms<true>) and the other kernels' gradient, stored back, so the
+// gradient array ends complete without per-atom atomics
+__device__ __forceinline__ void step_gradient(const Dev& d, int i, long long g[3]) {
   for (int c = 0; c < 3; ++c) {
-    g[c] += d.f[3 * i + c];
-    if (ATOM_TERMS) d.f[3 * i + c] = g[c];
+    g[c] = d.fterm[3 * i + c] + d.f[3 * i + c];
+    d.f[3 * i + c] = g[c];
   }
 }
 
@@ -2592,11 +2596,9 @@ __device__ __forceinline__ void finish_atom(const Dev& d, const WinDev& W, int i
 }
 
 // k_integrate: one block per run of whole integration units (Dev::ib), in
-// three phases.  1: one atom per thread, its gradient (with the per-atom
-// terms when ATOM_TERMS) and leapfrog, the new and old coordinates into LDS.
-// 2: one unit per thread, the SHAKE sweeps on the unit's coordinates in
-// LDS.  3: one atom per thread again, finish_atom.
-template <bool ATOM_TERMS>
+// three phases.  1: one atom per thread, its gradient and leapfrog, the new
+// and old coordinates into LDS.  2: one unit per thread, the SHAKE sweeps on
+// the unit's coordinates in LDS.  3: one atom per thread again, finish_atom.
 __global__ void __launch_bounds__(kIntBlock) k_integrate(Dev d, int post) {
 #pragma clang fp contract(off)
   __shared__ double s_x[kIntBlock][3], s_xx[kIntBlock][3];
@@ -2626,10 +2628,10 @@ __global__ void __launch_bounds__(kIntBlock) k_integrate(Dev d, int post) {
   }
   WinState& st = d.ws[w];
   const WinDev& W = d.win[w];
-  long long g[3];
-  step_gradient<ATOM_TERMS>(d, i, on, g);
   double x[3], xx[3], v[3];
   if (on) {
+    long long g[3];
+    step_gradient(d, i, g);
     leapfrog_vals(d, st, W, i, g, x, xx, v);
     for (int c = 0; c < 3; ++c) {
       s_x[t][c] = x[c];
@@ -2891,11 +2893,12 @@ void launch_nonbonded(const Dev& d, hipStream_t s, bool with_ww, bool q_energy) 
 // milliseconds of FP64 work: beside the row builds and pair kernels it
 // starves them (they wait for its long-running blocks), and cutting its
 // occupancy so that they fit slows it more than it frees them.  Only the
-// leapfrog kernel reads the expansions.
+// per-atom terms (k_atom_terms<true>) read the expansions.
 // All of them write the gradient and the energies only through integer
 // atomic adds, so the result does not depend on how they overlap.  The
-// per-atom terms (LRF evaluation, atom restraints) follow in the leapfrog
-// kernel of launch_integrate(..., true).
+// per-atom terms (LRF evaluation, atom restraints) go to their own array,
+// which the integrator adds: after the pair kernels of the water stream, or
+// after the LRF build on list steps.
 hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipStream_t s3, const hipEvent_t ev[6],
                               bool lists, int ngrp, int nwc, int nsc, bool q_energy) {
   hipError_t err = hipEventRecord(ev[0], s);
@@ -2928,12 +2931,17 @@ hipError_t launch_step_forces(const Dev& d, hipStream_t s, hipStream_t s2, hipSt
   launch_pw_side(d, s, q_energy, false);
   if (q_energy) launch_q_energy_qp(d, s);
   if (lists) {
-    // the LRF build once every pair kernel of the step is done
+    // the LRF build once every pair kernel of the step is done, then the
+    // per-atom terms that evaluate its expansions
     err = hipEventRecord(ev[4], s);
     if (err == hipSuccess) err = hipStreamWaitEvent(s2, ev[4], 0);
     if (err == hipSuccess) err = hipStreamWaitEvent(s2, ev[1], 0);
     if (err != hipSuccess) return err;
     launch_lrf_build(d, s2);
+    launch_atom_terms(d, s2);
+  } else {
+    // the stream of the water kernels finishes first
+    launch_atom_terms(d, s);
   }
   err = hipEventRecord(ev[3], s2);
   if (err == hipSuccess) err = hipStreamWaitEvent(s, ev[1], 0);
@@ -2949,7 +2957,7 @@ void launch_other_forces(const Dev& d, hipStream_t s, bool atom_terms) {
     k_watpol_shell<<<d.nwin * kWpolMaxShell, kWpolBlock, 0, s>>>(d);
     k_watpol_force<<<blocks(d.nwat_total, 256), 256, 0, s>>>(d);
   }
-  if (atom_terms) k_atom_terms<<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+  if (atom_terms) k_atom_terms<false><<<blocks(d.ntot, 256), 256, 0, s>>>(d);
   int nb = d.nbond + d.nang + d.ntor + d.nimp + d.nqb + d.nqa + d.nqt + d.nqi;
   launch_bonded(d, s);
   // the gather also collects the q-q pair slots written by k_qq
@@ -2957,10 +2965,12 @@ void launch_other_forces(const Dev& d, hipStream_t s, bool atom_terms) {
   if (d.nrst) k_prestr<<<d.nrst, 64, 0, s>>>(d);
 }
 
-void launch_integrate(const Dev& d, hipStream_t s, int shake_any, bool atom_terms) {
-  if (!d.nib) return;
-  if (atom_terms) k_integrate<true><<<d.nib, kIntBlock, 0, s>>>(d, shake_any ? 1 : 0);
-  else k_integrate<false><<<d.nib, kIntBlock, 0, s>>>(d, shake_any ? 1 : 0);
+void launch_atom_terms(const Dev& d, hipStream_t s) {
+  if (d.ntot) k_atom_terms<true><<<blocks(d.ntot, 256), 256, 0, s>>>(d);
+}
+
+void launch_integrate(const Dev& d, hipStream_t s, int shake_any) {
+  if (d.nib) k_integrate<<<d.nib, kIntBlock, 0, s>>>(d, shake_any ? 1 : 0);
   k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 1);
 }
 
