@@ -242,6 +242,7 @@ struct Dev {
   const double* q_scd;
   // topology LJ params in double per window type (energy path)
   const double* t_lj;    // [(t_off + t)*4]
+  const double4* ljd;    // per atom: its type's row of t_lj
   // tiles (static Q lists) and neighbour rows (dynamic lists, index T_WW..T_PP)
   TileList tl[T_NTYPES];
   RowList rl[4];

@@ -982,6 +982,17 @@ static int build_device(qmi_engine* e) {
   UP(fp_, q_sc); d.q_sc = fp_;
   UP(dp, q_scd); d.q_scd = dp;
   UP(dp, t_lj); d.t_lj = dp;
+  {
+    // per atom: its type's FP64 LJ columns, for the energy passes' gathers
+    std::vector<double4> ljd(std::max<size_t>(iac.size(), 1), make_double4(0, 0, 0, 0));
+    for (int wi = 0; wi < NW; ++wi)
+      for (int a = wd[wi].ao; a < wd[wi].ao + wd[wi].natom; ++a) {
+        const double* r = &t_lj[(size_t)(wd[wi].t_off + iac[a]) * 4];
+        ljd[a] = make_double4(r[0], r[1], r[2], r[3]);
+      }
+    double4* l4;
+    UP(l4, ljd); d.ljd = l4;
+  }
   d.nqq = (int)qq_ij.size();
   int4* i4;
   UP(i4, qq_ij); d.qq_ij = i4;

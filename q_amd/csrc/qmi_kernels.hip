@@ -823,16 +823,16 @@ __device__ __forceinline__ void pp_impl(const Dev& d) {
         g0 -= dv * dx; g1 -= dv * dy; g2 -= dv * dz;
         if (en && ea < eb) {
           const double ddx = d.x[3 * eb] - xs[0], ddy = d.x[3 * eb + 1] - xs[1], ddz = d.x[3 * eb + 2] - xs[2];
+          const double4 tlj = d.ljd[eb];   // the j atom's type row, without the type lookup
           const double rinv_d = rsqrt_e(ddx * ddx + ddy * ddy + ddz * ddz);
           const double r2i = rinv_d * rinv_d, r6d = r2i * r2i * r2i;
-          const double* tlj = &d.t_lj[(size_t)(W.t_off + d.iac[eb]) * 4];
           double ad, bd;
           double vd = (double)(pi.w * pj[u].w) * rinv_d;
           if (is14) {
             vd *= W.el14d;
-            lj_ab(tli[2], tli[3], tlj[2], tlj[3], arith, &ad, &bd);
+            lj_ab(tli[2], tli[3], tlj.z, tlj.w, arith, &ad, &bd);
           } else {
-            lj_ab(tli[0], tli[1], tlj[0], tlj[1], arith, &ad, &bd);
+            lj_ab(tli[0], tli[1], tlj.x, tlj.y, arith, &ad, &bd);
           }
           eel += vd;
           evdw += ad * r6d * r6d - bd * r6d;
@@ -2698,17 +2698,25 @@ __global__ void __launch_bounds__(256) k_ekin_part(Dev d) {
   }
 }
 
-// temperatures and Berendsen factors (one thread per window) from the
+// temperatures and Berendsen factors (one wave per window) from the
 // kinetic-energy partials of k_ekin_part, or with `advance` (the end of a
 // step) from the fixed-point sums of k_integrate, which it clears for the
 // next step; with `advance` also the ring records of the step and the step
-// count
+// count.  The lanes copy the energy accumulators into the step's ring slot,
+// lane 0 does the rest.
 __global__ void __launch_bounds__(64) k_temperature(Dev d, int advance) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.x;
   if (w >= d.nwin) return;
   WinState& st = d.ws[w];
   const WinDev& W = d.win[w];
-  if (!st.active && advance) return;
+  if (!st.active && advance) return;   // uniform over the block
+  if (advance && st.energy_now) {
+    const int slot = (st.istep - st.ring_base) % d.ring_n;
+    long long* er = &d.e_ring[((size_t)slot * d.nwin + w) * E_NSLOT];
+    for (int k = threadIdx.x; k < E_NSLOT; k += blockDim.x) er[k] = st.E[k];
+  }
+  __syncthreads();   // every lane has read istep before lane 0 advances it
+  if (threadIdx.x != 0) return;
   double t[4] = {0, 0, 0, 0};
   if (advance) {
     for (int k = 0; k < 4; ++k) {
@@ -2744,10 +2752,6 @@ __global__ void __launch_bounds__(64) k_temperature(Dev d, int advance) {
       tr[0] = st.temp; tr[1] = st.tfree; tr[2] = st.ekin;
       tr[3] = (double)st.istep + 4294967296.0 * (double)st.run_epoch;
       tr[4] = st.tfree_solute; tr[5] = st.tfree_solvent;
-      if (st.energy_now) {
-        long long* er = &d.e_ring[((size_t)slot * d.nwin + w) * E_NSLOT];
-        for (int k = 0; k < E_NSLOT; ++k) er[k] = st.E[k];
-      }
       st.istep += 1;
     }
   }
@@ -2971,12 +2975,12 @@ void launch_atom_terms(const Dev& d, hipStream_t s) {
 
 void launch_integrate(const Dev& d, hipStream_t s, int shake_any) {
   if (d.nib) k_integrate<<<d.nib, kIntBlock, 0, s>>>(d, shake_any ? 1 : 0);
-  k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 1);
+  k_temperature<<<d.nwin, 64, 0, s>>>(d, 1);
 }
 
 void launch_temperature(const Dev& d, hipStream_t s) {
   k_ekin_part<<<d.nwin * kTempParts, 256, 0, s>>>(d);
-  k_temperature<<<blocks(d.nwin, 64), 64, 0, s>>>(d, 0);
+  k_temperature<<<d.nwin, 64, 0, s>>>(d, 0);
 }
 
 void launch_refresh32(const Dev& d, hipStream_t s) { k_refresh32<<<blocks(d.ntot, 256), 256, 0, s>>>(d); }
