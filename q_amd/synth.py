@@ -1214,7 +1214,8 @@ def write_fep(s: System, path: str) -> None:
 def write_inp(path: str, top: str, fep: str | None, final: str, *, steps: int, stepsize: float = 1.0,
               temperature: float = 300.0, seed: int = 1, lambdas: tuple | None = None, lrf: bool = True,
               rc_solute: float = 11.0, rc_solvent: float = 10.0, rc_q: float = 99.0, rc_lrf: float = 99.0,
-              shake_solvent: bool = True, shake_hydrogens: bool = False, nb_cycle: int = 25,
+              shake_solvent: bool = True, shake_hydrogens: bool = False, shake_solute: bool = False,
+              nb_cycle: int = 25,
               output: int = 5, energy: int = 10, energy_file: str | None = None,
               restart: str | None = None, bath_coupling: float = 10.0,
               separate_scaling: bool = True, polarization: bool = True, trajectory: int = 0,
@@ -1226,7 +1227,7 @@ def write_inp(path: str, top: str, fep: str | None, final: str, *, steps: int, s
     if restart is None:
         out += [f"random_seed {seed}", f"initial_temperature {temperature}"]
     out += [f"shake_solvent {onoff(shake_solvent)}", f"shake_hydrogens {onoff(shake_hydrogens)}",
-            "shake_solute off", f"lrf {onoff(lrf)}", f"separate_scaling {onoff(separate_scaling)}", "",
+            f"shake_solute {onoff(shake_solute)}", f"lrf {onoff(lrf)}", f"separate_scaling {onoff(separate_scaling)}", "",
             "[cut-offs]", f"solute_solute {rc_solute}", f"solute_solvent {rc_solute}",
             f"solvent_solvent {rc_solvent}", f"q_atom {rc_q}"]
     if lrf:
@@ -1292,6 +1293,9 @@ CONFIGS = {
     # sphere's exclusion radius (nbqplist / nbqwlist, md.f90:7903, 8086)
     "ligqcut": (ligand_in_water, {"radius": 20.0}, {"lrf": True, "rc_q": 9.0}),
     "protqcut": (protein_leg, {"radius": 25.0}, {"lrf": True, "shake_hydrogens": True, "rc_q": 12.0}),
+    # every solute bond constrained (shake_solute): the ligand is one SHAKE
+    # component of some fifty constraints
+    "ligshake": (ligand_in_water, {"radius": 20.0}, {"lrf": True, "shake_solute": True}),
 }
 
 

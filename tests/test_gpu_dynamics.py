@@ -51,7 +51,7 @@ def rel_err(a, b):
 
 @pytest.mark.parametrize("cfg,seed", [("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2), ("ligtip3", 0),
                                       ("ligfepx", 0), ("liglibq", 0), ("ligarithlibq", 0), ("ligrestr", 0),
-                                      ("ligdual", 0), ("ligqcut", 0), ("protqcut", 2)])
+                                      ("ligdual", 0), ("ligqcut", 0), ("protqcut", 2), ("ligshake", 0)])
 def test_100_steps(golden_inp, cfg, seed):
     from q_amd.engine import Engine
     g = dict(np.load(os.path.join(GOLD, f"{cfg}_s{seed}.npz")))

@@ -32,7 +32,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # exclusion radius, nbqplist / nbqwlist)
 CASES = {"water": 0, "ligwat": 0, "protein": 2, "kat": 0, "protarith": 2, "ligmaxpot": 0, "ligarithmax": 0,
          "ligtip3": 0, "ligfepx": 0, "liglibq": 0, "ligarithlibq": 0, "ligrestr": 0, "ligdual": 0,
-         "ligqcut": 0, "protqcut": 2}
+         "ligqcut": 0, "protqcut": 2, "ligshake": 0}
 
 
 def _rel(a, b, floor=1.0):

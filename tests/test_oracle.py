@@ -26,7 +26,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # and with a q_atom cut-off that selects the Q-atom lists) and the
 # reference's own test system
 CASES = [("water", 0), ("ligwat", 0), ("protein", 2), ("kat", 0), ("protarith", 2), ("ligtip3", 0), ("ligfepx", 0),
-         ("liglibq", 0), ("ligarithlibq", 0), ("ligrestr", 0), ("ligqcut", 0)]
+         ("liglibq", 0), ("ligarithlibq", 0), ("ligrestr", 0), ("ligqcut", 0), ("ligshake", 0)]
 
 
 def load_gold(cfg, seed):
