@@ -603,7 +603,8 @@ def main(argv=None):
     v.add_argument("--pilot-steps", type=int, default=1000)
     v.add_argument("--prod-steps", type=int, default=15000)
     v.add_argument("--pilot-skip", type=int, default=40)
-    v.add_argument("--prod-skip", type=int, default=1000)
+    v.add_argument("--prod-skip", type=int, default=1000,
+                   help="energy records (not steps) dropped from the start of each ladder window")
     v.add_argument("--procs", type=int, default=16)
     v.add_argument("--seed", type=int, default=0)
     v.add_argument("--legs", default="ligwat,protein", help="comma-separated legs to evaluate")
