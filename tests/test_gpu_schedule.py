@@ -5,15 +5,14 @@ energy is compared with the uniform 51-window ladder.
 
 Tolerance: the ladders' free energies carry sampling noise far above the
 0.3 kcal/mol of a converged comparison.  Measured on four held-out
-ligand-in-water legs with 150 ps windows (the last 50 ps analysed) and 4
-Maxwell seeds each (q_amd/data/schedule_eval_150ps.json), the paired
-difference of one seed's
-learned and 51-window ladders scatters with a standard deviation of 0.2-1.7
-kcal/mol per leg, 1.1 kcal/mol RMS over the legs; more sampled time did not
-shrink it (the 51-window ladder's own replica spread is 0.3-0.8 kcal/mol
-there, as with 30 ps analysed per window).  This test's 4 ps windows scatter more.  So agreement is
-asserted at 3 kcal/mol, 2.7 times that RMS, which a ladder with a gap in its
-overlap misses by far; the time budget and the
+ligand-in-water legs with 150 ps windows and 4 Maxwell seeds each
+(q_amd/data/schedule_eval_150ps.json), the paired difference of one seed's
+learned and 51-window ladders scatters with a standard deviation of 0.3-1.2
+kcal/mol per leg, 0.8 kcal/mol RMS over the legs; more sampled time did not
+shrink it (the 51-window ladder's own replica spread is 0.3-0.7 kcal/mol
+there, 0.4 with 30 ps analysed per window).  This test's 4 ps windows scatter
+more.  So agreement is asserted at 3 kcal/mol, under 4 times that RMS, which
+a ladder with a gap in its overlap misses by far; the time budget and the
 window count are exact."""
 import json
 
